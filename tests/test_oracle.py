@@ -1,0 +1,271 @@
+"""Pins the CPU oracle (oracle/) before it is trusted as the parity checker.
+
+Each check is independent of the oracle's own arithmetic: a brute-force tree
+generator written from the grammar (tests/treegen.py), Python big-integer
+evaluation, closed-form known answers, and invariants of the hash mixers.
+"""
+import random
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.treegen import TOK_OP, py_eval, trees
+
+
+@pytest.mark.parametrize("L", [1, 2, 3, 4])
+def test_count_table_matches_brute_force(L):
+    T = orc.count_table(L, 7)
+    for s in range(1, 8 if L <= 2 else 7):
+        ts = list(trees(s, L))
+        assert int(T[s][8]) == len(ts)
+        if s > 1:
+            for op in range(8):
+                assert int(T[s][op]) == sum(1 for t in ts if t[-1] == TOK_OP + op)
+        else:
+            assert all(int(T[1][op]) == 0 for op in range(8))
+
+
+def test_count_table_overflow_guard():
+    orc.count_table(5, 20)  # 3.38e18 < 2^64
+    with pytest.raises(OverflowError):
+        orc.count_table(5, 21)  # 3.48e19 > 2^64
+
+
+@pytest.mark.parametrize("L", [2, 3])
+def test_unrank_is_the_enumeration_order(L):
+    T = orc.count_table(L, 6)
+    for s in range(1, 7):
+        for r, t in enumerate(trees(s, L)):
+            got = orc.unrank(T, s, r)
+            assert tuple(got) == t, (s, r)
+            assert orc.rank(T, got) == r
+
+
+def test_unrank_round_trip_large_sizes():
+    rng = random.Random(7)
+    for L in (3, 5):
+        T = orc.count_table(L, 20)
+        for _ in range(300):
+            s = rng.randint(1, 20)
+            r = rng.randrange(int(T[s][8]))
+            assert orc.rank(T, orc.unrank(T, s, r)) == r
+
+
+@pytest.mark.parametrize("W", [32, 64])
+def test_eval_matches_python_integers(W):
+    rng = random.Random(W)
+    L = 4
+    T = orc.count_table(L, 12)
+    for _ in range(400):
+        s = rng.randint(1, 12)
+        rpn = orc.unrank(T, s, rng.randrange(int(T[s][8])))
+        vals = [rng.getrandbits(64) for _ in range(L)]
+        assert orc.evaluate(rpn, vals, W) == py_eval(rpn, vals, W)
+
+
+def test_eval_textbook_identities():
+    # (x ^ y) + 2*(x & y) == x + y ;  ~x == -x - 1  (mod 2^W)
+    rng = random.Random(3)
+    for W in (32, 64):
+        m = (1 << W) - 1
+        for _ in range(50):
+            x, y = rng.getrandbits(W), rng.getrandbits(W)
+            assert orc.lib().orc_apply(5, x ^ y, (2 * (x & y)) & m, W) == (x + y) & m
+            assert orc.lib().orc_apply(0, x, 0, W) == (-x - 1) & m
+            assert orc.lib().orc_apply(7, x, y, W) == (x * y) & m
+
+
+def _brute_first(inputs, outputs, n_vars, consts, W, min_size, max_size):
+    L = n_vars + len(consts)
+    n = len(outputs)
+    leafvals = [[inputs[i * n_vars + v] for v in range(n_vars)] + list(consts)
+                for i in range(n)]
+    for s in range(min_size, max_size + 1):
+        for r, t in enumerate(trees(s, L)):
+            if all(py_eval(t, leafvals[i], W) == outputs[i] & ((1 << W) - 1)
+                   for i in range(n)):
+                return s, r, t
+    return None
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_simba_search_is_min_rank_brute_force(seed):
+    rng = random.Random(seed)
+    W = 32 if seed % 2 else 64
+    n_vars = rng.randint(1, 3)
+    consts = [1] if seed % 3 == 0 else []
+    L = n_vars + len(consts)
+    n = 6
+    inputs = [rng.getrandbits(W) for _ in range(n * n_vars)]
+    T = orc.count_table(L, 6)
+    ts = rng.randint(1, 5)
+    target = orc.unrank(T, ts, rng.randrange(int(T[ts][8])))
+    leafvals = [[inputs[i * n_vars + v] for v in range(n_vars)] + consts for i in range(n)]
+    outputs = [py_eval(target, leafvals[i], W) for i in range(n)]
+    want = _brute_first(inputs, outputs, n_vars, consts, W, 1, 5)
+    for threads in (1, 4):
+        got = orc.simba_search(inputs, outputs, n_vars, consts, W, 1, 5, threads)
+        assert got["status"] == 0
+        assert (got["size"], got["rank"], tuple(got["rpn"])) == want
+
+
+def test_simba_known_answers():
+    rng = np.random.default_rng(5)
+    x = rng.integers(0, 2**32, size=16, dtype=np.uint64)
+    y = rng.integers(0, 2**32, size=16, dtype=np.uint64)
+    inp = np.stack([x, y], axis=1).reshape(-1)
+    m = np.uint64(0xFFFFFFFF)
+    # target v0 with the reference size range (compat: sizes 2..maxLen-1):
+    r = orc.simba_search(inp, x, 2, (), 32, 2, 19)
+    assert r["expr"] == "(~(~v0))" and r["size"] == 3 and r["rank"] == 0
+    # native range finds the leaf itself
+    r = orc.simba_search(inp, x, 2, (), 32, 1, 9)
+    assert r["expr"] == "v0" and r["size"] == 1
+    r = orc.simba_search(inp, (x + y) & m, 2, (), 32, 1, 9)
+    assert r["expr"] == "(v0 + v1)"
+    r = orc.simba_search(inp, (~y) & m, 2, (), 32, 1, 9)
+    assert r["expr"] == "(~v1)"
+    r = orc.simba_search(inp, (x * y) & m, 2, (), 32, 1, 9)
+    assert r["expr"] == "(v0 * v1)"
+    r = orc.simba_search(inp, (x - y) & m, 2, (), 32, 1, 9)
+    assert r["expr"] == "(v0 - v1)"
+    r = orc.simba_search(inp, (y - x) & m, 2, (), 32, 1, 9)
+    assert r["expr"] == "(v1 - v0)"
+    # a constant leaf (extension): x + 1 is (v0 + 1) in SIMBA order? -(~x) is
+    # size 3 too and comes first (neg block precedes '+').
+    r = orc.simba_search(x, (x + np.uint64(1)) & m, 1, (1,), 32, 1, 9)
+    assert r["expr"] == "(-(~v0))"
+
+
+def test_simba_infix_format():
+    T = orc.count_table(2, 5)
+    # rank 0 of size 3 is ~(~v0) (nocache.cu:162-195 prints unary as (~X))
+    assert orc.infix(orc.unrank(T, 3, 0), 2, W=32) == "(~(~v0))"
+    assert orc.to_reference_rpn(orc.unrank(T, 3, 0), 2) == "0~~"
+
+
+def test_fmix_are_bijections_and_fix_zero():
+    # MurmurHash3 finalizers: xorshift and odd multiplications are invertible.
+    lib = orc.lib()
+    assert lib.orc_fmix32(0) == 0 and lib.orc_fmix64(0) == 0
+    rng = random.Random(11)
+    xs32 = [rng.getrandbits(32) for _ in range(20000)]
+    assert len({lib.orc_fmix32(x) for x in xs32}) == len(set(xs32))
+    xs = [rng.getrandbits(64) for _ in range(20000)]
+    assert len({lib.orc_fmix64(x) for x in xs}) == len(set(xs))
+
+
+def test_fingerprint_depends_on_order_and_values():
+    a = [1, 2, 3, 4]
+    assert orc.fingerprint(a, 32) == orc.fingerprint(list(a), 32)
+    assert orc.fingerprint(a, 32) != orc.fingerprint([2, 1, 3, 4], 32)
+    assert orc.fingerprint(a, 64) != orc.fingerprint(a, 32)
+
+
+def _vfb_exact(inputs, outputs, n_vars, consts, W, max_len):
+    """VFB with exact-signature dedup (no hashing): independent restatement
+    of the intended keep-iff-new semantics, full-cache.cu:568-663."""
+    m = (1 << W) - 1
+    n = len(outputs)
+    L = n_vars + len(consts)
+    rows, levels = [], {1: []}
+    seen = set()
+    target = tuple(o & m for o in outputs)
+    for v in range(L):
+        sig = tuple((inputs[i * n_vars + v] if v < n_vars else consts[v - n_vars]) & m
+                    for i in range(n))
+        rows.append((sig, None))
+        levels[1].append(len(rows) - 1)
+        if sig == target:
+            return 1, [L], f"v{v}"
+        seen.add(sig)
+    counts = [0, L]
+    ops = [lambda a, b: ~a, lambda a, b: a & b, lambda a, b: a | b,
+           lambda a, b: a ^ b, lambda a, b: -a, lambda a, b: a + b,
+           lambda a, b: a - b, lambda a, b: a * b]
+    for lev in range(2, max_len + 1):
+        levels[lev] = []
+        for op in range(8):
+            pairs = []
+            if op in (0, 4):
+                pairs = [(l, None) for l in levels[lev - 1]]
+            else:
+                for i in range(1, (lev - 1) // 2 + 1):
+                    for l in levels[i]:
+                        for r in levels[lev - 1 - i]:
+                            pairs.append((l, r))
+                            if op == 6:
+                                pairs.append((r, l))
+            for l, r in pairs:
+                A = rows[l][0]
+                B = rows[r][0] if r is not None else A
+                sig = tuple(ops[op](a, b) & m for a, b in zip(A, B))
+                if sig in seen:
+                    continue
+                seen.add(sig)
+                rows.append((sig, (op, l, r)))
+                levels[lev].append(len(rows) - 1)
+                if sig == target:
+                    counts.append(len(levels[lev]))
+                    return lev, counts, _vfb_str(rows, len(rows) - 1, n_vars)
+        counts.append(len(levels[lev]))
+    return None, counts, "Not found !"
+
+
+def _vfb_str(rows, idx, n_vars):
+    d = rows[idx][1]
+    if d is None:
+        return f"v{idx}"
+    op, l, r = d
+    ch = "~&|^-+-*"[op]
+    if op in (0, 4):
+        return f"{ch}({_vfb_str(rows, l, n_vars)})"
+    return f"({_vfb_str(rows, l, n_vars)}){ch}({_vfb_str(rows, r, n_vars)})"
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_vfb_matches_exact_signature_dedup(seed):
+    rng = random.Random(100 + seed)
+    W = 32 if seed % 2 == 0 else 64
+    n_vars = 2 + seed % 2
+    n = 8
+    inputs = [rng.getrandbits(W) for _ in range(n * n_vars)]
+    T = orc.count_table(n_vars, 8)
+    ts = rng.randint(3, 6)
+    target = orc.unrank(T, ts, rng.randrange(int(T[ts][8])))
+    lv = [[inputs[i * n_vars + v] for v in range(n_vars)] for i in range(n)]
+    outputs = [py_eval(target, lv[i], W) for i in range(n)]
+    lev, counts, expr = _vfb_exact(inputs, outputs, n_vars, [], W, 7)
+    got = orc.vfb_search(inputs, outputs, n_vars, (), W, 7)
+    if lev is None:
+        assert got["status"] == 1
+    else:
+        assert got["status"] == 0 and got["size"] == lev
+        assert got["expr"] == expr
+    assert got["level_new"][1:len(counts)] == counts[1:]
+
+
+def test_vfb_exhaustive_level_counts_match_exact_dedup():
+    rng = random.Random(9)
+    n_vars, n, W = 2, 8, 32
+    inputs = [rng.getrandbits(W) for _ in range(n * n_vars)]
+    outputs = [0xDEADBEEF ^ i for i in range(n)]  # unreachable target
+    _, counts, _ = _vfb_exact(inputs, outputs, n_vars, [], W, 6)
+    got = orc.vfb_search(inputs, outputs, n_vars, (), W, 6, exhaustive=True)
+    assert got["level_new"][1:7] == counts[1:7]
+
+
+def test_vfb_literal_mode_keeps_second_occurrences():
+    # In the literal reading the first occurrence of a signature is dropped
+    # and later duplicates are kept: v0+v1 is dropped at its first (v0,v1)
+    # occurrence and found at the swapped pair (v1,v0).
+    rng = np.random.default_rng(2)
+    x = rng.integers(0, 2**32, size=8, dtype=np.uint64)
+    y = rng.integers(0, 2**32, size=8, dtype=np.uint64)
+    inp = np.stack([x, y], axis=1).reshape(-1)
+    t = (x + y) & np.uint64(0xFFFFFFFF)
+    a = orc.vfb_search(inp, t, 2, (), 32, 5, dedup="intended")
+    b = orc.vfb_search(inp, t, 2, (), 32, 5, dedup="literal")
+    assert a["expr"] == "(v0)+(v1)"
+    assert b["expr"] == "(v1)+(v0)"
