@@ -1,0 +1,147 @@
+/*
+ * mba_api.h -- C ABI of the MI355X (gfx950) MBA synthesis engine.
+ *
+ * Drop-in for the search of the reference's two tools:
+ *   - SIMBA plain search   NoCache::solve(const MbaInstance&, int max_size)
+ *                          simba/src/nocache.cu:197-289, CLI main :291-342
+ *   - VFB cached search    string MBA(int maxLen, int numExamples,
+ *                                     uint32_t* in, uint32_t* out)
+ *                          vfb/src/full-cache.cu:665-774, CLI main :776-853
+ * and the instance readers MbaInstance::from_json_file
+ * (simba/src/instance.h:19-29, instance.cpp:7-27) and readJsonFile
+ * (vfb/src/utils.h:15-44).
+ *
+ * Plain pointers and sizes only.  Inputs are owned by the caller; results are
+ * written into caller-provided structs.  No global mutable state: every
+ * context owns its device buffers, so distinct contexts may be used from
+ * distinct threads.  Return codes: 0 = solution found, 1 = not found within
+ * the size range, 2 = time budget or cancel flag hit, 3 = cache capacity
+ * exhausted (VFB), < 0 = error (see mba_strerror).
+ */
+#ifndef MBA_API_H
+#define MBA_API_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MBA_MAX_VARS 16      /* leaves = vars + consts <= 16            */
+#define MBA_MAX_SAMPLES 128  /* reference: 100 (globals.cuh:6)          */
+#define MBA_MAX_SIZE 32      /* reference: 32 (globals.cuh:7)           */
+#define MBA_EXPR_CAP 1024
+
+enum {
+  MBA_FOUND = 0,
+  MBA_NOT_FOUND = 1,
+  MBA_BUDGET = 2,
+  MBA_OVERFLOW = 3,
+  MBA_ERR_ARG = -1,
+  MBA_ERR_LIMIT = -2,   /* sizes/vars/samples beyond the supported limits */
+  MBA_ERR_COUNT = -3,   /* count table overflows uint64 at this max size  */
+  MBA_ERR_HIP = -4,     /* a HIP runtime call failed                       */
+  MBA_ERR_NOMEM = -5,
+  MBA_ERR_IO = -6,      /* instance file missing or malformed              */
+  MBA_ERR_NODEV = -7    /* no gfx950 device / HIP code object not loadable */
+};
+
+/* One synthesis instance.  Layout of `inputs` is [sample][var], as the
+ * reference (instance.cpp:14-24, utils.h:27-41).  Values are taken mod 2^width.
+ * Constant leaves (an extension; empty = reference grammar) follow the
+ * variables in leaf order. */
+typedef struct {
+  uint32_t width;      /* 32 (reference semantics) or 64 */
+  uint32_t n_vars;
+  uint32_t n_samples;
+  uint32_t n_consts;
+  const uint64_t *inputs;   /* n_samples * n_vars */
+  const uint64_t *outputs;  /* n_samples          */
+  const uint64_t *consts;   /* n_consts (may be NULL when 0) */
+} mba_instance;
+
+typedef struct {
+  int32_t min_size;         /* inclusive; SIMBA native default 1            */
+  int32_t max_size;         /* inclusive                                     */
+  int32_t compat;           /* 1: reference ranges: SIMBA sizes 2..max-1,    */
+                            /*    VFB levels 1..max (nocache.cu:246,         */
+                            /*    full-cache.cu:754)                         */
+  int32_t exhaustive;       /* VFB: keep enumerating after a solution        */
+  double time_budget_s;     /* <= 0: none                                    */
+  volatile int32_t *cancel; /* host-written; polled between launches; NULL ok */
+  uint64_t table_bytes;     /* SIMBA: value-table budget, 0 = default        */
+  uint64_t cache_bytes;     /* VFB: row-store budget, 0 = default            */
+  int32_t verbose;          /* progress on stderr                            */
+  int32_t reserved[7];
+} mba_opts;
+
+typedef struct {
+  int32_t status;
+  int32_t size;          /* size of the solution (SIMBA) / level (VFB)      */
+  uint64_t rank;         /* SIMBA: rank within its size, in the reference's */
+                         /* enumeration order; VFB: enumeration index       */
+                         /* within the level                                */
+  int32_t op;            /* root operator 0..7 = ~ & | ^ neg + - *, -1 leaf */
+  int32_t split;         /* SIMBA: left child size (0: unary or leaf)       */
+  char expr[MBA_EXPR_CAP];     /* reference printing conventions            */
+  char rpn[MBA_MAX_SIZE + 1];  /* SIMBA: reference RPN chars ('0'+leaf, op) */
+  double seconds;        /* search-only time (device work + host driver)    */
+  double kernel_seconds; /* sum of the hot kernels' HIP-event times         */
+  uint64_t candidates;   /* candidates of every completed size/level        */
+  int32_t size_reached;  /* last size/level completely searched             */
+  int32_t reserved[5];
+} mba_result;
+
+/* VFB per-level statistics (PRINT_CACHE_SIZE mode, full-cache.cu:763-765). */
+typedef struct {
+  uint64_t level_new[MBA_MAX_SIZE + 1];   /* rows kept at each level         */
+  uint64_t level_cand[MBA_MAX_SIZE + 1];  /* candidates enumerated per level */
+  double level_seconds[MBA_MAX_SIZE + 1];
+  uint64_t level_bytes[MBA_MAX_SIZE + 1]; /* algorithmic HBM bytes per level */
+  uint64_t table_slots;
+  uint64_t rows_capacity;
+} mba_vfb_stats;
+
+typedef struct mba_ctx mba_ctx;
+
+/* Context on HIP device `device`.  Fails with MBA_ERR_NODEV when the device
+ * is absent or the gfx950 code object cannot be loaded. */
+int mba_ctx_create(int device, mba_ctx **out);
+void mba_ctx_destroy(mba_ctx *ctx);
+const char *mba_strerror(int code);
+/* Text of the last error a context recorded (diagnostics). */
+const char *mba_ctx_last_error(mba_ctx *ctx);
+void mba_default_opts(mba_opts *o);
+
+/* SIMBA plain search (replaces NoCache::solve, nocache.cu:197-289).  First
+ * solution = minimum rank within the smallest size that has one. */
+int mba_simba_solve(mba_ctx *ctx, const mba_instance *inst, const mba_opts *opts,
+                    mba_result *res);
+
+/* VFB bottom-up cached search (replaces MBA(), full-cache.cu:665-774) with
+ * the intended keep-iff-fingerprint-new dedup and the first kept match in
+ * enumeration order as the solution.  `stats` may be NULL. */
+int mba_vfb_solve(mba_ctx *ctx, const mba_instance *inst, const mba_opts *opts,
+                  mba_result *res, mba_vfb_stats *stats);
+
+/* Batched SIMBA: `count` instances solved by one persistent kernel with a
+ * device work queue.  Each result equals mba_simba_solve's on that instance
+ * within the per-instance budget `opts->time_budget_s` (0 = none). */
+int mba_batch_solve(mba_ctx *ctx, const mba_instance *insts, size_t count,
+                    const mba_opts *opts, mba_result *results);
+
+/* Instance reader for the reference JSON schema (sample.py:43-65):
+ * {"num_variables", "num_examples", "examples":[{"inputs":[{"value","size"}],
+ *  "output":[{"value","size"}]}]}.  Width = the first value's "size" field
+ * (32 when absent).  Arrays are malloc'ed; free with mba_instance_free. */
+int mba_instance_read_json(const char *path, mba_instance *out);
+void mba_instance_free(mba_instance *inst);
+
+/* Reference printing of a SIMBA RPN string (nocache.cu:162-195). */
+int mba_rpn_to_infix(const char *rpn, uint32_t n_vars, const uint64_t *consts,
+                     uint32_t n_consts, uint32_t width, char *out, size_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,273 @@
+"""simba_amd -- MI355X (gfx950) MBA synthesis: SIMBA and VFB search.
+
+Python mirror of the reference's search interface over the C ABI of
+libsimba_amd.so (include/mba_api.h):
+
+  MbaInstance.from_json_file(path)     simba/src/instance.h:19-29
+  NoCache.solve(mba, max_size) -> str  simba/src/nocache.cu:197-289
+  MBA(maxLen, numExamples, inputData, outputData, numVar) -> str
+                                       vfb/src/full-cache.cu:665-774
+
+plus richer entry points (simba_solve, vfb_solve, batch_solve) that return
+the full result record.  There is no CPU fallback: without the built library
+or without a gfx950 device every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsimba_amd.so")
+
+MAX_SIZE = 32
+EXPR_CAP = 1024
+FOUND, NOT_FOUND, BUDGET, OVERFLOW = 0, 1, 2, 3
+
+
+class Instance(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("n_vars", ctypes.c_uint32),
+                ("n_samples", ctypes.c_uint32), ("n_consts", ctypes.c_uint32),
+                ("inputs", ctypes.POINTER(ctypes.c_uint64)),
+                ("outputs", ctypes.POINTER(ctypes.c_uint64)),
+                ("consts", ctypes.POINTER(ctypes.c_uint64))]
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("min_size", ctypes.c_int32), ("max_size", ctypes.c_int32),
+                ("compat", ctypes.c_int32), ("exhaustive", ctypes.c_int32),
+                ("time_budget_s", ctypes.c_double),
+                ("cancel", ctypes.POINTER(ctypes.c_int32)),
+                ("table_bytes", ctypes.c_uint64), ("cache_bytes", ctypes.c_uint64),
+                ("verbose", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("size", ctypes.c_int32),
+                ("rank", ctypes.c_uint64), ("op", ctypes.c_int32),
+                ("split", ctypes.c_int32), ("expr", ctypes.c_char * EXPR_CAP),
+                ("rpn", ctypes.c_char * (MAX_SIZE + 1)), ("seconds", ctypes.c_double),
+                ("kernel_seconds", ctypes.c_double), ("candidates", ctypes.c_uint64),
+                ("size_reached", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+
+    def as_dict(self):
+        return {"status": self.status, "size": self.size, "rank": int(self.rank),
+                "op": self.op, "split": self.split, "expr": self.expr.decode(),
+                "rpn": self.rpn.decode(), "seconds": self.seconds,
+                "kernel_seconds": self.kernel_seconds,
+                "candidates": int(self.candidates), "size_reached": self.size_reached}
+
+
+class VfbStats(ctypes.Structure):
+    _fields_ = [("level_new", ctypes.c_uint64 * (MAX_SIZE + 1)),
+                ("level_cand", ctypes.c_uint64 * (MAX_SIZE + 1)),
+                ("level_seconds", ctypes.c_double * (MAX_SIZE + 1)),
+                ("level_bytes", ctypes.c_uint64 * (MAX_SIZE + 1)),
+                ("table_slots", ctypes.c_uint64), ("rows_capacity", ctypes.c_uint64)]
+
+
+# Every symbol the C ABI exports (include/mba_api.h).
+EXPORTS = ["mba_ctx_create", "mba_ctx_destroy", "mba_strerror", "mba_ctx_last_error",
+           "mba_default_opts", "mba_simba_solve", "mba_vfb_solve", "mba_batch_solve",
+           "mba_instance_read_json", "mba_instance_free", "mba_rpn_to_infix"]
+
+_lib = None
+
+
+def lib():
+    """Loads libsimba_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: build with `python -m simba_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        L.mba_ctx_create.argtypes = [ctypes.c_int, P(ctypes.c_void_p)]
+        L.mba_ctx_destroy.argtypes = [ctypes.c_void_p]
+        L.mba_strerror.argtypes = [ctypes.c_int]
+        L.mba_strerror.restype = ctypes.c_char_p
+        L.mba_ctx_last_error.argtypes = [ctypes.c_void_p]
+        L.mba_ctx_last_error.restype = ctypes.c_char_p
+        L.mba_default_opts.argtypes = [P(Opts)]
+        L.mba_simba_solve.argtypes = [ctypes.c_void_p, P(Instance), P(Opts), P(Result)]
+        L.mba_vfb_solve.argtypes = [ctypes.c_void_p, P(Instance), P(Opts), P(Result),
+                                    P(VfbStats)]
+        L.mba_batch_solve.argtypes = [ctypes.c_void_p, P(Instance), ctypes.c_size_t,
+                                      P(Opts), P(Result)]
+        L.mba_instance_read_json.argtypes = [ctypes.c_char_p, P(Instance)]
+        L.mba_instance_free.argtypes = [P(Instance)]
+        L.mba_rpn_to_infix.argtypes = [ctypes.c_char_p, ctypes.c_uint32, P(ctypes.c_uint64),
+                                       ctypes.c_uint32, ctypes.c_uint32, ctypes.c_char_p,
+                                       ctypes.c_size_t]
+        _lib = L
+    return _lib
+
+
+class MbaError(RuntimeError):
+    pass
+
+
+class Context:
+    """One device context (stream + reusable device buffers)."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        rc = lib().mba_ctx_create(device, ctypes.byref(self._h))
+        if rc != 0:
+            raise MbaError(f"mba_ctx_create failed: {lib().mba_strerror(rc).decode()}")
+
+    def close(self):
+        if self._h:
+            lib().mba_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc < 0:
+            msg = lib().mba_ctx_last_error(self._h).decode()
+            raise MbaError(f"{lib().mba_strerror(rc).decode()}: {msg}")
+        return rc
+
+    def simba(self, inst, **kw):
+        ci, keep = inst.to_c()
+        o = make_opts(**kw)
+        r = Result()
+        self._check(lib().mba_simba_solve(self._h, ctypes.byref(ci), ctypes.byref(o),
+                                          ctypes.byref(r)))
+        return r.as_dict()
+
+    def vfb(self, inst, **kw):
+        ci, keep = inst.to_c()
+        o = make_opts(**kw)
+        r = Result()
+        st = VfbStats()
+        self._check(lib().mba_vfb_solve(self._h, ctypes.byref(ci), ctypes.byref(o),
+                                        ctypes.byref(r), ctypes.byref(st)))
+        d = r.as_dict()
+        d["level_new"] = list(st.level_new)
+        d["level_cand"] = list(st.level_cand)
+        d["level_seconds"] = list(st.level_seconds)
+        d["level_bytes"] = list(st.level_bytes)
+        d["table_slots"] = int(st.table_slots)
+        return d
+
+    def batch(self, insts, **kw):
+        n = len(insts)
+        arr = (Instance * n)()
+        keep = []
+        for i, x in enumerate(insts):
+            ci, k = x.to_c()
+            arr[i] = ci
+            keep.append(k)
+        o = make_opts(**kw)
+        res = (Result * n)()
+        self._check(lib().mba_batch_solve(self._h, arr, n, ctypes.byref(o), res))
+        return [r.as_dict() for r in res]
+
+
+def make_opts(min_size=1, max_size=9, compat=False, exhaustive=False, budget_s=0.0,
+              table_bytes=0, cache_bytes=0, verbose=False, cancel=None):
+    o = Opts()
+    lib().mba_default_opts(ctypes.byref(o))
+    o.min_size, o.max_size = min_size, max_size
+    o.compat, o.exhaustive = int(compat), int(exhaustive)
+    o.time_budget_s = budget_s
+    o.table_bytes, o.cache_bytes = table_bytes, cache_bytes
+    o.verbose = int(verbose)
+    if cancel is not None:
+        o.cancel = ctypes.cast(ctypes.addressof(cancel), ctypes.POINTER(ctypes.c_int32))
+    return o
+
+
+class MbaInstance:
+    """A synthesis instance: inputs [sample][var], outputs [sample] (mirror of
+    the reference's MbaInstance, simba/src/instance.h:10-30)."""
+
+    def __init__(self, inputs, outputs, n_variables, width=32, consts=()):
+        self.width = int(width)
+        m = np.uint64((1 << self.width) - 1) if self.width < 64 else np.uint64(~0 & (2**64 - 1))
+        self.inputs = (np.asarray(inputs, dtype=np.uint64).reshape(-1) & m).copy()
+        self.outputs = (np.asarray(outputs, dtype=np.uint64).reshape(-1) & m).copy()
+        self.n_variables = int(n_variables)
+        self.n_examples = int(self.outputs.shape[0])
+        self.consts = np.asarray(list(consts), dtype=np.uint64) & m
+        if self.inputs.shape[0] != self.n_variables * self.n_examples:
+            raise ValueError("inputs must hold n_examples * n_variables values")
+
+    @staticmethod
+    def from_json_file(path):
+        ci = Instance()
+        rc = lib().mba_instance_read_json(os.fsencode(path), ctypes.byref(ci))
+        if rc != 0:
+            raise MbaError(f"Error: Could not read {path}: {lib().mba_strerror(rc).decode()}")
+        try:
+            n, v = ci.n_samples, ci.n_vars
+            inp = np.ctypeslib.as_array(ci.inputs, shape=(max(n * v, 1),))[: n * v].copy()
+            out = np.ctypeslib.as_array(ci.outputs, shape=(n,)).copy()
+            return MbaInstance(inp, out, v, ci.width)
+        finally:
+            lib().mba_instance_free(ctypes.byref(ci))
+
+    def to_c(self):
+        ci = Instance()
+        ci.width = self.width
+        ci.n_vars = self.n_variables
+        ci.n_samples = self.n_examples
+        ci.n_consts = len(self.consts)
+        inp = np.ascontiguousarray(self.inputs) if self.inputs.size else np.zeros(1, np.uint64)
+        out = np.ascontiguousarray(self.outputs)
+        cst = np.ascontiguousarray(self.consts) if len(self.consts) else np.zeros(1, np.uint64)
+        P = ctypes.POINTER(ctypes.c_uint64)
+        ci.inputs = inp.ctypes.data_as(P)
+        ci.outputs = out.ctypes.data_as(P)
+        ci.consts = cst.ctypes.data_as(P)
+        return ci, (inp, out, cst)
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class NoCache:
+    """Mirror of the reference's SIMBA entry point (nocache.cu:197-289)."""
+
+    @staticmethod
+    def solve(mba, max_size):
+        """Reference semantics: sizes 1..max_size-1 (exclusive, size 1 holds no
+        candidates); returns the infix string or "Not found !"."""
+        if max_size > MAX_SIZE:
+            raise MbaError(f"This version supports formulas of size at most {MAX_SIZE}")
+        r = default_context().simba(mba, max_size=max_size, compat=True)
+        return r["expr"]
+
+
+def MBA(maxLen, numExamples, inputData, outputData, numVar, width=32):
+    """Mirror of VFB's MBA() (full-cache.cu:665-774): levels 1..maxLen."""
+    if numExamples > 100:
+        return "see_the_error"
+    inst = MbaInstance(inputData, outputData, numVar, width)
+    r = default_context().vfb(inst, max_size=maxLen, compat=True)
+    return r["expr"]
+
+
+def simba_solve(inst, **kw):
+    return default_context().simba(inst, **kw)
+
+
+def vfb_solve(inst, **kw):
+    return default_context().vfb(inst, **kw)
+
+
+def batch_solve(insts, **kw):
+    return default_context().batch(insts, **kw)

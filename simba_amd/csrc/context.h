@@ -1,0 +1,75 @@
+// context.h -- per-context device state shared by the SIMBA/VFB drivers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <string>
+
+#include "../../include/mba_api.h"
+#include "mba_common.h"
+
+namespace mba {
+
+// Grow-only device allocation owned by a context.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t want) {
+    if (want <= bytes) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      if (e != hipSuccess) return e;
+      p = nullptr;
+      bytes = 0;
+    }
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) bytes = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T *as() const { return (T *)p; }
+};
+
+struct Ctx {
+  int device = 0;
+  int cu_count = 256;
+  size_t hbm_bytes = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string last_error;
+  // SIMBA
+  DevBuf tables, tview, hits, hit_count, leafvals, outputs, vsizes, vranks, vok;
+  // VFB
+  DevBuf vfb_rows, vfb_left, vfb_right, vfb_keys, vfb_idx, vfb_flags, vfb_misc,
+      vfb_counts, vfb_target;
+  // batched / learned
+  DevBuf b_inst, b_data, b_queue, b_results, b_misc, mlp_w, mlp_misc;
+};
+
+using Clock = std::chrono::steady_clock;
+inline double seconds_since(Clock::time_point t0) {
+  return std::chrono::duration<double>(Clock::now() - t0).count();
+}
+
+#define MBA_HIP(call)                                                     \
+  do {                                                                    \
+    hipError_t _e = (call);                                               \
+    if (_e != hipSuccess) {                                               \
+      ctx.last_error = std::string(#call) + ": " + hipGetErrorString(_e); \
+      return MBA_ERR_HIP;                                                 \
+    }                                                                     \
+  } while (0)
+
+int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &opts,
+                mba_result &res);
+int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &opts,
+              mba_result &res, mba_vfb_stats *stats);
+int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count,
+                const mba_opts &opts, mba_result *results);
+
+}  // namespace mba

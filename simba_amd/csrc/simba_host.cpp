@@ -1,0 +1,395 @@
+// simba_host.cpp -- host driver of the SIMBA search on one gfx950 device.
+//
+// Replaces NoCache::solve (simba/src/nocache.cu:197-289).  Sizes are visited
+// in increasing order; the first solution is the minimum rank, in the
+// reference's enumeration order, within the smallest size that has one (the
+// reference keeps whichever thread wins atomicCAS inside the first
+// (size, op) launch with a match, nocache.cu:102-105, which is one of the
+// trees of that size and op block).
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "context.h"
+#include "simba_kernels.h"
+
+namespace mba {
+
+namespace {
+
+constexpr uint32_t kHitCap = 1u << 20;
+constexpr uint64_t kDefaultTableBytes = 40ull << 30;
+// Work per launch, in A x B value pairs, so that budgets are honoured.
+constexpr uint64_t kPairsPerLaunch = 1ull << 36;
+
+inline uint64_t wmask(int W) { return W == 64 ? ~0ull : 0xffffffffull; }
+
+inline int64_t as_signed(uint64_t x, int W) {
+  return W == 64 ? (int64_t)x : (int64_t)(int32_t)(uint32_t)x;
+}
+
+struct Candidate {
+  int size;
+  uint64_t rank;
+};
+
+// Rank at size sb of the binary-rooted tree X = inst(A[a], B[b]) of the
+// split {j, m} (see HitRec), or of leaf `a` when inst == 7.
+uint64_t base_rank(const CountTable &T, int sb, int j, int m, const HitRec &h) {
+  static const int kInstOp[7] = {AND, OR, XOR, ADD, MUL, SUB, SUB};
+  if (h.inst == 7) return h.a;
+  const int op = kInstOp[h.inst];
+  if (h.inst == 6)  // (B - A): '-' with left child of size m
+    return T.op_offset(sb, SUB) + T.split_offset(sb, m) + h.b * T.t[j][8] + h.a;
+  return T.op_offset(sb, op) + T.split_offset(sb, j) + h.a * T.t[m][8] + h.b;
+}
+
+// Turns one window hit into candidate trees.  X has size sb and value v on
+// sample 0.  A chain of u unary operators over X computes (-1)^u * x + k; the
+// target is met by the even map x + (t0 - v) or the odd map -x + (t0 + v).
+// For each map the smallest chain length whose size lies in [min, max] is
+// found, and among chains of that length the one with the smallest rank:
+// a unary node at size s adds op_offset(s, op) to the rank of its child
+// (nocache.cu:52-63), 0 for '~' and T[s][0..3] for neg.
+void resolve_hit(const CountTable &T, int W, uint64_t t0, int sb, uint64_t rX,
+                 uint64_t v, int min_size, int max_size, std::vector<Candidate> &out) {
+  const uint64_t M = wmask(W);
+  const int U = max_size - sb;
+  const int64_t ke = as_signed((t0 - v) & M, W);
+  const int64_t ko = as_signed((t0 + v) & M, W);
+  const bool want_e = ke >= -U && ke <= U, want_o = ko >= -U && ko <= U;
+  if (!want_e && !want_o) return;
+  // cost[k + off] = min rank increment of a chain of length i reaching k.
+  const int off = U + 2, width = 2 * U + 5;
+  const uint64_t INF = ~0ull;
+  std::vector<uint64_t> cost(width, INF), next(width, INF);
+  cost[off] = 0;
+  bool done_e = !want_e, done_o = !want_o;
+  for (int i = 0; sb + i <= max_size && !(done_e && done_o); ++i) {
+    const int size = sb + i;
+    if (size >= min_size) {
+      if (!done_e && (i % 2 == 0) && cost[ke + off] != INF) {
+        out.push_back({size, rX + cost[ke + off]});
+        done_e = true;
+      }
+      if (!done_o && (i % 2 == 1) && cost[ko + off] != INF) {
+        out.push_back({size, rX + cost[ko + off]});
+        done_o = true;
+      }
+    }
+    if (size + 1 > max_size) break;
+    std::fill(next.begin(), next.end(), INF);
+    const uint64_t neg_off = T.op_offset(size + 1, NEG);
+    for (int k = -U - 1; k <= U + 1; ++k) {
+      const uint64_t c = cost[k + off];
+      if (c == INF) continue;
+      const int kn = -k - 1, kg = -k;  // '~': -x-1 ; neg: -x
+      if (kn >= -U - 1 && kn <= U + 1) next[kn + off] = std::min(next[kn + off], c);
+      if (kg >= -U - 1 && kg <= U + 1)
+        next[kg + off] = std::min(next[kg + off], c + neg_off);
+    }
+    std::swap(cost, next);
+  }
+}
+
+int pick_filter_sample(const std::vector<uint64_t> &lv, const std::vector<uint64_t> &outv,
+                       int n, int L, int W) {
+  const uint64_t M = wmask(W);
+  auto weak = [&](uint64_t x) {
+    const uint64_t d = std::min(x & M, (0 - x) & M);
+    return d <= 2 || x == (1ull << (W - 1));
+  };
+  int best = 0, best_score = -1;
+  for (int s = 0; s < n; ++s) {
+    int score = weak(outv[s]) ? 0 : 2 * L;
+    for (int a = 0; a < L; ++a) {
+      const uint64_t x = lv[(size_t)s * L + a];
+      if (weak(x)) continue;
+      bool distinct = true;
+      for (int b = 0; b < a; ++b)
+        if (lv[(size_t)s * L + b] == x || lv[(size_t)s * L + b] == ((0 - x) & M) ||
+            lv[(size_t)s * L + b] == (~x & M))
+          distinct = false;
+      score += distinct ? 2 : 1;
+    }
+    if (score > best_score) best = s, best_score = score;
+  }
+  return best;
+}
+
+int choose_kt(const CountTable &T, int max_size, int W, uint64_t budget) {
+  const uint64_t wb = W / 8;
+  int kt = 1;
+  uint64_t used = T.t[1][8] * wb;
+  for (int m = 2; m <= std::max(1, max_size - 2); ++m) {
+    const uint64_t add = T.t[m][8] * wb;
+    if (used + add > budget) break;
+    used += add;
+    kt = m;
+  }
+  return kt;
+}
+
+}  // namespace
+
+int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result &res) {
+  const auto t_start = Clock::now();
+  std::memset(&res, 0, sizeof res);
+  res.status = MBA_NOT_FOUND;
+  res.op = -1;
+  const int W = (int)inst.width;
+  const int nv = (int)inst.n_vars, nc = (int)inst.n_consts, n = (int)inst.n_samples;
+  const int L = nv + nc;
+  if ((W != 32 && W != 64) || L < 1 || L > kMaxLeaves || n < 1 || n > kMaxSamples ||
+      !inst.inputs || !inst.outputs || (nc > 0 && !inst.consts)) {
+    ctx.last_error = "instance outside supported limits";
+    return MBA_ERR_LIMIT;
+  }
+  int min_size = o.min_size < 1 ? 1 : o.min_size;
+  int max_size = o.max_size;
+  if (o.compat) {  // reference: sizes 1..maxLen-1, size 1 holds no candidates
+    min_size = 2;
+    max_size = o.max_size - 1;
+  }
+  if (max_size > kMaxSize) {
+    ctx.last_error = "max size above 32";
+    return MBA_ERR_LIMIT;
+  }
+  if (max_size < min_size) {
+    std::snprintf(res.expr, sizeof res.expr, "Not found !");
+    res.seconds = seconds_since(t_start);
+    return MBA_NOT_FOUND;
+  }
+  CountTable T;
+  if (!T.build(L, max_size)) {
+    ctx.last_error = "count table overflows uint64 at this max size";
+    return MBA_ERR_COUNT;
+  }
+  const uint64_t M = wmask(W);
+  const size_t wb = (size_t)W / 8;
+  std::vector<uint64_t> lv((size_t)n * L);
+  for (int s = 0; s < n; ++s) {
+    for (int v = 0; v < nv; ++v) lv[(size_t)s * L + v] = inst.inputs[(size_t)s * nv + v] & M;
+    for (int c = 0; c < nc; ++c) lv[(size_t)s * L + nv + c] = inst.consts[c] & M;
+  }
+  std::vector<uint64_t> outv(n);
+  for (int s = 0; s < n; ++s) outv[s] = inst.outputs[s] & M;
+  // Filter sample: the tables and fused checks work on one sample; every
+  // hit is verified on all samples, so any choice gives the same answer.  A
+  // sample whose leaf values are distinct and far from 0/+-1 rejects the most
+  // candidates, so it is preferred.
+  const int fs = pick_filter_sample(lv, outv, n, L, W);
+  const uint64_t t0 = outv[fs];
+
+  // ---- value tables V[1..Kt] -------------------------------------------
+  const uint64_t budget = o.table_bytes ? o.table_bytes : kDefaultTableBytes;
+  const int Kt = choose_kt(T, max_size, W, budget);
+  TableView tv;
+  std::memset(&tv, 0, sizeof tv);
+  uint64_t elems = 0;
+  for (int m = 1; m <= Kt; ++m) {
+    tv.off[m] = elems;
+    elems += T.t[m][8];
+  }
+  std::memcpy(tv.T, T.t, sizeof tv.T);
+  tv.Kt = Kt;
+  tv.L = L;
+  MBA_HIP(ctx.tables.ensure(std::max<size_t>(elems * wb, 256)));
+  tv.base = ctx.tables.p;
+  MBA_HIP(ctx.tview.ensure(sizeof(TableView)));
+  MBA_HIP(ctx.hits.ensure(sizeof(HitRec) * kHitCap));
+  MBA_HIP(ctx.hit_count.ensure(256));
+  MBA_HIP(ctx.leafvals.ensure(sizeof(uint64_t) * lv.size()));
+  MBA_HIP(ctx.outputs.ensure(sizeof(uint64_t) * n));
+  hipStream_t st = ctx.stream;
+  TableView *d_tv = ctx.tview.as<TableView>();
+  MBA_HIP(hipMemcpyAsync(d_tv, &tv, sizeof tv, hipMemcpyHostToDevice, st));
+  MBA_HIP(hipMemcpyAsync(ctx.leafvals.p, lv.data(), sizeof(uint64_t) * lv.size(),
+                         hipMemcpyHostToDevice, st));
+  MBA_HIP(hipMemcpyAsync(ctx.outputs.p, outv.data(), sizeof(uint64_t) * n,
+                         hipMemcpyHostToDevice, st));
+  {  // V[1] = leaf values on sample 0
+    std::vector<uint64_t> v1(L);
+    std::vector<uint32_t> v1w(L);
+    for (int l = 0; l < L; ++l)
+      v1[l] = lv[(size_t)fs * L + l], v1w[l] = (uint32_t)lv[(size_t)fs * L + l];
+    if (W == 64)
+      MBA_HIP(hipMemcpyAsync(ctx.tables.p, v1.data(), 8 * L, hipMemcpyHostToDevice, st));
+    else
+      MBA_HIP(hipMemcpyAsync(ctx.tables.p, v1w.data(), 4 * L, hipMemcpyHostToDevice, st));
+  }
+  for (int m = 2; m <= Kt; ++m)
+    MBA_HIP(launch_table_build(W, d_tv, ctx.tables.p, tv.off[m], m, T.t[m][8], st));
+
+  // ---- size sweep --------------------------------------------------------
+  std::vector<uint64_t> found(kMaxSize + 2, ~0ull);
+  std::vector<HitRec> hrec;
+  std::vector<Candidate> cands;
+  uint32_t *d_hc = ctx.hit_count.as<uint32_t>();
+  HitRec *d_hits = ctx.hits.as<HitRec>();
+  const int grid = ctx.cu_count * 8;
+  double ktime = 0.0;
+  int answer = -1;
+  int reached = 0;
+  int status = MBA_NOT_FOUND;
+
+  auto targets_for = [&](int sb) {
+    const uint64_t U = (uint64_t)(max_size - sb);
+    CheckTargets tg;
+    tg.lo_even = (t0 - U) & M;
+    tg.lo_odd = ((0 - t0) - U) & M;
+    tg.span = 2 * U;
+    return tg;
+  };
+
+  // Resolve the hits of one step, verify the candidates on all samples.
+  auto process_hits = [&](int sb_step) -> int {
+    uint32_t cnt = 0;
+    MBA_HIP(hipMemcpyAsync(&cnt, d_hc, 4, hipMemcpyDeviceToHost, st));
+    MBA_HIP(hipStreamSynchronize(st));
+    if (cnt == 0) return 0;
+    if (cnt > kHitCap) {
+      ctx.last_error = "sample-0 hit buffer overflow";
+      return MBA_ERR_LIMIT;
+    }
+    hrec.resize(cnt);
+    MBA_HIP(hipMemcpy(hrec.data(), d_hits, sizeof(HitRec) * cnt, hipMemcpyDeviceToHost));
+    cands.clear();
+    for (const HitRec &h : hrec) {
+      const int sb = (int)(h.pair >> 16), j = (int)(h.pair & 0xffff);
+      const int m = sb - 1 - j;
+      const uint64_t rX = base_rank(T, sb, j, m, h);
+      resolve_hit(T, W, t0, sb, rX, h.v & M, min_size, max_size, cands);
+    }
+    (void)sb_step;
+    if (cands.empty()) return 0;
+    const uint32_t nc_ = (uint32_t)cands.size();
+    MBA_HIP(ctx.vsizes.ensure(4ull * nc_));
+    MBA_HIP(ctx.vranks.ensure(8ull * nc_));
+    MBA_HIP(ctx.vok.ensure(nc_));
+    std::vector<uint32_t> sz(nc_);
+    std::vector<uint64_t> rk(nc_);
+    for (uint32_t i = 0; i < nc_; ++i) sz[i] = (uint32_t)cands[i].size, rk[i] = cands[i].rank;
+    MBA_HIP(hipMemcpyAsync(ctx.vsizes.p, sz.data(), 4ull * nc_, hipMemcpyHostToDevice, st));
+    MBA_HIP(hipMemcpyAsync(ctx.vranks.p, rk.data(), 8ull * nc_, hipMemcpyHostToDevice, st));
+    MBA_HIP(launch_verify(W, d_tv, ctx.leafvals.as<uint64_t>(), ctx.outputs.as<uint64_t>(),
+                          n, ctx.vsizes.as<uint32_t>(), ctx.vranks.as<uint64_t>(), nc_,
+                          ctx.vok.as<uint8_t>(), st));
+    std::vector<uint8_t> okv(nc_);
+    MBA_HIP(hipMemcpyAsync(okv.data(), ctx.vok.p, nc_, hipMemcpyDeviceToHost, st));
+    MBA_HIP(hipStreamSynchronize(st));
+    for (uint32_t i = 0; i < nc_; ++i)
+      if (okv[i]) found[sz[i]] = std::min(found[sz[i]], rk[i]);
+    return 0;
+  };
+
+  auto over_budget = [&]() {
+    if (o.cancel && *o.cancel) return true;
+    return o.time_budget_s > 0 && seconds_since(t_start) > o.time_budget_s;
+  };
+
+  // Step 1: leaves (size 1) and, through unary chains, every unary tree over a
+  // leaf (sizes 2..max).
+  MBA_HIP(hipMemsetAsync(d_hc, 0, 4, st));
+  MBA_HIP(launch_leaf_scan(W, d_tv, L, targets_for(1), d_hits, d_hc, kHitCap, st));
+  if (int rc = process_hits(1)) return rc;
+  for (int s = 1; s <= std::min(2, max_size); ++s) {
+    reached = s;
+    if (s >= min_size && found[s] != ~0ull) {
+      answer = s;
+      break;
+    }
+  }
+
+  for (int s = 3; answer < 0 && s <= max_size; ++s) {
+    if (over_budget()) {
+      status = MBA_BUDGET;
+      break;
+    }
+    MBA_HIP(hipMemsetAsync(d_hc, 0, 4, st));
+    const CheckTargets tg = targets_for(s);
+    bool stopped = false;
+    for (int j = 1; j <= (s - 1) / 2 && !stopped; ++j) {
+      const int m = s - 1 - j;
+      PairLaunch p;
+      p.sb = s;
+      p.j = j;
+      p.m = m;
+      p.nA = T.t[j][8];
+      p.nB = T.t[m][8];
+      p.nTA = (p.nA + kPairTA - 1) / kPairTA;
+      const uint64_t nTB = (p.nB + kPairTB - 1) / kPairTB;
+      const uint64_t tiles = p.nTA * nTB;
+      p.gen_a = j > Kt;
+      p.gen_b = m > Kt;
+      const uint64_t per_tile = (uint64_t)kPairTA * kPairTB;
+      const uint64_t tiles_per_launch = std::max<uint64_t>(1, kPairsPerLaunch / per_tile);
+      for (uint64_t tb = 0; tb < tiles; tb += tiles_per_launch) {
+        p.tile_begin = tb;
+        p.tile_end = std::min(tiles, tb + tiles_per_launch);
+        const uint64_t ntl = p.tile_end - p.tile_begin;
+        const int g = (int)std::min<uint64_t>((uint64_t)grid, ntl);
+        MBA_HIP(hipEventRecord(ctx.ev0, st));
+        MBA_HIP(launch_pair_check(W, d_tv, p, tg, d_hits, d_hc, kHitCap, g, st));
+        MBA_HIP(hipEventRecord(ctx.ev1, st));
+        MBA_HIP(hipEventSynchronize(ctx.ev1));
+        float ms = 0.f;
+        MBA_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
+        ktime += ms * 1e-3;
+        if (over_budget()) {
+          stopped = true;
+          break;
+        }
+      }
+    }
+    if (stopped) {
+      status = MBA_BUDGET;
+      break;
+    }
+    if (int rc = process_hits(s)) return rc;
+    reached = s;
+    if (o.verbose)
+      std::fprintf(stderr, "Length %-2d | candidates %-20llu | %.3f s\n", s,
+                   (unsigned long long)T.t[s][8], seconds_since(t_start));
+    if (s >= min_size && found[s] != ~0ull) answer = s;
+  }
+
+  uint64_t cand = 0;
+  for (int s = min_size; s <= reached; ++s) cand += T.t[s][8];
+  res.candidates = cand;
+  res.size_reached = reached;
+  res.kernel_seconds = ktime;
+  if (answer > 0) {
+    uint8_t rpn[kMaxSize];
+    unrank_host(T, answer, found[answer], rpn);
+    res.status = MBA_FOUND;
+    res.size = answer;
+    res.rank = found[answer];
+    res.op = answer > 1 ? rpn[answer - 1] - kTokOp : -1;
+    res.split = 0;
+    if (answer > 1 && !is_unary(res.op)) {
+      // left child size: walk the right subtree back from the root
+      int need = 1, i = answer - 2;
+      while (need > 0) {
+        const uint8_t t = rpn[i--];
+        if (t < kTokOp) need -= 1;
+        else if (!is_unary(t - kTokOp)) need += 1;
+      }
+      res.split = i + 1;
+    }
+    const std::string e = infix_simba(rpn, answer, nv, inst.consts, W);
+    std::snprintf(res.expr, sizeof res.expr, "%s", e.c_str());
+    const std::string r = rpn_chars(rpn, answer);
+    std::snprintf(res.rpn, sizeof res.rpn, "%s", r.c_str());
+    status = MBA_FOUND;
+  } else {
+    std::snprintf(res.expr, sizeof res.expr, "Not found !");
+  }
+  res.status = status;
+  res.seconds = seconds_since(t_start);
+  return status;
+}
+
+}  // namespace mba

@@ -1,0 +1,390 @@
+// simba_kernels.hip -- gfx950 kernels of the SIMBA (cache-free, unranking)
+// search.
+//
+// The reference (nocache.cu:85-107, mba.cuh:6-69) gives one thread one rank:
+// unrank to RPN in local memory, then a divergent stack evaluation.  Here a
+// size level is instead swept as outer products.  Every binary-rooted tree of
+// size s is op(X, Y) with X of size j and Y of size m = s-1-j; for an
+// unordered split {j, m} the seven operator instances & | ^ + * (X-Y) (Y-X)
+// are fused and evaluated on sample 0 from the value tables V[j], V[m]
+// (value of every tree of that size on sample 0, in rank order).  A lane
+// holds kPairRB values of the B side in VGPRs; the A side streams through
+// wave-uniform scalar loads, so the inner loop is pure integer VALU work with
+// no divergence.  Unary-rooted trees are the composition of a chain of ~/neg
+// with a binary-rooted tree or a leaf; such a chain maps x to +-x + k with
+// |k| <= chain length, so two windowed compares per value cover every unary
+// tree of every larger size.  Hits on sample 0 are rare; they are appended to
+// a hit buffer, resolved to (size, rank) by the host and verified on all
+// samples by launch_verify.
+#include "simba_kernels.h"
+
+namespace mba {
+
+template <typename word>
+struct WordOf;
+
+// ---------------------------------------------------------------------------
+// Table-assisted value of the tree (m, r) on sample 0: unrank top-down in the
+// reference's order, but stop at any subtree of size <= Kt and read its value
+// from V[size]; then evaluate the remaining skeleton in postfix order.
+// ---------------------------------------------------------------------------
+template <typename word>
+__device__ word value_of(const TableView *__restrict__ tv, int m, uint64_t r) {
+  const word *V = (const word *)tv->base;
+  const int Kt = tv->Kt;
+  if (m <= Kt) return V[tv->off[m] + r];
+  uint8_t tok[kMaxSize];     // 0xFF: inside a tabulated subtree
+  uint64_t ref[kMaxSize];    // table index for a 0x40|size token
+  struct Item { uint64_t r; int s, pos; };
+  Item st[kMaxSize];
+  for (int i = 0; i < m; ++i) tok[i] = 0xFF;
+  int sp = 0;
+  st[sp++] = {r, m, 0};
+  while (sp > 0) {
+    Item it = st[--sp];
+    if (it.s <= Kt) {
+      tok[it.pos + it.s - 1] = (uint8_t)(0x40 | it.s);
+      ref[it.pos + it.s - 1] = it.r;
+      continue;
+    }
+    uint64_t n = it.r;
+    int op = 0;
+    while (n >= tv->T[it.s][op]) n -= tv->T[it.s][op++];
+    const int cs = it.s - 1;
+    tok[it.pos + cs] = (uint8_t)(0x80 | op);
+    if (is_unary(op)) {
+      st[sp++] = {n, cs, it.pos};
+      continue;
+    }
+    int j = 1;
+    for (;; ++j) {
+      uint64_t blk = tv->T[j][8] * tv->T[cs - j][8];
+      if (n < blk) break;
+      n -= blk;
+    }
+    const uint64_t rs = tv->T[cs - j][8];
+    const uint64_t a = n / rs;
+    st[sp++] = {a, j, it.pos};
+    st[sp++] = {n - a * rs, cs - j, it.pos + j};
+  }
+  word vs[kMaxSize];
+  int top = -1;
+  for (int i = 0; i < m; ++i) {
+    const uint8_t t = tok[i];
+    if (t == 0xFF) continue;
+    if (t & 0x40) {
+      vs[++top] = V[tv->off[t & 0x3F] + ref[i]];
+    } else {
+      const int op = t & 7;
+      if (is_unary(op)) {
+        vs[top] = apply_op<word>(op, vs[top], word(0));
+      } else {
+        const word rv = vs[top--];
+        vs[top] = apply_op<word>(op, vs[top], rv);
+      }
+    }
+  }
+  return vs[0];
+}
+
+// ---------------------------------------------------------------------------
+// Table build: V[m][r] for r in [0, T[m]).  One lane per rank; the op block
+// and split are located by prefix sums over the count table (the reference's
+// unrank order, nocache.cu:52-79), the children are read from V[<m].
+// ---------------------------------------------------------------------------
+template <typename word>
+__global__ void __launch_bounds__(256)
+k_table_build(const TableView *__restrict__ tv, word *__restrict__ out, int m,
+              uint64_t n) {
+  const word *V = (const word *)tv->base;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += stride) {
+    uint64_t x = r;
+    int op = 0;
+    while (x >= tv->T[m][op]) x -= tv->T[m][op++];
+    word v;
+    if (is_unary(op)) {
+      v = apply_op<word>(op, V[tv->off[m - 1] + x], word(0));
+    } else {
+      const int cs = m - 1;
+      int j = 1;
+      for (;; ++j) {
+        const uint64_t blk = tv->T[j][8] * tv->T[cs - j][8];
+        if (x < blk) break;
+        x -= blk;
+      }
+      const uint64_t rs = tv->T[cs - j][8];
+      const uint64_t a = x / rs;
+      v = apply_op<word>(op, V[tv->off[j] + a], V[tv->off[cs - j] + (x - a * rs)]);
+    }
+    out[r] = v;
+  }
+}
+
+template <typename word>
+__device__ __forceinline__ bool in_window(word c, word lo_e, word lo_o, word span) {
+  return (word)(c - lo_e) <= span || (word)(c - lo_o) <= span;
+}
+
+template <typename word>
+__device__ __forceinline__ word inst_value(int inst, word a, word b) {
+  switch (inst) {
+    case 0: return a & b;
+    case 1: return a | b;
+    case 2: return a ^ b;
+    case 3: return a + b;
+    case 4: return a * b;
+    case 5: return a - b;
+    default: return b - a;
+  }
+}
+
+template <typename word>
+__device__ __noinline__ void record_hit(HitRec *hits, uint32_t *count, uint32_t cap,
+                                        uint64_t a, uint64_t b, word v, int inst,
+                                        uint32_t pair) {
+  const uint32_t i = atomicAdd(count, 1u);
+  if (i < cap) {
+    HitRec h;
+    h.a = a;
+    h.b = b;
+    h.v = (uint64_t)v;
+    h.inst = (uint32_t)inst;
+    h.pair = pair;
+    hits[i] = h;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused pair check over a tile range.  Tile t covers A values
+// [at*TA, at*TA+TA) x B values [bt*TB, bt*TB+TB) with at = t % nTA.
+// SAME: j == m, so (Y - X) is the (X - Y) instance of the swapped pair and is
+// skipped.
+// ---------------------------------------------------------------------------
+template <typename word, bool SAME, bool GEN_A, bool GEN_B>
+__global__ void __launch_bounds__(kPairThreads)
+k_pair_check(const TableView *__restrict__ tv, PairLaunch p, uint64_t lo_e64,
+             uint64_t lo_o64, uint64_t span64, HitRec *__restrict__ hits,
+             uint32_t *__restrict__ hit_count, uint32_t hit_cap) {
+  const word *V = (const word *)tv->base;
+  const word *A = V + tv->off[p.j];
+  const word *B = V + tv->off[p.m];
+  const word lo_e = (word)lo_e64, lo_o = (word)lo_o64, span = (word)span64;
+  const uint32_t pair = ((uint32_t)p.sb << 16) | (uint32_t)p.j;
+  for (uint64_t t = p.tile_begin + blockIdx.x; t < p.tile_end; t += gridDim.x) {
+    const uint64_t at = t % p.nTA, bt = t / p.nTA;
+    const uint64_t a0 = at * kPairTA;
+    const uint64_t a1 = a0 + kPairTA < p.nA ? a0 + kPairTA : p.nA;
+    const uint64_t b0 = bt * kPairTB + threadIdx.x;
+    word bv[kPairRB];
+    bool ok[kPairRB];
+#pragma unroll
+    for (int k = 0; k < kPairRB; ++k) {
+      const uint64_t b = b0 + (uint64_t)k * kPairThreads;
+      ok[k] = b < p.nB;
+      bv[k] = ok[k] ? (GEN_B ? value_of<word>(tv, p.m, b) : B[b]) : word(0);
+    }
+    bool hit = false;
+    for (uint64_t a = a0; a < a1; ++a) {
+      const word av = GEN_A ? value_of<word>(tv, p.j, a) : A[a];
+#pragma unroll
+      for (int k = 0; k < kPairRB; ++k) {
+        const word b = bv[k];
+        bool h = in_window(word(av & b), lo_e, lo_o, span);
+        h |= in_window(word(av | b), lo_e, lo_o, span);
+        h |= in_window(word(av ^ b), lo_e, lo_o, span);
+        h |= in_window(word(av + b), lo_e, lo_o, span);
+        h |= in_window(word(av * b), lo_e, lo_o, span);
+        h |= in_window(word(av - b), lo_e, lo_o, span);
+        if (!SAME) h |= in_window(word(b - av), lo_e, lo_o, span);
+        hit |= h & ok[k];
+      }
+    }
+    // Rare: some lane saw a window hit.  Re-walk the tile and record them.
+    if (__any(hit)) {
+      if (hit) {
+        for (uint64_t a = a0; a < a1; ++a) {
+          const word av = GEN_A ? value_of<word>(tv, p.j, a) : A[a];
+          for (int k = 0; k < kPairRB; ++k) {
+            if (!ok[k]) continue;
+            const uint64_t b = b0 + (uint64_t)k * kPairThreads;
+            for (int inst = 0; inst < (SAME ? 6 : 7); ++inst) {
+              const word c = inst_value<word>(inst, av, bv[k]);
+              if (in_window(c, lo_e, lo_o, span))
+                record_hit<word>(hits, hit_count, hit_cap, a, b, c, inst, pair);
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// Leaves (size-1 trees) against the same windows: exact leaf solutions and
+// every unary chain over a leaf.
+template <typename word>
+__global__ void k_leaf_scan(const TableView *__restrict__ tv, int L, uint64_t lo_e64,
+                            uint64_t lo_o64, uint64_t span64, HitRec *hits,
+                            uint32_t *hit_count, uint32_t hit_cap) {
+  const word *V = (const word *)tv->base + tv->off[1];
+  const int i = threadIdx.x;
+  if (i < L) {
+    const word c = V[i];
+    if (in_window(c, (word)lo_e64, (word)lo_o64, (word)span64))
+      record_hit<word>(hits, hit_count, hit_cap, (uint64_t)i, 0, c, 7, 1u << 16);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Verification: the reference's per-candidate path (numberToFormula +
+// is_solution_rpn, nocache.cu:23-83 / mba.cuh:11-66): unrank the full tree
+// and evaluate it sample by sample, leaving at the first mismatch.  Leaf
+// values of all samples are staged in LDS.
+// ---------------------------------------------------------------------------
+template <typename word>
+__global__ void __launch_bounds__(256)
+k_verify(const TableView *__restrict__ tv, const uint64_t *__restrict__ leafvals,
+         const uint64_t *__restrict__ outputs, int n_samples,
+         const uint32_t *__restrict__ sizes, const uint64_t *__restrict__ ranks,
+         uint32_t n_cand, uint8_t *__restrict__ ok) {
+  __shared__ word s_leaf[kMaxSamples * kMaxLeaves];
+  __shared__ word s_out[kMaxSamples];
+  const int L = tv->L;
+  for (int i = threadIdx.x; i < n_samples * L; i += blockDim.x)
+    s_leaf[i] = (word)leafvals[i];
+  for (int i = threadIdx.x; i < n_samples; i += blockDim.x) s_out[i] = (word)outputs[i];
+  __syncthreads();
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cand) return;
+  const int size = (int)sizes[c];
+  uint8_t rpn[kMaxSize];
+  struct Item { uint64_t r; int s, pos; };
+  Item st[kMaxSize];
+  int sp = 0;
+  st[sp++] = {ranks[c], size, 0};
+  while (sp > 0) {
+    Item it = st[--sp];
+    if (it.s == 1) {
+      rpn[it.pos] = (uint8_t)it.r;
+      continue;
+    }
+    uint64_t n = it.r;
+    int op = 0;
+    while (n >= tv->T[it.s][op]) n -= tv->T[it.s][op++];
+    const int cs = it.s - 1;
+    rpn[it.pos + cs] = (uint8_t)(kTokOp + op);
+    if (is_unary(op)) {
+      st[sp++] = {n, cs, it.pos};
+      continue;
+    }
+    int j = 1;
+    for (;; ++j) {
+      const uint64_t blk = tv->T[j][8] * tv->T[cs - j][8];
+      if (n < blk) break;
+      n -= blk;
+    }
+    const uint64_t rs = tv->T[cs - j][8];
+    const uint64_t a = n / rs;
+    st[sp++] = {a, j, it.pos};
+    st[sp++] = {n - a * rs, cs - j, it.pos + j};
+  }
+  uint8_t good = 1;
+  for (int s = 0; s < n_samples && good; ++s) {
+    word vs[kMaxSize];
+    int top = -1;
+    for (int i = 0; i < size; ++i) {
+      const uint8_t t = rpn[i];
+      if (t < kTokOp) {
+        vs[++top] = s_leaf[s * L + t];
+      } else {
+        const int op = t - kTokOp;
+        if (is_unary(op)) {
+          vs[top] = apply_op<word>(op, vs[top], word(0));
+        } else {
+          const word rv = vs[top--];
+          vs[top] = apply_op<word>(op, vs[top], rv);
+        }
+      }
+    }
+    good = vs[0] == s_out[s];
+  }
+  ok[c] = good;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+// ---------------------------------------------------------------------------
+hipError_t launch_table_build(int width, const TableView *d_tv, void *d_base,
+                              uint64_t off, int m, uint64_t n, hipStream_t stream) {
+  uint64_t blocks = (n + 255) / 256;
+  int grid = (int)(blocks < 65536 ? blocks : 65536);
+  if (grid < 1) grid = 1;
+  if (width == 32)
+    hipLaunchKernelGGL(k_table_build<uint32_t>, dim3(grid), dim3(256), 0, stream, d_tv,
+                       (uint32_t *)d_base + off, m, n);
+  else
+    hipLaunchKernelGGL(k_table_build<uint64_t>, dim3(grid), dim3(256), 0, stream, d_tv,
+                       (uint64_t *)d_base + off, m, n);
+  return hipGetLastError();
+}
+
+template <typename word>
+static void pair_dispatch(const TableView *d_tv, const PairLaunch &p,
+                          const CheckTargets &tg, HitRec *h, uint32_t *hc,
+                          uint32_t cap, int grid, hipStream_t s) {
+  const bool same = p.j == p.m;
+#define PAIR_LAUNCH(S, GA, GB)                                                    \
+  hipLaunchKernelGGL((k_pair_check<word, S, GA, GB>), dim3(grid), dim3(kPairThreads), \
+                     0, s, d_tv, p, tg.lo_even, tg.lo_odd, tg.span, h, hc, cap)
+  if (same) {
+    if (p.gen_a) PAIR_LAUNCH(true, true, true);
+    else PAIR_LAUNCH(true, false, false);
+  } else {
+    if (p.gen_a) PAIR_LAUNCH(false, true, true);
+    else if (p.gen_b) PAIR_LAUNCH(false, false, true);
+    else PAIR_LAUNCH(false, false, false);
+  }
+#undef PAIR_LAUNCH
+}
+
+hipError_t launch_pair_check(int width, const TableView *d_tv, const PairLaunch &p,
+                             const CheckTargets &tg, HitRec *d_hits,
+                             uint32_t *d_hit_count, uint32_t hit_cap, int grid,
+                             hipStream_t stream) {
+  if (width == 32)
+    pair_dispatch<uint32_t>(d_tv, p, tg, d_hits, d_hit_count, hit_cap, grid, stream);
+  else
+    pair_dispatch<uint64_t>(d_tv, p, tg, d_hits, d_hit_count, hit_cap, grid, stream);
+  return hipGetLastError();
+}
+
+hipError_t launch_leaf_scan(int width, const TableView *d_tv, int L,
+                            const CheckTargets &tg, HitRec *d_hits,
+                            uint32_t *d_hit_count, uint32_t hit_cap,
+                            hipStream_t stream) {
+  if (width == 32)
+    hipLaunchKernelGGL(k_leaf_scan<uint32_t>, dim3(1), dim3(64), 0, stream, d_tv, L,
+                       tg.lo_even, tg.lo_odd, tg.span, d_hits, d_hit_count, hit_cap);
+  else
+    hipLaunchKernelGGL(k_leaf_scan<uint64_t>, dim3(1), dim3(64), 0, stream, d_tv, L,
+                       tg.lo_even, tg.lo_odd, tg.span, d_hits, d_hit_count, hit_cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_verify(int width, const TableView *d_tv, const uint64_t *d_leafvals,
+                         const uint64_t *d_outputs, int n_samples,
+                         const uint32_t *d_sizes, const uint64_t *d_ranks,
+                         uint32_t n_cand, uint8_t *d_ok, hipStream_t stream) {
+  if (n_cand == 0) return hipSuccess;
+  const int grid = (int)((n_cand + 255) / 256);
+  if (width == 32)
+    hipLaunchKernelGGL(k_verify<uint32_t>, dim3(grid), dim3(256), 0, stream, d_tv,
+                       d_leafvals, d_outputs, n_samples, d_sizes, d_ranks, n_cand, d_ok);
+  else
+    hipLaunchKernelGGL(k_verify<uint64_t>, dim3(grid), dim3(256), 0, stream, d_tv,
+                       d_leafvals, d_outputs, n_samples, d_sizes, d_ranks, n_cand, d_ok);
+  return hipGetLastError();
+}
+
+}  // namespace mba
