@@ -1,0 +1,207 @@
+"""Benchmark of the MI355X SIMBA/VFB search.
+
+Default workload (one JSON line on stdout): the largest single-instance SIMBA
+shape -- leaves {a, b, c, d, 1}, uint64 words, 64 sample points, a uniform
+random size-15 target -- swept exhaustively over every size 1..S (S = 14 by
+default, 3.82e12 candidate trees).  One step = one full sweep; inputs are
+resident on the device before the timed region.  `value` is candidates per
+second over the K timed steps (Gcand/s).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload simba|vfb]
+
+roofline: the fused pair kernel is integer-VALU bound.  Algorithmic work per
+candidate tree = sum over its internal nodes of c(op) (64-bit words:
+& | ^ ~ + - neg = 2 lane-ops, * = 4; 32-bit words: 1), times the samples
+evaluated (1 for all but the rare sample-0 hits).  The kernel obtains most of
+a tree's value from the per-size value tables, so this naive-equivalent rate
+can exceed the lane-op rate the hardware executes; the executed rate is
+reported beside it.  Peak: 256 CU x 64 lanes x 2.4 GHz = 39.3 T lane-ops/s.
+
+cpu_baseline: the CPU oracle (multithreaded C restatement of the reference
+semantics, oracle/) on the same instance for sizes 1..9.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+VALU_PEAK = 256 * 64 * 2.4e9  # integer lane-ops/s
+HBM_PEAK_GBS = 8000.0
+
+
+def naive_ops_per_size(L, max_size, W):
+    """C[s] = sum over all trees of size s of sum_{internal nodes} c(op)."""
+    from simba_amd.synth import count_table
+    T = count_table(L, max_size)
+    c = [2, 2, 2, 2, 2, 2, 2, 4] if W == 64 else [1] * 8
+    C = [0] * (max_size + 1)
+    for s in range(2, max_size + 1):
+        tot = 0
+        for op in (0, 4):
+            tot += c[op] * T[s - 1][8] + C[s - 1]
+        for op in (1, 2, 3, 5, 6, 7):
+            last = s - 2 if op == 6 else (s - 1) // 2
+            for j in range(1, last + 1):
+                m = s - 1 - j
+                tot += c[op] * T[j][8] * T[m][8] + C[j] * T[m][8] + T[j][8] * C[m]
+        C[s] = tot
+    return T, C
+
+
+def executed_ops_per_pair(W):
+    """VALU lane-ops the fused pair kernel issues per (A, B) value pair:
+    7 operator instances (& | ^ + * A-B B-A) each followed by two windowed
+    compares (64-bit: sub 2 + cmp 1 each)."""
+    if W == 64:
+        ops = 2 + 2 + 2 + 2 + 4 + 2 + 2
+        return ops + 7 * 2 * 3
+    return 7 + 7 * 2 * 2
+
+
+def cpu_baseline_simba(inst, max_size, threads):
+    from oracle import oracle as orc
+    r = orc.simba_search(inst.inputs, inst.outputs, inst.n_variables,
+                         [int(c) for c in inst.consts], inst.width, 1, max_size, threads)
+    return r
+
+
+def run_simba(args):
+    import simba_amd
+    from simba_amd.synth import make_instance
+
+    inst = make_instance(args.seed, 4, 64, 15, 64, (1,))
+    S = args.size
+    L = 5
+    T, C = naive_ops_per_size(L, S, 64)
+    cands = sum(T[s][8] for s in range(1, S + 1))
+    naive_ops = sum(C[s] for s in range(1, S + 1))
+    ctx = simba_amd.Context(0)
+    kw = dict(min_size=1, max_size=S, exhaustive=True, table_bytes=args.table_gb << 30)
+    for _ in range(args.warmup):
+        ctx.simba(inst, **kw)
+    times, ktimes = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = ctx.simba(inst, **kw)
+        times.append(r["seconds"])
+        ktimes.append(r["kernel_seconds"])
+        assert r["size_reached"] == S and r["candidates"] == cands, r
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    value = cands * args.steps / wall / 1e9
+    kavg = sum(ktimes) / len(ktimes)
+    # pair-kernel share of the work: binary-rooted trees of sizes 3..S
+    pairs = 0
+    for s in range(3, S + 1):
+        for j in range(1, (s - 1) // 2 + 1):
+            pairs += T[j][8] * T[s - 1 - j][8]
+    achieved = naive_ops / kavg / 1e12
+    executed = pairs * executed_ops_per_pair(64) / kavg / 1e12
+    out = {
+        "metric": "Gcand/s evaluated",
+        "value": round(value, 3),
+        "unit": "Gcand/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "simba_sweep_4vars_const1_u64_64samples_sizes1to%d" % S,
+                   "target": inst.original_formula, "target_size": 15,
+                   "candidates_per_step": cands, "first_solution": r["expr"],
+                   "first_solution_size": r["size"]},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3),
+                     "peak": round(VALU_PEAK / 1e12, 3), "unit": "TOP/s",
+                     "frac": round(achieved * 1e12 / VALU_PEAK, 4), "traffic": None,
+                     "definition": "naive-equivalent lane-ops (sum of c(op) over every "
+                                   "internal node of every candidate) / pair-kernel time",
+                     "executed_achieved": round(executed, 3),
+                     "executed_frac": round(executed * 1e12 / VALU_PEAK, 4),
+                     "kernel_seconds_per_step": round(kavg, 4)},
+    }
+    ctx.close()
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cb = cpu_baseline_simba(inst, args.cpu_size, threads)
+        out["cpu_baseline"] = {
+            "value": round(cb["candidates"] / cb["seconds"] / 1e9, 5), "unit": "Gcand/s",
+            "cores": threads, "kind": "port",
+            "sample": "oracle SIMBA search of the same instance, sizes 1..%d "
+                      "(%d candidates, %.1f s)" % (args.cpu_size, cb["candidates"], cb["seconds"])}
+    return out
+
+
+def run_vfb(args):
+    import simba_amd
+    from simba_amd.synth import make_instance
+
+    inst = make_instance(args.seed, 3, 32, 11, 64, (1, (1 << 64) - 1))
+    ctx = simba_amd.Context(0)
+    kw = dict(max_size=args.vfb_levels, exhaustive=True, cache_bytes=args.cache_gb << 30)
+    for _ in range(args.warmup):
+        ctx.vfb(inst, **kw)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = ctx.vfb(inst, **kw)
+    wall = time.perf_counter() - t0
+    cands = r["candidates"]
+    nbytes = sum(r["level_bytes"])
+    out = {
+        "metric": "Gcand/s evaluated", "value": round(cands * args.steps / wall / 1e9, 3),
+        "unit": "Gcand/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "vfb_3vars_consts1m1_u64_32samples_levels1to%d" % args.vfb_levels,
+                   "level_new": r["level_new"][1:args.vfb_levels + 1],
+                   "level_cand": r["level_cand"][1:args.vfb_levels + 1]},
+        "roofline": {"bound": "hbm", "achieved": round(nbytes / r["kernel_seconds"] / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(nbytes / r["kernel_seconds"] / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+    }
+    ctx.close()
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        lv = min(args.vfb_levels, 8)
+        t = time.perf_counter()
+        w = orc.vfb_search(inst.inputs, inst.outputs, 3, [1, (1 << 64) - 1], 64, lv,
+                           exhaustive=True)
+        dt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": round(w["candidates"] / dt / 1e9, 5), "unit": "Gcand/s",
+                               "cores": 1, "kind": "port",
+                               "sample": "oracle VFB, same instance, levels 1..%d" % lv}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="simba", choices=["simba", "vfb"])
+    ap.add_argument("--size", type=int, default=14)
+    ap.add_argument("--cpu-size", type=int, default=9)
+    ap.add_argument("--table-gb", type=int, default=40)
+    ap.add_argument("--vfb-levels", type=int, default=11)
+    ap.add_argument("--cache-gb", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=15)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every box here has a single MI355X; "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    out = run_simba(args) if args.workload == "simba" else run_vfb(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -66,7 +66,8 @@ typedef struct {
   int32_t compat;           /* 1: reference ranges: SIMBA sizes 2..max-1,    */
                             /*    VFB levels 1..max (nocache.cu:246,         */
                             /*    full-cache.cu:754)                         */
-  int32_t exhaustive;       /* VFB: keep enumerating after a solution        */
+  int32_t exhaustive;       /* keep enumerating up to max after a solution   */
+                            /* (the answer is still the first solution)      */
   double time_budget_s;     /* <= 0: none                                    */
   volatile int32_t *cancel; /* host-written; polled between launches; NULL ok */
   uint64_t table_bytes;     /* SIMBA: value-table budget, 0 = default        */

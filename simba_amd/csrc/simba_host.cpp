@@ -303,7 +303,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     }
   }
 
-  for (int s = 3; answer < 0 && s <= max_size; ++s) {
+  for (int s = 3; (answer < 0 || o.exhaustive) && s <= max_size; ++s) {
     if (over_budget()) {
       status = MBA_BUDGET;
       break;
@@ -332,7 +332,8 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
         const uint64_t ntl = p.tile_end - p.tile_begin;
         const int g = (int)std::min<uint64_t>((uint64_t)grid, ntl);
         MBA_HIP(hipEventRecord(ctx.ev0, st));
-        MBA_HIP(launch_pair_check(W, d_tv, p, tg, d_hits, d_hc, kHitCap, g, st));
+        MBA_HIP(launch_pair_check(W, d_tv, ctx.tables.p, tv.off, p, tg, d_hits, d_hc,
+                                  kHitCap, g, st));
         MBA_HIP(hipEventRecord(ctx.ev1, st));
         MBA_HIP(hipEventSynchronize(ctx.ev1));
         float ms = 0.f;
@@ -353,7 +354,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     if (o.verbose)
       std::fprintf(stderr, "Length %-2d | candidates %-20llu | %.3f s\n", s,
                    (unsigned long long)T.t[s][8], seconds_since(t_start));
-    if (s >= min_size && found[s] != ~0ull) answer = s;
+    if (answer < 0 && s >= min_size && found[s] != ~0ull) answer = s;
   }
 
   uint64_t cand = 0;

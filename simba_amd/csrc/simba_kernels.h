@@ -48,7 +48,9 @@ constexpr int kPairTA = 256;                    // A values per tile
 // All launches are asynchronous on `stream`.
 hipError_t launch_table_build(int width, const TableView *d_tv, void *d_base,
                               uint64_t off, int m, uint64_t n, hipStream_t stream);
-hipError_t launch_pair_check(int width, const TableView *d_tv, const PairLaunch &p,
+// d_base/off: the table buffer and the host copy of TableView::off.
+hipError_t launch_pair_check(int width, const TableView *d_tv, const void *d_base,
+                             const uint64_t *off, const PairLaunch &p,
                              const CheckTargets &tg, HitRec *d_hits,
                              uint32_t *d_hit_count, uint32_t hit_cap, int grid,
                              hipStream_t stream);

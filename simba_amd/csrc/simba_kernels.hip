@@ -161,16 +161,91 @@ __device__ __noinline__ void record_hit(HitRec *hits, uint32_t *count, uint32_t 
 // [at*TA, at*TA+TA) x B values [bt*TB, bt*TB+TB) with at = t % nTA.
 // SAME: j == m, so (Y - X) is the (X - Y) instance of the swapped pair and is
 // skipped.
+//
+// Inner loop (per A value, per B value held in VGPRs): the seven operator
+// results are folded into one running minimum of their distances to the two
+// windows, so each value costs its operator plus two subtractions and one
+// 3-way minimum; one compare per tile decides whether a lane saw a hit.
+// For 64-bit words only the high halves are formed and compared: a value in
+// a window of width span < 2^32 has hi(c) - hi(lo) in {0, 1} (mod 2^32), so
+// this is a necessary condition, and the rare lanes that pass it re-walk the
+// tile with the exact 64-bit window test.
 // ---------------------------------------------------------------------------
+// Per-B-value precomputation, hoisted out of the A loop: the window offsets
+// are folded into B so that the additive instances need one add per window.
+//   64-bit words: offsets are multiples of 2^32 (He = hi(lo_even) << 32),
+//   which leave the low halves and carries unchanged, so hi(a + be) equals
+//   hi(a + b) - hi(lo_even) exactly.
+template <typename word>
+struct BPre {
+  word b, be, bo, nbe, nbo;  // b, b-Oe, b-Oo, -b-Oe, -b-Oo
+};
+
+template <typename word>
+__device__ __forceinline__ word win_off(word lo);
+template <>
+__device__ __forceinline__ uint32_t win_off<uint32_t>(uint32_t lo) { return lo; }
+template <>
+__device__ __forceinline__ uint64_t win_off<uint64_t>(uint64_t lo) {
+  return lo & 0xffffffff00000000ull;
+}
+
+template <typename word>
+__device__ __forceinline__ BPre<word> bpre(word b, word oe, word oo) {
+  BPre<word> p;
+  p.b = b;
+  p.be = b - oe;
+  p.bo = b - oo;
+  p.nbe = word(0) - b - oe;
+  p.nbo = word(0) - b - oo;
+  return p;
+}
+
+// The distance that is compared: the whole word for 32-bit words, the high
+// half for 64-bit words.
+__device__ __forceinline__ uint32_t dpart(uint32_t x) { return x; }
+__device__ __forceinline__ uint32_t dpart(uint64_t x) { return (uint32_t)(x >> 32); }
+
+__device__ __forceinline__ uint32_t min3u(uint32_t a, uint32_t b, uint32_t c) {
+  return min(min(a, b), c);
+}
+
+template <typename word, bool SAME>
+__device__ __forceinline__ uint32_t fold7(uint32_t acc, word a, word na, const BPre<word> &p,
+                                          uint32_t de, uint32_t dof) {
+  const uint32_t x_and = dpart(word(a & p.b));
+  const uint32_t x_or = dpart(word(a | p.b));
+  const uint32_t x_xor = dpart(word(a ^ p.b));
+  const uint32_t x_mul = dpart(word(a * p.b));
+  acc = min3u(acc, x_and - de, x_and - dof);
+  acc = min3u(acc, x_or - de, x_or - dof);
+  acc = min3u(acc, x_xor - de, x_xor - dof);
+  acc = min3u(acc, x_mul - de, x_mul - dof);
+  acc = min3u(acc, dpart(word(a + p.be)), dpart(word(a + p.bo)));    // a + b
+  acc = min3u(acc, dpart(word(a + p.nbe)), dpart(word(a + p.nbo)));  // a - b
+  if (!SAME) acc = min3u(acc, dpart(word(p.be + na)), dpart(word(p.bo + na)));  // b - a
+  return acc;
+}
+
+// Moves a wave-uniform word into scalar registers, so that expressions such
+// as b + (-a) keep their single-add form.
+__device__ __forceinline__ uint32_t uniform(uint32_t x) {
+  return __builtin_amdgcn_readfirstlane(x);
+}
+__device__ __forceinline__ uint64_t uniform(uint64_t x) {
+  return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32) |
+         __builtin_amdgcn_readfirstlane((uint32_t)x);
+}
+
 template <typename word, bool SAME, bool GEN_A, bool GEN_B>
 __global__ void __launch_bounds__(kPairThreads)
-k_pair_check(const TableView *__restrict__ tv, PairLaunch p, uint64_t lo_e64,
-             uint64_t lo_o64, uint64_t span64, HitRec *__restrict__ hits,
-             uint32_t *__restrict__ hit_count, uint32_t hit_cap) {
-  const word *V = (const word *)tv->base;
-  const word *A = V + tv->off[p.j];
-  const word *B = V + tv->off[p.m];
+k_pair_check(const TableView *__restrict__ tv, const word *__restrict__ A,
+             const word *__restrict__ B, PairLaunch p, uint64_t lo_e64, uint64_t lo_o64,
+             uint64_t span64, HitRec *__restrict__ hits, uint32_t *__restrict__ hit_count,
+             uint32_t hit_cap) {
   const word lo_e = (word)lo_e64, lo_o = (word)lo_o64, span = (word)span64;
+  // prefilter threshold on the folded distance
+  const uint32_t thr = sizeof(word) == 8 ? 1u : (uint32_t)span64;
   const uint32_t pair = ((uint32_t)p.sb << 16) | (uint32_t)p.j;
   for (uint64_t t = p.tile_begin + blockIdx.x; t < p.tile_end; t += gridDim.x) {
     const uint64_t at = t % p.nTA, bt = t / p.nTA;
@@ -179,29 +254,33 @@ k_pair_check(const TableView *__restrict__ tv, PairLaunch p, uint64_t lo_e64,
     const uint64_t b0 = bt * kPairTB + threadIdx.x;
     word bv[kPairRB];
     bool ok[kPairRB];
+    bool any_ok = false;
 #pragma unroll
     for (int k = 0; k < kPairRB; ++k) {
       const uint64_t b = b0 + (uint64_t)k * kPairThreads;
       ok[k] = b < p.nB;
+      any_ok |= ok[k];
       bv[k] = ok[k] ? (GEN_B ? value_of<word>(tv, p.m, b) : B[b]) : word(0);
     }
-    bool hit = false;
+    // lanes past the end of B reuse a valid value of their own (recorded
+    // hits are re-derived with the exact masks below)
+#pragma unroll
+    for (int k = 1; k < kPairRB; ++k)
+      if (!ok[k]) bv[k] = bv[0];
+    const word oe = win_off<word>(lo_e), oo = win_off<word>(lo_o);
+    const uint32_t de = dpart(oe), dof = dpart(oo);
+    BPre<word> pre[kPairRB];
+#pragma unroll
+    for (int k = 0; k < kPairRB; ++k) pre[k] = bpre<word>(bv[k], oe, oo);
+    uint32_t acc = 0xffffffffu;
     for (uint64_t a = a0; a < a1; ++a) {
       const word av = GEN_A ? value_of<word>(tv, p.j, a) : A[a];
+      const word na = uniform(word(word(0) - av));
 #pragma unroll
-      for (int k = 0; k < kPairRB; ++k) {
-        const word b = bv[k];
-        bool h = in_window(word(av & b), lo_e, lo_o, span);
-        h |= in_window(word(av | b), lo_e, lo_o, span);
-        h |= in_window(word(av ^ b), lo_e, lo_o, span);
-        h |= in_window(word(av + b), lo_e, lo_o, span);
-        h |= in_window(word(av * b), lo_e, lo_o, span);
-        h |= in_window(word(av - b), lo_e, lo_o, span);
-        if (!SAME) h |= in_window(word(b - av), lo_e, lo_o, span);
-        hit |= h & ok[k];
-      }
+      for (int k = 0; k < kPairRB; ++k) acc = fold7<word, SAME>(acc, av, na, pre[k], de, dof);
     }
-    // Rare: some lane saw a window hit.  Re-walk the tile and record them.
+    const bool hit = any_ok && acc <= thr;
+    // Rare: some lane passed the prefilter.  Re-walk the tile, exact test.
     if (__any(hit)) {
       if (hit) {
         for (uint64_t a = a0; a < a1; ++a) {
@@ -330,13 +409,13 @@ hipError_t launch_table_build(int width, const TableView *d_tv, void *d_base,
 }
 
 template <typename word>
-static void pair_dispatch(const TableView *d_tv, const PairLaunch &p,
-                          const CheckTargets &tg, HitRec *h, uint32_t *hc,
-                          uint32_t cap, int grid, hipStream_t s) {
+static void pair_dispatch(const TableView *d_tv, const word *A, const word *B,
+                          const PairLaunch &p, const CheckTargets &tg, HitRec *h,
+                          uint32_t *hc, uint32_t cap, int grid, hipStream_t s) {
   const bool same = p.j == p.m;
 #define PAIR_LAUNCH(S, GA, GB)                                                    \
   hipLaunchKernelGGL((k_pair_check<word, S, GA, GB>), dim3(grid), dim3(kPairThreads), \
-                     0, s, d_tv, p, tg.lo_even, tg.lo_odd, tg.span, h, hc, cap)
+                     0, s, d_tv, A, B, p, tg.lo_even, tg.lo_odd, tg.span, h, hc, cap)
   if (same) {
     if (p.gen_a) PAIR_LAUNCH(true, true, true);
     else PAIR_LAUNCH(true, false, false);
@@ -348,14 +427,21 @@ static void pair_dispatch(const TableView *d_tv, const PairLaunch &p,
 #undef PAIR_LAUNCH
 }
 
-hipError_t launch_pair_check(int width, const TableView *d_tv, const PairLaunch &p,
+hipError_t launch_pair_check(int width, const TableView *d_tv, const void *d_base,
+                             const uint64_t *off, const PairLaunch &p,
                              const CheckTargets &tg, HitRec *d_hits,
                              uint32_t *d_hit_count, uint32_t hit_cap, int grid,
                              hipStream_t stream) {
-  if (width == 32)
-    pair_dispatch<uint32_t>(d_tv, p, tg, d_hits, d_hit_count, hit_cap, grid, stream);
-  else
-    pair_dispatch<uint64_t>(d_tv, p, tg, d_hits, d_hit_count, hit_cap, grid, stream);
+  // sides above Kt have no table: their pointer is unused by the kernel
+  if (width == 32) {
+    const uint32_t *base = (const uint32_t *)d_base;
+    pair_dispatch<uint32_t>(d_tv, base + (p.gen_a ? 0 : off[p.j]), base + (p.gen_b ? 0 : off[p.m]),
+                            p, tg, d_hits, d_hit_count, hit_cap, grid, stream);
+  } else {
+    const uint64_t *base = (const uint64_t *)d_base;
+    pair_dispatch<uint64_t>(d_tv, base + (p.gen_a ? 0 : off[p.j]), base + (p.gen_b ? 0 : off[p.m]),
+                            p, tg, d_hits, d_hit_count, hit_cap, grid, stream);
+  }
   return hipGetLastError();
 }
 

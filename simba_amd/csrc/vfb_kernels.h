@@ -1,0 +1,70 @@
+// vfb_kernels.h -- launch interface of the VFB (bottom-up, cached) kernels.
+#pragma once
+#include "mba_common.h"
+
+namespace mba {
+
+constexpr uint64_t kVfbEmpty = ~0ull;        // empty hash slot (full-cache.cu:738)
+constexpr uint64_t kVfbTomb = ~0ull - 1;     // tombstone key (full-cache.cu:739)
+constexpr int kVfbMaxBlocks = 96;
+constexpr int kVfbChunk = 4096;              // candidates per workgroup step
+constexpr int kVfbThreads = 256;
+
+// One (op, split[, swap]) block of a level's candidate space.  Candidate
+// indices of the level are the concatenation of the blocks in enumeration
+// order (full-cache.cu:568-663): unary blocks over the rows of level L-1,
+// binary blocks over left rows (outer) x right rows (inner); for '-' every
+// pair gives (l, r) at local index 2p and (r, l) at 2p + 1.
+struct VfbBlock {
+  uint64_t begin;  // first candidate index of the block
+  uint64_t count;  // candidates in the block
+  uint32_t l0, nl; // left rows [l0, l0+nl)
+  uint32_t r0, nr; // right rows (binary)
+  int32_t op;
+  int32_t pad;
+};
+
+struct VfbLevelArgs {
+  const void *rows;       // SoA row store: rows[sample * cap + row]
+  uint64_t cap;           // row capacity
+  int n;                  // samples
+  int nblocks;
+  const VfbBlock *blocks; // device copy
+  uint64_t total;         // candidates of the level
+  uint64_t *keys;         // hash table keys (fingerprints)
+  uint64_t *minidx;       // per slot: (level << 48) | min candidate index; a
+                          // key of an earlier level keeps a smaller value
+  uint64_t tag;           // level << 48 of the level being processed
+  uint64_t mask;          // table capacity - 1 (power of two)
+  uint32_t *slot_of;      // per candidate: slot of its fingerprint (or ~0u: invalid)
+  uint8_t *keep;          // per candidate: kept (first occurrence of a new fingerprint)
+  uint32_t *chunk_count;  // per chunk of kVfbChunk candidates: kept count
+  uint64_t *chunk_base;   // exclusive prefix of chunk_count
+  const void *target;     // n target words
+  unsigned long long *found;  // min index of a candidate equal to the target
+  unsigned int *fail;     // set when a probe sequence exceeds max_probes
+  uint64_t max_probes;
+  void *out_rows;         // rows of this level start at row `out_base`
+  uint64_t out_base;
+  int32_t *left, *right;  // child rows per row
+  uint8_t *row_op;        // root op per row
+  unsigned long long *probes;  // total probe steps (statistics)
+};
+
+hipError_t vfb_launch_insert(int width, const VfbLevelArgs &a, int grid, hipStream_t s);
+hipError_t vfb_launch_flag(int width, const VfbLevelArgs &a, int grid, hipStream_t s);
+hipError_t vfb_launch_scan(const VfbLevelArgs &a, uint64_t nchunks, hipStream_t s);
+hipError_t vfb_launch_write(int width, const VfbLevelArgs &a, int grid, hipStream_t s);
+// Leaves: fingerprint and insert rows [0, L) (hashSetInit, full-cache.cu:142-157).
+hipError_t vfb_launch_seed(int width, const VfbLevelArgs &a, int L, hipStream_t s);
+// Writes, for the fingerprint of the target, the slot's minidx into *out
+// (~0 if absent).
+hipError_t vfb_launch_lookup_target(int width, const VfbLevelArgs &a, uint64_t *out,
+                                    hipStream_t s);
+// Post-order walk of the solution tree into RPN tokens (at most `cap`).
+hipError_t vfb_launch_rpn(const int32_t *left, const int32_t *right, const uint8_t *op,
+                          int L, int32_t root_op, int32_t root_l, int32_t root_r,
+                          uint8_t *out, int32_t *out_len, int cap, hipStream_t s);
+hipError_t vfb_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s);
+
+}  // namespace mba

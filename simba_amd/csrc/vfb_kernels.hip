@@ -1,0 +1,419 @@
+// vfb_kernels.hip -- gfx950 kernels of the VFB bottom-up cached search.
+//
+// Reference: processOperator<op> (full-cache.cu:333-372) computes one
+// candidate per thread from two AoS cache rows, fingerprints it
+// (makeUnqChk, :126-134), inserts the fingerprint into a warpcore HashSet
+// (64-bit atomicCAS, hash_set.cuh:193-254) and writes the row, a per-sample
+// bool mask and the child indices to a temp cache that thrust compacts.
+//
+// Here a level is three passes over its candidate index space, which keeps
+// the representative of each new fingerprint deterministic (the first
+// candidate in enumeration order, where the reference takes whichever
+// thread wins the CAS race):
+//   insert: signature -> fingerprint -> open-addressing insert (64-bit CAS on
+//           the key, atomicMin of (level << 48 | candidate index) on the
+//           slot's companion word, so keys of earlier levels always win); the
+//           candidate's slot is stored, and candidates equal to the target
+//           record their index;
+//   flag:   keep = (companion == level << 48 | own index); per-chunk counts;
+//   write:  prefix over chunks + in-chunk prefix -> row position; kept
+//           candidates recompute their signature and append it in order.
+// Rows are stored SoA, rows[sample * cap + row], so the lanes of a wave that
+// walk consecutive right rows read consecutive words of every sample.
+#include "vfb_kernels.h"
+
+namespace mba {
+
+// --- fingerprint (full-cache.cu:113-134) -----------------------------------
+__device__ __forceinline__ uint32_t fmix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x85ebca6bu;
+  x ^= x >> 13;
+  x *= 0xc2b2ae35u;
+  x ^= x >> 16;
+  return x;
+}
+
+__device__ __forceinline__ uint64_t fmix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+
+// One step of makeUnqChk.  For 32-bit words the reference adds
+// simpleHash(x) + 0x9e3779b9 in 32-bit unsigned arithmetic (both operands
+// are unsigned int) before widening; the 64-bit extension uses the 64-bit
+// MurmurHash3 finalizer and a 64-bit sum.
+template <typename word>
+__device__ __forceinline__ uint64_t fp_step(uint64_t seed, word x, int i) {
+  uint64_t v;
+  if (sizeof(word) == 4) {
+    const uint32_t h = fmix32((uint32_t)x) + 0x9e3779b9u;
+    v = (uint64_t)h + (seed << 6) + (seed >> 2);
+  } else {
+    v = fmix64((uint64_t)x) + 0x9e3779b9ull + (seed << 6) + (seed >> 2);
+  }
+  if (i & 1) v = (v >> 32) | (v << 32);
+  return seed ^ v;
+}
+
+// --- candidate decoding ----------------------------------------------------
+struct Cand {
+  uint32_t l, r;
+  int op;
+};
+
+__device__ __forceinline__ Cand decode(const VfbBlock *blk, int nblocks, uint64_t idx) {
+  int b = 0;
+  while (b + 1 < nblocks && idx >= blk[b + 1].begin) ++b;
+  const VfbBlock &B = blk[b];
+  uint64_t local = idx - B.begin;
+  Cand c;
+  c.op = B.op;
+  if (is_unary(B.op)) {
+    c.l = B.l0 + (uint32_t)local;
+    c.r = c.l;
+    return c;
+  }
+  int sw = 0;
+  if (B.op == SUB) {
+    sw = (int)(local & 1);
+    local >>= 1;
+  }
+  const uint64_t q = local / B.nr;
+  c.l = B.l0 + (uint32_t)q;
+  c.r = B.r0 + (uint32_t)(local - q * B.nr);
+  if (sw) {
+    const uint32_t t = c.l;
+    c.l = c.r;
+    c.r = t;
+  }
+  return c;
+}
+
+template <typename word>
+__device__ __forceinline__ uint64_t signature_fp(const word *rows, uint64_t cap, int n,
+                                                 const Cand &c, const word *tgt,
+                                                 bool &eq) {
+  uint64_t seed = 0;
+  eq = true;
+  for (int i = 0; i < n; ++i) {
+    const word x = rows[(uint64_t)i * cap + c.l];
+    const word y = rows[(uint64_t)i * cap + c.r];
+    const word v = apply_op<word>(c.op, x, y);
+    eq &= (v == tgt[i]);
+    seed = fp_step<word>(seed, v, i);
+  }
+  return seed;
+}
+
+// Open addressing with linear probing; returns the slot of `key`, or ~0 when
+// the probe sequence exceeds `limit` (table too full: the host reports a
+// capacity overflow), so every lane terminates.
+__device__ __forceinline__ uint64_t table_insert(uint64_t *keys, uint64_t mask,
+                                                 uint64_t key, uint64_t &probes,
+                                                 uint64_t limit) {
+  uint64_t s = fmix64(key) & mask;
+  for (uint64_t step = 0;; ++step) {
+    if (step >= limit) return ~0ull;
+    ++probes;
+    uint64_t k = __atomic_load_n(&keys[s], __ATOMIC_RELAXED);
+    if (k == key) return s;
+    if (k == kVfbEmpty) {
+      const uint64_t old = atomicCAS((unsigned long long *)&keys[s],
+                                     (unsigned long long)kVfbEmpty,
+                                     (unsigned long long)key);
+      if (old == kVfbEmpty || old == key) return s;
+    }
+    s = (s + 1) & mask;
+  }
+}
+
+__device__ __forceinline__ uint64_t table_find(const uint64_t *keys, uint64_t mask,
+                                               uint64_t key) {
+  uint64_t s = fmix64(key) & mask;
+  for (uint64_t step = 0; step <= mask; ++step) {
+    const uint64_t k = keys[s];
+    if (k == key) return s;
+    if (k == kVfbEmpty) return ~0ull;
+    s = (s + 1) & mask;
+  }
+  return ~0ull;
+}
+
+template <typename word>
+__global__ void __launch_bounds__(kVfbThreads) k_vfb_insert(VfbLevelArgs a) {
+  __shared__ VfbBlock s_blk[kVfbMaxBlocks];
+  __shared__ word s_tgt[kMaxSamples];
+  for (int i = threadIdx.x; i < a.nblocks; i += blockDim.x) s_blk[i] = a.blocks[i];
+  for (int i = threadIdx.x; i < a.n; i += blockDim.x) s_tgt[i] = ((const word *)a.target)[i];
+  __syncthreads();
+  const word *rows = (const word *)a.rows;
+  uint64_t probes = 0;
+  const uint64_t nchunks = (a.total + kVfbChunk - 1) / kVfbChunk;
+  for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    for (int k = 0; k < kVfbChunk / kVfbThreads; ++k) {
+      const uint64_t idx = ch * kVfbChunk + (uint64_t)k * kVfbThreads + threadIdx.x;
+      if (idx >= a.total) break;
+      const Cand c = decode(s_blk, a.nblocks, idx);
+      bool eq;
+      const uint64_t fp = signature_fp<word>(rows, a.cap, a.n, c, s_tgt, eq);
+      if (fp >= kVfbTomb) {  // invalid key: dropped (hash_set.cuh:200-204)
+        a.slot_of[idx] = ~0u;
+        continue;
+      }
+      const uint64_t s = table_insert(a.keys, a.mask, fp, probes, a.max_probes);
+      if (s == ~0ull) {
+        atomicOr(a.fail, 1u);
+        a.slot_of[idx] = ~0u;
+        continue;
+      }
+      atomicMin((unsigned long long *)&a.minidx[s], (unsigned long long)(a.tag | idx));
+      a.slot_of[idx] = (uint32_t)s;
+      if (eq) atomicMin(a.found, (unsigned long long)idx);
+    }
+  }
+  if (a.probes) atomicAdd(a.probes, (unsigned long long)probes);
+}
+
+// keep = first occurrence in this level of a fingerprint unseen before.
+__global__ void __launch_bounds__(kVfbThreads) k_vfb_flag(VfbLevelArgs a) {
+  __shared__ uint32_t s_cnt;
+  const uint64_t nchunks = (a.total + kVfbChunk - 1) / kVfbChunk;
+  for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    uint32_t mine = 0;
+    for (int k = 0; k < kVfbChunk / kVfbThreads; ++k) {
+      const uint64_t idx = ch * kVfbChunk + (uint64_t)k * kVfbThreads + threadIdx.x;
+      if (idx >= a.total) break;
+      const uint32_t s = a.slot_of[idx];
+      const uint8_t kp = (s != ~0u) && (a.minidx[s] == (a.tag | idx));
+      a.keep[idx] = kp;
+      mine += kp;
+    }
+    // wave reduction, one LDS atomic per wave
+    for (int off = 32; off > 0; off >>= 1) mine += __shfl_down(mine, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&s_cnt, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) a.chunk_count[ch] = s_cnt;
+    __syncthreads();
+  }
+}
+
+// Exclusive prefix of the chunk counts (single workgroup).
+__global__ void __launch_bounds__(1024) k_vfb_scan(const uint32_t *cnt, uint64_t *base,
+                                                   uint64_t n) {
+  __shared__ uint64_t s_part[1024];
+  const uint64_t per = (n + 1023) / 1024;
+  const uint64_t b0 = threadIdx.x * per, b1 = b0 + per < n ? b0 + per : n;
+  uint64_t sum = 0;
+  for (uint64_t i = b0; i < b1; ++i) sum += cnt[i];
+  s_part[threadIdx.x] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t acc = 0;
+    for (int i = 0; i < 1024; ++i) {
+      const uint64_t v = s_part[i];
+      s_part[i] = acc;
+      acc += v;
+    }
+    base[n] = acc;
+  }
+  __syncthreads();
+  uint64_t acc = s_part[threadIdx.x];
+  for (uint64_t i = b0; i < b1; ++i) {
+    base[i] = acc;
+    acc += cnt[i];
+  }
+}
+
+template <typename word>
+__global__ void __launch_bounds__(kVfbThreads) k_vfb_write(VfbLevelArgs a) {
+  __shared__ VfbBlock s_blk[kVfbMaxBlocks];
+  __shared__ uint32_t s_wave[kVfbThreads / 64];
+  for (int i = threadIdx.x; i < a.nblocks; i += blockDim.x) s_blk[i] = a.blocks[i];
+  __syncthreads();
+  const word *rows = (const word *)a.rows;
+  word *out = (word *)a.out_rows;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t nchunks = (a.total + kVfbChunk - 1) / kVfbChunk;
+  for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    uint64_t pos = a.out_base + a.chunk_base[ch];
+    for (int k = 0; k < kVfbChunk / kVfbThreads; ++k) {
+      const uint64_t idx = ch * kVfbChunk + (uint64_t)k * kVfbThreads + threadIdx.x;
+      const bool kp = idx < a.total && a.keep[idx];
+      // prefix of kept flags in index order across the 4 waves of this step
+      const uint64_t bal = __ballot(kp);
+      const uint32_t below = __popcll(bal & ((1ull << lane) - 1));
+      if (lane == 0) s_wave[wave] = __popcll(bal);
+      __syncthreads();
+      uint32_t wbase = 0, tot = 0;
+      for (int w = 0; w < kVfbThreads / 64; ++w) {
+        if (w < wave) wbase += s_wave[w];
+        tot += s_wave[w];
+      }
+      if (kp) {
+        const Cand c = decode(s_blk, a.nblocks, idx);
+        const uint64_t row = pos + wbase + below;
+        for (int i = 0; i < a.n; ++i) {
+          const word x = rows[(uint64_t)i * a.cap + c.l];
+          const word y = rows[(uint64_t)i * a.cap + c.r];
+          out[(uint64_t)i * a.cap + row] = apply_op<word>(c.op, x, y);
+        }
+        a.left[row] = (int32_t)c.l;
+        a.right[row] = is_unary(c.op) ? -1 : (int32_t)c.r;
+        a.row_op[row] = (uint8_t)c.op;
+      }
+      pos += tot;
+      __syncthreads();
+    }
+  }
+}
+
+template <typename word>
+__global__ void k_vfb_seed(VfbLevelArgs a, int L) {
+  const word *rows = (const word *)a.rows;
+  const int v = threadIdx.x;
+  if (v >= L) return;
+  uint64_t seed = 0;
+  for (int i = 0; i < a.n; ++i) seed = fp_step<word>(seed, rows[(uint64_t)i * a.cap + v], i);
+  if (seed >= kVfbTomb) return;
+  uint64_t probes = 0;
+  const uint64_t s = table_insert(a.keys, a.mask, seed, probes, a.max_probes);
+  if (s == ~0ull) {
+    atomicOr(a.fail, 1u);
+    return;
+  }
+  a.minidx[s] = 0;  // present before any level >= 2
+}
+
+template <typename word>
+__global__ void k_vfb_lookup_target(VfbLevelArgs a, uint64_t *out) {
+  if (threadIdx.x != 0) return;
+  const word *t = (const word *)a.target;
+  uint64_t seed = 0;
+  for (int i = 0; i < a.n; ++i) seed = fp_step<word>(seed, t[i], i);
+  const uint64_t s = seed >= kVfbTomb ? ~0ull : table_find(a.keys, a.mask, seed);
+  *out = s == ~0ull ? ~0ull : a.minidx[s];
+}
+
+__global__ void k_vfb_rpn(const int32_t *left, const int32_t *right, const uint8_t *op,
+                          int L, int32_t root_op, int32_t root_l, int32_t root_r,
+                          uint8_t *out, int32_t *out_len, int cap) {
+  if (threadIdx.x != 0) return;
+  // iterative post-order: (row, state); the root is the unsaved candidate
+  int32_t st_row[256];
+  uint8_t st_state[256];
+  int sp = 0, n = 0;
+  auto push_child = [&](int32_t row) {
+    if (sp < 256) {
+      st_row[sp] = row;
+      st_state[sp] = 0;
+      ++sp;
+    }
+  };
+  const bool un = is_unary(root_op);
+  // children of the root, left then right, then the root op itself
+  int32_t kids[2] = {root_l, root_r};
+  for (int k = 0; k < (un ? 1 : 2); ++k) {
+    push_child(kids[k]);
+    while (sp > 0) {
+      const int32_t row = st_row[sp - 1];
+      if (row < L) {
+        if (n < cap) out[n] = (uint8_t)row;
+        ++n;
+        --sp;
+        continue;
+      }
+      const int o = op[row];
+      const uint8_t s = st_state[sp - 1];
+      if (s == 0) {
+        st_state[sp - 1] = 1;
+        push_child(left[row]);
+      } else if (s == 1 && !is_unary(o)) {
+        st_state[sp - 1] = 2;
+        push_child(right[row]);
+      } else {
+        if (n < cap) out[n] = (uint8_t)(kTokOp + o);
+        ++n;
+        --sp;
+      }
+    }
+  }
+  if (n < cap) out[n] = (uint8_t)(kTokOp + root_op);
+  ++n;
+  *out_len = n;
+}
+
+__global__ void k_fill_u64(uint64_t *p, uint64_t n, uint64_t v) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    p[i] = v;
+}
+
+// --- launchers -------------------------------------------------------------
+hipError_t vfb_launch_insert(int width, const VfbLevelArgs &a, int grid, hipStream_t s) {
+  if (width == 32)
+    hipLaunchKernelGGL(k_vfb_insert<uint32_t>, dim3(grid), dim3(kVfbThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_vfb_insert<uint64_t>, dim3(grid), dim3(kVfbThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t vfb_launch_flag(int, const VfbLevelArgs &a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_vfb_flag, dim3(grid), dim3(kVfbThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t vfb_launch_scan(const VfbLevelArgs &a, uint64_t nchunks, hipStream_t s) {
+  hipLaunchKernelGGL(k_vfb_scan, dim3(1), dim3(1024), 0, s, a.chunk_count, a.chunk_base,
+                     nchunks);
+  return hipGetLastError();
+}
+
+hipError_t vfb_launch_write(int width, const VfbLevelArgs &a, int grid, hipStream_t s) {
+  if (width == 32)
+    hipLaunchKernelGGL(k_vfb_write<uint32_t>, dim3(grid), dim3(kVfbThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_vfb_write<uint64_t>, dim3(grid), dim3(kVfbThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t vfb_launch_seed(int width, const VfbLevelArgs &a, int L, hipStream_t s) {
+  if (width == 32)
+    hipLaunchKernelGGL(k_vfb_seed<uint32_t>, dim3(1), dim3(64), 0, s, a, L);
+  else
+    hipLaunchKernelGGL(k_vfb_seed<uint64_t>, dim3(1), dim3(64), 0, s, a, L);
+  return hipGetLastError();
+}
+
+hipError_t vfb_launch_lookup_target(int width, const VfbLevelArgs &a, uint64_t *out,
+                                    hipStream_t s) {
+  if (width == 32)
+    hipLaunchKernelGGL(k_vfb_lookup_target<uint32_t>, dim3(1), dim3(64), 0, s, a, out);
+  else
+    hipLaunchKernelGGL(k_vfb_lookup_target<uint64_t>, dim3(1), dim3(64), 0, s, a, out);
+  return hipGetLastError();
+}
+
+hipError_t vfb_launch_rpn(const int32_t *left, const int32_t *right, const uint8_t *op,
+                          int L, int32_t root_op, int32_t root_l, int32_t root_r,
+                          uint8_t *out, int32_t *out_len, int cap, hipStream_t s) {
+  hipLaunchKernelGGL(k_vfb_rpn, dim3(1), dim3(64), 0, s, left, right, op, L, root_op,
+                     root_l, root_r, out, out_len, cap);
+  return hipGetLastError();
+}
+
+hipError_t vfb_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  uint64_t blocks = (n + 255) / 256;
+  const int grid = (int)(blocks < 16384 ? blocks : 16384);
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+}  // namespace mba

@@ -1,0 +1,69 @@
+"""GPU parity of the VFB bottom-up cached search against the CPU oracle:
+same level, same enumeration index of the solution, same expression, and the
+same number of kept rows per level (exhaustive mode)."""
+import pytest
+
+from oracle import oracle as orc
+from tests.gpu_util import note
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import simba_amd
+    c = simba_amd.Context(0)
+    yield c
+    c.close()
+
+
+def _oracle(inst, max_len, exhaustive=False):
+    return orc.vfb_search(inst.inputs, inst.outputs, inst.n_variables,
+                          [int(c) for c in inst.consts], inst.width, max_len,
+                          exhaustive=exhaustive)
+
+
+CASES = [(seed, 64 if seed % 2 else 32, 1 + seed % 3, [(), (1,), (1, -1)][seed % 3],
+          2 + seed % 6) for seed in range(24)]
+
+
+@pytest.mark.parametrize("seed,W,nv,consts,tsize", CASES)
+def test_vfb_matches_oracle(ctx, seed, W, nv, consts, tsize):
+    from simba_amd.synth import make_instance
+    consts = [c & ((1 << W) - 1) for c in consts]
+    inst = make_instance(3000 + seed, nv, 8 + seed % 7, tsize, W, consts)
+    note(f"vfb parity seed={seed} W={W} nv={nv} consts={consts} tsize={tsize}")
+    want = _oracle(inst, 8)
+    got = ctx.vfb(inst, max_size=8)
+    assert got["status"] == want["status"]
+    if want["status"] == 0:
+        assert (got["size"], got["rank"], got["expr"]) == (want["size"], want["rank"], want["expr"])
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_vfb_level_counts_exhaustive(ctx, seed):
+    from simba_amd.synth import make_instance
+    W = 32 if seed % 2 == 0 else 64
+    inst = make_instance(4000 + seed, 2 + seed % 2, 16, 9, W)
+    note(f"vfb exhaustive counts seed={seed}")
+    want = _oracle(inst, 7, exhaustive=True)
+    got = ctx.vfb(inst, max_size=7, exhaustive=True)
+    assert got["level_new"][1:7] == want["level_new"][1:7]
+    assert got["level_cand"][1:8] == want["level_cand"][1:8]
+
+
+def test_vfb_leaf_target(ctx):
+    from simba_amd.synth import make_instance
+    inst = make_instance(9, 3, 16, 1, 32)
+    note("vfb leaf")
+    got = ctx.vfb(inst, max_size=5)
+    assert got["size"] == 1 and got["expr"] == "v%d" % inst.target_rpn[0]
+
+
+def test_vfb_capacity_overflow_is_reported(ctx):
+    import simba_amd
+    from simba_amd.synth import make_instance
+    inst = make_instance(10, 3, 16, 12, 64)
+    note("vfb overflow")
+    got = ctx.vfb(inst, max_size=9, cache_bytes=200 * (16 * 8 + 9))
+    assert got["status"] == simba_amd.OVERFLOW
