@@ -1,11 +1,366 @@
-// batch_host.cpp -- batched SIMBA driver (placeholder until the persistent
-// kernel lands).
+// batch_host.cpp -- batched SIMBA search: many instances of uneven shape
+// (variables, samples, target size) solved together.
+//
+// Every size step of every active instance becomes a list of jobs (value
+// table builds, fused pair checks, leaf scans) in a device work queue; one
+// persistent kernel per step drains it (workgroups pull tiles from an atomic
+// counter), so small and large instances share the GPU.  The per-step launch
+// sequence -- job upload, counter reset, persistent kernel, hit-count read
+// back -- is captured once into a hipGraph and replayed per step.  Each
+// instance's answer is the one mba_simba_solve returns on it: the same
+// tables, windows, hit resolution and verification, per instance.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <vector>
+
 #include "context.h"
+#include "simba_host.h"
 
 namespace mba {
-int batch_solve(Ctx &ctx, const mba_instance *, size_t, const mba_opts &,
-                mba_result *) {
-  ctx.last_error = "batched search not built yet";
-  return MBA_ERR_ARG;
+
+namespace {
+
+constexpr uint32_t kBatchHitCap = 1u << 22;
+constexpr uint64_t kBatchTableBytes = 32ull << 30;
+
+struct InstState {
+  int L = 0, n = 0, fs = 0, Kt = 1;
+  uint64_t t0 = 0;
+  const CountTable *T = nullptr;
+  std::vector<uint64_t> found;
+  bool active = true;
+  int reached = 0;
+};
+
+}  // namespace
+
+int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opts &o,
+                mba_result *results) {
+  const auto t_start = Clock::now();
+  if (count == 0) return 0;
+  const int W = (int)insts[0].width;
+  int min_size = o.min_size < 1 ? 1 : o.min_size;
+  int max_size = o.max_size;
+  if (o.compat) {
+    min_size = 2;
+    max_size = o.max_size - 1;
+  }
+  if (max_size > kMaxSize || (W != 32 && W != 64)) {
+    ctx.last_error = "batch: unsupported width or size";
+    return MBA_ERR_LIMIT;
+  }
+  const uint64_t M = wmask(W);
+  const size_t wb = (size_t)W / 8;
+  std::map<int, CountTable> tables_by_L;
+  std::vector<InstState> S(count);
+  std::vector<TableView> tvs(count);
+  std::vector<BatchInstData> idat(count);
+  std::vector<uint64_t> leafvals, outputs;
+  const uint64_t per_budget =
+      std::max<uint64_t>((o.table_bytes ? o.table_bytes : kBatchTableBytes) / count, 1 << 16);
+  uint64_t pool_elems = 0;
+  for (size_t i = 0; i < count; ++i) {
+    const mba_instance &in = insts[i];
+    const int nv = (int)in.n_vars, nc = (int)in.n_consts, n = (int)in.n_samples;
+    const int L = nv + nc;
+    mba_result &r = results[i];
+    std::memset(&r, 0, sizeof r);
+    r.status = MBA_NOT_FOUND;
+    r.op = -1;
+    std::snprintf(r.expr, sizeof r.expr, "Not found !");
+    if ((int)in.width != W || L < 1 || L > kMaxLeaves || n < 1 || n > kMaxSamples ||
+        !in.inputs || !in.outputs || (nc > 0 && !in.consts)) {
+      ctx.last_error = "batch: instance outside supported limits or mixed widths";
+      return MBA_ERR_LIMIT;
+    }
+    auto it = tables_by_L.find(L);
+    if (it == tables_by_L.end()) {
+      CountTable T;
+      if (!T.build(L, std::max(max_size, 1))) {
+        ctx.last_error = "count table overflows uint64 at this max size";
+        return MBA_ERR_COUNT;
+      }
+      it = tables_by_L.emplace(L, T).first;
+    }
+    InstState &st = S[i];
+    st.L = L;
+    st.n = n;
+    st.T = &it->second;
+    st.found.assign(kMaxSize + 2, ~0ull);
+    std::vector<uint64_t> lv((size_t)n * L), outv(n);
+    for (int s = 0; s < n; ++s) {
+      for (int v = 0; v < nv; ++v) lv[(size_t)s * L + v] = in.inputs[(size_t)s * nv + v] & M;
+      for (int c = 0; c < nc; ++c) lv[(size_t)s * L + nv + c] = in.consts[c] & M;
+      outv[s] = in.outputs[s] & M;
+    }
+    st.fs = pick_filter_sample(lv, outv, n, L, W);
+    st.t0 = outv[st.fs];
+    st.Kt = choose_kt(*st.T, std::max(max_size, 1), W, per_budget);
+    TableView &tv = tvs[i];
+    std::memset(&tv, 0, sizeof tv);
+    uint64_t e = 0;
+    for (int m = 1; m <= st.Kt; ++m) {
+      tv.off[m] = e;
+      e += st.T->t[m][8];
+    }
+    std::memcpy(tv.T, st.T->t, sizeof tv.T);
+    tv.Kt = st.Kt;
+    tv.L = L;
+    tv.base = (const void *)(uintptr_t)(pool_elems * wb);  // patched to a pointer below
+    pool_elems += e;
+    idat[i].leaf_off = leafvals.size();
+    idat[i].out_off = outputs.size();
+    idat[i].n_samples = (uint32_t)n;
+    idat[i].pad = (uint32_t)st.fs;
+    leafvals.insert(leafvals.end(), lv.begin(), lv.end());
+    outputs.insert(outputs.end(), outv.begin(), outv.end());
+    if (max_size < min_size) st.active = false;
+  }
+
+  hipStream_t stq = ctx.stream;
+  MBA_HIP(ctx.b_data.ensure(std::max<size_t>(pool_elems * wb, 256)));
+  for (size_t i = 0; i < count; ++i)
+    tvs[i].base = ctx.b_data.as<char>() + (uintptr_t)tvs[i].base;
+  MBA_HIP(ctx.b_inst.ensure(sizeof(TableView) * count));
+  MBA_HIP(ctx.leafvals.ensure(8 * leafvals.size()));
+  MBA_HIP(ctx.outputs.ensure(8 * outputs.size()));
+  MBA_HIP(ctx.b_results.ensure(sizeof(BatchInstData) * count));
+  MBA_HIP(hipMemcpyAsync(ctx.b_inst.p, tvs.data(), sizeof(TableView) * count,
+                         hipMemcpyHostToDevice, stq));
+  MBA_HIP(hipMemcpyAsync(ctx.leafvals.p, leafvals.data(), 8 * leafvals.size(),
+                         hipMemcpyHostToDevice, stq));
+  MBA_HIP(hipMemcpyAsync(ctx.outputs.p, outputs.data(), 8 * outputs.size(),
+                         hipMemcpyHostToDevice, stq));
+  MBA_HIP(hipMemcpyAsync(ctx.b_results.p, idat.data(), sizeof(BatchInstData) * count,
+                         hipMemcpyHostToDevice, stq));
+  {  // V[1] of every instance: leaf values of its filter sample
+    std::vector<uint64_t> v1;
+    std::vector<uint32_t> v1w;
+    for (size_t i = 0; i < count; ++i) {
+      const uint64_t *src = leafvals.data() + idat[i].leaf_off + (size_t)S[i].fs * S[i].L;
+      for (int l = 0; l < S[i].L; ++l) {
+        v1.push_back(src[l]);
+        v1w.push_back((uint32_t)src[l]);
+      }
+    }
+    // scatter with one copy per instance from a single staging buffer
+    MBA_HIP(ctx.b_misc.ensure(8 * v1.size() + 64));
+    MBA_HIP(hipMemcpyAsync(ctx.b_misc.p, W == 64 ? (void *)v1.data() : (void *)v1w.data(),
+                           wb * v1.size(), hipMemcpyHostToDevice, stq));
+    size_t pos = 0;
+    for (size_t i = 0; i < count; ++i) {
+      MBA_HIP(hipMemcpyAsync((void *)tvs[i].base, ctx.b_misc.as<char>() + pos * wb,
+                             wb * S[i].L, hipMemcpyDeviceToDevice, stq));
+      pos += S[i].L;
+    }
+  }
+
+  // ---- per-step graph: upload jobs, reset counters, drain, read hit count --
+  size_t job_cap = 0;
+  for (size_t i = 0; i < count; ++i) job_cap += 2 + (size_t)std::max(0, (max_size - 1) / 2);
+  BatchJob *h_jobs = nullptr;
+  BatchStep *h_step = nullptr;
+  uint32_t *h_hc = nullptr;
+  MBA_HIP(hipHostMalloc((void **)&h_jobs, sizeof(BatchJob) * job_cap, hipHostMallocDefault));
+  MBA_HIP(hipHostMalloc((void **)&h_step, sizeof(BatchStep) + 64, hipHostMallocDefault));
+  MBA_HIP(hipHostMalloc((void **)&h_hc, 64, hipHostMallocDefault));
+  struct PinnedGuard {
+    void *a, *b, *c;
+    ~PinnedGuard() {
+      (void)hipHostFree(a);
+      (void)hipHostFree(b);
+      (void)hipHostFree(c);
+    }
+  } guard{h_jobs, h_step, h_hc};
+  MBA_HIP(ctx.b_queue.ensure(sizeof(BatchJob) * job_cap + 256));
+  MBA_HIP(ctx.hits.ensure(sizeof(HitRec) * kBatchHitCap));
+  MBA_HIP(ctx.hit_count.ensure(256));
+  BatchJob *d_jobs = (BatchJob *)(ctx.b_queue.as<char>() + 256);
+  BatchStep *d_step = ctx.b_queue.as<BatchStep>();
+  unsigned long long *d_head = (unsigned long long *)(ctx.hit_count.as<char>() + 64);
+  uint32_t *d_hc = ctx.hit_count.as<uint32_t>();
+  HitRec *d_hits = ctx.hits.as<HitRec>();
+  const TableView *d_tvs = ctx.b_inst.as<TableView>();
+  const int mult = o.reserved[0] > 0 ? o.reserved[0] : 8;  // workgroups per CU
+  const int grid = ctx.cu_count * mult;
+
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  MBA_HIP(hipStreamSynchronize(stq));
+  MBA_HIP(hipStreamBeginCapture(stq, hipStreamCaptureModeThreadLocal));
+  MBA_HIP(hipMemcpyAsync(d_step, h_step, sizeof(BatchStep), hipMemcpyHostToDevice, stq));
+  MBA_HIP(hipMemcpyAsync(d_jobs, h_jobs, sizeof(BatchJob) * job_cap, hipMemcpyHostToDevice,
+                         stq));
+  MBA_HIP(launch_batch_reset(d_head, d_hc, stq));
+  MBA_HIP(launch_batch_step(W, d_step, d_jobs, d_tvs, d_head, d_hits, d_hc, kBatchHitCap,
+                            grid, stq));
+  MBA_HIP(hipMemcpyAsync(h_hc, d_hc, 4, hipMemcpyDeviceToHost, stq));
+  MBA_HIP(hipStreamEndCapture(stq, &graph));
+  MBA_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+  struct GraphGuard {
+    hipGraph_t g;
+    hipGraphExec_t e;
+    ~GraphGuard() {
+      if (e) (void)hipGraphExecDestroy(e);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  } gguard{graph, gexec};
+
+  std::vector<HitRec> hrec;
+  std::vector<Candidate> cands;
+  std::vector<VerifyCand> vc;
+  double ktime = 0.0;
+  bool budget_hit = false;
+  for (int s = 0; s <= max_size; ++s) {
+    if ((o.cancel && *o.cancel) ||
+        (o.time_budget_s > 0 && seconds_since(t_start) > o.time_budget_s)) {
+      budget_hit = true;
+      break;
+    }
+    size_t nj = 0;
+    uint64_t tiles = 0;
+    for (size_t i = 0; i < count; ++i) {
+      InstState &st = S[i];
+      if (!st.active) continue;
+      auto push = [&](const BatchJob &J) {
+        h_jobs[nj] = J;
+        h_jobs[nj].tile0 = tiles;
+        tiles += J.ntiles;
+        ++nj;
+      };
+      const CountTable &T = *st.T;
+      if (s == 1) {  // leaves and every unary chain over them
+        BatchJob J;
+        std::memset(&J, 0, sizeof J);
+        const CheckTargets tg = targets_for(st.t0, 1, max_size, W);
+        J.ntiles = 1;
+        J.lo_e = tg.lo_even;
+        J.lo_o = tg.lo_odd;
+        J.span = tg.span;
+        J.inst = (uint32_t)i;
+        J.kind = 2;
+        push(J);
+      }
+      if (s >= 2 && s <= st.Kt) {  // value table V[s]
+        BatchJob J;
+        std::memset(&J, 0, sizeof J);
+        J.p.m = s;
+        J.p.nB = T.t[s][8];
+        J.ntiles = (J.p.nB + kBuildTile - 1) / kBuildTile;
+        J.inst = (uint32_t)i;
+        J.kind = 0;
+        push(J);
+      }
+      if (s >= 3) {
+        const CheckTargets tg = targets_for(st.t0, s, max_size, W);
+        for (int j = 1; j <= (s - 1) / 2; ++j) {
+          BatchJob J;
+          std::memset(&J, 0, sizeof J);
+          J.p.sb = s;
+          J.p.j = j;
+          J.p.m = s - 1 - j;
+          J.p.nA = T.t[j][8];
+          J.p.nB = T.t[J.p.m][8];
+          J.p.nTA = (J.p.nA + kPairTA - 1) / kPairTA;
+          J.p.gen_a = j > st.Kt;
+          J.p.gen_b = J.p.m > st.Kt;
+          J.p.mask = kAllInst;
+          J.ntiles = J.p.nTA * ((J.p.nB + kPairTB - 1) / kPairTB);
+          J.lo_e = tg.lo_even;
+          J.lo_o = tg.lo_odd;
+          J.span = tg.span;
+          J.inst = (uint32_t)i;
+          J.kind = 1;
+          push(J);
+        }
+      }
+    }
+    if (nj == 0) continue;
+    h_step->total_tiles = tiles;
+    h_step->njobs = (uint32_t)nj;
+    MBA_HIP(hipEventRecord(ctx.ev0, stq));
+    MBA_HIP(hipGraphLaunch(gexec, stq));
+    MBA_HIP(hipEventRecord(ctx.ev1, stq));
+    MBA_HIP(hipEventSynchronize(ctx.ev1));
+    float ms = 0.f;
+    MBA_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
+    ktime += ms * 1e-3;
+    const uint32_t nh = *h_hc;
+    if (nh > kBatchHitCap) {
+      ctx.last_error = "batch: sample hit buffer overflow";
+      return MBA_ERR_LIMIT;
+    }
+    if (nh > 0) {
+      hrec.resize(nh);
+      MBA_HIP(hipMemcpy(hrec.data(), d_hits, sizeof(HitRec) * nh, hipMemcpyDeviceToHost));
+      vc.clear();
+      for (const HitRec &h0 : hrec) {
+        HitRec h = h0;
+        const uint32_t i = h.inst >> 8;
+        h.inst &= 0xff;
+        const InstState &st = S[i];
+        const int sb = (int)(h.pair >> 16), j = (int)(h.pair & 0xffff);
+        const uint64_t rX = base_rank(*st.T, sb, j, sb - 1 - j, h);
+        cands.clear();
+        resolve_hit(*st.T, W, st.t0, sb, rX, h.v & M, min_size, max_size, cands);
+        for (const Candidate &c : cands) vc.push_back({i, (uint32_t)c.size, c.rank});
+      }
+      if (!vc.empty()) {
+        const uint32_t ncand = (uint32_t)vc.size();
+        MBA_HIP(ctx.vranks.ensure(sizeof(VerifyCand) * ncand));
+        MBA_HIP(ctx.vok.ensure(ncand));
+        MBA_HIP(hipMemcpyAsync(ctx.vranks.p, vc.data(), sizeof(VerifyCand) * ncand,
+                               hipMemcpyHostToDevice, stq));
+        MBA_HIP(launch_verify_batch(W, d_tvs, ctx.leafvals.as<uint64_t>(),
+                                    ctx.outputs.as<uint64_t>(),
+                                    ctx.b_results.as<BatchInstData>(),
+                                    ctx.vranks.as<VerifyCand>(), ncand, ctx.vok.as<uint8_t>(),
+                                    stq));
+        std::vector<uint8_t> okv(ncand);
+        MBA_HIP(hipMemcpyAsync(okv.data(), ctx.vok.p, ncand, hipMemcpyDeviceToHost, stq));
+        MBA_HIP(hipStreamSynchronize(stq));
+        for (uint32_t c = 0; c < ncand; ++c)
+          if (okv[c]) {
+            auto &f = S[vc[c].inst].found[vc[c].size];
+            f = std::min(f, vc[c].rank);
+          }
+      }
+    }
+    // sizes completed by this step: 1..2 after the leaf scan, s for s >= 3
+    const int lo = s == 1 ? 1 : s, hi = s == 1 ? std::min(2, max_size) : s;
+    if (s == 0 || s == 2) continue;
+    for (size_t i = 0; i < count; ++i) {
+      InstState &st = S[i];
+      if (!st.active) continue;
+      for (int z = lo; z <= hi; ++z) {
+        st.reached = z;
+        if (z >= min_size && st.found[z] != ~0ull) {
+          const int nv = (int)insts[i].n_vars;
+          fill_result(*st.T, z, st.found[z], nv, insts[i].consts, W, results[i]);
+          results[i].seconds = seconds_since(t_start);
+          st.active = false;
+          break;
+        }
+      }
+    }
+  }
+  const double total = seconds_since(t_start);
+  for (size_t i = 0; i < count; ++i) {
+    InstState &st = S[i];
+    mba_result &r = results[i];
+    uint64_t cand = 0;
+    for (int z = min_size; z <= st.reached; ++z) cand += st.T->t[z][8];
+    r.candidates = cand;
+    r.size_reached = st.reached;
+    r.kernel_seconds = ktime;
+    if (st.active) {
+      r.status = budget_hit ? MBA_BUDGET : MBA_NOT_FOUND;
+      r.seconds = total;
+    }
+  }
+  return 0;
 }
+
 }  // namespace mba

@@ -65,8 +65,11 @@ inline double seconds_since(Clock::time_point t0) {
     }                                                                     \
   } while (0)
 
+// block_scores: NULL for the plain (rank-order, minimum-rank) search, else
+// per (op, left size) scores at [op * 32 + split] that order the binary
+// blocks of each size (learned proposal; any solution of minimal size).
 int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &opts,
-                mba_result &res);
+                mba_result &res, const float *block_scores = nullptr);
 int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &opts,
               mba_result &res, mba_vfb_stats *stats);
 int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count,

@@ -13,27 +13,18 @@
 #include <vector>
 
 #include "context.h"
-#include "simba_kernels.h"
+#include "simba_host.h"
 
 namespace mba {
-
-namespace {
 
 constexpr uint32_t kHitCap = 1u << 20;
 constexpr uint64_t kDefaultTableBytes = 40ull << 30;
 // Work per launch, in A x B value pairs, so that budgets are honoured.
 constexpr uint64_t kPairsPerLaunch = 1ull << 36;
 
-inline uint64_t wmask(int W) { return W == 64 ? ~0ull : 0xffffffffull; }
-
-inline int64_t as_signed(uint64_t x, int W) {
+static inline int64_t as_signed(uint64_t x, int W) {
   return W == 64 ? (int64_t)x : (int64_t)(int32_t)(uint32_t)x;
 }
-
-struct Candidate {
-  int size;
-  uint64_t rank;
-};
 
 // Rank at size sb of the binary-rooted tree X = inst(A[a], B[b]) of the
 // split {j, m} (see HitRec), or of leaf `a` when inst == 7.
@@ -132,9 +123,76 @@ int choose_kt(const CountTable &T, int max_size, int W, uint64_t budget) {
   return kt;
 }
 
-}  // namespace
+std::vector<BlockRef> binary_blocks(const CountTable &T, int s) {
+  static const int kOpInst[8] = {-1, 0, 1, 2, -1, 3, -1, 4};
+  std::vector<BlockRef> out;
+  for (int op = 1; op < 8; ++op) {
+    if (op == NEG) continue;
+    const int last = op == SUB ? s - 2 : (s - 1) / 2;
+    uint64_t r = T.op_offset(s, op);
+    for (int j = 1; j <= last; ++j) {
+      const int m = s - 1 - j;
+      BlockRef b;
+      b.op = op;
+      b.split = j;
+      b.begin = r;
+      r += T.t[j][8] * T.t[m][8];
+      b.end = r;
+      if (op != SUB) {
+        b.pair_j = j;
+        b.inst = kOpInst[op];
+      } else if (j <= m) {
+        b.pair_j = j;
+        b.inst = 5;  // A - B with A of size j
+      } else {
+        b.pair_j = m;
+        b.inst = 6;  // B - A with B of size j
+      }
+      out.push_back(b);
+    }
+  }
+  return out;
+}
 
-int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result &res) {
+CheckTargets targets_for(uint64_t t0, int sb, int max_size, int W) {
+  const uint64_t M = wmask(W);
+  const uint64_t U = (uint64_t)(max_size - sb);
+  CheckTargets tg;
+  tg.lo_even = (t0 - U) & M;
+  tg.lo_odd = ((0 - t0) - U) & M;
+  tg.span = 2 * U;
+  return tg;
+}
+
+int root_split(const uint8_t *rpn, int size) {
+  if (size <= 1 || is_unary(rpn[size - 1] - kTokOp)) return 0;
+  // walk the right subtree back from the root
+  int need = 1, i = size - 2;
+  while (need > 0) {
+    const uint8_t t = rpn[i--];
+    if (t < kTokOp) need -= 1;
+    else if (!is_unary(t - kTokOp)) need += 1;
+  }
+  return i + 1;
+}
+
+void fill_result(const CountTable &T, int size, uint64_t rank, int n_vars,
+                 const uint64_t *consts, int W, mba_result &res) {
+  uint8_t rpn[kMaxSize];
+  unrank_host(T, size, rank, rpn);
+  res.status = MBA_FOUND;
+  res.size = size;
+  res.rank = rank;
+  res.op = size > 1 ? rpn[size - 1] - kTokOp : -1;
+  res.split = root_split(rpn, size);
+  const std::string e = infix_simba(rpn, size, n_vars, consts, W);
+  std::snprintf(res.expr, sizeof res.expr, "%s", e.c_str());
+  const std::string r = rpn_chars(rpn, size);
+  std::snprintf(res.rpn, sizeof res.rpn, "%s", r.c_str());
+}
+
+int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result &res,
+                const float *block_scores) {
   const auto t_start = Clock::now();
   std::memset(&res, 0, sizeof res);
   res.status = MBA_NOT_FOUND;
@@ -235,15 +293,6 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   int reached = 0;
   int status = MBA_NOT_FOUND;
 
-  auto targets_for = [&](int sb) {
-    const uint64_t U = (uint64_t)(max_size - sb);
-    CheckTargets tg;
-    tg.lo_even = (t0 - U) & M;
-    tg.lo_odd = ((0 - t0) - U) & M;
-    tg.span = 2 * U;
-    return tg;
-  };
-
   // Resolve the hits of one step, verify the candidates on all samples.
   auto process_hits = [&](int sb_step) -> int {
     uint32_t cnt = 0;
@@ -293,7 +342,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   // Step 1: leaves (size 1) and, through unary chains, every unary tree over a
   // leaf (sizes 2..max).
   MBA_HIP(hipMemsetAsync(d_hc, 0, 4, st));
-  MBA_HIP(launch_leaf_scan(W, d_tv, L, targets_for(1), d_hits, d_hc, kHitCap, st));
+  MBA_HIP(launch_leaf_scan(W, d_tv, L, targets_for(t0, 1, max_size, W), d_hits, d_hc, kHitCap, st));
   if (int rc = process_hits(1)) return rc;
   for (int s = 1; s <= std::min(2, max_size); ++s) {
     reached = s;
@@ -303,53 +352,107 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     }
   }
 
+  // Runs the pair {j, m} (j <= m) of size s for the operator instances in
+  // `mask`, in launches of bounded work.  Returns 1 when the budget stopped it.
+  auto run_pair = [&](int s, int j, uint32_t mask, const CheckTargets &tg) -> int {
+    const int m = s - 1 - j;
+    PairLaunch p;
+    p.sb = s;
+    p.j = j;
+    p.m = m;
+    p.nA = T.t[j][8];
+    p.nB = T.t[m][8];
+    p.nTA = (p.nA + kPairTA - 1) / kPairTA;
+    const uint64_t nTB = (p.nB + kPairTB - 1) / kPairTB;
+    const uint64_t tiles = p.nTA * nTB;
+    p.gen_a = j > Kt;
+    p.gen_b = m > Kt;
+    p.mask = mask;
+    const uint64_t per_tile = (uint64_t)kPairTA * kPairTB;
+    const uint64_t tiles_per_launch = std::max<uint64_t>(1, kPairsPerLaunch / per_tile);
+    for (uint64_t tb = 0; tb < tiles; tb += tiles_per_launch) {
+      p.tile_begin = tb;
+      p.tile_end = std::min(tiles, tb + tiles_per_launch);
+      const uint64_t ntl = p.tile_end - p.tile_begin;
+      const int g = (int)std::min<uint64_t>((uint64_t)grid, ntl);
+      hipError_t e = hipEventRecord(ctx.ev0, st);
+      if (e == hipSuccess)
+        e = launch_pair_check(W, d_tv, ctx.tables.p, tv.off, p, tg, d_hits, d_hc, kHitCap, g, st);
+      if (e == hipSuccess) e = hipEventRecord(ctx.ev1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(ctx.ev1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1);
+      if (e != hipSuccess) {
+        ctx.last_error = std::string("pair launch: ") + hipGetErrorString(e);
+        return -1;
+      }
+      ktime += ms * 1e-3;
+      if (over_budget()) return 1;
+    }
+    return 0;
+  };
+
   for (int s = 3; (answer < 0 || o.exhaustive) && s <= max_size; ++s) {
     if (over_budget()) {
       status = MBA_BUDGET;
       break;
     }
     MBA_HIP(hipMemsetAsync(d_hc, 0, 4, st));
-    const CheckTargets tg = targets_for(s);
+    const CheckTargets tg = targets_for(t0, s, max_size, W);
     bool stopped = false;
-    for (int j = 1; j <= (s - 1) / 2 && !stopped; ++j) {
-      const int m = s - 1 - j;
-      PairLaunch p;
-      p.sb = s;
-      p.j = j;
-      p.m = m;
-      p.nA = T.t[j][8];
-      p.nB = T.t[m][8];
-      p.nTA = (p.nA + kPairTA - 1) / kPairTA;
-      const uint64_t nTB = (p.nB + kPairTB - 1) / kPairTB;
-      const uint64_t tiles = p.nTA * nTB;
-      p.gen_a = j > Kt;
-      p.gen_b = m > Kt;
-      const uint64_t per_tile = (uint64_t)kPairTA * kPairTB;
-      const uint64_t tiles_per_launch = std::max<uint64_t>(1, kPairsPerLaunch / per_tile);
-      for (uint64_t tb = 0; tb < tiles; tb += tiles_per_launch) {
-        p.tile_begin = tb;
-        p.tile_end = std::min(tiles, tb + tiles_per_launch);
-        const uint64_t ntl = p.tile_end - p.tile_begin;
-        const int g = (int)std::min<uint64_t>((uint64_t)grid, ntl);
-        MBA_HIP(hipEventRecord(ctx.ev0, st));
-        MBA_HIP(launch_pair_check(W, d_tv, ctx.tables.p, tv.off, p, tg, d_hits, d_hc,
-                                  kHitCap, g, st));
-        MBA_HIP(hipEventRecord(ctx.ev1, st));
-        MBA_HIP(hipEventSynchronize(ctx.ev1));
-        float ms = 0.f;
-        MBA_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
-        ktime += ms * 1e-3;
-        if (over_budget()) {
+    const bool by_block = !o.exhaustive && (block_scores || T.t[s][8] >= kBlockModeMin);
+    if (s >= min_size && found[s] != ~0ull &&
+        (block_scores || found[s] < T.t[s][NOT])) {
+      // a unary-rooted solution of this size is already verified: with the
+      // learned order any solution of the minimal size is accepted; in rank
+      // order one in the '~' block precedes every binary-rooted tree
+      answer = s;
+      reached = s;
+      break;
+    }
+    if (!by_block) {
+      // every split of this size, all seven operator instances fused
+      for (int j = 1; j <= (s - 1) / 2 && !stopped; ++j) {
+        const int rc = run_pair(s, j, kAllInst, tg);
+        if (rc < 0) return MBA_ERR_HIP;
+        stopped = rc == 1;
+      }
+      if (!stopped) {
+        if (int rc = process_hits(s)) return rc;
+      }
+    } else {
+      // one (op, split) block at a time, in rank order (plain) or in the
+      // order of the learned scores; stop once the answer is settled
+      std::vector<BlockRef> blocks = binary_blocks(T, s);
+      if (block_scores) {
+        std::stable_sort(blocks.begin(), blocks.end(), [&](const BlockRef &x, const BlockRef &y) {
+          return block_scores[x.op * 32 + x.split] > block_scores[y.op * 32 + y.split];
+        });
+      }
+      std::vector<char> done(blocks.size(), 0);
+      for (size_t bi = 0; bi < blocks.size() && !stopped; ++bi) {
+        const BlockRef &b = blocks[bi];
+        const int rc = run_pair(s, b.pair_j, 1u << b.inst, tg);
+        if (rc < 0) return MBA_ERR_HIP;
+        if (rc == 1) {
           stopped = true;
           break;
         }
+        if (int rc2 = process_hits(s)) return rc2;
+        MBA_HIP(hipMemsetAsync(d_hc, 0, 4, st));
+        done[bi] = 1;
+        if (found[s] == ~0ull || s < min_size) continue;
+        if (block_scores) break;  // any verified tree of the minimal size
+        // rank order: settled when every block below the best rank is done;
+        // the neg block after '^' is complete already (unary chains)
+        const uint64_t prefix = b.end == T.op_offset(s, NEG) ? T.op_offset(s, ADD) : b.end;
+        if (found[s] < prefix) break;
       }
     }
     if (stopped) {
       status = MBA_BUDGET;
       break;
     }
-    if (int rc = process_hits(s)) return rc;
     reached = s;
     if (o.verbose)
       std::fprintf(stderr, "Length %-2d | candidates %-20llu | %.3f s\n", s,
@@ -363,27 +466,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   res.size_reached = reached;
   res.kernel_seconds = ktime;
   if (answer > 0) {
-    uint8_t rpn[kMaxSize];
-    unrank_host(T, answer, found[answer], rpn);
-    res.status = MBA_FOUND;
-    res.size = answer;
-    res.rank = found[answer];
-    res.op = answer > 1 ? rpn[answer - 1] - kTokOp : -1;
-    res.split = 0;
-    if (answer > 1 && !is_unary(res.op)) {
-      // left child size: walk the right subtree back from the root
-      int need = 1, i = answer - 2;
-      while (need > 0) {
-        const uint8_t t = rpn[i--];
-        if (t < kTokOp) need -= 1;
-        else if (!is_unary(t - kTokOp)) need += 1;
-      }
-      res.split = i + 1;
-    }
-    const std::string e = infix_simba(rpn, answer, nv, inst.consts, W);
-    std::snprintf(res.expr, sizeof res.expr, "%s", e.c_str());
-    const std::string r = rpn_chars(rpn, answer);
-    std::snprintf(res.rpn, sizeof res.rpn, "%s", r.c_str());
+    fill_result(T, answer, found[answer], nv, inst.consts, W, res);
     status = MBA_FOUND;
   } else {
     std::snprintf(res.expr, sizeof res.expr, "Not found !");

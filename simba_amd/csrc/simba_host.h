@@ -1,0 +1,47 @@
+// simba_host.h -- host-side pieces of the SIMBA search shared by the
+// per-instance and batched drivers.
+#pragma once
+#include <vector>
+
+#include "simba_kernels.h"
+
+namespace mba {
+
+inline uint64_t wmask(int W) { return W == 64 ? ~0ull : 0xffffffffull; }
+
+struct Candidate {
+  int size;
+  uint64_t rank;
+};
+
+// Rank, at size sb, of the binary-rooted tree recorded by a pair hit (or of
+// the leaf of a leaf hit).
+uint64_t base_rank(const CountTable &T, int sb, int j, int m, const HitRec &h);
+// Candidate trees (chains of ~/neg over the hit tree) that can meet the
+// target; appended to `out`.
+void resolve_hit(const CountTable &T, int W, uint64_t t0, int sb, uint64_t rX, uint64_t v,
+                 int min_size, int max_size, std::vector<Candidate> &out);
+int pick_filter_sample(const std::vector<uint64_t> &lv, const std::vector<uint64_t> &outv,
+                       int n, int L, int W);
+int choose_kt(const CountTable &T, int max_size, int W, uint64_t budget);
+// A binary (op, left size) block of size s and the pair launch covering it.
+struct BlockRef {
+  int op, split;        // root operator, left child size
+  uint64_t begin, end;  // rank range within the size
+  int pair_j;           // the pair {pair_j, s-1-pair_j}
+  int inst;             // operator instance within that pair
+};
+// Binary blocks of size s in rank order (nocache.cu:52-79).
+std::vector<BlockRef> binary_blocks(const CountTable &T, int s);
+// Below this many trees a size is swept with fused pairs and one sync.
+constexpr uint64_t kBlockModeMin = 1ull << 31;
+
+// Windows for the checks of base size sb (chains up to max_size).
+CheckTargets targets_for(uint64_t t0, int sb, int max_size, int W);
+// Left child size of the root of an RPN (0 for unary roots and leaves).
+int root_split(const uint8_t *rpn, int size);
+// Fills the tree fields of a result from (size, rank).
+void fill_result(const CountTable &T, int size, uint64_t rank, int n_vars,
+                 const uint64_t *consts, int W, mba_result &res);
+
+}  // namespace mba

@@ -38,12 +38,59 @@ struct PairLaunch {
   uint64_t tile_begin, tile_end;  // tile range of this launch
   uint64_t nTA;            // number of A tiles
   bool gen_a, gen_b;       // side is above Kt: values computed per lane
+  uint32_t mask;           // operator instances to check (bit i = inst i)
 };
+
+constexpr uint32_t kAllInst = 0x7f;
 
 constexpr int kPairThreads = 256;
 constexpr int kPairRB = 4;                      // B values held per lane
 constexpr int kPairTB = kPairThreads * kPairRB; // B values per tile
 constexpr int kPairTA = 256;                    // A values per tile
+
+// ---- batched search --------------------------------------------------------
+constexpr int kBuildTile = 1024;  // table values per build tile
+
+// One unit of a batch step: kind 0 builds V[p.m] of instance `inst` (p.nB
+// values), kind 1 checks the pair {p.j, p.m} at size p.sb, kind 2 scans the
+// leaves.  Jobs own the global tile range [tile0, tile0 + ntiles).
+struct BatchJob {
+  uint64_t tile0, ntiles;
+  uint64_t lo_e, lo_o, span;  // windows (kinds 1 and 2)
+  PairLaunch p;
+  uint32_t inst;
+  uint32_t kind;
+};
+
+// Rewritten by the host before each (graph) replay of a step.
+struct BatchStep {
+  uint64_t total_tiles;
+  uint32_t njobs;
+  uint32_t pad;
+};
+
+struct BatchInstData {
+  uint64_t leaf_off;   // [sample][leaf] values of this instance in the batch array
+  uint64_t out_off;    // outputs of this instance
+  uint32_t n_samples;
+  uint32_t pad;
+};
+
+struct VerifyCand {
+  uint32_t inst, size;
+  uint64_t rank;
+};
+
+hipError_t launch_batch_reset(unsigned long long *d_head, uint32_t *d_hit_count,
+                              hipStream_t stream);
+hipError_t launch_batch_step(int width, const BatchStep *d_step, const BatchJob *d_jobs,
+                             const TableView *d_tvs, unsigned long long *d_head,
+                             HitRec *d_hits, uint32_t *d_hit_count, uint32_t hit_cap,
+                             int grid, hipStream_t stream);
+hipError_t launch_verify_batch(int width, const TableView *d_tvs, const uint64_t *d_leafvals,
+                               const uint64_t *d_outputs, const BatchInstData *d_idat,
+                               const VerifyCand *d_cand, uint32_t n_cand, uint8_t *d_ok,
+                               hipStream_t stream);
 
 // All launches are asynchronous on `stream`.
 hipError_t launch_table_build(int width, const TableView *d_tv, void *d_base,

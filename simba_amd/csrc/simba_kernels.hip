@@ -93,33 +93,33 @@ __device__ word value_of(const TableView *__restrict__ tv, int m, uint64_t r) {
 // unrank order, nocache.cu:52-79), the children are read from V[<m].
 // ---------------------------------------------------------------------------
 template <typename word>
+__device__ __forceinline__ word build_value(const TableView *__restrict__ tv, int m,
+                                            uint64_t r) {
+  const word *V = (const word *)tv->base;
+  uint64_t x = r;
+  int op = 0;
+  while (x >= tv->T[m][op]) x -= tv->T[m][op++];
+  if (is_unary(op)) return apply_op<word>(op, V[tv->off[m - 1] + x], word(0));
+  const int cs = m - 1;
+  int j = 1;
+  for (;; ++j) {
+    const uint64_t blk = tv->T[j][8] * tv->T[cs - j][8];
+    if (x < blk) break;
+    x -= blk;
+  }
+  const uint64_t rs = tv->T[cs - j][8];
+  const uint64_t a = x / rs;
+  return apply_op<word>(op, V[tv->off[j] + a], V[tv->off[cs - j] + (x - a * rs)]);
+}
+
+template <typename word>
 __global__ void __launch_bounds__(256)
 k_table_build(const TableView *__restrict__ tv, word *__restrict__ out, int m,
               uint64_t n) {
-  const word *V = (const word *)tv->base;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
-       r += stride) {
-    uint64_t x = r;
-    int op = 0;
-    while (x >= tv->T[m][op]) x -= tv->T[m][op++];
-    word v;
-    if (is_unary(op)) {
-      v = apply_op<word>(op, V[tv->off[m - 1] + x], word(0));
-    } else {
-      const int cs = m - 1;
-      int j = 1;
-      for (;; ++j) {
-        const uint64_t blk = tv->T[j][8] * tv->T[cs - j][8];
-        if (x < blk) break;
-        x -= blk;
-      }
-      const uint64_t rs = tv->T[cs - j][8];
-      const uint64_t a = x / rs;
-      v = apply_op<word>(op, V[tv->off[j] + a], V[tv->off[cs - j] + (x - a * rs)]);
-    }
-    out[r] = v;
-  }
+       r += stride)
+    out[r] = build_value<word>(tv, m, r);
 }
 
 template <typename word>
@@ -237,17 +237,50 @@ __device__ __forceinline__ uint64_t uniform(uint64_t x) {
          __builtin_amdgcn_readfirstlane((uint32_t)x);
 }
 
+// One operator instance over the A tile: its distances folded into acc.
+template <typename word, int INST, bool GEN_A>
+__device__ __forceinline__ uint32_t fold_inst(uint32_t acc, const TableView *__restrict__ tv,
+                                              const word *__restrict__ A, int j, uint64_t a0,
+                                              uint64_t a1, const BPre<word> *pre, uint32_t de,
+                                              uint32_t dof) {
+  for (uint64_t a = a0; a < a1; ++a) {
+    const word av = GEN_A ? value_of<word>(tv, j, a) : A[a];
+    const word na = uniform(word(word(0) - av));
+#pragma unroll
+    for (int k = 0; k < kPairRB; ++k) {
+      const BPre<word> &q = pre[k];
+      if (INST <= 2 || INST == 4) {
+        const word c = INST == 0 ? word(av & q.b) : INST == 1 ? word(av | q.b)
+                     : INST == 2 ? word(av ^ q.b) : word(av * q.b);
+        const uint32_t x = dpart(c);
+        acc = min3u(acc, x - de, x - dof);
+      } else if (INST == 3) {
+        acc = min3u(acc, dpart(word(av + q.be)), dpart(word(av + q.bo)));
+      } else if (INST == 5) {
+        acc = min3u(acc, dpart(word(av + q.nbe)), dpart(word(av + q.nbo)));
+      } else {
+        acc = min3u(acc, dpart(word(q.be + na)), dpart(word(q.bo + na)));
+      }
+    }
+  }
+  return acc;
+}
+
+// One tile of a pair check (shared by the per-instance and batched kernels).
+// p.mask selects the operator instances (bit i: instance i, see HitRec).
+// `tag` is or-ed into HitRec::inst above bit 8 (the batch's instance index).
 template <typename word, bool SAME, bool GEN_A, bool GEN_B>
-__global__ void __launch_bounds__(kPairThreads)
-k_pair_check(const TableView *__restrict__ tv, const word *__restrict__ A,
-             const word *__restrict__ B, PairLaunch p, uint64_t lo_e64, uint64_t lo_o64,
-             uint64_t span64, HitRec *__restrict__ hits, uint32_t *__restrict__ hit_count,
-             uint32_t hit_cap) {
-  const word lo_e = (word)lo_e64, lo_o = (word)lo_o64, span = (word)span64;
+__device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
+                                          const word *__restrict__ A,
+                                          const word *__restrict__ B, const PairLaunch &p,
+                                          uint64_t t, word lo_e, word lo_o, word span,
+                                          HitRec *__restrict__ hits,
+                                          uint32_t *__restrict__ hit_count, uint32_t hit_cap,
+                                          uint32_t tag) {
   // prefilter threshold on the folded distance
-  const uint32_t thr = sizeof(word) == 8 ? 1u : (uint32_t)span64;
+  const uint32_t thr = sizeof(word) == 8 ? 1u : (uint32_t)span;
   const uint32_t pair = ((uint32_t)p.sb << 16) | (uint32_t)p.j;
-  for (uint64_t t = p.tile_begin + blockIdx.x; t < p.tile_end; t += gridDim.x) {
+  {
     const uint64_t at = t % p.nTA, bt = t / p.nTA;
     const uint64_t a0 = at * kPairTA;
     const uint64_t a1 = a0 + kPairTA < p.nA ? a0 + kPairTA : p.nA;
@@ -273,11 +306,29 @@ k_pair_check(const TableView *__restrict__ tv, const word *__restrict__ A,
 #pragma unroll
     for (int k = 0; k < kPairRB; ++k) pre[k] = bpre<word>(bv[k], oe, oo);
     uint32_t acc = 0xffffffffu;
-    for (uint64_t a = a0; a < a1; ++a) {
-      const word av = GEN_A ? value_of<word>(tv, p.j, a) : A[a];
-      const word na = uniform(word(word(0) - av));
+    const uint32_t full = SAME ? 0x3fu : 0x7fu;
+    if ((p.mask & full) == full) {
+      for (uint64_t a = a0; a < a1; ++a) {
+        const word av = GEN_A ? value_of<word>(tv, p.j, a) : A[a];
+        const word na = uniform(word(word(0) - av));
 #pragma unroll
-      for (int k = 0; k < kPairRB; ++k) acc = fold7<word, SAME>(acc, av, na, pre[k], de, dof);
+        for (int k = 0; k < kPairRB; ++k) acc = fold7<word, SAME>(acc, av, na, pre[k], de, dof);
+      }
+    } else {
+      // a subset of the operator instances (one block at a time): one pass
+      // over the A tile per selected instance
+      for (int inst = 0; inst < 7; ++inst) {
+        if (!((p.mask & full) >> inst & 1u)) continue;
+        switch (inst) {
+          case 0: acc = fold_inst<word, 0, GEN_A>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
+          case 1: acc = fold_inst<word, 1, GEN_A>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
+          case 2: acc = fold_inst<word, 2, GEN_A>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
+          case 3: acc = fold_inst<word, 3, GEN_A>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
+          case 4: acc = fold_inst<word, 4, GEN_A>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
+          case 5: acc = fold_inst<word, 5, GEN_A>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
+          default: acc = fold_inst<word, 6, GEN_A>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
+        }
+      }
     }
     const bool hit = any_ok && acc <= thr;
     // Rare: some lane passed the prefilter.  Re-walk the tile, exact test.
@@ -289,15 +340,27 @@ k_pair_check(const TableView *__restrict__ tv, const word *__restrict__ A,
             if (!ok[k]) continue;
             const uint64_t b = b0 + (uint64_t)k * kPairThreads;
             for (int inst = 0; inst < (SAME ? 6 : 7); ++inst) {
+              if (!(p.mask >> inst & 1u)) continue;
               const word c = inst_value<word>(inst, av, bv[k]);
               if (in_window(c, lo_e, lo_o, span))
-                record_hit<word>(hits, hit_count, hit_cap, a, b, c, inst, pair);
+                record_hit<word>(hits, hit_count, hit_cap, a, b, c, inst | tag, pair);
             }
           }
         }
       }
     }
   }
+}
+
+template <typename word, bool SAME, bool GEN_A, bool GEN_B>
+__global__ void __launch_bounds__(kPairThreads)
+k_pair_check(const TableView *__restrict__ tv, const word *__restrict__ A,
+             const word *__restrict__ B, PairLaunch p, uint64_t lo_e64, uint64_t lo_o64,
+             uint64_t span64, HitRec *__restrict__ hits, uint32_t *__restrict__ hit_count,
+             uint32_t hit_cap) {
+  for (uint64_t t = p.tile_begin + blockIdx.x; t < p.tile_end; t += gridDim.x)
+    pair_tile<word, SAME, GEN_A, GEN_B>(tv, A, B, p, t, (word)lo_e64, (word)lo_o64,
+                                        (word)span64, hits, hit_count, hit_cap, 0u);
 }
 
 // Leaves (size-1 trees) against the same windows: exact leaf solutions and
@@ -392,6 +455,160 @@ k_verify(const TableView *__restrict__ tv, const uint64_t *__restrict__ leafvals
 }
 
 // ---------------------------------------------------------------------------
+// Batched search: one persistent kernel per size step over the jobs of every
+// active instance.  Workgroups pull tile indices from a device counter; a
+// binary search over the jobs' first tiles locates the job.  Every workgroup
+// leaves once the counter passes the step's total, so the grid drains.
+// ---------------------------------------------------------------------------
+template <typename word>
+__device__ void batch_pair(const TableView *tv, const BatchJob &J, uint64_t lt,
+                           HitRec *hits, uint32_t *hc, uint32_t cap) {
+  const word *base = (const word *)tv->base;
+  const PairLaunch &p = J.p;
+  const word *A = base + (p.gen_a ? 0 : tv->off[p.j]);
+  const word *B = base + (p.gen_b ? 0 : tv->off[p.m]);
+  const word lo_e = (word)J.lo_e, lo_o = (word)J.lo_o, span = (word)J.span;
+  const uint32_t tag = J.inst << 8;
+  if (p.j == p.m) {
+    if (p.gen_a)
+      pair_tile<word, true, true, true>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
+    else
+      pair_tile<word, true, false, false>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
+  } else if (p.gen_a) {
+    pair_tile<word, false, true, true>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
+  } else if (p.gen_b) {
+    pair_tile<word, false, false, true>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
+  } else {
+    pair_tile<word, false, false, false>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
+  }
+}
+
+template <typename word>
+__global__ void __launch_bounds__(kPairThreads)
+k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jobs,
+             const TableView *__restrict__ tvs, unsigned long long *head, HitRec *hits,
+             uint32_t *hit_count, uint32_t hit_cap) {
+  __shared__ unsigned long long s_t;
+  __shared__ int s_job;
+  const uint64_t total = step->total_tiles;
+  const int njobs = (int)step->njobs;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const unsigned long long t = atomicAdd(head, 1ull);
+      int lo = 0, hi = njobs - 1;
+      if (t < total) {
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (jobs[mid].tile0 <= t) lo = mid;
+          else hi = mid - 1;
+        }
+      }
+      s_t = t;
+      s_job = lo;
+    }
+    __syncthreads();
+    const uint64_t t = s_t;
+    const int jb = s_job;
+    __syncthreads();
+    if (t >= total) break;
+    const BatchJob J = jobs[jb];
+    const TableView *tv = tvs + J.inst;
+    const uint64_t lt = t - J.tile0;
+    if (J.kind == 0) {  // table build: V[m] ranks [lt*kBuildTile, +kBuildTile)
+      word *out = (word *)tv->base + tv->off[J.p.m];
+      const uint64_t r0 = lt * kBuildTile;
+      const uint64_t r1 = r0 + kBuildTile < J.p.nB ? r0 + kBuildTile : J.p.nB;
+      for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x)
+        out[r] = build_value<word>(tv, J.p.m, r);
+    } else if (J.kind == 1) {
+      batch_pair<word>(tv, J, lt, hits, hit_count, hit_cap);
+    } else {  // leaves against the windows of base size 1
+      const word *V = (const word *)tv->base + tv->off[1];
+      if ((int)threadIdx.x < tv->L) {
+        const word c = V[threadIdx.x];
+        if (in_window(c, (word)J.lo_e, (word)J.lo_o, (word)J.span))
+          record_hit<word>(hits, hit_count, hit_cap, threadIdx.x, 0, c, 7 | (J.inst << 8),
+                           1u << 16);
+      }
+    }
+  }
+}
+
+__global__ void k_batch_reset(unsigned long long *head, uint32_t *hit_count) {
+  if (threadIdx.x == 0) {
+    *head = 0;
+    *hit_count = 0;
+  }
+}
+
+// Verification of candidates of many instances (global-memory samples).
+template <typename word>
+__global__ void __launch_bounds__(256)
+k_verify_batch(const TableView *__restrict__ tvs, const uint64_t *__restrict__ leafvals,
+               const uint64_t *__restrict__ outputs, const BatchInstData *__restrict__ idat,
+               const VerifyCand *__restrict__ cand, uint32_t n_cand, uint8_t *__restrict__ ok) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cand) return;
+  const VerifyCand vc = cand[c];
+  const TableView *tv = tvs + vc.inst;
+  const BatchInstData d = idat[vc.inst];
+  const int L = tv->L;
+  const int size = (int)vc.size;
+  uint8_t rpn[kMaxSize];
+  struct Item { uint64_t r; int s, pos; };
+  Item st[kMaxSize];
+  int sp = 0;
+  st[sp++] = {vc.rank, size, 0};
+  while (sp > 0) {
+    Item it = st[--sp];
+    if (it.s == 1) {
+      rpn[it.pos] = (uint8_t)it.r;
+      continue;
+    }
+    uint64_t n = it.r;
+    int op = 0;
+    while (n >= tv->T[it.s][op]) n -= tv->T[it.s][op++];
+    const int cs = it.s - 1;
+    rpn[it.pos + cs] = (uint8_t)(kTokOp + op);
+    if (is_unary(op)) {
+      st[sp++] = {n, cs, it.pos};
+      continue;
+    }
+    int j = 1;
+    for (;; ++j) {
+      const uint64_t blk = tv->T[j][8] * tv->T[cs - j][8];
+      if (n < blk) break;
+      n -= blk;
+    }
+    const uint64_t rs = tv->T[cs - j][8];
+    const uint64_t a = n / rs;
+    st[sp++] = {a, j, it.pos};
+    st[sp++] = {n - a * rs, cs - j, it.pos + j};
+  }
+  uint8_t good = 1;
+  for (int s = 0; s < (int)d.n_samples && good; ++s) {
+    word vs[kMaxSize];
+    int top = -1;
+    for (int i = 0; i < size; ++i) {
+      const uint8_t t = rpn[i];
+      if (t < kTokOp) {
+        vs[++top] = (word)leafvals[d.leaf_off + (uint64_t)s * L + t];
+      } else {
+        const int op = t - kTokOp;
+        if (is_unary(op)) {
+          vs[top] = apply_op<word>(op, vs[top], word(0));
+        } else {
+          const word rv = vs[top--];
+          vs[top] = apply_op<word>(op, vs[top], rv);
+        }
+      }
+    }
+    good = vs[0] == (word)outputs[d.out_off + s];
+  }
+  ok[c] = good;
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers.
 // ---------------------------------------------------------------------------
 hipError_t launch_table_build(int width, const TableView *d_tv, void *d_base,
@@ -455,6 +672,40 @@ hipError_t launch_leaf_scan(int width, const TableView *d_tv, int L,
   else
     hipLaunchKernelGGL(k_leaf_scan<uint64_t>, dim3(1), dim3(64), 0, stream, d_tv, L,
                        tg.lo_even, tg.lo_odd, tg.span, d_hits, d_hit_count, hit_cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_reset(unsigned long long *d_head, uint32_t *d_hit_count,
+                              hipStream_t stream) {
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, stream, d_head, d_hit_count);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_step(int width, const BatchStep *d_step, const BatchJob *d_jobs,
+                             const TableView *d_tvs, unsigned long long *d_head,
+                             HitRec *d_hits, uint32_t *d_hit_count, uint32_t hit_cap,
+                             int grid, hipStream_t stream) {
+  if (width == 32)
+    hipLaunchKernelGGL(k_batch_step<uint32_t>, dim3(grid), dim3(kPairThreads), 0, stream,
+                       d_step, d_jobs, d_tvs, d_head, d_hits, d_hit_count, hit_cap);
+  else
+    hipLaunchKernelGGL(k_batch_step<uint64_t>, dim3(grid), dim3(kPairThreads), 0, stream,
+                       d_step, d_jobs, d_tvs, d_head, d_hits, d_hit_count, hit_cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_verify_batch(int width, const TableView *d_tvs, const uint64_t *d_leafvals,
+                               const uint64_t *d_outputs, const BatchInstData *d_idat,
+                               const VerifyCand *d_cand, uint32_t n_cand, uint8_t *d_ok,
+                               hipStream_t stream) {
+  if (n_cand == 0) return hipSuccess;
+  const int grid = (int)((n_cand + 255) / 256);
+  if (width == 32)
+    hipLaunchKernelGGL(k_verify_batch<uint32_t>, dim3(grid), dim3(256), 0, stream, d_tvs,
+                       d_leafvals, d_outputs, d_idat, d_cand, n_cand, d_ok);
+  else
+    hipLaunchKernelGGL(k_verify_batch<uint64_t>, dim3(grid), dim3(256), 0, stream, d_tvs,
+                       d_leafvals, d_outputs, d_idat, d_cand, n_cand, d_ok);
   return hipGetLastError();
 }
 

@@ -25,37 +25,6 @@ namespace {
 
 constexpr uint64_t kDefaultCacheBytes = 8ull << 30;
 
-uint64_t fp_host(const std::vector<uint64_t> &v, int W) {
-  auto fmix32 = [](uint32_t x) {
-    x ^= x >> 16;
-    x *= 0x85ebca6bu;
-    x ^= x >> 13;
-    x *= 0xc2b2ae35u;
-    x ^= x >> 16;
-    return x;
-  };
-  auto fmix64 = [](uint64_t x) {
-    x ^= x >> 33;
-    x *= 0xff51afd7ed558ccdull;
-    x ^= x >> 33;
-    x *= 0xc4ceb9fe1a85ec53ull;
-    x ^= x >> 33;
-    return x;
-  };
-  uint64_t seed = 0;
-  for (size_t i = 0; i < v.size(); ++i) {
-    uint64_t val;
-    if (W == 32)
-      val = (uint64_t)(uint32_t)(fmix32((uint32_t)v[i]) + 0x9e3779b9u) + (seed << 6) +
-            (seed >> 2);
-    else
-      val = fmix64(v[i]) + 0x9e3779b9ull + (seed << 6) + (seed >> 2);
-    if (i & 1) val = (val >> 32) | (val << 32);
-    seed ^= val;
-  }
-  return seed;
-}
-
 }  // namespace
 
 int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result &res,
@@ -92,19 +61,21 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
       leaf[v][i] = (v < nv ? inst.inputs[(size_t)i * nv + v] : inst.consts[v - nv]) & M;
   S.level_new[1] = (uint64_t)L;
   S.level_cand[1] = (uint64_t)L;
-  for (int v = 0; v < L; ++v) {
-    if (leaf[v] == tgt) {
-      uint8_t rpn[1] = {(uint8_t)v};
-      std::snprintf(res.expr, sizeof res.expr, "%s",
-                    infix_vfb(rpn, 1, nv, inst.consts, W).c_str());
-      res.status = MBA_FOUND;
-      res.size = 1;
-      res.rank = (uint64_t)v;
-      res.candidates = (uint64_t)v + 1;
-      res.size_reached = 1;
-      res.seconds = seconds_since(t_start);
-      return MBA_FOUND;
-    }
+  int leaf_sol = -1;
+  for (int v = 0; v < L && leaf_sol < 0; ++v) {
+    if (leaf[v] == tgt) leaf_sol = v;
+  }
+  if (leaf_sol >= 0) {
+    uint8_t rpn[1] = {(uint8_t)leaf_sol};
+    std::snprintf(res.expr, sizeof res.expr, "%s",
+                  infix_vfb(rpn, 1, nv, inst.consts, W).c_str());
+    res.status = MBA_FOUND;
+    res.size = 1;
+    res.rank = (uint64_t)leaf_sol;
+    res.candidates = (uint64_t)leaf_sol + 1;
+    res.size_reached = 1;
+    res.seconds = seconds_since(t_start);
+    if (!o.exhaustive) return MBA_FOUND;
   }
 
   // ---- capacity -------------------------------------------------------------
@@ -181,7 +152,7 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
   int status = MBA_NOT_FOUND;
   int reached = 1;
   uint64_t cand_total = (uint64_t)L;
-  int64_t sol_idx = -1;
+  int64_t sol_idx = leaf_sol >= 0 ? -2 : -1;  // -2: the solution is a leaf
   int sol_level = 0;
   std::vector<VfbBlock> blocks;
 
@@ -262,7 +233,7 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
     cand_total += total;
     S.level_cand[lev] = total;
     const uint64_t found = cells[0], tmin = cells[1];
-    if (sol_idx < 0 && found != ~0ull && tmin == (a.tag | found)) {
+    if (sol_idx == -1 && found != ~0ull && tmin == (a.tag | found)) {
       sol_idx = (int64_t)found;
       sol_level = lev;
       reached = lev;
@@ -379,12 +350,15 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
                     infix_vfb(tok.data(), len, nv, inst.consts, W).c_str());
     }
     status = MBA_FOUND;
+  } else if (sol_idx == -2) {  // a leaf, found before the levels (exhaustive run)
+    status = MBA_FOUND;
+    res.size = 1;
+    res.rank = (uint64_t)leaf_sol;
   } else {
     std::snprintf(res.expr, sizeof res.expr, "Not found !");
   }
   res.status = status;
   res.seconds = seconds_since(t_start);
-  (void)fp_host;
   return status;
 }
 

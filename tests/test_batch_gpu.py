@@ -1,0 +1,65 @@
+"""Batched search: every instance's answer equals the per-instance plain
+search's (and the CPU oracle's) on the same size budget."""
+import random
+
+import pytest
+
+from tests.gpu_util import note, oracle_simba
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import simba_amd
+    c = simba_amd.Context(0)
+    yield c
+    c.close()
+
+
+def _instances(seed, count, W, max_target):
+    from simba_amd.synth import make_instance
+    rng = random.Random(seed)
+    out = []
+    for i in range(count):
+        nv = rng.randint(2, 5)
+        n = rng.randint(16, 64)
+        ts = rng.randint(3, max_target)
+        out.append(make_instance(seed * 100000 + i, nv, n, ts, W))
+    return out
+
+
+@pytest.mark.parametrize("W", [64, 32])
+def test_batch_matches_plain_and_oracle(ctx, W):
+    insts = _instances(7 + W, 48, W, 7)
+    note(f"batch vs plain W={W}")
+    got = ctx.batch(insts, min_size=1, max_size=6)
+    for inst, g in zip(insts, got):
+        p = ctx.simba(inst, min_size=1, max_size=6)
+        assert (g["status"], g["size"], g["rank"], g["expr"]) == \
+            (p["status"], p["size"], p["rank"], p["expr"])
+    for inst, g in list(zip(insts, got))[:16]:
+        w = oracle_simba(inst, 1, 6)
+        assert g["status"] == w["status"]
+        if w["status"] == 0:
+            assert (g["size"], g["rank"], g["expr"]) == (w["size"], w["rank"], w["expr"])
+
+
+def test_batch_small_table_budget_and_compat(ctx):
+    # tiny table budget: most pairs use the per-lane table-assisted values
+    insts = _instances(99, 24, 64, 6)
+    note("batch small tables + compat")
+    got = ctx.batch(insts, max_size=8, compat=True, table_bytes=24 * 64)
+    for inst, g in zip(insts, got):
+        p = ctx.simba(inst, max_size=8, compat=True)
+        assert (g["status"], g["size"], g["rank"], g["expr"]) == \
+            (p["status"], p["size"], p["rank"], p["expr"])
+
+
+def test_batch_single_and_repeated(ctx):
+    insts = _instances(5, 1, 64, 5)
+    note("batch single")
+    a = ctx.batch(insts, max_size=7)
+    b = ctx.batch(insts, max_size=7)
+    key = lambda r: (r["status"], r["size"], r["rank"], r["expr"], r["candidates"])
+    assert [key(x) for x in a] == [key(x) for x in b]
