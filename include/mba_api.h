@@ -131,6 +131,42 @@ int mba_vfb_solve(mba_ctx *ctx, const mba_instance *inst, const mba_opts *opts,
 int mba_batch_solve(mba_ctx *ctx, const mba_instance *insts, size_t count,
                     const mba_opts *opts, mba_result *results);
 
+/* ---- learned block proposal (no reference counterpart: the north-star's
+ * learned proposal).  A 2-layer MLP 256 -> 512 -> 128, BF16 MFMA operands
+ * with FP32 accumulation, trained on the device with Adam.  Output class =
+ * op * 16 + min(left size, 15) of a block's root (op 0..7, left size 0 for
+ * unary roots).  An mba_mlp belongs to the context it was created on. */
+typedef struct mba_mlp mba_mlp;
+#define MBA_MLP_IN 256
+#define MBA_MLP_HIDDEN 512
+#define MBA_MLP_OUT 128
+
+/* 256 features of an instance from its hashed sample input/output bits. */
+int mba_features(const mba_instance *inst, float *out);
+int mba_mlp_create(mba_ctx *ctx, uint64_t seed, mba_mlp **out);
+void mba_mlp_destroy(mba_mlp *m);
+/* Host copies of the FP32 master weights: w1 [256 x 512], b1 [512],
+ * w2 [512 x 128], b2 [128], row-major (input-major). */
+int mba_mlp_get_weights(mba_mlp *m, float *w1, float *b1, float *w2, float *b2);
+int mba_mlp_set_weights(mba_mlp *m, const float *w1, const float *b1, const float *w2,
+                        const float *b2);
+/* logits [batch x 128] of features x [batch x 256]. */
+int mba_mlp_forward(mba_mlp *m, const float *x, int batch, float *logits);
+/* Mean softmax cross-entropy over rows with label >= 0 and its gradients
+ * (any output pointer may be NULL). */
+int mba_mlp_grads(mba_mlp *m, const float *x, const int32_t *labels, int batch, float *loss,
+                  float *gw1, float *gb1, float *gw2, float *gb2);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on a batch; returns the loss. */
+int mba_mlp_train_step(mba_mlp *m, const float *x, const int32_t *labels, int batch, float lr,
+                       float *loss);
+/* SIMBA search with the binary blocks of each size visited in descending
+ * learned score; returns a verified solution of the minimal size. */
+int mba_learned_solve(mba_ctx *ctx, mba_mlp *m, const mba_instance *inst,
+                      const mba_opts *opts, mba_result *res);
+/* Batched form of mba_learned_solve (one persistent kernel per round). */
+int mba_learned_batch_solve(mba_ctx *ctx, mba_mlp *m, const mba_instance *insts,
+                            size_t count, const mba_opts *opts, mba_result *results);
+
 /* Instance reader for the reference JSON schema (sample.py:43-65):
  * {"num_variables", "num_examples", "examples":[{"inputs":[{"value","size"}],
  *  "output":[{"value","size"}]}]}.  Width = the first value's "size" field

@@ -69,7 +69,11 @@ class VfbStats(ctypes.Structure):
 # Every symbol the C ABI exports (include/mba_api.h).
 EXPORTS = ["mba_ctx_create", "mba_ctx_destroy", "mba_strerror", "mba_ctx_last_error",
            "mba_default_opts", "mba_simba_solve", "mba_vfb_solve", "mba_batch_solve",
-           "mba_instance_read_json", "mba_instance_free", "mba_rpn_to_infix"]
+           "mba_instance_read_json", "mba_instance_free", "mba_rpn_to_infix",
+           "mba_features", "mba_mlp_create", "mba_mlp_destroy", "mba_mlp_get_weights",
+           "mba_mlp_set_weights", "mba_mlp_forward", "mba_mlp_grads", "mba_mlp_train_step",
+           "mba_learned_solve", "mba_learned_batch_solve"]
+MLP_IN, MLP_HIDDEN, MLP_OUT = 256, 512, 128
 
 _lib = None
 
@@ -99,6 +103,21 @@ def lib():
         L.mba_rpn_to_infix.argtypes = [ctypes.c_char_p, ctypes.c_uint32, P(ctypes.c_uint64),
                                        ctypes.c_uint32, ctypes.c_uint32, ctypes.c_char_p,
                                        ctypes.c_size_t]
+        fp = P(ctypes.c_float)
+        L.mba_features.argtypes = [P(Instance), fp]
+        L.mba_mlp_create.argtypes = [ctypes.c_void_p, ctypes.c_uint64, P(ctypes.c_void_p)]
+        L.mba_mlp_destroy.argtypes = [ctypes.c_void_p]
+        L.mba_mlp_get_weights.argtypes = [ctypes.c_void_p, fp, fp, fp, fp]
+        L.mba_mlp_set_weights.argtypes = [ctypes.c_void_p, fp, fp, fp, fp]
+        L.mba_mlp_forward.argtypes = [ctypes.c_void_p, fp, ctypes.c_int, fp]
+        L.mba_mlp_grads.argtypes = [ctypes.c_void_p, fp, P(ctypes.c_int32), ctypes.c_int, fp,
+                                    fp, fp, fp, fp]
+        L.mba_mlp_train_step.argtypes = [ctypes.c_void_p, fp, P(ctypes.c_int32), ctypes.c_int,
+                                         ctypes.c_float, fp]
+        L.mba_learned_solve.argtypes = [ctypes.c_void_p, ctypes.c_void_p, P(Instance), P(Opts),
+                                        P(Result)]
+        L.mba_learned_batch_solve.argtypes = [ctypes.c_void_p, ctypes.c_void_p, P(Instance),
+                                              ctypes.c_size_t, P(Opts), P(Result)]
         _lib = L
     return _lib
 
@@ -167,6 +186,108 @@ class Context:
         o = make_opts(**kw)
         res = (Result * n)()
         self._check(lib().mba_batch_solve(self._h, arr, n, ctypes.byref(o), res))
+        return [r.as_dict() for r in res]
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def features(inst):
+    """256 hashed input/output-bit features of an instance (float32)."""
+    ci, keep = inst.to_c()
+    out = np.zeros(MLP_IN, dtype=np.float32)
+    rc = lib().mba_features(ctypes.byref(ci), _fptr(out))
+    if rc != 0:
+        raise MbaError(lib().mba_strerror(rc).decode())
+    return out
+
+
+def block_class(op, split):
+    """Output class of a root block: op * 16 + min(left size, 15)."""
+    return op * 16 + min(max(split, 0), 15)
+
+
+class Mlp:
+    """The learned block-proposal network on one context (device weights)."""
+
+    def __init__(self, ctx, seed=0):
+        self.ctx = ctx
+        self._h = ctypes.c_void_p()
+        rc = lib().mba_mlp_create(ctx._h, seed, ctypes.byref(self._h))
+        if rc != 0:
+            raise MbaError(lib().mba_strerror(rc).decode())
+
+    def close(self):
+        if self._h:
+            lib().mba_mlp_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_weights(self):
+        w1 = np.zeros((MLP_IN, MLP_HIDDEN), np.float32)
+        b1 = np.zeros(MLP_HIDDEN, np.float32)
+        w2 = np.zeros((MLP_HIDDEN, MLP_OUT), np.float32)
+        b2 = np.zeros(MLP_OUT, np.float32)
+        self.ctx._check(lib().mba_mlp_get_weights(self._h, _fptr(w1), _fptr(b1), _fptr(w2),
+                                                  _fptr(b2)))
+        return w1, b1, w2, b2
+
+    def set_weights(self, w1, b1, w2, b2):
+        arrs = [np.ascontiguousarray(x, dtype=np.float32) for x in (w1, b1, w2, b2)]
+        self.ctx._check(lib().mba_mlp_set_weights(self._h, *[_fptr(x) for x in arrs]))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_IN)
+        out = np.zeros((x.shape[0], MLP_OUT), np.float32)
+        self.ctx._check(lib().mba_mlp_forward(self._h, _fptr(x), x.shape[0], _fptr(out)))
+        return out
+
+    def grads(self, x, labels):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_IN)
+        y = np.ascontiguousarray(labels, dtype=np.int32)
+        loss = np.zeros(1, np.float32)
+        g = [np.zeros((MLP_IN, MLP_HIDDEN), np.float32), np.zeros(MLP_HIDDEN, np.float32),
+             np.zeros((MLP_HIDDEN, MLP_OUT), np.float32), np.zeros(MLP_OUT, np.float32)]
+        self.ctx._check(lib().mba_mlp_grads(
+            self._h, _fptr(x), y.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), x.shape[0],
+            _fptr(loss), *[_fptr(a) for a in g]))
+        return float(loss[0]), g
+
+    def train_step(self, x, labels, lr):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_IN)
+        y = np.ascontiguousarray(labels, dtype=np.int32)
+        loss = np.zeros(1, np.float32)
+        self.ctx._check(lib().mba_mlp_train_step(
+            self._h, _fptr(x), y.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), x.shape[0],
+            ctypes.c_float(lr), _fptr(loss)))
+        return float(loss[0])
+
+    def solve(self, inst, **kw):
+        ci, keep = inst.to_c()
+        o = make_opts(**kw)
+        r = Result()
+        self.ctx._check(lib().mba_learned_solve(self.ctx._h, self._h, ctypes.byref(ci),
+                                                ctypes.byref(o), ctypes.byref(r)))
+        return r.as_dict()
+
+    def batch_solve(self, insts, **kw):
+        n = len(insts)
+        arr = (Instance * n)()
+        keep = []
+        for i, x in enumerate(insts):
+            ci, k = x.to_c()
+            arr[i] = ci
+            keep.append(k)
+        o = make_opts(**kw)
+        res = (Result * n)()
+        self.ctx._check(lib().mba_learned_batch_solve(self.ctx._h, self._h, arr, n,
+                                                      ctypes.byref(o), res))
         return [r.as_dict() for r in res]
 
 

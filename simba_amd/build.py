@@ -27,8 +27,8 @@ HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops"]
 
 LIB_SOURCES = [
-    "simba_kernels.hip", "vfb_kernels.hip", "batch_kernels.hip", "mlp_kernels.hip",
-    "simba_host.cpp", "vfb_host.cpp", "batch_host.cpp", "api.cpp", "expr.cpp",
+    "simba_kernels.hip", "vfb_kernels.hip", "mlp_kernels.hip",
+    "simba_host.cpp", "vfb_host.cpp", "batch_host.cpp", "mlp_host.cpp", "api.cpp", "expr.cpp",
     "instance_io.cpp",
 ]
 CLI_SOURCES = {"simba_exe": "cli_simba.cpp", "vfb_exe": "cli_vfb.cpp"}

@@ -9,9 +9,6 @@
 
 using namespace mba;
 
-struct mba_ctx {
-  Ctx c;
-};
 
 extern "C" {
 

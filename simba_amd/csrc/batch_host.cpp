@@ -37,7 +37,7 @@ struct InstState {
 }  // namespace
 
 int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opts &o,
-                mba_result *results) {
+                mba_result *results, const float *scores) {
   const auto t_start = Clock::now();
   if (count == 0) return 0;
   const int W = (int)insts[0].width;
@@ -158,8 +158,11 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   }
 
   // ---- per-step graph: upload jobs, reset counters, drain, read hit count --
+  // jobs per instance and step: a leaf scan or a table build, plus the pairs
+  // (plain) or at most half of the binary blocks of the size (learned rounds)
   size_t job_cap = 0;
-  for (size_t i = 0; i < count; ++i) job_cap += 2 + (size_t)std::max(0, (max_size - 1) / 2);
+  for (size_t i = 0; i < count; ++i)
+    job_cap += 2 + (size_t)std::max(0, scores ? 3 * max_size : (max_size - 1) / 2);
   BatchJob *h_jobs = nullptr;
   BatchStep *h_step = nullptr;
   uint32_t *h_hc = nullptr;
@@ -213,49 +216,58 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   std::vector<VerifyCand> vc;
   double ktime = 0.0;
   bool budget_hit = false;
+  auto finish = [&](size_t i, int z) {
+    InstState &st = S[i];
+    fill_result(*st.T, z, st.found[z], (int)insts[i].n_vars, insts[i].consts, W, results[i]);
+    results[i].seconds = seconds_since(t_start);
+    st.reached = z;
+    st.active = false;
+  };
+  std::vector<std::vector<BlockRef>> order(count);
   for (int s = 0; s <= max_size; ++s) {
     if ((o.cancel && *o.cancel) ||
         (o.time_budget_s > 0 && seconds_since(t_start) > o.time_budget_s)) {
       budget_hit = true;
       break;
     }
-    size_t nj = 0;
-    uint64_t tiles = 0;
-    for (size_t i = 0; i < count; ++i) {
-      InstState &st = S[i];
-      if (!st.active) continue;
-      auto push = [&](const BatchJob &J) {
-        h_jobs[nj] = J;
-        h_jobs[nj].tile0 = tiles;
-        tiles += J.ntiles;
-        ++nj;
-      };
-      const CountTable &T = *st.T;
-      if (s == 1) {  // leaves and every unary chain over them
-        BatchJob J;
-        std::memset(&J, 0, sizeof J);
-        const CheckTargets tg = targets_for(st.t0, 1, max_size, W);
-        J.ntiles = 1;
-        J.lo_e = tg.lo_even;
-        J.lo_o = tg.lo_odd;
-        J.span = tg.span;
-        J.inst = (uint32_t)i;
-        J.kind = 2;
-        push(J);
+    const bool learned = scores && s >= 3;
+    size_t max_blocks = 0;
+    if (learned) {
+      for (size_t i = 0; i < count; ++i) {
+        InstState &st = S[i];
+        if (!st.active) continue;
+        // a unary-rooted solution of this size is verified already
+        if (s >= min_size && st.found[s] != ~0ull) {
+          finish(i, s);
+          continue;
+        }
+        order[i] = binary_blocks(*st.T, s);
+        const float *sc = scores + i * 256;
+        std::stable_sort(order[i].begin(), order[i].end(),
+                         [&](const BlockRef &x, const BlockRef &y) {
+                           return sc[x.op * 32 + x.split] > sc[y.op * 32 + y.split];
+                         });
+        max_blocks = std::max(max_blocks, order[i].size());
       }
-      if (s >= 2 && s <= st.Kt) {  // value table V[s]
-        BatchJob J;
-        std::memset(&J, 0, sizeof J);
-        J.p.m = s;
-        J.p.nB = T.t[s][8];
-        J.ntiles = (J.p.nB + kBuildTile - 1) / kBuildTile;
-        J.inst = (uint32_t)i;
-        J.kind = 0;
-        push(J);
-      }
-      if (s >= 3) {
-        const CheckTargets tg = targets_for(st.t0, s, max_size, W);
-        for (int j = 1; j <= (s - 1) / 2; ++j) {
+    }
+    // learned order: rounds of 1, 1, 2, 4, ... blocks per instance
+    for (int round = 0;; ++round) {
+      const size_t rb = round == 0 ? 0 : (size_t)1 << (round - 1);
+      const size_t re = (size_t)1 << round;
+      size_t nj = 0;
+      uint64_t tiles = 0;
+      for (size_t i = 0; i < count; ++i) {
+        InstState &st = S[i];
+        if (!st.active) continue;
+        auto push = [&](const BatchJob &J) {
+          h_jobs[nj] = J;
+          h_jobs[nj].tile0 = tiles;
+          tiles += J.ntiles;
+          ++nj;
+        };
+        const CountTable &T = *st.T;
+        auto pair_job = [&](int j, uint32_t mask) {
+          const CheckTargets tg = targets_for(st.t0, s, max_size, W);
           BatchJob J;
           std::memset(&J, 0, sizeof J);
           J.p.sb = s;
@@ -266,7 +278,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
           J.p.nTA = (J.p.nA + kPairTA - 1) / kPairTA;
           J.p.gen_a = j > st.Kt;
           J.p.gen_b = J.p.m > st.Kt;
-          J.p.mask = kAllInst;
+          J.p.mask = mask;
           J.ntiles = J.p.nTA * ((J.p.nB + kPairTB - 1) / kPairTB);
           J.lo_e = tg.lo_even;
           J.lo_o = tg.lo_odd;
@@ -274,73 +286,106 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
           J.inst = (uint32_t)i;
           J.kind = 1;
           push(J);
+        };
+        if (round == 0 && s == 1) {  // leaves and every unary chain over them
+          BatchJob J;
+          std::memset(&J, 0, sizeof J);
+          const CheckTargets tg = targets_for(st.t0, 1, max_size, W);
+          J.ntiles = 1;
+          J.lo_e = tg.lo_even;
+          J.lo_o = tg.lo_odd;
+          J.span = tg.span;
+          J.inst = (uint32_t)i;
+          J.kind = 2;
+          push(J);
+        }
+        if (round == 0 && s >= 2 && s <= st.Kt) {  // value table V[s]
+          BatchJob J;
+          std::memset(&J, 0, sizeof J);
+          J.p.m = s;
+          J.p.nB = T.t[s][8];
+          J.ntiles = (J.p.nB + kBuildTile - 1) / kBuildTile;
+          J.inst = (uint32_t)i;
+          J.kind = 0;
+          push(J);
+        }
+        if (s >= 3 && !learned && round == 0) {
+          for (int j = 1; j <= (s - 1) / 2; ++j) pair_job(j, kAllInst);
+        }
+        if (learned) {
+          for (size_t k = rb; k < re && k < order[i].size(); ++k)
+            pair_job(order[i][k].pair_j, 1u << order[i][k].inst);
         }
       }
-    }
-    if (nj == 0) continue;
-    h_step->total_tiles = tiles;
-    h_step->njobs = (uint32_t)nj;
-    MBA_HIP(hipEventRecord(ctx.ev0, stq));
-    MBA_HIP(hipGraphLaunch(gexec, stq));
-    MBA_HIP(hipEventRecord(ctx.ev1, stq));
-    MBA_HIP(hipEventSynchronize(ctx.ev1));
-    float ms = 0.f;
-    MBA_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
-    ktime += ms * 1e-3;
-    const uint32_t nh = *h_hc;
-    if (nh > kBatchHitCap) {
-      ctx.last_error = "batch: sample hit buffer overflow";
-      return MBA_ERR_LIMIT;
-    }
-    if (nh > 0) {
-      hrec.resize(nh);
-      MBA_HIP(hipMemcpy(hrec.data(), d_hits, sizeof(HitRec) * nh, hipMemcpyDeviceToHost));
-      vc.clear();
-      for (const HitRec &h0 : hrec) {
-        HitRec h = h0;
-        const uint32_t i = h.inst >> 8;
-        h.inst &= 0xff;
-        const InstState &st = S[i];
-        const int sb = (int)(h.pair >> 16), j = (int)(h.pair & 0xffff);
-        const uint64_t rX = base_rank(*st.T, sb, j, sb - 1 - j, h);
-        cands.clear();
-        resolve_hit(*st.T, W, st.t0, sb, rX, h.v & M, min_size, max_size, cands);
-        for (const Candidate &c : cands) vc.push_back({i, (uint32_t)c.size, c.rank});
-      }
-      if (!vc.empty()) {
-        const uint32_t ncand = (uint32_t)vc.size();
-        MBA_HIP(ctx.vranks.ensure(sizeof(VerifyCand) * ncand));
-        MBA_HIP(ctx.vok.ensure(ncand));
-        MBA_HIP(hipMemcpyAsync(ctx.vranks.p, vc.data(), sizeof(VerifyCand) * ncand,
-                               hipMemcpyHostToDevice, stq));
-        MBA_HIP(launch_verify_batch(W, d_tvs, ctx.leafvals.as<uint64_t>(),
-                                    ctx.outputs.as<uint64_t>(),
-                                    ctx.b_results.as<BatchInstData>(),
-                                    ctx.vranks.as<VerifyCand>(), ncand, ctx.vok.as<uint8_t>(),
-                                    stq));
-        std::vector<uint8_t> okv(ncand);
-        MBA_HIP(hipMemcpyAsync(okv.data(), ctx.vok.p, ncand, hipMemcpyDeviceToHost, stq));
-        MBA_HIP(hipStreamSynchronize(stq));
-        for (uint32_t c = 0; c < ncand; ++c)
-          if (okv[c]) {
-            auto &f = S[vc[c].inst].found[vc[c].size];
-            f = std::min(f, vc[c].rank);
+      if (nj > 0) {
+        h_step->total_tiles = tiles;
+        h_step->njobs = (uint32_t)nj;
+        MBA_HIP(hipEventRecord(ctx.ev0, stq));
+        MBA_HIP(hipGraphLaunch(gexec, stq));
+        MBA_HIP(hipEventRecord(ctx.ev1, stq));
+        MBA_HIP(hipEventSynchronize(ctx.ev1));
+        float ms = 0.f;
+        MBA_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
+        ktime += ms * 1e-3;
+        const uint32_t nh = *h_hc;
+        if (nh > kBatchHitCap) {
+          ctx.last_error = "batch: sample hit buffer overflow";
+          return MBA_ERR_LIMIT;
+        }
+        if (nh > 0) {
+          hrec.resize(nh);
+          MBA_HIP(hipMemcpy(hrec.data(), d_hits, sizeof(HitRec) * nh, hipMemcpyDeviceToHost));
+          vc.clear();
+          for (const HitRec &h0 : hrec) {
+            HitRec h = h0;
+            const uint32_t i = h.inst >> 8;
+            h.inst &= 0xff;
+            const InstState &st = S[i];
+            const int sb = (int)(h.pair >> 16), j = (int)(h.pair & 0xffff);
+            const uint64_t rX = base_rank(*st.T, sb, j, sb - 1 - j, h);
+            cands.clear();
+            resolve_hit(*st.T, W, st.t0, sb, rX, h.v & M, min_size, max_size, cands);
+            for (const Candidate &c : cands) vc.push_back({i, (uint32_t)c.size, c.rank});
           }
+          if (!vc.empty()) {
+            const uint32_t ncand = (uint32_t)vc.size();
+            MBA_HIP(ctx.vranks.ensure(sizeof(VerifyCand) * ncand));
+            MBA_HIP(ctx.vok.ensure(ncand));
+            MBA_HIP(hipMemcpyAsync(ctx.vranks.p, vc.data(), sizeof(VerifyCand) * ncand,
+                                   hipMemcpyHostToDevice, stq));
+            MBA_HIP(launch_verify_batch(W, d_tvs, ctx.leafvals.as<uint64_t>(),
+                                        ctx.outputs.as<uint64_t>(),
+                                        ctx.b_results.as<BatchInstData>(),
+                                        ctx.vranks.as<VerifyCand>(), ncand,
+                                        ctx.vok.as<uint8_t>(), stq));
+            std::vector<uint8_t> okv(ncand);
+            MBA_HIP(hipMemcpyAsync(okv.data(), ctx.vok.p, ncand, hipMemcpyDeviceToHost, stq));
+            MBA_HIP(hipStreamSynchronize(stq));
+            for (uint32_t c = 0; c < ncand; ++c)
+              if (okv[c]) {
+                auto &f = S[vc[c].inst].found[vc[c].size];
+                f = std::min(f, vc[c].rank);
+              }
+          }
+        }
       }
+      if (!learned) break;
+      // learned order: any verified solution of this (minimal) size settles
+      if (s >= min_size)
+        for (size_t i = 0; i < count; ++i)
+          if (S[i].active && S[i].found[s] != ~0ull) finish(i, s);
+      if (re >= max_blocks) break;
     }
     // sizes completed by this step: 1..2 after the leaf scan, s for s >= 3
-    const int lo = s == 1 ? 1 : s, hi = s == 1 ? std::min(2, max_size) : s;
     if (s == 0 || s == 2) continue;
+    const int lo = s == 1 ? 1 : s, hi = s == 1 ? std::min(2, max_size) : s;
     for (size_t i = 0; i < count; ++i) {
       InstState &st = S[i];
       if (!st.active) continue;
       for (int z = lo; z <= hi; ++z) {
         st.reached = z;
         if (z >= min_size && st.found[z] != ~0ull) {
-          const int nv = (int)insts[i].n_vars;
-          fill_result(*st.T, z, st.found[z], nv, insts[i].consts, W, results[i]);
-          results[i].seconds = seconds_since(t_start);
-          st.active = false;
+          finish(i, z);
           break;
         }
       }

@@ -72,7 +72,17 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &opts,
                 mba_result &res, const float *block_scores = nullptr);
 int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &opts,
               mba_result &res, mba_vfb_stats *stats);
+// scores: NULL (plain, minimum rank per instance) or count x 256 block
+// scores as for simba_solve (learned order, rounds of 1, 1, 2, 4 ... blocks).
 int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count,
-                const mba_opts &opts, mba_result *results);
+                const mba_opts &opts, mba_result *results, const float *scores = nullptr);
+
+}  // namespace mba
+
+struct mba_ctx {
+  mba::Ctx c;
+};
+
+namespace mba {
 
 }  // namespace mba

@@ -1,0 +1,32 @@
+// mlp_kernels.h -- launch interface of the block-proposal MLP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mba {
+
+constexpr int kMlpIn = 256;
+constexpr int kMlpHidden = 512;
+constexpr int kMlpOut = 128;   // class = op * 16 + min(left size, 15)
+
+// C[M x N] (row stride ldc) = A . B with A(i, k) = A[i*sa_i + k*sa_k] and
+// B(k, j) = B[k*sb_k + j*sb_j]; epilogue: + bias[j], ReLU, or zero where
+// gate[i][j] <= 0 (ReLU backward).  M, N, K multiples of 16.
+struct GemmArgs {
+  const float *A, *B;
+  float *C;
+  int M, N, K;
+  int64_t sa_i, sa_k, sb_k, sb_j, ldc;
+  const float *bias;
+  const float *gate;
+  int relu;
+};
+
+hipError_t launch_gemm(const GemmArgs &g, hipStream_t s);
+hipError_t launch_softmax_ce(const float *Z, const int32_t *labels, int B, int K, float inv_n,
+                             float *dZ, float *loss_rows, hipStream_t s);
+hipError_t launch_colsum(const float *D, int rows, int cols, float *out, hipStream_t s);
+hipError_t launch_adam(float *w, const float *g, float *m, float *v, int n, float lr, float b1,
+                       float b2, float eps, float c1, float c2, hipStream_t s);
+
+}  // namespace mba

@@ -1,0 +1,129 @@
+// mlp_kernels.hip -- the learned block-proposal network on gfx950: a 2-layer
+// MLP (256 -> 512 -> 128), BF16 MFMA operands with FP32 accumulation, and
+// its training kernels (softmax cross-entropy, backward GEMMs, Adam).
+//
+// GEMM: C[M x N] = A[M x K] . B[K x N] with arbitrary element strides, so
+// the transposed products of the backward pass need no copies.  Operands are
+// held in FP32 and rounded to BF16 (round to nearest even) as they are
+// staged into the MFMA registers; accumulation is FP32.  One wave computes
+// one 16 x 16 tile with v_mfma_f32_16x16x16_bf16: per 16-deep K step lane l
+// supplies A[l % 16][k0 + 4(l/16) .. +3] and B[k0 + 4(l/16) .. +3][l % 16],
+// and receives D[4(l/16) + i][l % 16] in its i-th accumulator.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mlp_kernels.h"
+
+namespace mba {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (short)(u >> 16);  // inf / nan
+  u += 0x7fffu + ((u >> 16) & 1u);                                // nearest even
+  return (short)(u >> 16);
+}
+
+__global__ void __launch_bounds__(256)
+k_gemm_bf16(GemmArgs g) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tiles_n = g.N / 16;
+  const int tile = blockIdx.x * 4 + wave;
+  if (tile >= (g.M / 16) * tiles_n) return;
+  const int ti = tile / tiles_n, tj = tile % tiles_n;
+  const int r = lane & 15, q = lane >> 4;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < g.K; k0 += 16) {
+    bf16x4 a, b;
+    for (int e = 0; e < 4; ++e) {
+      const int k = k0 + 4 * q + e;
+      a[e] = f32_to_bf16(g.A[(int64_t)(ti * 16 + r) * g.sa_i + (int64_t)k * g.sa_k]);
+      b[e] = f32_to_bf16(g.B[(int64_t)k * g.sb_k + (int64_t)(tj * 16 + r) * g.sb_j]);
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  }
+  const int col = tj * 16 + r;
+  for (int i = 0; i < 4; ++i) {
+    const int row = ti * 16 + 4 * q + i;
+    float v = acc[i];
+    if (g.bias) v += g.bias[col];
+    if (g.relu) v = v > 0.f ? v : 0.f;
+    if (g.gate) v = g.gate[(int64_t)row * g.ldc + col] > 0.f ? v : 0.f;
+    g.C[(int64_t)row * g.ldc + col] = v;
+  }
+}
+
+// Softmax cross-entropy over rows of Z [B x K]: loss (mean over the rows
+// with label >= 0) and dZ = (softmax - onehot) / n_labelled.
+__global__ void __launch_bounds__(64)
+k_softmax_ce(const float *Z, const int32_t *labels, int B, int K, float inv_n, float *dZ,
+             float *loss_rows) {
+  const int row = blockIdx.x;
+  if (row >= B) return;
+  const float *z = Z + (int64_t)row * K;
+  float m = -3.0e38f;
+  for (int k = threadIdx.x; k < K; k += 64) m = fmaxf(m, z[k]);
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  float s = 0.f;
+  for (int k = threadIdx.x; k < K; k += 64) s += __expf(z[k] - m);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  const int y = labels[row];
+  const float lse = m + __logf(s);
+  for (int k = threadIdx.x; k < K; k += 64) {
+    const float p = __expf(z[k] - lse);
+    dZ[(int64_t)row * K + k] = y >= 0 ? (p - (k == y ? 1.f : 0.f)) * inv_n : 0.f;
+  }
+  if (threadIdx.x == 0) loss_rows[row] = y >= 0 ? (lse - z[y]) : 0.f;
+}
+
+// db[j] = sum_i D[i][j]
+__global__ void k_colsum(const float *D, int rows, int cols, float *out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cols) return;
+  float s = 0.f;
+  for (int i = 0; i < rows; ++i) s += D[(int64_t)i * cols + j];
+  out[j] = s;
+}
+
+// Adam (Kingma & Ba, ICLR 2015) on FP32 master weights.
+__global__ void k_adam(float *w, const float *g, float *m, float *v, int n, float lr,
+                       float b1, float b2, float eps, float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+hipError_t launch_gemm(const GemmArgs &g, hipStream_t s) {
+  if (g.M % 16 || g.N % 16 || g.K % 16) return hipErrorInvalidValue;
+  const int tiles = (g.M / 16) * (g.N / 16);
+  hipLaunchKernelGGL(k_gemm_bf16, dim3((tiles + 3) / 4), dim3(256), 0, s, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_softmax_ce(const float *Z, const int32_t *labels, int B, int K, float inv_n,
+                             float *dZ, float *loss_rows, hipStream_t s) {
+  hipLaunchKernelGGL(k_softmax_ce, dim3(B), dim3(64), 0, s, Z, labels, B, K, inv_n, dZ,
+                     loss_rows);
+  return hipGetLastError();
+}
+
+hipError_t launch_colsum(const float *D, int rows, int cols, float *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_colsum, dim3((cols + 255) / 256), dim3(256), 0, s, D, rows, cols, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam(float *w, const float *g, float *m, float *v, int n, float lr, float b1,
+                       float b2, float eps, float c1, float c2, hipStream_t s) {
+  hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, s, w, g, m, v, n, lr, b1, b2,
+                     eps, c1, c2);
+  return hipGetLastError();
+}
+
+}  // namespace mba

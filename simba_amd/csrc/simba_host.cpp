@@ -401,7 +401,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     const CheckTargets tg = targets_for(t0, s, max_size, W);
     bool stopped = false;
     const bool by_block = !o.exhaustive && (block_scores || T.t[s][8] >= kBlockModeMin);
-    if (s >= min_size && found[s] != ~0ull &&
+    if (!o.exhaustive && s >= min_size && found[s] != ~0ull &&
         (block_scores || found[s] < T.t[s][NOT])) {
       // a unary-rooted solution of this size is already verified: with the
       // learned order any solution of the minimal size is accepted; in rank

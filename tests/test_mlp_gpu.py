@@ -1,0 +1,97 @@
+"""The block-proposal MLP on the GPU: forward and gradients (BF16 MFMA, FP32
+accumulation) against the CPU reference with the same bf16 operand
+rounding; training lowers the loss; the learned search returns verified
+solutions of the plain search's size."""
+import numpy as np
+import pytest
+
+from oracle import mlp_ref
+from tests.gpu_util import note, verify_rpn_chars
+
+pytestmark = pytest.mark.gpu
+
+# Tolerance: operands are rounded identically on both sides; only the FP32
+# accumulation order differs (K <= 512 terms), so 1e-3 relative to the
+# largest magnitude of each tensor bounds the difference.
+RTOL = 1e-3
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import simba_amd
+    c = simba_amd.Context(0)
+    yield c
+    c.close()
+
+
+def _close(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    scale = max(np.abs(b).max(), 1e-6)
+    return np.abs(a - b).max() / scale
+
+
+def test_forward_matches_reference(ctx):
+    import simba_amd
+    note("mlp forward")
+    m = simba_amd.Mlp(ctx, seed=3)
+    rng = np.random.default_rng(1)
+    for batch in (1, 16, 37, 256):
+        x = rng.random((batch, 256), dtype=np.float32)
+        z = m.forward(x)
+        _, zr = mlp_ref.forward(x, *m.get_weights())
+        assert _close(z, zr) < RTOL, batch
+    m.close()
+
+
+def test_gradients_match_reference(ctx):
+    import simba_amd
+    note("mlp grads")
+    m = simba_amd.Mlp(ctx, seed=4)
+    rng = np.random.default_rng(2)
+    x = rng.random((48, 256), dtype=np.float32)
+    y = rng.integers(0, 128, size=48).astype(np.int32)
+    y[5] = -1  # unlabelled row
+    loss, g = m.grads(x, y)
+    lr, gr = mlp_ref.loss_and_grads(x, y, *m.get_weights())
+    assert abs(loss - lr) < 1e-4 * max(1, abs(lr))
+    for a, b in zip(g, gr):
+        assert _close(a, b) < RTOL
+    m.close()
+
+
+def test_training_reduces_loss(ctx):
+    import simba_amd
+    note("mlp train")
+    m = simba_amd.Mlp(ctx, seed=5)
+    rng = np.random.default_rng(3)
+    x = rng.random((64, 256), dtype=np.float32)
+    y = (np.arange(64) % 10).astype(np.int32)
+    l0 = m.train_step(x, y, 1e-3)
+    for _ in range(60):
+        l1 = m.train_step(x, y, 1e-3)
+    assert l1 < 0.5 * l0
+    m.close()
+
+
+def test_learned_solve_is_minimal_size_and_verified(ctx):
+    import simba_amd
+    from simba_amd.synth import make_instance
+    m = simba_amd.Mlp(ctx, seed=6)
+    for seed in range(12):
+        inst = make_instance(7000 + seed, 2 + seed % 3, 32, 4 + seed % 4, 64)
+        note(f"learned solve seed={seed}")
+        plain = ctx.simba(inst, max_size=8)
+        got = m.solve(inst, max_size=8)
+        assert got["status"] == plain["status"]
+        if plain["status"] == 0:
+            assert got["size"] == plain["size"]
+            assert verify_rpn_chars(got["rpn"], inst)
+    insts = [make_instance(7100 + s, 3, 32, 6, 64) for s in range(16)]
+    note("learned batch solve")
+    got = m.batch_solve(insts, max_size=8)
+    for inst, g in zip(insts, got):
+        p = ctx.simba(inst, max_size=8)
+        assert g["status"] == p["status"]
+        if p["status"] == 0:
+            assert g["size"] == p["size"] and verify_rpn_chars(g["rpn"], inst)
+    m.close()
