@@ -181,12 +181,128 @@ def run_vfb(args):
     return out
 
 
+def _key(r):
+    return (r["status"], r["size"], r["rank"], r["expr"])
+
+
+def _verified(r, inst):
+    """Re-evaluates a returned SIMBA RPN on every sample (product evaluator)."""
+    from simba_amd.synth import TOK_OP, evaluate
+    L = inst.n_variables + len(inst.consts)
+    toks = [ord(c) - 48 if c.isdigit() and ord(c) - 48 < L else TOK_OP + "~&|^#+-*".index(c)
+            for c in r["rpn"]]
+    for i in range(inst.n_examples):
+        lv = [int(inst.inputs[i * inst.n_variables + v]) for v in range(inst.n_variables)]
+        lv += [int(c) for c in inst.consts]
+        if evaluate(toks, lv, inst.width) != int(inst.outputs[i]):
+            return False
+    return True
+
+
+def run_batch(args):
+    """4096 instances (vars 2..5, target size 5..17, samples 16..64) solved by
+    one batched call vs one plain call per instance, same size budget."""
+    import simba_amd
+    from simba_amd.learn import sample_instances
+
+    insts = sample_instances(args.seed, args.count, (5, 17), (2, 5), (16, 64))
+    ctx = simba_amd.Context(0)
+    kw = dict(max_size=args.budget_size)
+    ctx.batch(insts[:64], **kw)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rb = ctx.batch(insts, **kw)
+    t_batch = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    rp = [ctx.simba(x, **kw) for x in insts]
+    t_plain = time.perf_counter() - t0
+    solved = sum(r["status"] == 0 for r in rb)
+    exact = sum(_key(a) == _key(b) for a, b in zip(rb, rp)) / len(insts)
+    ctx.close()
+    return {
+        "metric": "batched solves/s", "value": round(solved / t_batch, 2), "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": round(1e3 * t_batch, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic",
+        "config": {"workload": "batch_%d_vars2to5_size5to17_samples16to64_budget%d"
+                               % (args.count, args.budget_size),
+                   "solved": solved, "exact_match_vs_plain": exact,
+                   "plain_per_instance_solves_per_s": round(solved / t_plain, 2),
+                   "plain_seconds": round(t_plain, 3),
+                   "speedup_vs_plain": round(t_plain / t_batch, 2)},
+    }
+
+
+def run_learned(args):
+    """Learned block proposal: trained on plain-solved instances (sizes 3..13,
+    2..4 vars, 32 samples), evaluated on held-out instances (sizes 9..19)."""
+    import numpy as np
+
+    import simba_amd
+    from simba_amd import learn
+
+    ctx = simba_amd.Context(0)
+    t0 = time.perf_counter()
+    X, Y = learn.build_dataset(ctx, args.train, args.seed, max_size=args.label_size)
+    t_label = time.perf_counter() - t0
+    mlp = simba_amd.Mlp(ctx, seed=args.seed)
+    untrained = simba_amd.Mlp(ctx, seed=args.seed + 1)
+    t0 = time.perf_counter()
+    losses = learn.train(mlp, X, Y, epochs=args.epochs)
+    t_train = time.perf_counter() - t0
+    ev = learn.sample_instances(args.seed + 10**6, args.count, (9, 19), (2, 4), (32, 32))
+    kw = dict(max_size=args.budget_size)
+    mlp.solve(ev[0], **kw)
+    plain = [ctx.simba(x, **kw) for x in ev]
+    lrn = [mlp.solve(x, **kw) for x in ev]
+    unt = [untrained.solve(x, **kw) for x in ev]
+    solved = [i for i, r in enumerate(plain) if r["status"] == 0]
+    same_size = all(lrn[i]["status"] == 0 and lrn[i]["size"] == plain[i]["size"] for i in solved)
+    verified = all(_verified(lrn[i], ev[i]) for i in solved)
+    tp = np.array([plain[i]["seconds"] for i in solved])
+    tl = np.array([lrn[i]["seconds"] for i in solved])
+    tu = np.array([unt[i]["seconds"] for i in solved])
+    t0 = time.perf_counter()
+    bl = mlp.batch_solve(ev, **kw)
+    t_bl = time.perf_counter() - t0
+    b_solved = sum(r["status"] == 0 for r in bl)
+    b_same = all(bl[i]["status"] == 0 and bl[i]["size"] == plain[i]["size"] for i in solved)
+    ctx.close()
+    pct = lambda a, q: round(float(np.percentile(a, q)) * 1e3, 3) if len(a) else None
+    plain_rate = len(solved) / tp.sum() if len(solved) else 0.0
+    return {
+        "metric": "learned vs plain p50 time-to-solution", "value": round(
+            float(np.percentile(tp, 50) / np.percentile(tl, 50)), 3) if len(solved) else None,
+        "unit": "x (plain p50 / learned p50)", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": pct(tl, 50), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64 search, bf16 MFMA MLP", "data": "synthetic",
+        "config": {"workload": "learned_train%d_eval%d_sizes9to19_budget%d"
+                               % (args.train, args.count, args.budget_size),
+                   "train_examples": int(len(Y)), "label_seconds": round(t_label, 2),
+                   "train_seconds": round(t_train, 2), "final_loss": round(losses[-1], 4),
+                   "solved_plain": len(solved), "same_size": same_size,
+                   "verified": verified,
+                   "p50_ms": {"plain": pct(tp, 50), "learned": pct(tl, 50),
+                              "untrained": pct(tu, 50)},
+                   "p95_ms": {"plain": pct(tp, 95), "learned": pct(tl, 95),
+                              "untrained": pct(tu, 95)},
+                   "batched_learned_solves_per_s": round(b_solved / t_bl, 2),
+                   "plain_per_instance_solves_per_s": round(plain_rate, 2),
+                   "batched_learned_same_size": b_same},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="simba", choices=["simba", "vfb"])
+    ap.add_argument("--workload", default="simba", choices=["simba", "vfb", "batch", "learned"])
+    ap.add_argument("--count", type=int, default=4096)
+    ap.add_argument("--budget-size", type=int, default=9)
+    ap.add_argument("--train", type=int, default=200000)
+    ap.add_argument("--label-size", type=int, default=9)
+    ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--size", type=int, default=14)
     ap.add_argument("--cpu-size", type=int, default=9)
     ap.add_argument("--table-gb", type=int, default=40)
@@ -199,7 +315,8 @@ def main():
         print("bench.py measures one GPU: every box here has a single MI355X; "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    out = run_simba(args) if args.workload == "simba" else run_vfb(args)
+    out = {"simba": run_simba, "vfb": run_vfb, "batch": run_batch,
+           "learned": run_learned}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

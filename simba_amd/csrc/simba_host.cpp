@@ -279,7 +279,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
       MBA_HIP(hipMemcpyAsync(ctx.tables.p, v1w.data(), 4 * L, hipMemcpyHostToDevice, st));
   }
   for (int m = 2; m <= Kt; ++m)
-    MBA_HIP(launch_table_build(W, d_tv, ctx.tables.p, tv.off[m], m, T.t[m][8], st));
+    MBA_HIP(launch_table_build(W, ctx.tables.p, tv.off, T, m, st));
 
   // ---- size sweep --------------------------------------------------------
   std::vector<uint64_t> found(kMaxSize + 2, ~0ull);

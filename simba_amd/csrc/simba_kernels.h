@@ -47,6 +47,7 @@ constexpr int kPairThreads = 256;
 constexpr int kPairRB = 4;                      // B values held per lane
 constexpr int kPairTB = kPairThreads * kPairRB; // B values per tile
 constexpr int kPairTA = 256;                    // A values per tile
+constexpr int kAGroup = 8;                      // A values loaded per group
 
 // ---- batched search --------------------------------------------------------
 constexpr int kBuildTile = 1024;  // table values per build tile
@@ -93,8 +94,9 @@ hipError_t launch_verify_batch(int width, const TableView *d_tvs, const uint64_t
                                hipStream_t stream);
 
 // All launches are asynchronous on `stream`.
-hipError_t launch_table_build(int width, const TableView *d_tv, void *d_base,
-                              uint64_t off, int m, uint64_t n, hipStream_t stream);
+// V[m] from V[<m], one launch per (op, left size) block of size m.
+hipError_t launch_table_build(int width, void *d_base, const uint64_t *off, const CountTable &T,
+                              int m, hipStream_t stream);
 // d_base/off: the table buffer and the host copy of TableView::off.
 hipError_t launch_pair_check(int width, const TableView *d_tv, const void *d_base,
                              const uint64_t *off, const PairLaunch &p,

@@ -29,10 +29,16 @@ struct WordOf;
 // from V[size]; then evaluate the remaining skeleton in postfix order.
 // ---------------------------------------------------------------------------
 template <typename word>
+__device__ __forceinline__ word build_value(const TableView *__restrict__ tv, int m,
+                                            uint64_t r);
+
+template <typename word>
 __device__ word value_of(const TableView *__restrict__ tv, int m, uint64_t r) {
   const word *V = (const word *)tv->base;
   const int Kt = tv->Kt;
   if (m <= Kt) return V[tv->off[m] + r];
+  // one level above the tables: the root's children are tabulated
+  if (m - 1 <= Kt) return build_value<word>(tv, m, r);
   uint8_t tok[kMaxSize];     // 0xFF: inside a tabulated subtree
   uint64_t ref[kMaxSize];    // table index for a 0x40|size token
   struct Item { uint64_t r; int s, pos; };
@@ -120,6 +126,29 @@ k_table_build(const TableView *__restrict__ tv, word *__restrict__ out, int m,
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
        r += stride)
     out[r] = build_value<word>(tv, m, r);
+}
+
+// One (op, left size) block of V[m]: out[a * nB + b] = op(VA[a], VB[b]) for
+// a < nA, b < nB (unary blocks: nB = 1, VB unused).  A tile is 1024 outputs,
+// 2^logTB columns by 1024 >> logTB rows, so lanes need no division and the
+// stores of a wave are contiguous.
+template <typename word>
+__global__ void __launch_bounds__(256)
+k_table_block(const word *__restrict__ VA, const word *__restrict__ VB, word *__restrict__ out,
+              int op, uint64_t nA, uint64_t nB, int logTB, uint64_t nBT, uint64_t ntiles) {
+  const uint32_t TB = 1u << logTB;
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint64_t at = t / nBT, bt = t - at * nBT;
+    for (uint32_t k = threadIdx.x; k < 1024; k += 256) {
+      const uint64_t a = at * (1024 >> logTB) + (k >> logTB);
+      const uint64_t b = bt * TB + (k & (TB - 1));
+      if (a < nA && b < nB) {
+        const word x = VA[a];
+        out[a * nB + b] = is_unary(op) ? apply_op<word>(op, x, word(0))
+                                       : apply_op<word>(op, x, VB[b]);
+      }
+    }
+  }
 }
 
 template <typename word>
@@ -281,7 +310,7 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
   const uint32_t thr = sizeof(word) == 8 ? 1u : (uint32_t)span;
   const uint32_t pair = ((uint32_t)p.sb << 16) | (uint32_t)p.j;
   {
-    const uint64_t at = t % p.nTA, bt = t / p.nTA;
+    const uint64_t at = p.nTA == 1 ? 0 : t % p.nTA, bt = p.nTA == 1 ? t : t / p.nTA;
     const uint64_t a0 = at * kPairTA;
     const uint64_t a1 = a0 + kPairTA < p.nA ? a0 + kPairTA : p.nA;
     const uint64_t b0 = bt * kPairTB + threadIdx.x;
@@ -308,7 +337,23 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
     uint32_t acc = 0xffffffffu;
     const uint32_t full = SAME ? 0x3fu : 0x7fu;
     if ((p.mask & full) == full) {
-      for (uint64_t a = a0; a < a1; ++a) {
+      uint64_t a = a0;
+      // A values in groups of kAGroup: their scalar loads are issued together
+      // and waited for once
+      for (; a + kAGroup <= a1; a += kAGroup) {
+        word av[kAGroup];
+#pragma unroll
+        for (int i = 0; i < kAGroup; ++i)
+          av[i] = GEN_A ? value_of<word>(tv, p.j, a + i) : A[a + i];
+#pragma unroll
+        for (int i = 0; i < kAGroup; ++i) {
+          const word na = uniform(word(word(0) - av[i]));
+#pragma unroll
+          for (int k = 0; k < kPairRB; ++k)
+            acc = fold7<word, SAME>(acc, av[i], na, pre[k], de, dof);
+        }
+      }
+      for (; a < a1; ++a) {
         const word av = GEN_A ? value_of<word>(tv, p.j, a) : A[a];
         const word na = uniform(word(word(0) - av));
 #pragma unroll
@@ -358,6 +403,17 @@ k_pair_check(const TableView *__restrict__ tv, const word *__restrict__ A,
              const word *__restrict__ B, PairLaunch p, uint64_t lo_e64, uint64_t lo_o64,
              uint64_t span64, HitRec *__restrict__ hits, uint32_t *__restrict__ hit_count,
              uint32_t hit_cap) {
+  if (!GEN_A && p.nTA == 1) {
+    // the whole A side fits one tile: stage it in LDS once per workgroup
+    // and reuse it for every B tile this workgroup takes
+    __shared__ word sA[kPairTA];
+    for (int i = threadIdx.x; i < (int)p.nA; i += blockDim.x) sA[i] = A[i];
+    __syncthreads();
+    for (uint64_t t = p.tile_begin + blockIdx.x; t < p.tile_end; t += gridDim.x)
+      pair_tile<word, SAME, GEN_A, GEN_B>(tv, sA, B, p, t, (word)lo_e64, (word)lo_o64,
+                                          (word)span64, hits, hit_count, hit_cap, 0u);
+    return;
+  }
   for (uint64_t t = p.tile_begin + blockIdx.x; t < p.tile_end; t += gridDim.x)
     pair_tile<word, SAME, GEN_A, GEN_B>(tv, A, B, p, t, (word)lo_e64, (word)lo_o64,
                                         (word)span64, hits, hit_count, hit_cap, 0u);
@@ -609,19 +665,150 @@ k_verify_batch(const TableView *__restrict__ tvs, const uint64_t *__restrict__ l
 }
 
 // ---------------------------------------------------------------------------
+// Wavefront verification: one wave per candidate, one lane per sample.  Lane
+// 0 unranks the candidate into RPN in LDS (the reference's numberToFormula
+// order, nocache.cu:23-83); every lane then evaluates the RPN on its sample
+// with an LDS value stack (mba.cuh:11-66 semantics), the samples' leaf values
+// staged in LDS, and a wave ballot ends the check at the first chunk of
+// samples with a mismatch.
+// ---------------------------------------------------------------------------
+constexpr int kVerifyDepth = kMaxSize / 2 + 2;  // max RPN stack depth
+
+template <typename word>
+__device__ void verify_wave(const TableView *__restrict__ tv, const uint64_t *__restrict__ lv,
+                            const uint64_t *__restrict__ outv, int n_samples, int size,
+                            uint64_t rank, uint8_t *__restrict__ ok_out) {
+  __shared__ word s_leaf[kMaxSamples * kMaxLeaves];
+  __shared__ word s_out[kMaxSamples];
+  __shared__ word s_stack[kVerifyDepth][64];
+  __shared__ uint8_t s_rpn[kMaxSize];
+  __shared__ uint64_t s_wr[kMaxSize + 1];
+  __shared__ int s_ws[kMaxSize + 1], s_wp[kMaxSize + 1];
+  const int lane = threadIdx.x;
+  const int L = tv->L;
+  for (int i = lane; i < n_samples * L; i += 64) s_leaf[i] = (word)lv[i];
+  for (int i = lane; i < n_samples; i += 64) s_out[i] = (word)outv[i];
+  if (lane == 0) {
+    int sp = 0;
+    s_wr[0] = rank;
+    s_ws[0] = size;
+    s_wp[0] = 0;
+    sp = 1;
+    while (sp > 0) {
+      --sp;
+      uint64_t n = s_wr[sp];
+      const int s = s_ws[sp], pos = s_wp[sp];
+      if (s == 1) {
+        s_rpn[pos] = (uint8_t)n;
+        continue;
+      }
+      int op = 0;
+      while (n >= tv->T[s][op]) n -= tv->T[s][op++];
+      const int cs = s - 1;
+      s_rpn[pos + cs] = (uint8_t)(kTokOp + op);
+      if (is_unary(op)) {
+        s_wr[sp] = n, s_ws[sp] = cs, s_wp[sp] = pos, ++sp;
+        continue;
+      }
+      int j = 1;
+      for (;; ++j) {
+        const uint64_t blk = tv->T[j][8] * tv->T[cs - j][8];
+        if (n < blk) break;
+        n -= blk;
+      }
+      const uint64_t rs = tv->T[cs - j][8];
+      const uint64_t a = n / rs;
+      s_wr[sp] = a, s_ws[sp] = j, s_wp[sp] = pos, ++sp;
+      s_wr[sp] = n - a * rs, s_ws[sp] = cs - j, s_wp[sp] = pos + j, ++sp;
+    }
+  }
+  __syncthreads();
+  bool good = true;
+  for (int base = 0; base < n_samples; base += 64) {
+    const int smp = base + lane;
+    bool mismatch = false;
+    if (smp < n_samples) {
+      int top = -1;
+      for (int i = 0; i < size; ++i) {
+        const uint8_t t = s_rpn[i];
+        if (t < kTokOp) {
+          s_stack[++top][lane] = s_leaf[smp * L + t];
+        } else {
+          const int op = t - kTokOp;
+          if (is_unary(op)) {
+            s_stack[top][lane] = apply_op<word>(op, s_stack[top][lane], word(0));
+          } else {
+            const word rv = s_stack[top--][lane];
+            s_stack[top][lane] = apply_op<word>(op, s_stack[top][lane], rv);
+          }
+        }
+      }
+      mismatch = s_stack[0][lane] != s_out[smp];
+    }
+    if (__any(mismatch)) {
+      good = false;
+      break;
+    }
+  }
+  if (lane == 0) *ok_out = good ? 1 : 0;
+}
+
+template <typename word>
+__global__ void __launch_bounds__(64)
+k_verify_wave(const TableView *__restrict__ tv, const uint64_t *__restrict__ leafvals,
+              const uint64_t *__restrict__ outputs, int n_samples,
+              const uint32_t *__restrict__ sizes, const uint64_t *__restrict__ ranks,
+              uint8_t *__restrict__ ok) {
+  const uint32_t c = blockIdx.x;
+  verify_wave<word>(tv, leafvals, outputs, n_samples, (int)sizes[c], ranks[c], ok + c);
+}
+
+template <typename word>
+__global__ void __launch_bounds__(64)
+k_verify_batch_wave(const TableView *__restrict__ tvs, const uint64_t *__restrict__ leafvals,
+                    const uint64_t *__restrict__ outputs,
+                    const BatchInstData *__restrict__ idat,
+                    const VerifyCand *__restrict__ cand, uint8_t *__restrict__ ok) {
+  const VerifyCand vc = cand[blockIdx.x];
+  const BatchInstData d = idat[vc.inst];
+  verify_wave<word>(tvs + vc.inst, leafvals + d.leaf_off, outputs + d.out_off,
+                    (int)d.n_samples, (int)vc.size, vc.rank, ok + blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers.
 // ---------------------------------------------------------------------------
-hipError_t launch_table_build(int width, const TableView *d_tv, void *d_base,
-                              uint64_t off, int m, uint64_t n, hipStream_t stream) {
-  uint64_t blocks = (n + 255) / 256;
-  int grid = (int)(blocks < 65536 ? blocks : 65536);
-  if (grid < 1) grid = 1;
+template <typename word>
+static void table_blocks(word *base, const uint64_t *off, const CountTable &T, int m,
+                         int max_grid, hipStream_t stream) {
+  uint64_t r = 0;
+  for (int op = 0; op < 8; ++op) {
+    const bool un = is_unary(op);
+    const int last = un ? 1 : (op == SUB ? m - 2 : (m - 1) / 2);
+    for (int j = 1; j <= last; ++j) {
+      const int jl = un ? m - 1 : j, jr = un ? 0 : m - 1 - j;
+      const uint64_t nA = T.t[jl][8], nB = un ? 1 : T.t[jr][8];
+      if (nA * nB == 0) continue;
+      int logTB = 0;
+      while ((1ull << logTB) < nB && logTB < 10) ++logTB;
+      const uint64_t nBT = (nB + (1ull << logTB) - 1) >> logTB;
+      const uint64_t nAT = (nA + (1024 >> logTB) - 1) / (1024 >> logTB);
+      const uint64_t ntiles = nAT * nBT;
+      const int grid = (int)(ntiles < (uint64_t)max_grid ? ntiles : (uint64_t)max_grid);
+      hipLaunchKernelGGL(k_table_block<word>, dim3(grid), dim3(256), 0, stream,
+                         base + off[jl], un ? base : base + off[jr], base + off[m] + r, op, nA,
+                         nB, logTB, nBT, ntiles);
+      r += nA * nB;
+    }
+  }
+}
+
+hipError_t launch_table_build(int width, void *d_base, const uint64_t *off, const CountTable &T,
+                              int m, hipStream_t stream) {
   if (width == 32)
-    hipLaunchKernelGGL(k_table_build<uint32_t>, dim3(grid), dim3(256), 0, stream, d_tv,
-                       (uint32_t *)d_base + off, m, n);
+    table_blocks<uint32_t>((uint32_t *)d_base, off, T, m, 8192, stream);
   else
-    hipLaunchKernelGGL(k_table_build<uint64_t>, dim3(grid), dim3(256), 0, stream, d_tv,
-                       (uint64_t *)d_base + off, m, n);
+    table_blocks<uint64_t>((uint64_t *)d_base, off, T, m, 8192, stream);
   return hipGetLastError();
 }
 
@@ -699,13 +886,12 @@ hipError_t launch_verify_batch(int width, const TableView *d_tvs, const uint64_t
                                const VerifyCand *d_cand, uint32_t n_cand, uint8_t *d_ok,
                                hipStream_t stream) {
   if (n_cand == 0) return hipSuccess;
-  const int grid = (int)((n_cand + 255) / 256);
   if (width == 32)
-    hipLaunchKernelGGL(k_verify_batch<uint32_t>, dim3(grid), dim3(256), 0, stream, d_tvs,
-                       d_leafvals, d_outputs, d_idat, d_cand, n_cand, d_ok);
+    hipLaunchKernelGGL(k_verify_batch_wave<uint32_t>, dim3(n_cand), dim3(64), 0, stream, d_tvs,
+                       d_leafvals, d_outputs, d_idat, d_cand, d_ok);
   else
-    hipLaunchKernelGGL(k_verify_batch<uint64_t>, dim3(grid), dim3(256), 0, stream, d_tvs,
-                       d_leafvals, d_outputs, d_idat, d_cand, n_cand, d_ok);
+    hipLaunchKernelGGL(k_verify_batch_wave<uint64_t>, dim3(n_cand), dim3(64), 0, stream, d_tvs,
+                       d_leafvals, d_outputs, d_idat, d_cand, d_ok);
   return hipGetLastError();
 }
 
@@ -714,13 +900,12 @@ hipError_t launch_verify(int width, const TableView *d_tv, const uint64_t *d_lea
                          const uint32_t *d_sizes, const uint64_t *d_ranks,
                          uint32_t n_cand, uint8_t *d_ok, hipStream_t stream) {
   if (n_cand == 0) return hipSuccess;
-  const int grid = (int)((n_cand + 255) / 256);
   if (width == 32)
-    hipLaunchKernelGGL(k_verify<uint32_t>, dim3(grid), dim3(256), 0, stream, d_tv,
-                       d_leafvals, d_outputs, n_samples, d_sizes, d_ranks, n_cand, d_ok);
+    hipLaunchKernelGGL(k_verify_wave<uint32_t>, dim3(n_cand), dim3(64), 0, stream, d_tv,
+                       d_leafvals, d_outputs, n_samples, d_sizes, d_ranks, d_ok);
   else
-    hipLaunchKernelGGL(k_verify<uint64_t>, dim3(grid), dim3(256), 0, stream, d_tv,
-                       d_leafvals, d_outputs, n_samples, d_sizes, d_ranks, n_cand, d_ok);
+    hipLaunchKernelGGL(k_verify_wave<uint64_t>, dim3(n_cand), dim3(64), 0, stream, d_tv,
+                       d_leafvals, d_outputs, n_samples, d_sizes, d_ranks, d_ok);
   return hipGetLastError();
 }
 

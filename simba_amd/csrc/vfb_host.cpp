@@ -84,7 +84,9 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
   uint64_t cap = std::max<uint64_t>(budget / row_bytes, (uint64_t)L + 1);
   cap = std::min<uint64_t>(cap, (1ull << 31) - 1);  // int32 child indices
   uint64_t slots = 1ull << 16;
-  while (slots < 4 * cap && slots < (1ull << 32)) slots <<= 1;
+  // the table also holds the new fingerprints of the last, hash-only level
+  // (about 6x the previous level's rows): 8 slots per stored row
+  while (slots < 8 * cap && slots < (1ull << 32)) slots <<= 1;
   S.table_slots = slots;
   S.rows_capacity = cap;
 
