@@ -266,6 +266,68 @@ __device__ __forceinline__ uint64_t uniform(uint64_t x) {
          __builtin_amdgcn_readfirstlane((uint32_t)x);
 }
 
+// Values of V[m] for m = Kt + 1 (children tabulated) at the B indices of a
+// tile: the tile's first index `first` is decoded once (op block, split,
+// left/right rank -- the same for every lane), and each lane steps from it:
+// inside one (op, split) block whose right side has at least a tile's worth
+// of trees, the lane's right rank is first's plus its offset with at most
+// one carry into the left rank.  Lanes past that block decode on their own.
+template <typename word>
+__device__ __forceinline__ void gen_b_tile(const TableView *__restrict__ tv, int m,
+                                           uint64_t first, uint64_t lane_b0, word *bv,
+                                           const bool *ok) {
+  const word *V = (const word *)tv->base;
+  uint64_t x = first;
+  int op = 0;
+  while (x >= tv->T[m][op]) x -= tv->T[m][op++];
+  const bool un = is_unary(op);
+  int j = 0;
+  uint64_t rs = 1, a0 = 0, r0 = 0, rem;
+  if (un) {
+    rem = tv->T[m][op] - x;
+  } else {
+    const int cs = m - 1;
+    j = 1;
+    for (;; ++j) {
+      const uint64_t blk = tv->T[j][8] * tv->T[cs - j][8];
+      if (x < blk) {
+        rem = blk - x;
+        break;
+      }
+      x -= blk;
+    }
+    rs = tv->T[cs - j][8];
+    a0 = x / rs;
+    r0 = x - a0 * rs;
+  }
+  const bool step_ok = un || rs >= (uint64_t)kPairTB;
+  const word *VA = V + tv->off[un ? m - 1 : j];
+  const word *VB = V + tv->off[un ? 0 : m - 1 - j];
+#pragma unroll
+  for (int k = 0; k < kPairRB; ++k) {
+    const uint64_t b = lane_b0 + (uint64_t)k * kPairThreads;
+    if (!ok[k]) {
+      bv[k] = word(0);
+      continue;
+    }
+    const uint64_t off = b - first;
+    if (step_ok && off < rem) {
+      if (un) {
+        bv[k] = apply_op<word>(op, VA[x + off], word(0));
+      } else {
+        uint64_t rb = r0 + off, ra = a0;
+        if (rb >= rs) {
+          rb -= rs;
+          ra += 1;
+        }
+        bv[k] = apply_op<word>(op, VA[ra], VB[rb]);
+      }
+    } else {
+      bv[k] = build_value<word>(tv, m, b);
+    }
+  }
+}
+
 // One operator instance over the A tile: its distances folded into acc.
 template <typename word, int INST, bool GEN_A>
 __device__ __forceinline__ uint32_t fold_inst(uint32_t acc, const TableView *__restrict__ tv,
@@ -322,7 +384,16 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
       const uint64_t b = b0 + (uint64_t)k * kPairThreads;
       ok[k] = b < p.nB;
       any_ok |= ok[k];
-      bv[k] = ok[k] ? (GEN_B ? value_of<word>(tv, p.m, b) : B[b]) : word(0);
+      if (!GEN_B) bv[k] = ok[k] ? B[b] : word(0);
+    }
+    if (GEN_B) {
+      if (p.m == tv->Kt + 1) {
+        gen_b_tile<word>(tv, p.m, bt * kPairTB, b0, bv, ok);
+      } else {
+#pragma unroll
+        for (int k = 0; k < kPairRB; ++k)
+          bv[k] = ok[k] ? value_of<word>(tv, p.m, b0 + (uint64_t)k * kPairThreads) : word(0);
+      }
     }
     // lanes past the end of B reuse a valid value of their own (recorded
     // hits are re-derived with the exact masks below)

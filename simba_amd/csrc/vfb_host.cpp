@@ -95,12 +95,11 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
   MBA_HIP(ctx.vfb_left.ensure(cap * 4));
   MBA_HIP(ctx.vfb_right.ensure(cap * 4));
   MBA_HIP(ctx.vfb_flags.ensure(cap));  // row ops
-  MBA_HIP(ctx.vfb_keys.ensure(slots * 8));
-  MBA_HIP(ctx.vfb_idx.ensure(slots * 8));
+  // slot s = {key at [2s], companion at [2s+1]}: one line per probe
+  MBA_HIP(ctx.vfb_keys.ensure(slots * 16));
   MBA_HIP(ctx.vfb_target.ensure(n * 8 + 64));
   MBA_HIP(ctx.vfb_misc.ensure(4096));
-  MBA_HIP(vfb_fill_u64(ctx.vfb_keys.as<uint64_t>(), slots, kVfbEmpty, st));
-  MBA_HIP(vfb_fill_u64(ctx.vfb_idx.as<uint64_t>(), slots, ~0ull, st));
+  MBA_HIP(vfb_fill_u64(ctx.vfb_keys.as<uint64_t>(), 2 * slots, kVfbEmpty, st));  // = ~0
   {  // leaves -> rows 0..L-1 (SoA), target words
     std::vector<uint64_t> col(L);
     std::vector<uint32_t> col32(L);
@@ -132,7 +131,7 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
   a.cap = cap;
   a.n = n;
   a.keys = ctx.vfb_keys.as<uint64_t>();
-  a.minidx = ctx.vfb_idx.as<uint64_t>();
+  a.minidx = ctx.vfb_keys.as<uint64_t>() + 1;  // companion words, stride 2
   a.mask = slots - 1;
   a.target = ctx.vfb_target.p;
   a.found = d_found;
@@ -195,6 +194,34 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
       ctx.last_error = "too many enumeration blocks in one level";
       return MBA_ERR_LIMIT;
     }
+    // row-pair view of the same blocks (one per split, plus the unary rows)
+    std::vector<VfbPairBlock> pblocks;
+    uint64_t npairs = 0;
+    for (const VfbBlock &B : blocks) {
+      const bool un = is_unary(B.op);
+      VfbPairBlock *pb = nullptr;
+      for (VfbPairBlock &x : pblocks)
+        if (x.unary == (int)un && x.l0 == B.l0 && x.nl == B.nl && x.r0 == B.r0 && x.nr == B.nr)
+          pb = &x;
+      if (!pb) {
+        VfbPairBlock nb;
+        std::memset(&nb, 0, sizeof nb);
+        nb.pbegin = npairs;
+        nb.l0 = B.l0;
+        nb.nl = B.nl;
+        nb.r0 = B.r0;
+        nb.nr = B.nr;
+        nb.unary = un;
+        npairs += un ? (uint64_t)B.nl : (uint64_t)B.nl * B.nr;
+        pblocks.push_back(nb);
+        pb = &pblocks.back();
+      }
+      pb->opbegin[B.op] = B.begin;
+    }
+    if ((int)pblocks.size() > kVfbMaxPairBlocks) {
+      ctx.last_error = "too many row-pair blocks in one level";
+      return MBA_ERR_LIMIT;
+    }
     if (total == 0) {
       lstart[lev + 1] = rows;
       reached = lev;
@@ -212,6 +239,9 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
     MBA_HIP(hipMemcpyAsync(ctx.b_inst.p, blocks.data(), sizeof(VfbBlock) * blocks.size(),
                            hipMemcpyHostToDevice, st));
     a.blocks = ctx.b_inst.as<VfbBlock>();
+    MBA_HIP(ctx.b_results.ensure(sizeof(VfbPairBlock) * pblocks.size() + 64));
+    MBA_HIP(hipMemcpyAsync(ctx.b_results.p, pblocks.data(),
+                           sizeof(VfbPairBlock) * pblocks.size(), hipMemcpyHostToDevice, st));
     a.nblocks = (int)blocks.size();
     a.total = total;
     a.tag = (uint64_t)lev << 48;
@@ -223,7 +253,12 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
     const int g = (int)std::min<uint64_t>((uint64_t)grid, nchunks);
     MBA_HIP(hipMemsetAsync(d_found, 0xff, 8, st));
     MBA_HIP(hipEventRecord(ctx.ev0, st));
-    MBA_HIP(vfb_launch_insert(W, a, g, st));
+    {
+      const uint64_t pgroups = (npairs + kVfbThreads - 1) / kVfbThreads;
+      const int gp = (int)std::min<uint64_t>((uint64_t)grid, std::max<uint64_t>(pgroups, 1));
+      MBA_HIP(vfb_launch_insert_pairs(W, a, ctx.b_results.as<VfbPairBlock>(),
+                                      (int)pblocks.size(), npairs, gp, st));
+    }
     MBA_HIP(vfb_launch_lookup_target(W, a, d_tmin, st));
     uint64_t cells[4];
     MBA_HIP(hipMemcpyAsync(cells, d_misc, 32, hipMemcpyDeviceToHost, st));
@@ -267,7 +302,10 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
     ktime += ms * 1e-3;
     S.level_new[lev] = kept;
     S.level_seconds[lev] = seconds_since(t_lev);
-    S.level_bytes[lev] = total * 8 + kept * (8 + n * wb + 8) + stream_bytes;
+    // algorithmic bytes: one slot read per candidate, a key write per new
+    // fingerprint, row + child indices per stored row, child rows streamed
+    // once per block (the last level stores no rows)
+    S.level_bytes[lev] = total * 8 + kept * 8 + (last ? 0 : kept * (n * wb + 8)) + stream_bytes;
     if (o.verbose)
       std::fprintf(stderr, "Length %-2d | candidates %-14llu | new %-12llu | %.3f s\n", lev,
                    (unsigned long long)total, (unsigned long long)kept,

@@ -31,8 +31,9 @@ struct VfbLevelArgs {
   int nblocks;
   const VfbBlock *blocks; // device copy
   uint64_t total;         // candidates of the level
-  uint64_t *keys;         // hash table keys (fingerprints)
-  uint64_t *minidx;       // per slot: (level << 48) | min candidate index; a
+  uint64_t *keys;         // hash slots, 16 B each: key (fingerprint) at [2s]
+  uint64_t *minidx;       // = keys + 1: companion of slot s at minidx[2s],
+                          // (level << 48) | min candidate index; a
                           // key of an earlier level keeps a smaller value
   uint64_t tag;           // level << 48 of the level being processed
   uint64_t mask;          // table capacity - 1 (power of two)
@@ -51,7 +52,22 @@ struct VfbLevelArgs {
   unsigned long long *probes;  // total probe steps (statistics)
 };
 
+// The row pairs of a level: the unary block (rows of level L-1, ops ~ and
+// neg) and one block per split i (left rows of level i x right rows of level
+// L-1-i, every binary op).  opbegin[op] = first candidate index of that op's
+// block for this split (the order of full-cache.cu:568-663).
+constexpr int kVfbMaxPairBlocks = 24;
+struct VfbPairBlock {
+  uint64_t pbegin;       // first pair index of the block
+  uint64_t opbegin[8];
+  uint32_t l0, nl, r0, nr;
+  int32_t unary;
+  int32_t pad;
+};
+
 hipError_t vfb_launch_insert(int width, const VfbLevelArgs &a, int grid, hipStream_t s);
+hipError_t vfb_launch_insert_pairs(int width, const VfbLevelArgs &a, const VfbPairBlock *d_pb,
+                                   int npb, uint64_t npairs, int grid, hipStream_t s);
 hipError_t vfb_launch_flag(int width, const VfbLevelArgs &a, int grid, hipStream_t s);
 hipError_t vfb_launch_scan(const VfbLevelArgs &a, uint64_t nchunks, hipStream_t s);
 hipError_t vfb_launch_write(int width, const VfbLevelArgs &a, int grid, hipStream_t s);

@@ -116,14 +116,14 @@ __device__ __forceinline__ uint64_t signature_fp(const word *rows, uint64_t cap,
 __device__ __forceinline__ uint64_t table_insert(uint64_t *keys, uint64_t mask,
                                                  uint64_t key, uint64_t &probes,
                                                  uint64_t limit) {
-  uint64_t s = fmix64(key) & mask;
+  uint64_t s = (key ^ (key >> 29)) & mask;
   for (uint64_t step = 0;; ++step) {
     if (step >= limit) return ~0ull;
     ++probes;
-    uint64_t k = __atomic_load_n(&keys[s], __ATOMIC_RELAXED);
+    uint64_t k = __atomic_load_n(&keys[2 * s], __ATOMIC_RELAXED);
     if (k == key) return s;
     if (k == kVfbEmpty) {
-      const uint64_t old = atomicCAS((unsigned long long *)&keys[s],
+      const uint64_t old = atomicCAS((unsigned long long *)&keys[2 * s],
                                      (unsigned long long)kVfbEmpty,
                                      (unsigned long long)key);
       if (old == kVfbEmpty || old == key) return s;
@@ -134,9 +134,9 @@ __device__ __forceinline__ uint64_t table_insert(uint64_t *keys, uint64_t mask,
 
 __device__ __forceinline__ uint64_t table_find(const uint64_t *keys, uint64_t mask,
                                                uint64_t key) {
-  uint64_t s = fmix64(key) & mask;
+  uint64_t s = (key ^ (key >> 29)) & mask;
   for (uint64_t step = 0; step <= mask; ++step) {
-    const uint64_t k = keys[s];
+    const uint64_t k = keys[2 * s];
     if (k == key) return s;
     if (k == kVfbEmpty) return ~0ull;
     s = (s + 1) & mask;
@@ -171,10 +171,94 @@ __global__ void __launch_bounds__(kVfbThreads) k_vfb_insert(VfbLevelArgs a) {
         a.slot_of[idx] = ~0u;
         continue;
       }
-      atomicMin((unsigned long long *)&a.minidx[s], (unsigned long long)(a.tag | idx));
+      if (a.minidx[2 * s] > (a.tag | idx))
+    atomicMin((unsigned long long *)&a.minidx[2 * s], (unsigned long long)(a.tag | idx));
       a.slot_of[idx] = (uint32_t)s;
       if (eq) atomicMin(a.found, (unsigned long long)idx);
     }
+  }
+  if (a.probes) atomicAdd(a.probes, (unsigned long long)probes);
+}
+
+// Pair-major form of the insert pass: one lane takes one (left, right) row
+// pair of a split (or one row for the unary ops) and produces every
+// candidate built on it -- & | ^ + * and both orders of '-' (or ~ and neg) --
+// so each pair of rows is read once for up to seven candidates.  Candidate
+// indices are those of the enumeration order (opbegin[op] + local index).
+template <typename word>
+__device__ __forceinline__ void insert_one(const VfbLevelArgs &a, uint64_t idx, uint64_t fp,
+                                           bool eq, uint64_t &probes) {
+  if (fp >= kVfbTomb) {  // invalid key: dropped (hash_set.cuh:200-204)
+    a.slot_of[idx] = ~0u;
+    return;
+  }
+  const uint64_t s = table_insert(a.keys, a.mask, fp, probes, a.max_probes);
+  if (s == ~0ull) {
+    atomicOr(a.fail, 1u);
+    a.slot_of[idx] = ~0u;
+    return;
+  }
+  if (a.minidx[2 * s] > (a.tag | idx))
+    atomicMin((unsigned long long *)&a.minidx[2 * s], (unsigned long long)(a.tag | idx));
+  a.slot_of[idx] = (uint32_t)s;
+  if (eq) atomicMin(a.found, (unsigned long long)idx);
+}
+
+template <typename word>
+__global__ void __launch_bounds__(kVfbThreads)
+k_vfb_insert_pairs(VfbLevelArgs a, const VfbPairBlock *pblocks, int npb, uint64_t npairs) {
+  __shared__ VfbPairBlock s_pb[kVfbMaxPairBlocks];
+  __shared__ word s_tgt[kMaxSamples];
+  for (int i = threadIdx.x; i < npb; i += blockDim.x) s_pb[i] = pblocks[i];
+  for (int i = threadIdx.x; i < a.n; i += blockDim.x) s_tgt[i] = ((const word *)a.target)[i];
+  __syncthreads();
+  const word *rows = (const word *)a.rows;
+  uint64_t probes = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
+    int b = 0;
+    while (b + 1 < npb && p >= s_pb[b + 1].pbegin) ++b;
+    const VfbPairBlock &B = s_pb[b];
+    const uint64_t local = p - B.pbegin;
+    if (B.unary) {
+      const uint32_t l = B.l0 + (uint32_t)local;
+      uint64_t sn = 0, sg = 0;
+      bool en = true, eg = true;
+      for (int i = 0; i < a.n; ++i) {
+        const word x = rows[(uint64_t)i * a.cap + l];
+        const word vn = ~x, vg = word(0) - x;
+        en &= vn == s_tgt[i];
+        eg &= vg == s_tgt[i];
+        sn = fp_step<word>(sn, vn, i);
+        sg = fp_step<word>(sg, vg, i);
+      }
+      insert_one<word>(a, B.opbegin[NOT] + local, sn, en, probes);
+      insert_one<word>(a, B.opbegin[NEG] + local, sg, eg, probes);
+      continue;
+    }
+    const uint64_t q = local / B.nr;
+    const uint32_t l = B.l0 + (uint32_t)q, r = B.r0 + (uint32_t)(local - q * B.nr);
+    uint64_t sd[7] = {0, 0, 0, 0, 0, 0, 0};
+    bool eq[7] = {true, true, true, true, true, true, true};
+    for (int i = 0; i < a.n; ++i) {
+      const word x = rows[(uint64_t)i * a.cap + l];
+      const word y = rows[(uint64_t)i * a.cap + r];
+      const word v[7] = {word(x & y), word(x | y), word(x ^ y), word(x + y),
+                         word(x * y), word(x - y), word(y - x)};
+      const word t = s_tgt[i];
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        eq[k] &= v[k] == t;
+        sd[k] = fp_step<word>(sd[k], v[k], i);
+      }
+    }
+    insert_one<word>(a, B.opbegin[AND] + local, sd[0], eq[0], probes);
+    insert_one<word>(a, B.opbegin[OR] + local, sd[1], eq[1], probes);
+    insert_one<word>(a, B.opbegin[XOR] + local, sd[2], eq[2], probes);
+    insert_one<word>(a, B.opbegin[ADD] + local, sd[3], eq[3], probes);
+    insert_one<word>(a, B.opbegin[MUL] + local, sd[4], eq[4], probes);
+    insert_one<word>(a, B.opbegin[SUB] + 2 * local, sd[5], eq[5], probes);
+    insert_one<word>(a, B.opbegin[SUB] + 2 * local + 1, sd[6], eq[6], probes);
   }
   if (a.probes) atomicAdd(a.probes, (unsigned long long)probes);
 }
@@ -191,7 +275,7 @@ __global__ void __launch_bounds__(kVfbThreads) k_vfb_flag(VfbLevelArgs a) {
       const uint64_t idx = ch * kVfbChunk + (uint64_t)k * kVfbThreads + threadIdx.x;
       if (idx >= a.total) break;
       const uint32_t s = a.slot_of[idx];
-      const uint8_t kp = (s != ~0u) && (a.minidx[s] == (a.tag | idx));
+      const uint8_t kp = (s != ~0u) && (a.minidx[2 * s] == (a.tag | idx));
       a.keep[idx] = kp;
       mine += kp;
     }
@@ -288,7 +372,7 @@ __global__ void k_vfb_seed(VfbLevelArgs a, int L) {
     atomicOr(a.fail, 1u);
     return;
   }
-  a.minidx[s] = 0;  // present before any level >= 2
+  a.minidx[2 * s] = 0;  // present before any level >= 2
 }
 
 template <typename word>
@@ -298,7 +382,7 @@ __global__ void k_vfb_lookup_target(VfbLevelArgs a, uint64_t *out) {
   uint64_t seed = 0;
   for (int i = 0; i < a.n; ++i) seed = fp_step<word>(seed, t[i], i);
   const uint64_t s = seed >= kVfbTomb ? ~0ull : table_find(a.keys, a.mask, seed);
-  *out = s == ~0ull ? ~0ull : a.minidx[s];
+  *out = s == ~0ull ? ~0ull : a.minidx[2 * s];
 }
 
 __global__ void k_vfb_rpn(const int32_t *left, const int32_t *right, const uint8_t *op,
@@ -361,6 +445,17 @@ hipError_t vfb_launch_insert(int width, const VfbLevelArgs &a, int grid, hipStre
     hipLaunchKernelGGL(k_vfb_insert<uint32_t>, dim3(grid), dim3(kVfbThreads), 0, s, a);
   else
     hipLaunchKernelGGL(k_vfb_insert<uint64_t>, dim3(grid), dim3(kVfbThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t vfb_launch_insert_pairs(int width, const VfbLevelArgs &a, const VfbPairBlock *d_pb,
+                                   int npb, uint64_t npairs, int grid, hipStream_t s) {
+  if (width == 32)
+    hipLaunchKernelGGL(k_vfb_insert_pairs<uint32_t>, dim3(grid), dim3(kVfbThreads), 0, s, a,
+                       d_pb, npb, npairs);
+  else
+    hipLaunchKernelGGL(k_vfb_insert_pairs<uint64_t>, dim3(grid), dim3(kVfbThreads), 0, s, a,
+                       d_pb, npb, npairs);
   return hipGetLastError();
 }
 
