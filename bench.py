@@ -81,6 +81,11 @@ def run_simba(args):
     naive_ops = sum(C[s] for s in range(1, S + 1))
     ctx = simba_amd.Context(0)
     kw = dict(min_size=1, max_size=S, exhaustive=True, table_bytes=args.table_gb << 30)
+    tuned = {}
+    if args.autotune:
+        from simba_amd import autotune
+        tuned, _ = autotune.tune_simba(ctx, inst, S - 1)
+        kw.update(tuned)
     for _ in range(args.warmup):
         ctx.simba(inst, **kw)
     times, ktimes = [], []
@@ -117,7 +122,7 @@ def run_simba(args):
         "config": {"workload": "simba_sweep_4vars_const1_u64_64samples_sizes1to%d" % S,
                    "target": inst.original_formula, "target_size": 15,
                    "candidates_per_step": cands, "first_solution": r["expr"],
-                   "first_solution_size": r["size"]},
+                   "first_solution_size": r["size"], "launch": tuned or "default"},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": round(VALU_PEAK / 1e12, 3), "unit": "TOP/s",
                      "frac": round(achieved * 1e12 / VALU_PEAK, 4), "traffic": None,
@@ -208,6 +213,11 @@ def run_batch(args):
     insts = sample_instances(args.seed, args.count, (5, 17), (2, 5), (16, 64))
     ctx = simba_amd.Context(0)
     kw = dict(max_size=args.budget_size)
+    tuned = {}
+    if args.autotune:
+        from simba_amd import autotune
+        tuned, _ = autotune.tune_batch(ctx, insts[:1024], args.budget_size)
+        kw.update(tuned)
     ctx.batch(insts[:64], **kw)
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -229,7 +239,8 @@ def run_batch(args):
                    "solved": solved, "exact_match_vs_plain": exact,
                    "plain_per_instance_solves_per_s": round(solved / t_plain, 2),
                    "plain_seconds": round(t_plain, 3),
-                   "speedup_vs_plain": round(t_plain / t_batch, 2)},
+                   "speedup_vs_plain": round(t_plain / t_batch, 2),
+                   "launch": tuned or "default"},
     }
 
 
@@ -241,24 +252,29 @@ def run_learned(args):
     import simba_amd
     from simba_amd import learn
 
+    log = lambda m: print(m, file=sys.stderr, flush=True)
     ctx = simba_amd.Context(0)
     t0 = time.perf_counter()
-    X, Y = learn.build_dataset(ctx, args.train, args.seed, max_size=args.label_size)
+    X, Y = learn.build_dataset(ctx, args.train, args.seed, max_size=args.label_size, log=log)
     t_label = time.perf_counter() - t0
     mlp = simba_amd.Mlp(ctx, seed=args.seed)
     untrained = simba_amd.Mlp(ctx, seed=args.seed + 1)
     t0 = time.perf_counter()
-    losses = learn.train(mlp, X, Y, epochs=args.epochs)
+    losses = learn.train(mlp, X, Y, epochs=args.epochs, log=log)
     t_train = time.perf_counter() - t0
     ev = learn.sample_instances(args.seed + 10**6, args.count, (9, 19), (2, 4), (32, 32))
     kw = dict(max_size=args.budget_size)
     mlp.solve(ev[0], **kw)
     plain = [ctx.simba(x, **kw) for x in ev]
+    log("plain per-instance done")
     lrn = [mlp.solve(x, **kw) for x in ev]
+    log("learned per-instance done")
     unt = [untrained.solve(x, **kw) for x in ev]
+    log("untrained per-instance done")
     solved = [i for i, r in enumerate(plain) if r["status"] == 0]
     same_size = all(lrn[i]["status"] == 0 and lrn[i]["size"] == plain[i]["size"] for i in solved)
-    verified = all(_verified(lrn[i], ev[i]) for i in solved)
+    verified = all(lrn[i]["status"] == 0 and _verified(lrn[i], ev[i]) for i in solved)
+    missed = [i for i in solved if lrn[i]["status"] != 0 or lrn[i]["size"] != plain[i]["size"]]
     tp = np.array([plain[i]["seconds"] for i in solved])
     tl = np.array([lrn[i]["seconds"] for i in solved])
     tu = np.array([unt[i]["seconds"] for i in solved])
@@ -281,7 +297,7 @@ def run_learned(args):
                    "train_examples": int(len(Y)), "label_seconds": round(t_label, 2),
                    "train_seconds": round(t_train, 2), "final_loss": round(losses[-1], 4),
                    "solved_plain": len(solved), "same_size": same_size,
-                   "verified": verified,
+                   "verified": verified, "mismatched": len(missed),
                    "p50_ms": {"plain": pct(tp, 50), "learned": pct(tl, 50),
                               "untrained": pct(tu, 50)},
                    "p95_ms": {"plain": pct(tp, 95), "learned": pct(tl, 95),
@@ -310,6 +326,8 @@ def main():
     ap.add_argument("--cache-gb", type=int, default=64)
     ap.add_argument("--seed", type=int, default=15)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune launch parameters on the workload before timing")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every box here has a single MI355X; "

@@ -39,7 +39,9 @@ class Opts(ctypes.Structure):
                 ("time_budget_s", ctypes.c_double),
                 ("cancel", ctypes.POINTER(ctypes.c_int32)),
                 ("table_bytes", ctypes.c_uint64), ("cache_bytes", ctypes.c_uint64),
-                ("verbose", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
+                ("verbose", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32),
+                ("lanes_b", ctypes.c_int32), ("table_delta", ctypes.c_int32),
+                ("block_log2", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
 
 
 class Result(ctypes.Structure):
@@ -292,9 +294,12 @@ class Mlp:
 
 
 def make_opts(min_size=1, max_size=9, compat=False, exhaustive=False, budget_s=0.0,
-              table_bytes=0, cache_bytes=0, verbose=False, cancel=None):
+              table_bytes=0, cache_bytes=0, verbose=False, cancel=None, wg_per_cu=0,
+              lanes_b=0, table_delta=0, block_log2=0):
     o = Opts()
     lib().mba_default_opts(ctypes.byref(o))
+    o.wg_per_cu, o.lanes_b, o.table_delta = wg_per_cu, lanes_b, table_delta
+    o.block_log2 = block_log2
     o.min_size, o.max_size = min_size, max_size
     o.compat, o.exhaustive = int(compat), int(exhaustive)
     o.time_budget_s = budget_s

@@ -186,7 +186,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   uint32_t *d_hc = ctx.hit_count.as<uint32_t>();
   HitRec *d_hits = ctx.hits.as<HitRec>();
   const TableView *d_tvs = ctx.b_inst.as<TableView>();
-  const int mult = o.reserved[0] > 0 ? o.reserved[0] : 8;  // workgroups per CU
+  const int mult = o.wg_per_cu > 0 ? o.wg_per_cu : 8;  // workgroups per CU
   const int grid = ctx.cu_count * mult;
 
   hipGraph_t graph = nullptr;
@@ -279,6 +279,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
           J.p.gen_a = j > st.Kt;
           J.p.gen_b = J.p.m > st.Kt;
           J.p.mask = mask;
+          J.p.rb = kPairRB;
           J.ntiles = J.p.nTA * ((J.p.nB + kPairTB - 1) / kPairTB);
           J.lo_e = tg.lo_even;
           J.lo_o = tg.lo_odd;

@@ -243,7 +243,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
 
   // ---- value tables V[1..Kt] -------------------------------------------
   const uint64_t budget = o.table_bytes ? o.table_bytes : kDefaultTableBytes;
-  const int Kt = choose_kt(T, max_size, W, budget);
+  const int Kt = std::max(1, choose_kt(T, max_size, W, budget) - std::max(0, o.table_delta));
   TableView tv;
   std::memset(&tv, 0, sizeof tv);
   uint64_t elems = 0;
@@ -287,7 +287,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   std::vector<Candidate> cands;
   uint32_t *d_hc = ctx.hit_count.as<uint32_t>();
   HitRec *d_hits = ctx.hits.as<HitRec>();
-  const int grid = ctx.cu_count * 8;
+  const int grid = ctx.cu_count * (o.wg_per_cu > 0 ? o.wg_per_cu : 8);
   double ktime = 0.0;
   int answer = -1;
   int reached = 0;
@@ -363,12 +363,14 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     p.nA = T.t[j][8];
     p.nB = T.t[m][8];
     p.nTA = (p.nA + kPairTA - 1) / kPairTA;
-    const uint64_t nTB = (p.nB + kPairTB - 1) / kPairTB;
+    p.rb = o.lanes_b == 8 ? 8 : 4;
+    const uint64_t tb = (uint64_t)kPairThreads * p.rb;
+    const uint64_t nTB = (p.nB + tb - 1) / tb;
     const uint64_t tiles = p.nTA * nTB;
     p.gen_a = j > Kt;
     p.gen_b = m > Kt;
     p.mask = mask;
-    const uint64_t per_tile = (uint64_t)kPairTA * kPairTB;
+    const uint64_t per_tile = (uint64_t)kPairTA * tb;
     const uint64_t tiles_per_launch = std::max<uint64_t>(1, kPairsPerLaunch / per_tile);
     for (uint64_t tb = 0; tb < tiles; tb += tiles_per_launch) {
       p.tile_begin = tb;
@@ -400,7 +402,12 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     MBA_HIP(hipMemsetAsync(d_hc, 0, 4, st));
     const CheckTargets tg = targets_for(t0, s, max_size, W);
     bool stopped = false;
-    const bool by_block = !o.exhaustive && (block_scores || T.t[s][8] >= kBlockModeMin);
+    // block-at-a-time only pays where a size's sweep outweighs a launch and
+    // a sync per block; below that the fused sweep (minimum rank, also a
+    // valid answer for the learned order) is used
+    const uint64_t block_min = o.block_log2 > 0 ? (1ull << std::min(o.block_log2, 63))
+                               : block_scores ? kLearnedBlockMin : kBlockModeMin;
+    const bool by_block = !o.exhaustive && T.t[s][8] >= block_min;
     if (!o.exhaustive && s >= min_size && found[s] != ~0ull &&
         (block_scores || found[s] < T.t[s][NOT])) {
       // a unary-rooted solution of this size is already verified: with the

@@ -35,6 +35,8 @@ struct BlockRef {
 std::vector<BlockRef> binary_blocks(const CountTable &T, int s);
 // Below this many trees a size is swept with fused pairs and one sync.
 constexpr uint64_t kBlockModeMin = 1ull << 31;
+// The same threshold for the learned order, whose point is to stop early.
+constexpr uint64_t kLearnedBlockMin = 1ull << 26;
 
 // Windows for the checks of base size sb (chains up to max_size).
 CheckTargets targets_for(uint64_t t0, int sb, int max_size, int W);

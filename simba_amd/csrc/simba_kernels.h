@@ -39,6 +39,7 @@ struct PairLaunch {
   uint64_t nTA;            // number of A tiles
   bool gen_a, gen_b;       // side is above Kt: values computed per lane
   uint32_t mask;           // operator instances to check (bit i = inst i)
+  int rb;                  // B values per lane: 4 or 8 (tile = 256 * rb B values)
 };
 
 constexpr uint32_t kAllInst = 0x7f;

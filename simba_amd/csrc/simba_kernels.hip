@@ -8,7 +8,7 @@
 // unordered split {j, m} the seven operator instances & | ^ + * (X-Y) (Y-X)
 // are fused and evaluated on sample 0 from the value tables V[j], V[m]
 // (value of every tree of that size on sample 0, in rank order).  A lane
-// holds kPairRB values of the B side in VGPRs; the A side streams through
+// holds RB (4 or 8) values of the B side in VGPRs; the A side streams through
 // wave-uniform scalar loads, so the inner loop is pure integer VALU work with
 // no divergence.  Unary-rooted trees are the composition of a chain of ~/neg
 // with a binary-rooted tree or a leaf; such a chain maps x to +-x + k with
@@ -272,7 +272,7 @@ __device__ __forceinline__ uint64_t uniform(uint64_t x) {
 // inside one (op, split) block whose right side has at least a tile's worth
 // of trees, the lane's right rank is first's plus its offset with at most
 // one carry into the left rank.  Lanes past that block decode on their own.
-template <typename word>
+template <typename word, int RB>
 __device__ __forceinline__ void gen_b_tile(const TableView *__restrict__ tv, int m,
                                            uint64_t first, uint64_t lane_b0, word *bv,
                                            const bool *ok) {
@@ -300,11 +300,11 @@ __device__ __forceinline__ void gen_b_tile(const TableView *__restrict__ tv, int
     a0 = x / rs;
     r0 = x - a0 * rs;
   }
-  const bool step_ok = un || rs >= (uint64_t)kPairTB;
+  const bool step_ok = un || rs >= (uint64_t)(kPairThreads * RB);
   const word *VA = V + tv->off[un ? m - 1 : j];
   const word *VB = V + tv->off[un ? 0 : m - 1 - j];
 #pragma unroll
-  for (int k = 0; k < kPairRB; ++k) {
+  for (int k = 0; k < RB; ++k) {
     const uint64_t b = lane_b0 + (uint64_t)k * kPairThreads;
     if (!ok[k]) {
       bv[k] = word(0);
@@ -329,7 +329,7 @@ __device__ __forceinline__ void gen_b_tile(const TableView *__restrict__ tv, int
 }
 
 // One operator instance over the A tile: its distances folded into acc.
-template <typename word, int INST, bool GEN_A>
+template <typename word, int INST, bool GEN_A, int RB>
 __device__ __forceinline__ uint32_t fold_inst(uint32_t acc, const TableView *__restrict__ tv,
                                               const word *__restrict__ A, int j, uint64_t a0,
                                               uint64_t a1, const BPre<word> *pre, uint32_t de,
@@ -338,7 +338,7 @@ __device__ __forceinline__ uint32_t fold_inst(uint32_t acc, const TableView *__r
     const word av = GEN_A ? value_of<word>(tv, j, a) : A[a];
     const word na = uniform(word(word(0) - av));
 #pragma unroll
-    for (int k = 0; k < kPairRB; ++k) {
+    for (int k = 0; k < RB; ++k) {
       const BPre<word> &q = pre[k];
       if (INST <= 2 || INST == 4) {
         const word c = INST == 0 ? word(av & q.b) : INST == 1 ? word(av | q.b)
@@ -360,7 +360,7 @@ __device__ __forceinline__ uint32_t fold_inst(uint32_t acc, const TableView *__r
 // One tile of a pair check (shared by the per-instance and batched kernels).
 // p.mask selects the operator instances (bit i: instance i, see HitRec).
 // `tag` is or-ed into HitRec::inst above bit 8 (the batch's instance index).
-template <typename word, bool SAME, bool GEN_A, bool GEN_B>
+template <typename word, bool SAME, bool GEN_A, bool GEN_B, int RB>
 __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
                                           const word *__restrict__ A,
                                           const word *__restrict__ B, const PairLaunch &p,
@@ -375,12 +375,12 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
     const uint64_t at = p.nTA == 1 ? 0 : t % p.nTA, bt = p.nTA == 1 ? t : t / p.nTA;
     const uint64_t a0 = at * kPairTA;
     const uint64_t a1 = a0 + kPairTA < p.nA ? a0 + kPairTA : p.nA;
-    const uint64_t b0 = bt * kPairTB + threadIdx.x;
-    word bv[kPairRB];
-    bool ok[kPairRB];
+    const uint64_t b0 = bt * (kPairThreads * RB) + threadIdx.x;
+    word bv[RB];
+    bool ok[RB];
     bool any_ok = false;
 #pragma unroll
-    for (int k = 0; k < kPairRB; ++k) {
+    for (int k = 0; k < RB; ++k) {
       const uint64_t b = b0 + (uint64_t)k * kPairThreads;
       ok[k] = b < p.nB;
       any_ok |= ok[k];
@@ -388,23 +388,23 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
     }
     if (GEN_B) {
       if (p.m == tv->Kt + 1) {
-        gen_b_tile<word>(tv, p.m, bt * kPairTB, b0, bv, ok);
+        gen_b_tile<word, RB>(tv, p.m, bt * (kPairThreads * RB), b0, bv, ok);
       } else {
 #pragma unroll
-        for (int k = 0; k < kPairRB; ++k)
+        for (int k = 0; k < RB; ++k)
           bv[k] = ok[k] ? value_of<word>(tv, p.m, b0 + (uint64_t)k * kPairThreads) : word(0);
       }
     }
     // lanes past the end of B reuse a valid value of their own (recorded
     // hits are re-derived with the exact masks below)
 #pragma unroll
-    for (int k = 1; k < kPairRB; ++k)
+    for (int k = 1; k < RB; ++k)
       if (!ok[k]) bv[k] = bv[0];
     const word oe = win_off<word>(lo_e), oo = win_off<word>(lo_o);
     const uint32_t de = dpart(oe), dof = dpart(oo);
-    BPre<word> pre[kPairRB];
+    BPre<word> pre[RB];
 #pragma unroll
-    for (int k = 0; k < kPairRB; ++k) pre[k] = bpre<word>(bv[k], oe, oo);
+    for (int k = 0; k < RB; ++k) pre[k] = bpre<word>(bv[k], oe, oo);
     uint32_t acc = 0xffffffffu;
     const uint32_t full = SAME ? 0x3fu : 0x7fu;
     if ((p.mask & full) == full) {
@@ -420,7 +420,7 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
         for (int i = 0; i < kAGroup; ++i) {
           const word na = uniform(word(word(0) - av[i]));
 #pragma unroll
-          for (int k = 0; k < kPairRB; ++k)
+          for (int k = 0; k < RB; ++k)
             acc = fold7<word, SAME>(acc, av[i], na, pre[k], de, dof);
         }
       }
@@ -428,7 +428,7 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
         const word av = GEN_A ? value_of<word>(tv, p.j, a) : A[a];
         const word na = uniform(word(word(0) - av));
 #pragma unroll
-        for (int k = 0; k < kPairRB; ++k) acc = fold7<word, SAME>(acc, av, na, pre[k], de, dof);
+        for (int k = 0; k < RB; ++k) acc = fold7<word, SAME>(acc, av, na, pre[k], de, dof);
       }
     } else {
       // a subset of the operator instances (one block at a time): one pass
@@ -436,13 +436,13 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
       for (int inst = 0; inst < 7; ++inst) {
         if (!((p.mask & full) >> inst & 1u)) continue;
         switch (inst) {
-          case 0: acc = fold_inst<word, 0, GEN_A>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
-          case 1: acc = fold_inst<word, 1, GEN_A>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
-          case 2: acc = fold_inst<word, 2, GEN_A>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
-          case 3: acc = fold_inst<word, 3, GEN_A>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
-          case 4: acc = fold_inst<word, 4, GEN_A>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
-          case 5: acc = fold_inst<word, 5, GEN_A>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
-          default: acc = fold_inst<word, 6, GEN_A>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
+          case 0: acc = fold_inst<word, 0, GEN_A, RB>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
+          case 1: acc = fold_inst<word, 1, GEN_A, RB>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
+          case 2: acc = fold_inst<word, 2, GEN_A, RB>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
+          case 3: acc = fold_inst<word, 3, GEN_A, RB>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
+          case 4: acc = fold_inst<word, 4, GEN_A, RB>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
+          case 5: acc = fold_inst<word, 5, GEN_A, RB>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
+          default: acc = fold_inst<word, 6, GEN_A, RB>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
         }
       }
     }
@@ -452,7 +452,7 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
       if (hit) {
         for (uint64_t a = a0; a < a1; ++a) {
           const word av = GEN_A ? value_of<word>(tv, p.j, a) : A[a];
-          for (int k = 0; k < kPairRB; ++k) {
+          for (int k = 0; k < RB; ++k) {
             if (!ok[k]) continue;
             const uint64_t b = b0 + (uint64_t)k * kPairThreads;
             for (int inst = 0; inst < (SAME ? 6 : 7); ++inst) {
@@ -468,7 +468,7 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
   }
 }
 
-template <typename word, bool SAME, bool GEN_A, bool GEN_B>
+template <typename word, bool SAME, bool GEN_A, bool GEN_B, int RB>
 __global__ void __launch_bounds__(kPairThreads)
 k_pair_check(const TableView *__restrict__ tv, const word *__restrict__ A,
              const word *__restrict__ B, PairLaunch p, uint64_t lo_e64, uint64_t lo_o64,
@@ -481,12 +481,12 @@ k_pair_check(const TableView *__restrict__ tv, const word *__restrict__ A,
     for (int i = threadIdx.x; i < (int)p.nA; i += blockDim.x) sA[i] = A[i];
     __syncthreads();
     for (uint64_t t = p.tile_begin + blockIdx.x; t < p.tile_end; t += gridDim.x)
-      pair_tile<word, SAME, GEN_A, GEN_B>(tv, sA, B, p, t, (word)lo_e64, (word)lo_o64,
+      pair_tile<word, SAME, GEN_A, GEN_B, RB>(tv, sA, B, p, t, (word)lo_e64, (word)lo_o64,
                                           (word)span64, hits, hit_count, hit_cap, 0u);
     return;
   }
   for (uint64_t t = p.tile_begin + blockIdx.x; t < p.tile_end; t += gridDim.x)
-    pair_tile<word, SAME, GEN_A, GEN_B>(tv, A, B, p, t, (word)lo_e64, (word)lo_o64,
+    pair_tile<word, SAME, GEN_A, GEN_B, RB>(tv, A, B, p, t, (word)lo_e64, (word)lo_o64,
                                         (word)span64, hits, hit_count, hit_cap, 0u);
 }
 
@@ -598,15 +598,15 @@ __device__ void batch_pair(const TableView *tv, const BatchJob &J, uint64_t lt,
   const uint32_t tag = J.inst << 8;
   if (p.j == p.m) {
     if (p.gen_a)
-      pair_tile<word, true, true, true>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
+      pair_tile<word, true, true, true, kPairRB>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
     else
-      pair_tile<word, true, false, false>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
+      pair_tile<word, true, false, false, kPairRB>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
   } else if (p.gen_a) {
-    pair_tile<word, false, true, true>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
+    pair_tile<word, false, true, true, kPairRB>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
   } else if (p.gen_b) {
-    pair_tile<word, false, false, true>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
+    pair_tile<word, false, false, true, kPairRB>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
   } else {
-    pair_tile<word, false, false, false>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
+    pair_tile<word, false, false, false, kPairRB>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
   }
 }
 
@@ -888,9 +888,15 @@ static void pair_dispatch(const TableView *d_tv, const word *A, const word *B,
                           const PairLaunch &p, const CheckTargets &tg, HitRec *h,
                           uint32_t *hc, uint32_t cap, int grid, hipStream_t s) {
   const bool same = p.j == p.m;
-#define PAIR_LAUNCH(S, GA, GB)                                                    \
-  hipLaunchKernelGGL((k_pair_check<word, S, GA, GB>), dim3(grid), dim3(kPairThreads), \
-                     0, s, d_tv, A, B, p, tg.lo_even, tg.lo_odd, tg.span, h, hc, cap)
+#define PAIR_LAUNCH(S, GA, GB)                                                       \
+  do {                                                                               \
+    if (p.rb == 8)                                                                   \
+      hipLaunchKernelGGL((k_pair_check<word, S, GA, GB, 8>), dim3(grid), dim3(kPairThreads), \
+                         0, s, d_tv, A, B, p, tg.lo_even, tg.lo_odd, tg.span, h, hc, cap); \
+    else                                                                             \
+      hipLaunchKernelGGL((k_pair_check<word, S, GA, GB, 4>), dim3(grid), dim3(kPairThreads), \
+                         0, s, d_tv, A, B, p, tg.lo_even, tg.lo_odd, tg.span, h, hc, cap); \
+  } while (0)
   if (same) {
     if (p.gen_a) PAIR_LAUNCH(true, true, true);
     else PAIR_LAUNCH(true, false, false);

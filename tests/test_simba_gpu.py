@@ -98,3 +98,53 @@ def test_simba_degenerate_first_sample(ctx):
     inst = MbaInstance(inp, out, 2, 32)
     note("simba degenerate sample 0")
     _check(ctx, inst, 1, 7)
+
+
+@pytest.mark.parametrize("cfg", [dict(lanes_b=8), dict(wg_per_cu=4), dict(wg_per_cu=16),
+                                 dict(table_delta=1), dict(lanes_b=8, table_delta=1)])
+def test_simba_launch_configs_keep_answers(ctx, cfg):
+    from simba_amd.synth import make_instance
+    for seed in range(4):
+        inst = make_instance(5000 + seed, 2 + seed % 2, 16, 6 + seed % 2, 64 if seed % 2 else 32,
+                             (1,) if seed == 3 else ())
+        note(f"simba cfg {cfg} seed={seed}")
+        _check(ctx, inst, 1, 8, **cfg)
+
+
+def test_autotune_picks_a_config_with_identical_answers(ctx, tmp_path, monkeypatch):
+    from simba_amd import autotune
+    from simba_amd.synth import make_instance
+    monkeypatch.setenv("SIMBA_AMD_TUNE", str(tmp_path / "tune.json"))
+    inst = make_instance(77, 3, 16, 9, 64)
+    note("autotune")
+    best, timing = autotune.tune_simba(ctx, inst, 8)
+    assert set(best) == {"lanes_b", "wg_per_cu", "table_delta"}
+    assert len(timing) == 12
+    assert autotune.tuned(autotune.simba_key(inst, 8)) == best
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_simba_block_mode_matches_oracle(ctx, seed):
+    # one (op, split) block per launch with the rank-order early exit
+    from simba_amd.synth import make_instance
+    inst = make_instance(6000 + seed, 1 + seed % 3, 12, 3 + seed % 5, 64 if seed % 2 else 32,
+                         (1,) if seed % 4 == 0 else ())
+    note(f"simba block mode seed={seed}")
+    _check(ctx, inst, 1, 8, block_log2=1)
+    _check(ctx, inst, 1, 8, block_log2=6)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_learned_block_mode_is_minimal_and_verified(ctx, seed):
+    import simba_amd
+    from simba_amd.synth import make_instance
+    m = simba_amd.Mlp(ctx, seed=seed)
+    inst = make_instance(6100 + seed, 2 + seed % 3, 16, 4 + seed % 5, 64)
+    note(f"learned block mode seed={seed}")
+    plain = ctx.simba(inst, max_size=8)
+    got = m.solve(inst, max_size=8, block_log2=1)
+    assert got["status"] == plain["status"], (got, plain)
+    if plain["status"] == 0:
+        assert got["size"] == plain["size"]
+        assert verify_rpn_chars(got["rpn"], inst)
+    m.close()
