@@ -9,16 +9,20 @@ second over the K timed steps (Gcand/s).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload simba|vfb]
 
-roofline: the fused pair kernel is integer-VALU bound.  Algorithmic work per
-candidate tree = sum over its internal nodes of c(op) (64-bit words:
-& | ^ ~ + - neg = 2 lane-ops, * = 4; 32-bit words: 1), times the samples
-evaluated (1 for all but the rare sample-0 hits).  The kernel obtains most of
-a tree's value from the per-size value tables, so this naive-equivalent rate
-can exceed the lane-op rate the hardware executes; the executed rate is
-reported beside it.  Peak: 256 CU x 64 lanes x 2.4 GHz = 39.3 T lane-ops/s.
+roofline: the fused pair kernel (k_pair_check) is integer-VALU bound.
+Algorithmic work per candidate tree = sum over its internal nodes of c(op)
+(64-bit words: & | ^ ~ + - neg = 2 lane-ops, * = 4; 32-bit words: 1), times
+the samples evaluated (1 for all but the rare sample-0 hits).  `achieved` sums
+that over the candidates the pair kernel decides and divides by the kernel's
+HIP-event time on the library stream (every launch of a step).  The kernel
+takes most of a tree's value from the per-size value tables, so this
+naive-equivalent rate can exceed the lane-op rate the hardware executes; the
+counter figures (VALU busy, issued VALU instructions) and the HBM traffic per
+launch come from the rocprofv3 summary under profiles/ for this workload.
+Peak: 256 CU x 64 lanes x 2.4 GHz = 39.3 T lane-ops/s.
 
 cpu_baseline: the CPU oracle (multithreaded C restatement of the reference
-semantics, oracle/) on the same instance for sizes 1..9.
+semantics, oracle/) on the same instance for sizes 1..11.
 """
 import argparse
 import json
@@ -52,14 +56,29 @@ def naive_ops_per_size(L, max_size, W):
     return T, C
 
 
-def executed_ops_per_pair(W):
-    """VALU lane-ops the fused pair kernel issues per (A, B) value pair:
-    7 operator instances (& | ^ + * A-B B-A) each followed by two windowed
-    compares (64-bit: sub 2 + cmp 1 each)."""
-    if W == 64:
-        ops = 2 + 2 + 2 + 2 + 4 + 2 + 2
-        return ops + 7 * 2 * 3
-    return 7 + 7 * 2 * 2
+def pair_kernel_ops(T, C, L, max_size, W):
+    """Naive-equivalent lane-ops of the trees the pair kernel decides: every
+    tree of sizes 3..max_size except the unary chains over a single leaf
+    (L * 2^(s-1) trees of s-1 unary nodes each), which the leaf scan covers."""
+    c_un = 2 if W == 64 else 1
+    tot = 0
+    for s in range(3, max_size + 1):
+        tot += C[s] - L * (1 << (s - 1)) * (s - 1) * c_un
+    return tot
+
+
+def profile_fields(name):
+    """Counter figures recorded by rocprofv3 for the pair kernel on this
+    workload: the `bench-fields:` JSON line of profiles/<name>, or {}."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("bench-fields:"):
+                    return json.loads(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return {}
 
 
 def cpu_baseline_simba(inst, max_size, threads):
@@ -78,7 +97,6 @@ def run_simba(args):
     L = 5
     T, C = naive_ops_per_size(L, S, 64)
     cands = sum(T[s][8] for s in range(1, S + 1))
-    naive_ops = sum(C[s] for s in range(1, S + 1))
     ctx = simba_amd.Context(0)
     kw = dict(min_size=1, max_size=S, exhaustive=True, table_bytes=args.table_gb << 30)
     tuned = {}
@@ -99,13 +117,10 @@ def run_simba(args):
     ms_per_step = 1e3 * wall / args.steps
     value = cands * args.steps / wall / 1e9
     kavg = sum(ktimes) / len(ktimes)
-    # pair-kernel share of the work: binary-rooted trees of sizes 3..S
-    pairs = 0
-    for s in range(3, S + 1):
-        for j in range(1, (s - 1) // 2 + 1):
-            pairs += T[j][8] * T[s - 1 - j][8]
-    achieved = naive_ops / kavg / 1e12
-    executed = pairs * executed_ops_per_pair(64) / kavg / 1e12
+    launches = r["kernel_launches"]
+    pk_ops = pair_kernel_ops(T, C, L, S, 64)
+    achieved = pk_ops / kavg / 1e12
+    prof = profile_fields("k_pair_check_simba_sweep_sizes1to%d.md" % S)
     out = {
         "metric": "Gcand/s evaluated",
         "value": round(value, 3),
@@ -125,12 +140,20 @@ def run_simba(args):
                    "first_solution_size": r["size"], "launch": tuned or "default"},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": round(VALU_PEAK / 1e12, 3), "unit": "TOP/s",
-                     "frac": round(achieved * 1e12 / VALU_PEAK, 4), "traffic": None,
-                     "definition": "naive-equivalent lane-ops (sum of c(op) over every "
-                                   "internal node of every candidate) / pair-kernel time",
-                     "executed_achieved": round(executed, 3),
-                     "executed_frac": round(executed * 1e12 / VALU_PEAK, 4),
-                     "kernel_seconds_per_step": round(kavg, 4)},
+                     "frac": round(achieved * 1e12 / VALU_PEAK, 4),
+                     "traffic": prof.get("traffic_bytes_per_launch"),
+                     "kernel": "k_pair_check",
+                     "definition": "naive-equivalent 32-bit lane-ops of the candidates the "
+                                   "pair kernel decides (sum of c(op) over every internal "
+                                   "node; u64: 2 per op, 4 per *) / pair-kernel HIP-event "
+                                   "time; the kernel reuses tabulated subtree values, so "
+                                   "this can exceed the executed rate",
+                     "launches_per_step": launches,
+                     "avg_launch_ms": round(1e3 * kavg / max(launches, 1), 4),
+                     "kernel_seconds_per_step": round(kavg, 4),
+                     "counter_valu_busy": prof.get("valu_busy"),
+                     "counter_valu_inst_rate_frac": prof.get("valu_inst_frac"),
+                     "counter_source": prof.get("source")},
     }
     ctx.close()
     if not args.no_cpu:
@@ -308,19 +331,43 @@ def run_learned(args):
     }
 
 
+def run_copy(args):
+    """Device-to-device copy of 1 GiB per launch: the kernel of known traffic
+    (1 GiB read, 1 GiB written) that scales the FETCH_SIZE / WRITE_SIZE
+    counters of the profiled kernels."""
+    import torch
+    n = 1 << 30
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    return {"metric": "copy GB/s", "value": round(2 * n / dt / 1e9, 1), "unit": "GB/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(1e3 * dt, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+            "config": {"workload": "d2d_copy_1GiB"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="simba", choices=["simba", "vfb", "batch", "learned"])
+    ap.add_argument("--workload", default="simba",
+                    choices=["simba", "vfb", "batch", "learned", "copy"])
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--budget-size", type=int, default=9)
     ap.add_argument("--train", type=int, default=200000)
     ap.add_argument("--label-size", type=int, default=9)
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--size", type=int, default=14)
-    ap.add_argument("--cpu-size", type=int, default=9)
+    ap.add_argument("--cpu-size", type=int, default=11)
     ap.add_argument("--table-gb", type=int, default=40)
     ap.add_argument("--vfb-levels", type=int, default=11)
     ap.add_argument("--cache-gb", type=int, default=64)
@@ -334,7 +381,7 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     out = {"simba": run_simba, "vfb": run_vfb, "batch": run_batch,
-           "learned": run_learned}[args.workload](args)
+           "learned": run_learned, "copy": run_copy}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

@@ -95,7 +95,10 @@ typedef struct {
   double kernel_seconds; /* sum of the hot kernels' HIP-event times         */
   uint64_t candidates;   /* candidates of every completed size/level        */
   int32_t size_reached;  /* last size/level completely searched             */
-  int32_t reserved[5];
+  uint32_t kernel_launches; /* intervals summed in kernel_seconds (SIMBA:   */
+                            /* one per pair-kernel launch; VFB: one per     */
+                            /* level; batch: one per graph replay)          */
+  int32_t reserved[4];
 } mba_result;
 
 /* VFB per-level statistics (PRINT_CACHE_SIZE mode, full-cache.cu:763-765). */

@@ -50,13 +50,15 @@ class Result(ctypes.Structure):
                 ("split", ctypes.c_int32), ("expr", ctypes.c_char * EXPR_CAP),
                 ("rpn", ctypes.c_char * (MAX_SIZE + 1)), ("seconds", ctypes.c_double),
                 ("kernel_seconds", ctypes.c_double), ("candidates", ctypes.c_uint64),
-                ("size_reached", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+                ("size_reached", ctypes.c_int32), ("kernel_launches", ctypes.c_uint32),
+                ("reserved", ctypes.c_int32 * 4)]
 
     def as_dict(self):
         return {"status": self.status, "size": self.size, "rank": int(self.rank),
                 "op": self.op, "split": self.split, "expr": self.expr.decode(),
                 "rpn": self.rpn.decode(), "seconds": self.seconds,
                 "kernel_seconds": self.kernel_seconds,
+                "kernel_launches": self.kernel_launches,
                 "candidates": int(self.candidates), "size_reached": self.size_reached}
 
 

@@ -215,6 +215,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   std::vector<Candidate> cands;
   std::vector<VerifyCand> vc;
   double ktime = 0.0;
+  uint32_t klaunch = 0;
   bool budget_hit = false;
   auto finish = [&](size_t i, int z) {
     InstState &st = S[i];
@@ -328,6 +329,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
         float ms = 0.f;
         MBA_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
         ktime += ms * 1e-3;
+        ++klaunch;
         const uint32_t nh = *h_hc;
         if (nh > kBatchHitCap) {
           ctx.last_error = "batch: sample hit buffer overflow";
@@ -401,6 +403,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     r.candidates = cand;
     r.size_reached = st.reached;
     r.kernel_seconds = ktime;
+    r.kernel_launches = klaunch;
     if (st.active) {
       r.status = budget_hit ? MBA_BUDGET : MBA_NOT_FOUND;
       r.seconds = total;

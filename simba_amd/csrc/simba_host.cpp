@@ -289,6 +289,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   HitRec *d_hits = ctx.hits.as<HitRec>();
   const int grid = ctx.cu_count * (o.wg_per_cu > 0 ? o.wg_per_cu : 8);
   double ktime = 0.0;
+  uint32_t klaunch = 0;
   int answer = -1;
   int reached = 0;
   int status = MBA_NOT_FOUND;
@@ -389,6 +390,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
         return -1;
       }
       ktime += ms * 1e-3;
+      ++klaunch;
       if (over_budget()) return 1;
     }
     return 0;
@@ -472,6 +474,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   res.candidates = cand;
   res.size_reached = reached;
   res.kernel_seconds = ktime;
+  res.kernel_launches = klaunch;
   if (answer > 0) {
     fill_result(T, answer, found[answer], nv, inst.consts, W, res);
     status = MBA_FOUND;

@@ -150,6 +150,7 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
   uint64_t rows = (uint64_t)L;
   const int grid = ctx.cu_count * 8;
   double ktime = 0.0;
+  uint32_t klaunch = 0;
   int status = MBA_NOT_FOUND;
   int reached = 1;
   uint64_t cand_total = (uint64_t)L;
@@ -274,12 +275,15 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
       sol_idx = (int64_t)found;
       sol_level = lev;
       reached = lev;
-      float ms = 0.f;
-      MBA_HIP(hipEventRecord(ctx.ev1, st));
-      MBA_HIP(hipEventSynchronize(ctx.ev1));
-      MBA_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
-      ktime += ms * 1e-3;
-      if (!o.exhaustive) break;
+      if (!o.exhaustive) {
+        float ms = 0.f;
+        MBA_HIP(hipEventRecord(ctx.ev1, st));
+        MBA_HIP(hipEventSynchronize(ctx.ev1));
+        MBA_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
+        ktime += ms * 1e-3;
+        ++klaunch;
+        break;
+      }
     }
     const bool last = lev == max_len;
     MBA_HIP(vfb_launch_flag(W, a, g, st));
@@ -300,6 +304,7 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
     float ms = 0.f;
     MBA_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
     ktime += ms * 1e-3;
+    ++klaunch;
     S.level_new[lev] = kept;
     S.level_seconds[lev] = seconds_since(t_lev);
     // algorithmic bytes: one slot read per candidate, a key write per new
@@ -320,6 +325,7 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
   }
 
   res.kernel_seconds = ktime;
+  res.kernel_launches = klaunch;
   res.candidates = cand_total;
   res.size_reached = reached;
   if (sol_idx >= 0) {

@@ -28,6 +28,32 @@ def test_header_and_exports_agree():
         assert hasattr(lib, n), n
 
 
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes mirror has the header's sizes and field offsets (gcc)."""
+    import ctypes
+    structs = {"mba_instance": simba_amd.Instance, "mba_opts": simba_amd.Opts,
+               "mba_result": simba_amd.Result, "mba_vfb_stats": simba_amd.VfbStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mba_api.h"',
+             'int main(void) {']
+    for cname, py in structs.items():
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f in py._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));'
+                         % (cname, f[0], cname, f[0]))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)],
+                   check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                 check=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == ctypes.sizeof(py), cname
+        for f in py._fields_:
+            assert int(got[cname + "." + f[0]]) == getattr(py, f[0]).offset, (cname, f[0])
+
+
 def test_context_refuses_without_gpu():
     import torch
     if torch.cuda.is_available():
