@@ -2,8 +2,8 @@
 
 Default workload (one JSON line on stdout): the largest single-instance SIMBA
 shape -- leaves {a, b, c, d, 1}, uint64 words, 64 sample points, a uniform
-random size-15 target -- swept exhaustively over every size 1..S (S = 14 by
-default, 3.82e12 candidate trees).  One step = one full sweep; inputs are
+random size-15 target -- swept exhaustively over every size 1..S (S = 15 by
+default, 3.90e13 candidate trees).  One step = one full sweep; inputs are
 resident on the device before the timed region.  `value` is candidates per
 second over the K timed steps (Gcand/s).
 
@@ -84,7 +84,8 @@ def profile_fields(name):
 def cpu_baseline_simba(inst, max_size, threads):
     from oracle import oracle as orc
     r = orc.simba_search(inst.inputs, inst.outputs, inst.n_variables,
-                         [int(c) for c in inst.consts], inst.width, 1, max_size, threads)
+                         [int(c) for c in inst.consts], inst.width, 1, max_size, threads,
+                         exhaustive=True)
     return r
 
 
@@ -99,7 +100,8 @@ def run_simba(args):
     cands = sum(T[s][8] for s in range(1, S + 1))
     ctx = simba_amd.Context(0)
     kw = dict(min_size=1, max_size=S, exhaustive=True, table_bytes=args.table_gb << 30)
-    tuned = {}
+    tuned = {k: v for k, v in (("lanes_b", args.lanes_b), ("wg_per_cu", args.wg_per_cu)) if v}
+    kw.update(tuned)
     if args.autotune:
         from simba_amd import autotune
         tuned, _ = autotune.tune_simba(ctx, inst, S - 1)
@@ -162,7 +164,7 @@ def run_simba(args):
         out["cpu_baseline"] = {
             "value": round(cb["candidates"] / cb["seconds"] / 1e9, 5), "unit": "Gcand/s",
             "cores": threads, "kind": "port",
-            "sample": "oracle SIMBA search of the same instance, sizes 1..%d "
+            "sample": "oracle SIMBA sweep of the same instance, every size 1..%d "
                       "(%d candidates, %.1f s)" % (args.cpu_size, cb["candidates"], cb["seconds"])}
     return out
 
@@ -366,13 +368,15 @@ def main():
     ap.add_argument("--train", type=int, default=200000)
     ap.add_argument("--label-size", type=int, default=9)
     ap.add_argument("--epochs", type=int, default=4)
-    ap.add_argument("--size", type=int, default=14)
+    ap.add_argument("--size", type=int, default=15)
     ap.add_argument("--cpu-size", type=int, default=11)
     ap.add_argument("--table-gb", type=int, default=40)
     ap.add_argument("--vfb-levels", type=int, default=11)
     ap.add_argument("--cache-gb", type=int, default=64)
     ap.add_argument("--seed", type=int, default=15)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lanes-b", type=int, default=0, help="B values per lane (4 or 8)")
+    ap.add_argument("--wg-per-cu", type=int, default=0, help="workgroups per CU")
     ap.add_argument("--autotune", action="store_true",
                     help="tune launch parameters on the workload before timing")
     args = ap.parse_args()

@@ -74,7 +74,7 @@ typedef struct {
   uint64_t cache_bytes;     /* VFB: row-store budget, 0 = default            */
   int32_t verbose;          /* progress on stderr                            */
   int32_t wg_per_cu;        /* launch tuning: workgroups per CU (0: 8)       */
-  int32_t lanes_b;          /* launch tuning: B values per lane, 4 or 8 (0: 4) */
+  int32_t lanes_b;          /* launch tuning: B values per lane, 4 or 8 (0: 8) */
   int32_t table_delta;      /* tabulate Kt - table_delta sizes (0: budget)   */
   int32_t block_log2;       /* sweep one (op, split) block per launch from   */
                             /* 2^block_log2 trees per size (0: defaults)    */
@@ -140,8 +140,9 @@ int mba_batch_solve(mba_ctx *ctx, const mba_instance *insts, size_t count,
                     const mba_opts *opts, mba_result *results);
 
 /* ---- learned block proposal (no reference counterpart: an extension that
- * orders the plain search's blocks by a learned score).  A 2-layer MLP 256 -> 512 -> 128, BF16 MFMA operands
- * with FP32 accumulation, trained on the device with Adam.  Output class =
+ * orders the plain search's blocks by a learned score).  A 2-layer MLP
+ * 256 -> 512 -> 128, BF16 MFMA operands with FP32 accumulation, trained on
+ * the device with Adam.  Output class =
  * op * 16 + min(left size, 15) of a block's root (op 0..7, left size 0 for
  * unary roots).  An mba_mlp belongs to the context it was created on. */
 typedef struct mba_mlp mba_mlp;

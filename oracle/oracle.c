@@ -272,6 +272,7 @@ typedef struct {
   _Atomic uint64_t evaluated;
   _Atomic int stop;
   double deadline;
+  int exhaustive;
 } simba_job;
 
 #define ORC_CHUNK 16384ull
@@ -284,7 +285,7 @@ static void *simba_worker(void *arg) {
     if (atomic_load(&J->stop)) break;
     uint64_t c = atomic_fetch_add(&J->next_chunk, 1);
     uint64_t lo = c * ORC_CHUNK;
-    if (lo >= J->total || lo > atomic_load(&J->best)) break;
+    if (lo >= J->total || (!J->exhaustive && lo > atomic_load(&J->best))) break;
     uint64_t hi = lo + ORC_CHUNK;
     if (hi > J->total) hi = J->total;
     for (uint64_t r = lo; r < hi; ++r) {
@@ -295,7 +296,7 @@ static void *simba_worker(void *arg) {
         uint64_t b = atomic_load(&J->best);
         while (r < b && !atomic_compare_exchange_weak(&J->best, &b, r)) {
         }
-        break; /* later ranks of this chunk cannot be smaller */
+        if (!J->exhaustive) break; /* later ranks of this chunk cannot be smaller */
       }
     }
     if (J->deadline > 0 && now_s() > J->deadline) {
@@ -310,7 +311,7 @@ static void *simba_worker(void *arg) {
 int orc_simba_search(int n_vars, int n_consts, const uint64_t *consts, int W,
                      int n_samples, const uint64_t *inputs,
                      const uint64_t *outputs, int min_size, int max_size,
-                     int nthreads, double budget_s, orc_result *res) {
+                     int nthreads, double budget_s, int exhaustive, orc_result *res) {
   memset(res, 0, sizeof *res);
   int L = n_vars + n_consts;
   if (L < 1 || L > ORC_MAX_LEAVES || max_size > ORC_MAX_SIZE || min_size < 1 ||
@@ -336,6 +337,7 @@ int orc_simba_search(int n_vars, int n_consts, const uint64_t *consts, int W,
     atomic_init(&J.evaluated, 0);
     atomic_init(&J.stop, 0);
     J.deadline = budget_s > 0 ? t0 + budget_s : 0;
+    J.exhaustive = exhaustive;
     pthread_t th[256];
     int nt = nthreads > 256 ? 256 : nthreads;
     for (int k = 0; k < nt; ++k) pthread_create(&th[k], NULL, simba_worker, &J);
@@ -347,7 +349,7 @@ int orc_simba_search(int n_vars, int n_consts, const uint64_t *consts, int W,
       res->size = s; /* size reached */
       break;
     }
-    if (b != UINT64_MAX) {
+    if (b != UINT64_MAX && status != ORC_FOUND) {
       status = ORC_FOUND;
       res->size = s;
       res->rank = b;
@@ -360,7 +362,7 @@ int orc_simba_search(int n_vars, int n_consts, const uint64_t *consts, int W,
         res->op = -1;
       }
       orc_infix(res->rpn, s, n_vars, consts, W, 0, res->expr, sizeof res->expr);
-      break;
+      if (!exhaustive) break;
     }
   }
   res->status = status;
@@ -603,8 +605,10 @@ int orc_vfb_search(int n_vars, int n_consts, const uint64_t *consts, int W,
                 found_row = (int64_t)C.rows - 1;
                 found_level = lev;
                 found_index = idx;
-                if (!exhaustive) stop_level = 1;
-                break;
+                if (!exhaustive) {
+                  stop_level = 1;
+                  break;
+                }
               }
             }
           }

@@ -44,7 +44,7 @@ int orc_infix(const uint8_t *rpn, int size, int n_vars, const uint64_t *consts,
 int orc_simba_search(int n_vars, int n_consts, const uint64_t *consts, int W,
                      int n_samples, const uint64_t *inputs, const uint64_t *outputs,
                      int min_size, int max_size, int nthreads, double budget_s,
-                     orc_result *res);
+                     int exhaustive, orc_result *res);
 uint32_t orc_fmix32(uint32_t x);
 uint64_t orc_fmix64(uint64_t x);
 uint64_t orc_fingerprint(const uint64_t *cs, int n, int W);

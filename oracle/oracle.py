@@ -54,7 +54,7 @@ def lib():
                                 u64p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
         L.orc_simba_search.argtypes = [ctypes.c_int, ctypes.c_int, u64p, ctypes.c_int,
                                        ctypes.c_int, u64p, u64p, ctypes.c_int, ctypes.c_int,
-                                       ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                        ctypes.POINTER(OrcResult)]
         L.orc_fmix32.argtypes = [ctypes.c_uint32]
         L.orc_fmix32.restype = ctypes.c_uint32
@@ -132,7 +132,9 @@ def _result(r):
 
 
 def simba_search(inputs, outputs, n_vars, consts=(), W=64, min_size=1, max_size=9,
-                 threads=1, budget_s=0.0):
+                 threads=1, budget_s=0.0, exhaustive=False):
+    """Minimum-rank first solution; `exhaustive` evaluates every size up to
+    max_size (the answer is unchanged)."""
     inputs = np.asarray(inputs, dtype=np.uint64).reshape(-1)
     outputs = np.asarray(outputs, dtype=np.uint64)
     n = outputs.shape[0]
@@ -141,7 +143,7 @@ def simba_search(inputs, outputs, n_vars, consts=(), W=64, min_size=1, max_size=
     ca, cp = _u64(list(consts) if len(consts) else [0])
     r = OrcResult()
     lib().orc_simba_search(n_vars, len(consts), cp, W, n, ip, op, min_size, max_size,
-                           threads, budget_s, ctypes.byref(r))
+                           threads, budget_s, int(exhaustive), ctypes.byref(r))
     return _result(r)
 
 

@@ -22,6 +22,85 @@ constexpr uint64_t kDefaultTableBytes = 40ull << 30;
 // Work per launch, in A x B value pairs, so that budgets are honoured.
 constexpr uint64_t kPairsPerLaunch = 1ull << 36;
 
+// Tile geometry of a B block of nA x nB values (see BBlock): columns run
+// over the larger dimension, 2^logTB of them, and the tile's remaining
+// (256*rb) >> logTB lanes take successive rows.
+static BBlock shape_block(BBlock b, int rb) {
+  const uint64_t tile = (uint64_t)kPairThreads * rb;
+  b.trans = b.nB >= b.nA ? 0 : 1;
+  const uint64_t nX = b.trans ? b.nA : b.nB;
+  int lg = 0;
+  while ((1ull << lg) < nX && (1ull << lg) < tile) ++lg;
+  b.logTB = lg;
+  b.ntc = (nX + (1ull << lg) - 1) >> lg;
+  return b;
+}
+
+static uint64_t block_tiles(const BBlock &b, int rb) {
+  const uint64_t tile = (uint64_t)kPairThreads * rb;
+  if (b.generic) return (b.nA + tile - 1) / tile;
+  const uint64_t nY = b.trans ? b.nB : b.nA;
+  const uint64_t rows = tile >> b.logTB;
+  return ((nY + rows - 1) / rows) * b.ntc;
+}
+
+// The B-side trees of size m > Kt in rank order, as blocks whose values come
+// from at most two tables under the affine map (neg, k) of the unary chain
+// above them; blocks with a child above Kt are generic.  `start` is the rank
+// (at the B side's own size) of this subtree's rank 0.
+static void b_blocks(const CountTable &T, int Kt, const uint64_t *off, int m, uint64_t start,
+                     bool neg, uint64_t k, int rb, std::vector<BBlock> &out) {
+  uint64_t r = start;
+  for (int op = 0; op < 8; ++op) {
+    const uint64_t cnt = T.t[m][op];
+    if (cnt == 0) continue;
+    if (is_unary(op)) {
+      // s*u(x) + k with u(x) = -x - c: the map becomes (-s, k - s*c)
+      const uint64_t c = op == NOT ? 1 : 0;
+      const bool nneg = !neg;
+      const uint64_t nk = neg ? k + c : k - c;
+      if (m - 1 <= Kt) {
+        BBlock b{};
+        b.start = r;
+        b.offA = off[m - 1];
+        b.nA = T.t[m - 1][8];
+        b.nB = 1;
+        b.k = nk;
+        b.op = -1;
+        b.neg = nneg;
+        out.push_back(shape_block(b, rb));
+      } else {
+        b_blocks(T, Kt, off, m - 1, r, nneg, nk, rb, out);
+      }
+    } else {
+      const int cs = m - 1;
+      const int last = op == SUB ? cs - 1 : cs / 2;
+      uint64_t q = r;
+      for (int j = 1; j <= last; ++j) {
+        const uint64_t nl = T.t[j][8], nr = T.t[cs - j][8];
+        BBlock b{};
+        b.start = q;
+        if (j <= Kt && cs - j <= Kt) {
+          b.offA = off[j];
+          b.offB = off[cs - j];
+          b.nA = nl;
+          b.nB = nr;
+          b.k = k;
+          b.op = op;
+          b.neg = neg;
+          out.push_back(shape_block(b, rb));
+        } else {
+          b.nA = nl * nr;
+          b.generic = 1;
+          out.push_back(b);
+        }
+        q += nl * nr;
+      }
+    }
+    r += cnt;
+  }
+}
+
 static inline int64_t as_signed(uint64_t x, int W) {
   return W == 64 ? (int64_t)x : (int64_t)(int32_t)(uint32_t)x;
 }
@@ -354,44 +433,61 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   }
 
   // Runs the pair {j, m} (j <= m) of size s for the operator instances in
-  // `mask`, in launches of bounded work.  Returns 1 when the budget stopped it.
+  // `mask`, in launches of bounded work (B sides above Kt one block at a
+  // time).  Returns 1 when the budget stopped it.
+  std::map<int, std::vector<BBlock>> bcache;  // B blocks per size above Kt
   auto run_pair = [&](int s, int j, uint32_t mask, const CheckTargets &tg) -> int {
     const int m = s - 1 - j;
-    PairLaunch p;
+    PairLaunch p{};
     p.sb = s;
     p.j = j;
     p.m = m;
     p.nA = T.t[j][8];
     p.nB = T.t[m][8];
     p.nTA = (p.nA + kPairTA - 1) / kPairTA;
-    p.rb = o.lanes_b == 8 ? 8 : 4;
+    p.rb = o.lanes_b == 4 ? 4 : 8;
     const uint64_t tb = (uint64_t)kPairThreads * p.rb;
-    const uint64_t nTB = (p.nB + tb - 1) / tb;
-    const uint64_t tiles = p.nTA * nTB;
     p.gen_a = j > Kt;
     p.gen_b = m > Kt;
     p.mask = mask;
+    std::vector<BBlock> one(1);
+    const std::vector<BBlock> *blocks = &one;
+    if (p.gen_b) {
+      auto it = bcache.find(m * 16 + p.rb);
+      if (it == bcache.end()) {
+        std::vector<BBlock> v;
+        b_blocks(T, Kt, tv.off, m, 0, false, 0, p.rb, v);
+        it = bcache.emplace(m * 16 + p.rb, std::move(v)).first;
+      }
+      blocks = &it->second;
+    }
     const uint64_t per_tile = (uint64_t)kPairTA * tb;
     const uint64_t tiles_per_launch = std::max<uint64_t>(1, kPairsPerLaunch / per_tile);
-    for (uint64_t tb = 0; tb < tiles; tb += tiles_per_launch) {
-      p.tile_begin = tb;
-      p.tile_end = std::min(tiles, tb + tiles_per_launch);
-      const uint64_t ntl = p.tile_end - p.tile_begin;
-      const int g = (int)std::min<uint64_t>((uint64_t)grid, ntl);
-      hipError_t e = hipEventRecord(ctx.ev0, st);
-      if (e == hipSuccess)
-        e = launch_pair_check(W, d_tv, ctx.tables.p, tv.off, p, tg, d_hits, d_hc, kHitCap, g, st);
-      if (e == hipSuccess) e = hipEventRecord(ctx.ev1, st);
-      if (e == hipSuccess) e = hipEventSynchronize(ctx.ev1);
-      float ms = 0.f;
-      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1);
-      if (e != hipSuccess) {
-        ctx.last_error = std::string("pair launch: ") + hipGetErrorString(e);
-        return -1;
+    for (const BBlock &bb : *blocks) {
+      p.bb = bb;
+      p.nTB = p.gen_b ? block_tiles(bb, p.rb) : (p.nB + tb - 1) / tb;
+      const uint64_t tiles = p.nTA * p.nTB;
+      for (uint64_t t0 = 0; t0 < tiles; t0 += tiles_per_launch) {
+        p.tile_begin = t0;
+        p.tile_end = std::min(tiles, t0 + tiles_per_launch);
+        const uint64_t ntl = p.tile_end - p.tile_begin;
+        const int g = (int)std::min<uint64_t>((uint64_t)grid, ntl);
+        hipError_t e = hipEventRecord(ctx.ev0, st);
+        if (e == hipSuccess)
+          e = launch_pair_check(W, d_tv, ctx.tables.p, tv.off, p, tg, d_hits, d_hc, kHitCap, g,
+                                st);
+        if (e == hipSuccess) e = hipEventRecord(ctx.ev1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(ctx.ev1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1);
+        if (e != hipSuccess) {
+          ctx.last_error = std::string("pair launch: ") + hipGetErrorString(e);
+          return -1;
+        }
+        ktime += ms * 1e-3;
+        ++klaunch;
+        if (over_budget()) return 1;
       }
-      ktime += ms * 1e-3;
-      ++klaunch;
-      if (over_budget()) return 1;
     }
     return 0;
   };

@@ -32,14 +32,34 @@ struct CheckTargets {
   uint64_t lo_even, lo_odd, span;
 };
 
+// A block of B-side trees of size m > Kt whose sample-0 values each lane
+// forms from at most two tables:
+//   v = s * op(VA[ra], VB[rb]) + k   (op >= 0),   v = s * VA[ra] + k   (op < 0)
+// for ra < nA, rb < nB; the tree's rank within size m is start + ra*nB + rb.
+// (s, k) is the affine map x -> s*x + k of the unary chain above the block's
+// root (~x = -x - 1, neg x = -x; s = -1 when neg).  A tile covers 2^logTB
+// columns (over ra when trans, else over rb) by (256*RB) >> logTB rows; ntc
+// tiles span a row band.  generic: ranks [start, start + nA) of size m, each
+// evaluated by a table-assisted unrank (children above Kt).
+struct BBlock {
+  uint64_t start;
+  uint64_t offA, offB;
+  uint64_t nA, nB;
+  uint64_t k;
+  uint64_t ntc;
+  int32_t op, neg, generic, trans, logTB, pad;
+};
+
 struct PairLaunch {
   int sb, j, m;            // candidate size sb = j + m + 1, j <= m
   uint64_t nA, nB;         // T[j], T[m]
   uint64_t tile_begin, tile_end;  // tile range of this launch
   uint64_t nTA;            // number of A tiles
+  uint64_t nTB;            // number of B tiles (of the block when gen_b)
   bool gen_a, gen_b;       // side is above Kt: values computed per lane
   uint32_t mask;           // operator instances to check (bit i = inst i)
   int rb;                  // B values per lane: 4 or 8 (tile = 256 * rb B values)
+  BBlock bb;               // the B block (gen_b, per-instance launches)
 };
 
 constexpr uint32_t kAllInst = 0x7f;
@@ -48,7 +68,6 @@ constexpr int kPairThreads = 256;
 constexpr int kPairRB = 4;                      // B values held per lane
 constexpr int kPairTB = kPairThreads * kPairRB; // B values per tile
 constexpr int kPairTA = 256;                    // A values per tile
-constexpr int kAGroup = 8;                      // A values loaded per group
 
 // ---- batched search --------------------------------------------------------
 constexpr int kBuildTile = 1024;  // table values per build tile

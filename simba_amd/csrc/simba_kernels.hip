@@ -191,73 +191,36 @@ __device__ __noinline__ void record_hit(HitRec *hits, uint32_t *count, uint32_t 
 // SAME: j == m, so (Y - X) is the (X - Y) instance of the swapped pair and is
 // skipped.
 //
-// Inner loop (per A value, per B value held in VGPRs): the seven operator
-// results are folded into one running minimum of their distances to the two
-// windows, so each value costs its operator plus two subtractions and one
-// 3-way minimum; one compare per tile decides whether a lane saw a hit.
-// For 64-bit words only the high halves are formed and compared: a value in
-// a window of width span < 2^32 has hi(c) - hi(lo) in {0, 1} (mod 2^32), so
-// this is a necessary condition, and the rare lanes that pass it re-walk the
-// tile with the exact 64-bit window test.
+// Each lane holds RB B values; the A values of the tile are wave-uniform
+// (scalar registers), so everything that depends on A alone is formed once
+// per A value on the scalar unit.  Only the low 32 bits of a result are
+// tested in the loop: a value c inside a window [lo, lo + span] (span < 2^32)
+// has lo32(c) - lo32(lo) <= span (mod 2^32), and lo32 of & | ^ + - * needs
+// only the low halves of the operands (the 64-bit product's low half is one
+// 32-bit multiply).  Lanes that pass re-walk their tile with the exact
+// full-width test.
+//
+// Window mode (unary chains above the root still possible, span > 0): the
+// seven distances of a pair are folded into one running minimum; the
+// additive instances fold A into the window offset, so each costs one
+// subtraction per window:
+//   a + b - lo = b - (lo - a),  a - b - lo = (a - lo) - b,  b - a - lo = b - (a + lo).
+// Exact mode (span = 0: the candidate itself must equal the target t; the
+// odd window is empty because no unary node fits above the root): the
+// invertible instances become one compare of b against a uniform value
+// (b = a ^ t, t - a, a - t, t + a), * is one multiply and a compare, and
+// a & b = t (a | b = t) is only possible when t's bits lie inside a's (a's
+// inside t's), a uniform test that skips the instance for the whole tile row.
 // ---------------------------------------------------------------------------
-// Per-B-value precomputation, hoisted out of the A loop: the window offsets
-// are folded into B so that the additive instances need one add per window.
-//   64-bit words: offsets are multiples of 2^32 (He = hi(lo_even) << 32),
-//   which leave the low halves and carries unchanged, so hi(a + be) equals
-//   hi(a + b) - hi(lo_even) exactly.
-template <typename word>
-struct BPre {
-  word b, be, bo, nbe, nbo;  // b, b-Oe, b-Oo, -b-Oe, -b-Oo
-};
-
-template <typename word>
-__device__ __forceinline__ word win_off(word lo);
-template <>
-__device__ __forceinline__ uint32_t win_off<uint32_t>(uint32_t lo) { return lo; }
-template <>
-__device__ __forceinline__ uint64_t win_off<uint64_t>(uint64_t lo) {
-  return lo & 0xffffffff00000000ull;
-}
-
-template <typename word>
-__device__ __forceinline__ BPre<word> bpre(word b, word oe, word oo) {
-  BPre<word> p;
-  p.b = b;
-  p.be = b - oe;
-  p.bo = b - oo;
-  p.nbe = word(0) - b - oe;
-  p.nbo = word(0) - b - oo;
-  return p;
-}
-
-// The distance that is compared: the whole word for 32-bit words, the high
-// half for 64-bit words.
-__device__ __forceinline__ uint32_t dpart(uint32_t x) { return x; }
-__device__ __forceinline__ uint32_t dpart(uint64_t x) { return (uint32_t)(x >> 32); }
+__device__ __forceinline__ uint32_t lo32(uint32_t x) { return x; }
+__device__ __forceinline__ uint32_t lo32(uint64_t x) { return (uint32_t)x; }
 
 __device__ __forceinline__ uint32_t min3u(uint32_t a, uint32_t b, uint32_t c) {
   return min(min(a, b), c);
 }
 
-template <typename word, bool SAME>
-__device__ __forceinline__ uint32_t fold7(uint32_t acc, word a, word na, const BPre<word> &p,
-                                          uint32_t de, uint32_t dof) {
-  const uint32_t x_and = dpart(word(a & p.b));
-  const uint32_t x_or = dpart(word(a | p.b));
-  const uint32_t x_xor = dpart(word(a ^ p.b));
-  const uint32_t x_mul = dpart(word(a * p.b));
-  acc = min3u(acc, x_and - de, x_and - dof);
-  acc = min3u(acc, x_or - de, x_or - dof);
-  acc = min3u(acc, x_xor - de, x_xor - dof);
-  acc = min3u(acc, x_mul - de, x_mul - dof);
-  acc = min3u(acc, dpart(word(a + p.be)), dpart(word(a + p.bo)));    // a + b
-  acc = min3u(acc, dpart(word(a + p.nbe)), dpart(word(a + p.nbo)));  // a - b
-  if (!SAME) acc = min3u(acc, dpart(word(p.be + na)), dpart(word(p.bo + na)));  // b - a
-  return acc;
-}
-
-// Moves a wave-uniform word into scalar registers, so that expressions such
-// as b + (-a) keep their single-add form.
+// Moves a wave-uniform value into scalar registers, so that the expressions
+// built from it stay on the scalar unit.
 __device__ __forceinline__ uint32_t uniform(uint32_t x) {
   return __builtin_amdgcn_readfirstlane(x);
 }
@@ -328,139 +291,348 @@ __device__ __forceinline__ void gen_b_tile(const TableView *__restrict__ tv, int
   }
 }
 
-// One operator instance over the A tile: its distances folded into acc.
-template <typename word, int INST, bool GEN_A, int RB>
-__device__ __forceinline__ uint32_t fold_inst(uint32_t acc, const TableView *__restrict__ tv,
-                                              const word *__restrict__ A, int j, uint64_t a0,
-                                              uint64_t a1, const BPre<word> *pre, uint32_t de,
-                                              uint32_t dof) {
-  for (uint64_t a = a0; a < a1; ++a) {
-    const word av = GEN_A ? value_of<word>(tv, j, a) : A[a];
-    const word na = uniform(word(word(0) - av));
+// Per-A values of a tile, staged in LDS by the workgroup (one A value per
+// lane) and read back as wave-uniform broadcasts inside the loop:
+//   exact mode:  q0 = {a, a ^ t, t - a, a - t},  q1 = {t + a, bit-op gap, -, -}
+//   window mode: q0 = {a, lo_e - a, lo_o - a, a - lo_e},
+//                q1 = {a - lo_o, a + lo_e, a + lo_o, -}
+// (low halves; t = lo_e in exact mode).
+struct AStage {
+  uint4 q0[kPairTA];
+  uint4 q1[kPairTA];
+};
+
+// a & b = t needs t inside a, a | b = t needs a inside t: zero when either
+// instance can hit for some b.
+__device__ __forceinline__ uint32_t bitops_gap(uint32_t al, uint32_t tl) {
+  return min(tl & ~al, al & ~tl);
+}
+
+// The exact-mode layout is staged when every operator instance is checked
+// against a single target (span 0); otherwise the window layout.
+__device__ __forceinline__ bool exact_layout(uint32_t mask, bool same, uint32_t span) {
+  const uint32_t full = same ? 0x3fu : 0x7fu;
+  return span == 0u && (mask & full) == full;
+}
+
+// Stages A values [a0, a1) (GEN_A: computed per lane from the tables).
+// Returns, for the exact mode, whether any A value admits & or | hits.
+template <typename word>
+__device__ __forceinline__ bool stage_a(AStage &S, const TableView *__restrict__ tv,
+                                        const word *__restrict__ A, bool gen_a, int j,
+                                        uint64_t a0, uint64_t a1, bool exact, uint32_t le,
+                                        uint32_t lo) {
+  __syncthreads();  // the previous tile's readers are done
+  const uint32_t i = threadIdx.x;
+  bool bit = false;
+  if (a0 + i < a1) {
+    const word av = gen_a ? value_of<word>(tv, j, a0 + i) : A[a0 + i];
+    const uint32_t al = lo32(av);
+    if (exact) {
+      const uint32_t gap = bitops_gap(al, le);
+      S.q0[i] = make_uint4(al, al ^ le, le - al, al - le);
+      S.q1[i] = make_uint4(le + al, gap, 0u, 0u);
+      bit = gap == 0u;
+    } else {
+      S.q0[i] = make_uint4(al, le - al, lo - al, al - le);
+      S.q1[i] = make_uint4(al - lo, al + le, al + lo, 0u);
+    }
+  }
+  return __syncthreads_or(bit) != 0;
+}
+
+// ---- B side sources --------------------------------------------------------
+enum : int { kBTable = 0, kBDesc = 1, kBGeneric = 2, kBStep = 3 };
+
+template <typename word>
+__device__ __forceinline__ uint32_t lo32_at(const word *p, uint64_t i) {
+  return lo32(p[i]);
+}
+template <>
+__device__ __forceinline__ uint32_t lo32_at<uint64_t>(const uint64_t *p, uint64_t i) {
+  return ((const uint32_t *)p)[2 * i];  // little-endian low half
+}
+
+// Coordinates of the k-th B value of this lane in block tile bt.
+template <int RB>
+__device__ __forceinline__ bool desc_coords(const BBlock &bb, uint64_t bt, int k, uint64_t &ra,
+                                            uint64_t &rb) {
+  const uint64_t tr = bb.ntc == 1 ? bt : bt / bb.ntc;
+  const uint64_t tc = bt - tr * bb.ntc;
+  const uint32_t idx = threadIdx.x + (uint32_t)k * kPairThreads;
+  const uint32_t TB = 1u << bb.logTB;
+  const uint64_t x = tc * TB + (idx & (TB - 1));
+  const uint64_t y = tr * (uint64_t)((kPairThreads * RB) >> bb.logTB) + (idx >> bb.logTB);
+  ra = bb.trans ? x : y;
+  rb = bb.trans ? y : x;
+  return ra < bb.nA && rb < bb.nB;
+}
+
+template <typename word>
+__device__ __forceinline__ word desc_value(const BBlock &bb, const word *V, uint64_t ra,
+                                           uint64_t rb) {
+  word v = V[bb.offA + ra];
+  if (bb.op >= 0) v = apply_op<word>(bb.op, v, V[bb.offB + rb]);
+  if (bb.neg) v = word(0) - v;
+  return word(v + (word)bb.k);
+}
+
+// Low 32 bits of the RB B values of this lane in a block tile.  Tiles of one
+// row (the block's columns span at least a tile) take the row's value with a
+// uniform load and the columns with contiguous loads; smaller blocks use the
+// general 2-D mapping.  The unary chain: -v = (v ^ ~0) - ~0, so
+// s*v + k = (v ^ sg) + (k - sg) with sg = 0 or ~0.
+template <typename word, int RB, int OP>
+__device__ __forceinline__ void desc_lo_op(const BBlock &bb, const word *V, uint64_t bt,
+                                           uint32_t *bl, bool *ok) {
+  constexpr uint32_t tile = kPairThreads * RB;
+  const uint32_t sg = bb.neg ? 0xffffffffu : 0u;
+  const uint32_t kk = (uint32_t)bb.k - sg;
+  auto comb = [](uint32_t a, uint32_t b) -> uint32_t {
+    return OP == AND ? (a & b) : OP == OR ? (a | b) : OP == XOR ? (a ^ b)
+         : OP == ADD ? (a + b) : OP == SUB ? (a - b) : OP == MUL ? (a * b) : a;
+  };
+  if ((1u << bb.logTB) == tile) {
+    const uint64_t tr = bb.ntc == 1 ? bt : bt / bb.ntc;
+    const uint64_t x0 = (bt - tr * bb.ntc) * tile;
+    const uint64_t nX = bb.trans ? bb.nA : bb.nB;
+    const uint64_t left = nX - x0;
+    const uint32_t rem = left > tile ? tile : (uint32_t)left;
+    // columns run over ra (trans) or rb; the row is the other child
+    const word *col = V + (bb.trans ? bb.offA : bb.offB) + x0;
+    const uint32_t rowv = OP >= 0 ? lo32_at<word>(V, (bb.trans ? bb.offB : bb.offA) + tr) : 0u;
 #pragma unroll
     for (int k = 0; k < RB; ++k) {
-      const BPre<word> &q = pre[k];
-      if (INST <= 2 || INST == 4) {
-        const word c = INST == 0 ? word(av & q.b) : INST == 1 ? word(av | q.b)
-                     : INST == 2 ? word(av ^ q.b) : word(av * q.b);
-        const uint32_t x = dpart(c);
-        acc = min3u(acc, x - de, x - dof);
-      } else if (INST == 3) {
-        acc = min3u(acc, dpart(word(av + q.be)), dpart(word(av + q.bo)));
-      } else if (INST == 5) {
-        acc = min3u(acc, dpart(word(av + q.nbe)), dpart(word(av + q.nbo)));
-      } else {
-        acc = min3u(acc, dpart(word(q.be + na)), dpart(word(q.bo + na)));
+      const uint32_t off = threadIdx.x + (uint32_t)k * kPairThreads;
+      ok[k] = off < rem;
+      const uint32_t c = lo32_at<word>(col, ok[k] ? off : 0u);
+      const uint32_t v = OP < 0 ? c : bb.trans ? comb(c, rowv) : comb(rowv, c);
+      bl[k] = (v ^ sg) + kk;
+    }
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < RB; ++k) {
+    uint64_t ra, rb;
+    ok[k] = desc_coords<RB>(bb, bt, k, ra, rb);
+    if (!ok[k]) ra = 0, rb = 0;
+    uint32_t v = lo32_at<word>(V, bb.offA + ra);
+    if (OP >= 0) v = comb(v, lo32_at<word>(V, bb.offB + rb));
+    bl[k] = (v ^ sg) + kk;
+  }
+}
+
+template <typename word, int RB>
+__device__ __forceinline__ void desc_lo(const BBlock &bb, const word *V, uint64_t bt,
+                                        uint32_t *bl, bool *ok) {
+  switch (bb.op) {
+    case AND: desc_lo_op<word, RB, AND>(bb, V, bt, bl, ok); break;
+    case OR: desc_lo_op<word, RB, OR>(bb, V, bt, bl, ok); break;
+    case XOR: desc_lo_op<word, RB, XOR>(bb, V, bt, bl, ok); break;
+    case ADD: desc_lo_op<word, RB, ADD>(bb, V, bt, bl, ok); break;
+    case SUB: desc_lo_op<word, RB, SUB>(bb, V, bt, bl, ok); break;
+    case MUL: desc_lo_op<word, RB, MUL>(bb, V, bt, bl, ok); break;
+    default: desc_lo_op<word, RB, -1>(bb, V, bt, bl, ok); break;
+  }
+}
+
+// Full B value and rank (within size m) of the k-th value of this lane
+// (used by the exact re-walk only).
+template <typename word, int BSRC, int RB>
+__device__ __forceinline__ word b_full(const TableView *__restrict__ tv,
+                                       const word *__restrict__ B, const PairLaunch &p,
+                                       uint64_t bt, int k, uint64_t &rank) {
+  if (BSRC == kBDesc) {
+    uint64_t ra, rb;
+    desc_coords<RB>(p.bb, bt, k, ra, rb);
+    rank = p.bb.start + ra * p.bb.nB + rb;
+    return desc_value<word>(p.bb, (const word *)tv->base, ra, rb);
+  }
+  const uint64_t local = bt * (kPairThreads * RB) + threadIdx.x + (uint64_t)k * kPairThreads;
+  if (BSRC == kBGeneric) {
+    rank = p.bb.start + local;
+    return value_of<word>(tv, p.m, rank);
+  }
+  rank = local;
+  if (BSRC == kBTable) return B[local];
+  return value_of<word>(tv, p.m, local);
+}
+
+// Low halves of this lane's B values of tile bt (and their validity).
+template <typename word, int BSRC, int RB>
+__device__ __forceinline__ void b_lo(const TableView *__restrict__ tv,
+                                     const word *__restrict__ B, const PairLaunch &p,
+                                     uint64_t bt, uint32_t *bl, bool *ok) {
+  if (BSRC == kBDesc) {
+    desc_lo<word, RB>(p.bb, (const word *)tv->base, bt, bl, ok);
+    return;
+  }
+  const uint64_t b0 = bt * (kPairThreads * RB) + threadIdx.x;
+  const uint64_t n = BSRC == kBGeneric ? p.bb.nA : p.nB;
+  if (BSRC == kBStep) {
+    word bv[RB];
+#pragma unroll
+    for (int k = 0; k < RB; ++k) ok[k] = b0 + (uint64_t)k * kPairThreads < n;
+    if (p.m == tv->Kt + 1) {
+      gen_b_tile<word, RB>(tv, p.m, bt * (kPairThreads * RB), b0, bv, ok);
+    } else {
+#pragma unroll
+      for (int k = 0; k < RB; ++k)
+        bv[k] = ok[k] ? value_of<word>(tv, p.m, b0 + (uint64_t)k * kPairThreads) : word(0);
+    }
+#pragma unroll
+    for (int k = 0; k < RB; ++k) bl[k] = lo32(bv[k]);
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < RB; ++k) {
+    const uint64_t b = b0 + (uint64_t)k * kPairThreads;
+    ok[k] = b < n;
+    if (BSRC == kBTable)
+      bl[k] = ok[k] ? lo32_at<word>(B, b) : 0u;
+    else
+      bl[k] = ok[k] ? lo32(value_of<word>(tv, p.m, p.bb.start + b)) : 0u;
+  }
+}
+
+// ---- the fused checks -------------------------------------------------------
+// Exact mode: each compare's lane mask is or-ed into one wave-wide mask on
+// the scalar unit (one VALU compare + one scalar or per candidate).
+template <bool SAME, int RB>
+__device__ __forceinline__ void exact_a(uint64_t &m, const uint4 q0, uint32_t q_rsub,
+                                        const uint32_t *bl, uint32_t tl) {
+#pragma unroll
+  for (int k = 0; k < RB; ++k) {
+    const uint32_t b = bl[k];
+    m |= __builtin_amdgcn_ballot_w64(b == q0.y);            // a ^ b = t
+    m |= __builtin_amdgcn_ballot_w64(b == q0.z);            // a + b = t
+    m |= __builtin_amdgcn_ballot_w64(b == q0.w);            // a - b = t
+    if (!SAME) m |= __builtin_amdgcn_ballot_w64(b == q_rsub);  // b - a = t
+    m |= __builtin_amdgcn_ballot_w64(q0.x * b == tl);       // a * b = t
+  }
+}
+
+template <bool SAME, int RB>
+__device__ __forceinline__ bool exact_tile(const AStage &S, uint32_t na, const uint32_t *bl,
+                                           uint32_t tl, bool bitops) {
+  uint64_t m = 0;
+  uint32_t i;
+#pragma unroll 1
+  for (i = 0; i < na; ++i) exact_a<SAME, RB>(m, S.q0[i], SAME ? 0u : S.q1[i].x, bl, tl);
+  if (bitops) {  // rare: & and | for the A values that admit them
+    for (i = 0; i < na; ++i) {
+      if (S.q1[i].y != 0u) continue;
+      const uint32_t al = S.q0[i].x;
+#pragma unroll
+      for (int k = 0; k < RB; ++k) {
+        m |= __builtin_amdgcn_ballot_w64((al & bl[k]) == tl);
+        m |= __builtin_amdgcn_ballot_w64((al | bl[k]) == tl);
       }
+    }
+  }
+  return (m >> (threadIdx.x & 63)) & 1ull;
+}
+
+template <bool SAME, int RB>
+__device__ __forceinline__ uint32_t window_tile(const AStage &S, uint32_t na,
+                                                const uint32_t *bl, uint32_t le, uint32_t lo) {
+  uint32_t acc = 0xffffffffu;
+  for (uint32_t i = 0; i < na; ++i) {
+    const uint4 q0 = S.q0[i];
+    const uint4 q1 = S.q1[i];
+#pragma unroll
+    for (int k = 0; k < RB; ++k) {
+      const uint32_t b = bl[k];
+      const uint32_t x_and = q0.x & b, x_or = q0.x | b, x_xor = q0.x ^ b, x_mul = q0.x * b;
+      acc = min3u(acc, x_and - le, x_and - lo);
+      acc = min3u(acc, x_or - le, x_or - lo);
+      acc = min3u(acc, x_xor - le, x_xor - lo);
+      acc = min3u(acc, x_mul - le, x_mul - lo);
+      acc = min3u(acc, b - q0.y, b - q0.z);                // a + b
+      acc = min3u(acc, q0.w - b, q1.x - b);                // a - b
+      if (!SAME) acc = min3u(acc, b - q1.y, b - q1.z);     // b - a
+    }
+  }
+  return acc;
+}
+
+// One operator instance (block-at-a-time mode, window form, any span).
+template <int RB>
+__device__ __forceinline__ uint32_t inst_tile(const AStage &S, uint32_t na, int inst,
+                                              const uint32_t *bl, uint32_t le, uint32_t lo,
+                                              uint32_t acc) {
+  for (uint32_t i = 0; i < na; ++i) {
+    const uint4 q0 = S.q0[i];
+    const uint4 q1 = S.q1[i];
+#pragma unroll
+    for (int k = 0; k < RB; ++k) {
+      const uint32_t b = bl[k];
+      uint32_t d0, d1;
+      switch (inst) {
+        case 0: d0 = (q0.x & b) - le; d1 = (q0.x & b) - lo; break;
+        case 1: d0 = (q0.x | b) - le; d1 = (q0.x | b) - lo; break;
+        case 2: d0 = (q0.x ^ b) - le; d1 = (q0.x ^ b) - lo; break;
+        case 3: d0 = b - q0.y; d1 = b - q0.z; break;
+        case 4: d0 = q0.x * b - le; d1 = q0.x * b - lo; break;
+        case 5: d0 = q0.w - b; d1 = q1.x - b; break;
+        default: d0 = b - q1.y; d1 = b - q1.z; break;
+      }
+      acc = min3u(acc, d0, d1);
     }
   }
   return acc;
 }
 
 // One tile of a pair check (shared by the per-instance and batched kernels).
-// p.mask selects the operator instances (bit i: instance i, see HitRec).
-// `tag` is or-ed into HitRec::inst above bit 8 (the batch's instance index).
-template <typename word, bool SAME, bool GEN_A, bool GEN_B, int RB>
+// The A values of the tile must be staged in S (stage_a).  p.mask selects the
+// operator instances (bit i: instance i, see HitRec); `tag` is or-ed into
+// HitRec::inst above bit 8 (the batch's instance index).
+template <typename word, bool SAME, int BSRC, int RB>
 __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
-                                          const word *__restrict__ A,
+                                          const AStage &S, const word *__restrict__ A,
                                           const word *__restrict__ B, const PairLaunch &p,
-                                          uint64_t t, word lo_e, word lo_o, word span,
-                                          HitRec *__restrict__ hits,
+                                          uint64_t at, uint64_t bt, bool bitops, word lo_e,
+                                          word lo_o, word span, HitRec *__restrict__ hits,
                                           uint32_t *__restrict__ hit_count, uint32_t hit_cap,
                                           uint32_t tag) {
-  // prefilter threshold on the folded distance
-  const uint32_t thr = sizeof(word) == 8 ? 1u : (uint32_t)span;
   const uint32_t pair = ((uint32_t)p.sb << 16) | (uint32_t)p.j;
-  {
-    const uint64_t at = p.nTA == 1 ? 0 : t % p.nTA, bt = p.nTA == 1 ? t : t / p.nTA;
-    const uint64_t a0 = at * kPairTA;
-    const uint64_t a1 = a0 + kPairTA < p.nA ? a0 + kPairTA : p.nA;
-    const uint64_t b0 = bt * (kPairThreads * RB) + threadIdx.x;
-    word bv[RB];
-    bool ok[RB];
-    bool any_ok = false;
+  const uint64_t a0 = at * kPairTA;
+  const uint64_t a1 = a0 + kPairTA < p.nA ? a0 + kPairTA : p.nA;
+  const uint32_t na = (uint32_t)(a1 - a0);
+  uint32_t bl[RB];
+  bool ok[RB];
+  b_lo<word, BSRC, RB>(tv, B, p, bt, bl, ok);
+  bool any_ok = false;
 #pragma unroll
-    for (int k = 0; k < RB; ++k) {
-      const uint64_t b = b0 + (uint64_t)k * kPairThreads;
-      ok[k] = b < p.nB;
-      any_ok |= ok[k];
-      if (!GEN_B) bv[k] = ok[k] ? B[b] : word(0);
-    }
-    if (GEN_B) {
-      if (p.m == tv->Kt + 1) {
-        gen_b_tile<word, RB>(tv, p.m, bt * (kPairThreads * RB), b0, bv, ok);
-      } else {
-#pragma unroll
-        for (int k = 0; k < RB; ++k)
-          bv[k] = ok[k] ? value_of<word>(tv, p.m, b0 + (uint64_t)k * kPairThreads) : word(0);
-      }
-    }
-    // lanes past the end of B reuse a valid value of their own (recorded
-    // hits are re-derived with the exact masks below)
-#pragma unroll
-    for (int k = 1; k < RB; ++k)
-      if (!ok[k]) bv[k] = bv[0];
-    const word oe = win_off<word>(lo_e), oo = win_off<word>(lo_o);
-    const uint32_t de = dpart(oe), dof = dpart(oo);
-    BPre<word> pre[RB];
-#pragma unroll
-    for (int k = 0; k < RB; ++k) pre[k] = bpre<word>(bv[k], oe, oo);
+  for (int k = 0; k < RB; ++k) any_ok |= ok[k];
+  const uint32_t le = lo32(lo_e), lo = lo32(lo_o), sp = (uint32_t)span;
+  const uint32_t full = SAME ? 0x3fu : 0x7fu;
+  bool hit;
+  if (exact_layout(p.mask, SAME, sp)) {
+    hit = exact_tile<SAME, RB>(S, na, bl, le, bitops);
+  } else if ((p.mask & full) == full) {
+    hit = window_tile<SAME, RB>(S, na, bl, le, lo) <= sp;
+  } else {
     uint32_t acc = 0xffffffffu;
-    const uint32_t full = SAME ? 0x3fu : 0x7fu;
-    if ((p.mask & full) == full) {
-      uint64_t a = a0;
-      // A values in groups of kAGroup: their scalar loads are issued together
-      // and waited for once
-      for (; a + kAGroup <= a1; a += kAGroup) {
-        word av[kAGroup];
-#pragma unroll
-        for (int i = 0; i < kAGroup; ++i)
-          av[i] = GEN_A ? value_of<word>(tv, p.j, a + i) : A[a + i];
-#pragma unroll
-        for (int i = 0; i < kAGroup; ++i) {
-          const word na = uniform(word(word(0) - av[i]));
-#pragma unroll
-          for (int k = 0; k < RB; ++k)
-            acc = fold7<word, SAME>(acc, av[i], na, pre[k], de, dof);
-        }
-      }
-      for (; a < a1; ++a) {
-        const word av = GEN_A ? value_of<word>(tv, p.j, a) : A[a];
-        const word na = uniform(word(word(0) - av));
-#pragma unroll
-        for (int k = 0; k < RB; ++k) acc = fold7<word, SAME>(acc, av, na, pre[k], de, dof);
-      }
-    } else {
-      // a subset of the operator instances (one block at a time): one pass
-      // over the A tile per selected instance
-      for (int inst = 0; inst < 7; ++inst) {
-        if (!((p.mask & full) >> inst & 1u)) continue;
-        switch (inst) {
-          case 0: acc = fold_inst<word, 0, GEN_A, RB>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
-          case 1: acc = fold_inst<word, 1, GEN_A, RB>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
-          case 2: acc = fold_inst<word, 2, GEN_A, RB>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
-          case 3: acc = fold_inst<word, 3, GEN_A, RB>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
-          case 4: acc = fold_inst<word, 4, GEN_A, RB>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
-          case 5: acc = fold_inst<word, 5, GEN_A, RB>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
-          default: acc = fold_inst<word, 6, GEN_A, RB>(acc, tv, A, p.j, a0, a1, pre, de, dof); break;
-        }
-      }
-    }
-    const bool hit = any_ok && acc <= thr;
-    // Rare: some lane passed the prefilter.  Re-walk the tile, exact test.
-    if (__any(hit)) {
-      if (hit) {
+    for (int inst = 0; inst < 7; ++inst)
+      if ((p.mask & full) >> inst & 1u) acc = inst_tile<RB>(S, na, inst, bl, le, lo, acc);
+    hit = acc <= sp;
+  }
+  hit = hit && any_ok;
+  // Rare: some lane passed the prefilter.  Re-walk the tile, exact test.
+  if (__any(hit)) {
+    if (hit) {
+      for (int k = 0; k < RB; ++k) {
+        if (!ok[k]) continue;
+        uint64_t b;
+        const word bv = b_full<word, BSRC, RB>(tv, B, p, bt, k, b);
         for (uint64_t a = a0; a < a1; ++a) {
-          const word av = GEN_A ? value_of<word>(tv, p.j, a) : A[a];
-          for (int k = 0; k < RB; ++k) {
-            if (!ok[k]) continue;
-            const uint64_t b = b0 + (uint64_t)k * kPairThreads;
-            for (int inst = 0; inst < (SAME ? 6 : 7); ++inst) {
-              if (!(p.mask >> inst & 1u)) continue;
-              const word c = inst_value<word>(inst, av, bv[k]);
-              if (in_window(c, lo_e, lo_o, span))
-                record_hit<word>(hits, hit_count, hit_cap, a, b, c, inst | tag, pair);
-            }
+          const word av = p.gen_a ? value_of<word>(tv, p.j, a) : A[a];
+          for (int inst = 0; inst < (SAME ? 6 : 7); ++inst) {
+            if (!(p.mask >> inst & 1u)) continue;
+            const word c = inst_value<word>(inst, av, bv);
+            if (in_window(c, lo_e, lo_o, span))
+              record_hit<word>(hits, hit_count, hit_cap, a, b, c, inst | tag, pair);
           }
         }
       }
@@ -468,26 +640,29 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
   }
 }
 
-template <typename word, bool SAME, bool GEN_A, bool GEN_B, int RB>
+template <typename word, bool SAME, int BSRC, int RB>
 __global__ void __launch_bounds__(kPairThreads)
 k_pair_check(const TableView *__restrict__ tv, const word *__restrict__ A,
              const word *__restrict__ B, PairLaunch p, uint64_t lo_e64, uint64_t lo_o64,
              uint64_t span64, HitRec *__restrict__ hits, uint32_t *__restrict__ hit_count,
              uint32_t hit_cap) {
-  if (!GEN_A && p.nTA == 1) {
-    // the whole A side fits one tile: stage it in LDS once per workgroup
-    // and reuse it for every B tile this workgroup takes
-    __shared__ word sA[kPairTA];
-    for (int i = threadIdx.x; i < (int)p.nA; i += blockDim.x) sA[i] = A[i];
-    __syncthreads();
-    for (uint64_t t = p.tile_begin + blockIdx.x; t < p.tile_end; t += gridDim.x)
-      pair_tile<word, SAME, GEN_A, GEN_B, RB>(tv, sA, B, p, t, (word)lo_e64, (word)lo_o64,
-                                          (word)span64, hits, hit_count, hit_cap, 0u);
-    return;
+  __shared__ AStage S;
+  const word lo_e = (word)lo_e64, lo_o = (word)lo_o64, span = (word)span64;
+  const bool exact = exact_layout(p.mask, SAME, (uint32_t)span);
+  const uint32_t le = lo32(lo_e), lo = lo32(lo_o);
+  uint64_t staged = ~0ull;
+  bool bitops = false;
+  for (uint64_t t = p.tile_begin + blockIdx.x; t < p.tile_end; t += gridDim.x) {
+    const uint64_t at = p.nTA == 1 ? 0 : t % p.nTA, bt = p.nTA == 1 ? t : t / p.nTA;
+    if (at != staged) {  // workgroup-uniform: every wave takes the same tiles
+      const uint64_t a0 = at * kPairTA;
+      const uint64_t a1 = a0 + kPairTA < p.nA ? a0 + kPairTA : p.nA;
+      bitops = stage_a<word>(S, tv, A, p.gen_a, p.j, a0, a1, exact, le, lo);
+      staged = at;
+    }
+    pair_tile<word, SAME, BSRC, RB>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o, span, hits,
+                                    hit_count, hit_cap, 0u);
   }
-  for (uint64_t t = p.tile_begin + blockIdx.x; t < p.tile_end; t += gridDim.x)
-    pair_tile<word, SAME, GEN_A, GEN_B, RB>(tv, A, B, p, t, (word)lo_e64, (word)lo_o64,
-                                        (word)span64, hits, hit_count, hit_cap, 0u);
 }
 
 // Leaves (size-1 trees) against the same windows: exact leaf solutions and
@@ -588,7 +763,7 @@ k_verify(const TableView *__restrict__ tv, const uint64_t *__restrict__ leafvals
 // leaves once the counter passes the step's total, so the grid drains.
 // ---------------------------------------------------------------------------
 template <typename word>
-__device__ void batch_pair(const TableView *tv, const BatchJob &J, uint64_t lt,
+__device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J, uint64_t lt,
                            HitRec *hits, uint32_t *hc, uint32_t cap) {
   const word *base = (const word *)tv->base;
   const PairLaunch &p = J.p;
@@ -596,17 +771,25 @@ __device__ void batch_pair(const TableView *tv, const BatchJob &J, uint64_t lt,
   const word *B = base + (p.gen_b ? 0 : tv->off[p.m]);
   const word lo_e = (word)J.lo_e, lo_o = (word)J.lo_o, span = (word)J.span;
   const uint32_t tag = J.inst << 8;
+  const uint64_t at = p.nTA == 1 ? 0 : lt % p.nTA, bt = p.nTA == 1 ? lt : lt / p.nTA;
+  const uint64_t a0 = at * kPairTA;
+  const uint64_t a1 = a0 + kPairTA < p.nA ? a0 + kPairTA : p.nA;
+  const bool bitops = stage_a<word>(S, tv, A, p.gen_a, p.j, a0, a1,
+                                    exact_layout(p.mask, p.j == p.m, (uint32_t)span),
+                                    lo32(lo_e), lo32(lo_o));
   if (p.j == p.m) {
-    if (p.gen_a)
-      pair_tile<word, true, true, true, kPairRB>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
+    if (p.gen_b)
+      pair_tile<word, true, kBStep, kPairRB>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o, span,
+                                             hits, hc, cap, tag);
     else
-      pair_tile<word, true, false, false, kPairRB>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
-  } else if (p.gen_a) {
-    pair_tile<word, false, true, true, kPairRB>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
+      pair_tile<word, true, kBTable, kPairRB>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o, span,
+                                              hits, hc, cap, tag);
   } else if (p.gen_b) {
-    pair_tile<word, false, false, true, kPairRB>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
+    pair_tile<word, false, kBStep, kPairRB>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o, span,
+                                            hits, hc, cap, tag);
   } else {
-    pair_tile<word, false, false, false, kPairRB>(tv, A, B, p, lt, lo_e, lo_o, span, hits, hc, cap, tag);
+    pair_tile<word, false, kBTable, kPairRB>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o, span,
+                                             hits, hc, cap, tag);
   }
 }
 
@@ -617,6 +800,7 @@ k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jo
              uint32_t *hit_count, uint32_t hit_cap) {
   __shared__ unsigned long long s_t;
   __shared__ int s_job;
+  __shared__ AStage S;
   const uint64_t total = step->total_tiles;
   const int njobs = (int)step->njobs;
   for (;;) {
@@ -648,7 +832,7 @@ k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jo
       for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x)
         out[r] = build_value<word>(tv, J.p.m, r);
     } else if (J.kind == 1) {
-      batch_pair<word>(tv, J, lt, hits, hit_count, hit_cap);
+      batch_pair<word>(S, tv, J, lt, hits, hit_count, hit_cap);
     } else {  // leaves against the windows of base size 1
       const word *V = (const word *)tv->base + tv->off[1];
       if ((int)threadIdx.x < tv->L) {
@@ -888,22 +1072,25 @@ static void pair_dispatch(const TableView *d_tv, const word *A, const word *B,
                           const PairLaunch &p, const CheckTargets &tg, HitRec *h,
                           uint32_t *hc, uint32_t cap, int grid, hipStream_t s) {
   const bool same = p.j == p.m;
-#define PAIR_LAUNCH(S, GA, GB)                                                       \
+#define PAIR_LAUNCH(S, BS)                                                           \
   do {                                                                               \
     if (p.rb == 8)                                                                   \
-      hipLaunchKernelGGL((k_pair_check<word, S, GA, GB, 8>), dim3(grid), dim3(kPairThreads), \
+      hipLaunchKernelGGL((k_pair_check<word, S, BS, 8>), dim3(grid), dim3(kPairThreads), \
                          0, s, d_tv, A, B, p, tg.lo_even, tg.lo_odd, tg.span, h, hc, cap); \
     else                                                                             \
-      hipLaunchKernelGGL((k_pair_check<word, S, GA, GB, 4>), dim3(grid), dim3(kPairThreads), \
+      hipLaunchKernelGGL((k_pair_check<word, S, BS, 4>), dim3(grid), dim3(kPairThreads), \
                          0, s, d_tv, A, B, p, tg.lo_even, tg.lo_odd, tg.span, h, hc, cap); \
   } while (0)
+  // the B side: its table, a block formed from tables, or a generic rank range
+  const int bs = !p.gen_b ? kBTable : p.bb.generic ? kBGeneric : kBDesc;
   if (same) {
-    if (p.gen_a) PAIR_LAUNCH(true, true, true);
-    else PAIR_LAUNCH(true, false, false);
+    if (bs == kBTable) PAIR_LAUNCH(true, kBTable);
+    else if (bs == kBDesc) PAIR_LAUNCH(true, kBDesc);
+    else PAIR_LAUNCH(true, kBGeneric);
   } else {
-    if (p.gen_a) PAIR_LAUNCH(false, true, true);
-    else if (p.gen_b) PAIR_LAUNCH(false, false, true);
-    else PAIR_LAUNCH(false, false, false);
+    if (bs == kBTable) PAIR_LAUNCH(false, kBTable);
+    else if (bs == kBDesc) PAIR_LAUNCH(false, kBDesc);
+    else PAIR_LAUNCH(false, kBGeneric);
   }
 #undef PAIR_LAUNCH
 }

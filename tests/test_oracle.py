@@ -110,6 +110,21 @@ def test_simba_search_is_min_rank_brute_force(seed):
         assert (got["size"], got["rank"], tuple(got["rpn"])) == want
 
 
+def test_simba_exhaustive_sweep_keeps_first_solution():
+    rng = np.random.default_rng(9)
+    x = rng.integers(0, 2**64, size=8, dtype=np.uint64)
+    y = rng.integers(0, 2**64, size=8, dtype=np.uint64)
+    inp = np.stack([x, y], axis=1).reshape(-1)
+    first = orc.simba_search(inp, x + y, 2, (), 64, 1, 6, 4)
+    full = orc.simba_search(inp, x + y, 2, (), 64, 1, 6, 4, exhaustive=True)
+    T = orc.count_table(2, 6)
+    assert (full["size"], full["rank"], full["expr"]) == (first["size"], first["rank"],
+                                                          first["expr"]) == (3, first["rank"],
+                                                                             "(v0 + v1)")
+    assert full["candidates"] == sum(int(T[s][8]) for s in range(1, 7))
+    assert first["candidates"] < full["candidates"]
+
+
 def test_simba_known_answers():
     rng = np.random.default_rng(5)
     x = rng.integers(0, 2**32, size=16, dtype=np.uint64)
