@@ -127,6 +127,16 @@ void mba_default_opts(mba_opts *o);
 int mba_simba_solve(mba_ctx *ctx, const mba_instance *inst, const mba_opts *opts,
                     mba_result *res);
 
+/* SIMBA-RTid ablation (nocache_rtid.cu, the same driver as nocache.cu with
+ * the kernel of :85-95): in each (size, op) block of N trees, thread g of
+ * the ceil(N/1024)*1024 launched checks rank ((g * 2246822507) mod 2^64)
+ * mod N, so some ranks are visited twice and others never when
+ * gcd(2246822507, N) > 1.  The first block with a match decides; within it
+ * the tree of the smallest such g (the reference keeps whichever thread
+ * wins atomicCAS).  Size 1 has no op block and is not searched. */
+int mba_simba_rtid_solve(mba_ctx *ctx, const mba_instance *inst, const mba_opts *opts,
+                         mba_result *res);
+
 /* VFB bottom-up cached search (replaces MBA(), full-cache.cu:665-774) with
  * the intended keep-iff-fingerprint-new dedup and the first kept match in
  * enumeration order as the solution.  `stats` may be NULL. */

@@ -373,6 +373,114 @@ int orc_simba_search(int n_vars, int n_consts, const uint64_t *consts, int W,
 }
 
 /* ------------------------------------------------------------------------ */
+/* SIMBA-RTid: nocache_rtid.cu:85-95.  Each (size, op) block of N trees is    */
+/* launched as ceil(N/1024) x 1024 threads; thread g checks the tree of rank  */
+/* offset + ((g * C) mod 2^64) mod N with C = 2246822507, so the visited      */
+/* ranks are a multiset that may miss ranks when gcd(C, N) > 1.  The first    */
+/* (size, op) block, in order, with a visited solution decides; the          */
+/* reference keeps the thread that wins atomicCAS, restated here as the      */
+/* smallest thread index g (`rank` = its rank, `candidates` = threads run).   */
+/* ------------------------------------------------------------------------ */
+#define ORC_RTID_C 2246822507ull
+
+typedef struct {
+  const uint64_t *T;
+  int size, L, n_samples, W;
+  const uint64_t *leafvals, *outputs;
+  uint64_t offset, N, G;
+  _Atomic uint64_t next_chunk;
+  _Atomic uint64_t best_g;
+  _Atomic uint64_t evaluated;
+} rtid_job;
+
+static void *rtid_worker(void *arg) {
+  rtid_job *J = (rtid_job *)arg;
+  uint8_t rpn[ORC_MAX_SIZE];
+  uint64_t done = 0;
+  for (;;) {
+    uint64_t c = atomic_fetch_add(&J->next_chunk, 1);
+    uint64_t lo = c * ORC_CHUNK;
+    if (lo >= J->G || lo > atomic_load(&J->best_g)) break;
+    uint64_t hi = lo + ORC_CHUNK;
+    if (hi > J->G) hi = J->G;
+    for (uint64_t g = lo; g < hi; ++g) {
+      uint64_t r = J->offset + (g * ORC_RTID_C) % J->N;
+      unrank_rec(J->T, J->size, r, rpn);
+      ++done;
+      if (orc_is_solution(rpn, J->size, J->L, J->n_samples, J->leafvals, J->outputs, J->W)) {
+        uint64_t b = atomic_load(&J->best_g);
+        while (g < b && !atomic_compare_exchange_weak(&J->best_g, &b, g)) {
+        }
+        break;
+      }
+    }
+  }
+  atomic_fetch_add(&J->evaluated, done);
+  return NULL;
+}
+
+int orc_simba_rtid_search(int n_vars, int n_consts, const uint64_t *consts, int W,
+                          int n_samples, const uint64_t *inputs, const uint64_t *outputs,
+                          int min_size, int max_size, int nthreads, orc_result *res,
+                          uint64_t *thread_index) {
+  memset(res, 0, sizeof *res);
+  int L = n_vars + n_consts;
+  if (L < 1 || L > ORC_MAX_LEAVES || max_size > ORC_MAX_SIZE || min_size < 1 ||
+      n_samples < 1)
+    return -2;
+  uint64_t T[(ORC_MAX_SIZE + 1) * 9];
+  if (orc_count_table(L, max_size, T) != 0) return -1;
+  uint64_t *lv = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)n_samples * L);
+  for (int i = 0; i < n_samples; ++i) {
+    for (int v = 0; v < n_vars; ++v) lv[i * L + v] = inputs[i * n_vars + v] & wmask(W);
+    for (int c = 0; c < n_consts; ++c) lv[i * L + n_vars + c] = consts[c] & wmask(W);
+  }
+  if (nthreads < 1) nthreads = 1;
+  double t0 = now_s();
+  uint64_t evaluated = 0;
+  int status = ORC_NOT_FOUND;
+  for (int s = min_size; s <= max_size && status != ORC_FOUND; ++s) {
+    uint64_t offset = 0;
+    for (int op = 0; op < 8; ++op) {
+      uint64_t N = T[s * 9 + op];
+      if (N == 0) continue;
+      rtid_job J;
+      J.T = T; J.size = s; J.L = L; J.n_samples = n_samples; J.W = W;
+      J.leafvals = lv; J.outputs = outputs;
+      J.offset = offset; J.N = N; J.G = (N + 1023) / 1024 * 1024;
+      atomic_init(&J.next_chunk, 0);
+      atomic_init(&J.best_g, UINT64_MAX);
+      atomic_init(&J.evaluated, 0);
+      pthread_t th[256];
+      int nt = nthreads > 256 ? 256 : nthreads;
+      for (int k = 0; k < nt; ++k) pthread_create(&th[k], NULL, rtid_worker, &J);
+      for (int k = 0; k < nt; ++k) pthread_join(th[k], NULL);
+      evaluated += atomic_load(&J.evaluated);
+      uint64_t g = atomic_load(&J.best_g);
+      if (g != UINT64_MAX) {
+        uint64_t r = offset + (g * ORC_RTID_C) % N;
+        status = ORC_FOUND;
+        res->size = s;
+        res->rank = r;
+        if (thread_index) *thread_index = g;
+        unrank_rec(T, s, r, res->rpn);
+        res->op = op;
+        res->split = (op == ORC_NOT || op == ORC_NEG) ? 0 : s - 1 - subtree_len(res->rpn, s - 2);
+        orc_infix(res->rpn, s, n_vars, consts, W, 0, res->expr, sizeof res->expr);
+        break;
+      }
+      offset += N;
+    }
+  }
+  if (status != ORC_FOUND) strcpy(res->expr, "Not found !");
+  res->status = status;
+  res->seconds = now_s() - t0;
+  res->candidates = evaluated;
+  free(lv);
+  return status;
+}
+
+/* ------------------------------------------------------------------------ */
 /* VFB fingerprint: full-cache.cu:113-134 (simpleHash, makeUnqChk).           */
 /* W=32 is the reference.  Note that `simpleHash(x) + 0x9e3779b9` is a sum   */
 /* of two 32-bit unsigned ints, so it wraps mod 2^32 before being widened.  */

@@ -45,6 +45,12 @@ int orc_simba_search(int n_vars, int n_consts, const uint64_t *consts, int W,
                      int n_samples, const uint64_t *inputs, const uint64_t *outputs,
                      int min_size, int max_size, int nthreads, double budget_s,
                      int exhaustive, orc_result *res);
+/* SIMBA-RTid (nocache_rtid.cu:85-95): first (size, op) block with a visited
+ * solution; within it the smallest thread index (written to *thread_index). */
+int orc_simba_rtid_search(int n_vars, int n_consts, const uint64_t *consts, int W,
+                          int n_samples, const uint64_t *inputs, const uint64_t *outputs,
+                          int min_size, int max_size, int nthreads, orc_result *res,
+                          uint64_t *thread_index);
 uint32_t orc_fmix32(uint32_t x);
 uint64_t orc_fmix64(uint64_t x);
 uint64_t orc_fingerprint(const uint64_t *cs, int n, int W);

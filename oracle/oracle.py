@@ -56,6 +56,10 @@ def lib():
                                        ctypes.c_int, u64p, u64p, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                        ctypes.POINTER(OrcResult)]
+        L.orc_simba_rtid_search.argtypes = [ctypes.c_int, ctypes.c_int, u64p, ctypes.c_int,
+                                            ctypes.c_int, u64p, u64p, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(OrcResult), u64p]
         L.orc_fmix32.argtypes = [ctypes.c_uint32]
         L.orc_fmix32.restype = ctypes.c_uint32
         L.orc_fmix64.argtypes = [ctypes.c_uint64]
@@ -145,6 +149,28 @@ def simba_search(inputs, outputs, n_vars, consts=(), W=64, min_size=1, max_size=
     lib().orc_simba_search(n_vars, len(consts), cp, W, n, ip, op, min_size, max_size,
                            threads, budget_s, int(exhaustive), ctypes.byref(r))
     return _result(r)
+
+
+RTID_C = 2246822507
+
+
+def simba_rtid_search(inputs, outputs, n_vars, consts=(), W=64, min_size=1, max_size=9,
+                      threads=1):
+    """SIMBA-RTid order (nocache_rtid.cu:85-95); `thread` is the smallest
+    thread index that checks the returned tree."""
+    inputs = np.asarray(inputs, dtype=np.uint64).reshape(-1)
+    outputs = np.asarray(outputs, dtype=np.uint64)
+    ia, ip = _u64(inputs)
+    oa, op = _u64(outputs)
+    ca, cp = _u64(list(consts) if len(consts) else [0])
+    g = np.zeros(1, dtype=np.uint64)
+    r = OrcResult()
+    lib().orc_simba_rtid_search(n_vars, len(consts), cp, W, outputs.shape[0], ip, op,
+                                min_size, max_size, threads, ctypes.byref(r),
+                                g.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+    d = _result(r)
+    d["thread"] = int(g[0]) if d["status"] == 0 else None
+    return d
 
 
 def fingerprint(values, W):

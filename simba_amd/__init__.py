@@ -72,7 +72,8 @@ class VfbStats(ctypes.Structure):
 
 # Every symbol the C ABI exports (include/mba_api.h).
 EXPORTS = ["mba_ctx_create", "mba_ctx_destroy", "mba_strerror", "mba_ctx_last_error",
-           "mba_default_opts", "mba_simba_solve", "mba_vfb_solve", "mba_batch_solve",
+           "mba_default_opts", "mba_simba_solve", "mba_simba_rtid_solve", "mba_vfb_solve",
+           "mba_batch_solve",
            "mba_instance_read_json", "mba_instance_free", "mba_rpn_to_infix",
            "mba_features", "mba_mlp_create", "mba_mlp_destroy", "mba_mlp_get_weights",
            "mba_mlp_set_weights", "mba_mlp_forward", "mba_mlp_grads", "mba_mlp_train_step",
@@ -98,6 +99,7 @@ def lib():
         L.mba_ctx_last_error.restype = ctypes.c_char_p
         L.mba_default_opts.argtypes = [P(Opts)]
         L.mba_simba_solve.argtypes = [ctypes.c_void_p, P(Instance), P(Opts), P(Result)]
+        L.mba_simba_rtid_solve.argtypes = [ctypes.c_void_p, P(Instance), P(Opts), P(Result)]
         L.mba_vfb_solve.argtypes = [ctypes.c_void_p, P(Instance), P(Opts), P(Result),
                                     P(VfbStats)]
         L.mba_batch_solve.argtypes = [ctypes.c_void_p, P(Instance), ctypes.c_size_t,
@@ -162,6 +164,15 @@ class Context:
         r = Result()
         self._check(lib().mba_simba_solve(self._h, ctypes.byref(ci), ctypes.byref(o),
                                           ctypes.byref(r)))
+        return r.as_dict()
+
+    def simba_rtid(self, inst, **kw):
+        """SIMBA-RTid order (nocache_rtid.cu:85-95)."""
+        ci, keep = inst.to_c()
+        o = make_opts(**kw)
+        r = Result()
+        self._check(lib().mba_simba_rtid_solve(self._h, ctypes.byref(ci), ctypes.byref(o),
+                                               ctypes.byref(r)))
         return r.as_dict()
 
     def vfb(self, inst, **kw):

@@ -31,7 +31,11 @@ LIB_SOURCES = [
     "simba_host.cpp", "vfb_host.cpp", "batch_host.cpp", "mlp_host.cpp", "api.cpp", "expr.cpp",
     "instance_io.cpp",
 ]
-CLI_SOURCES = {"simba_exe": "cli_simba.cpp", "vfb_exe": "cli_vfb.cpp"}
+# CLI binaries: (source, extra defines).  simba_rtid_exe is simba_exe with
+# the SIMBA-RTid order (the reference's nocache_rtid.cu build).
+CLI_SOURCES = {"simba_exe": ("cli_simba.cpp", []),
+               "simba_rtid_exe": ("cli_simba.cpp", ["-DMBA_CLI_RTID=1"]),
+               "vfb_exe": ("cli_vfb.cpp", [])}
 
 
 def hipcc():
@@ -67,13 +71,13 @@ def build(jobs=None, verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{r.stderr}")
     os.replace(tmp, LIB)
-    for exe, src in CLI_SOURCES.items():
+    for exe, (src, defs) in CLI_SOURCES.items():
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
         out = os.path.join(BIN, exe)
-        cmd = [hipcc()] + COMMON + ["-o", out, path, f"-L{HERE}", "-lsimba_amd",
-                                    "-Wl,-rpath,$ORIGIN/.."]
+        cmd = [hipcc()] + COMMON + defs + ["-o", out, path, f"-L{HERE}", "-lsimba_amd",
+                                           "-Wl,-rpath,$ORIGIN/.."]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"CLI build failed: {exe}\n{r.stderr}")

@@ -92,6 +92,18 @@ int mba_simba_solve(mba_ctx *c, const mba_instance *inst, const mba_opts *opts,
   return simba_solve(c->c, *inst, *opts, *res);
 }
 
+int mba_simba_rtid_solve(mba_ctx *c, const mba_instance *inst, const mba_opts *opts,
+                         mba_result *res) {
+  if (!c || !inst || !res) return MBA_ERR_ARG;
+  mba_opts d;
+  if (!opts) {
+    mba_default_opts(&d);
+    opts = &d;
+  }
+  if (hipSetDevice(c->c.device) != hipSuccess) return MBA_ERR_HIP;
+  return simba_solve(c->c, *inst, *opts, *res, nullptr, true);
+}
+
 int mba_vfb_solve(mba_ctx *c, const mba_instance *inst, const mba_opts *opts,
                   mba_result *res, mba_vfb_stats *stats) {
   if (!c || !inst || !res) return MBA_ERR_ARG;

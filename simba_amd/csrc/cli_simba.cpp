@@ -1,5 +1,7 @@
 // cli_simba.cpp -- simba_exe: drop-in for the reference's SIMBA CLI
 // (simba/src/nocache.cu:291-342).  stdout: "<path>,<formula>,<seconds>".
+// Built with MBA_CLI_RTID as simba_rtid_exe: the SIMBA-RTid order
+// (simba/src/nocache_rtid.cu, same CLI).
 #include <chrono>
 #include <iostream>
 
@@ -47,7 +49,11 @@ int main(int argc, char **argv) {
   o.time_budget_s = a.budget;
   o.verbose = a.verbose;
   mba_result r;
+#ifdef MBA_CLI_RTID
+  rc = mba_simba_rtid_solve(ctx, &inst, &o, &r);
+#else
   rc = mba_simba_solve(ctx, &inst, &o, &r);
+#endif
   auto t1 = std::chrono::high_resolution_clock::now();
   if (rc < 0) {
     std::fprintf(stderr, "error: %s: %s\n", mba_strerror(rc), mba_ctx_last_error(ctx));

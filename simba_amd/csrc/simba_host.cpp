@@ -271,7 +271,7 @@ void fill_result(const CountTable &T, int size, uint64_t rank, int n_vars,
 }
 
 int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result &res,
-                const float *block_scores) {
+                const float *block_scores, bool rtid) {
   const auto t_start = Clock::now();
   std::memset(&res, 0, sizeof res);
   res.status = MBA_NOT_FOUND;
@@ -359,6 +359,61 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   }
   for (int m = 2; m <= Kt; ++m)
     MBA_HIP(launch_table_build(W, ctx.tables.p, tv.off, T, m, st));
+
+  if (rtid) {
+    // SIMBA-RTid: every (size, op) block in order, thread g of the block
+    // checking rank ((g * C) mod 2^64) mod N; the first block with a match
+    // decides, and within it the smallest such g (size 1 has no op block).
+    constexpr uint64_t kC = 2246822507ull;
+    constexpr uint64_t kChunk = 1ull << 32;
+    MBA_HIP(ctx.hit_count.ensure(256));
+    auto *d_best = (unsigned long long *)ctx.hit_count.p;
+    const int grid_r = ctx.cu_count * 16;
+    uint64_t cand = 0;
+    int reached = 0;
+    for (int s = std::max(min_size, 2); s <= max_size; ++s) {
+      uint64_t offset = 0;
+      for (int op = 0; op < 8; ++op) {
+        const uint64_t N = T.t[s][op];
+        if (N == 0) continue;
+        const uint64_t G = (N + 1023) / 1024 * 1024;
+        MBA_HIP(hipMemsetAsync(d_best, 0xff, 8, st));
+        for (uint64_t g0 = 0; g0 < G; g0 += kChunk) {
+          const uint64_t g1 = std::min(G, g0 + kChunk);
+          MBA_HIP(launch_rtid_scan(W, d_tv, ctx.leafvals.as<uint64_t>(),
+                                   ctx.outputs.as<uint64_t>(), n, s, offset, N, g0, g1, t0,
+                                   d_best, grid_r, st));
+          unsigned long long best = ~0ull;
+          MBA_HIP(hipMemcpyAsync(&best, d_best, 8, hipMemcpyDeviceToHost, st));
+          MBA_HIP(hipStreamSynchronize(st));
+          cand += g1 - g0;
+          if (best != ~0ull) {
+            fill_result(T, s, offset + (best * kC) % N, nv, inst.consts, W, res);
+            res.candidates = cand;
+            res.size_reached = reached;
+            res.seconds = seconds_since(t_start);
+            return MBA_FOUND;
+          }
+          const bool over = (o.cancel && *o.cancel) ||
+                            (o.time_budget_s > 0 && seconds_since(t_start) > o.time_budget_s);
+          if (over) {
+            std::snprintf(res.expr, sizeof res.expr, "Not found !");
+            res.candidates = cand;
+            res.size_reached = reached;
+            res.seconds = seconds_since(t_start);
+            return MBA_BUDGET;
+          }
+        }
+        offset += N;
+      }
+      reached = s;
+    }
+    std::snprintf(res.expr, sizeof res.expr, "Not found !");
+    res.candidates = cand;
+    res.size_reached = reached;
+    res.seconds = seconds_since(t_start);
+    return MBA_NOT_FOUND;
+  }
 
   // ---- size sweep --------------------------------------------------------
   std::vector<uint64_t> found(kMaxSize + 2, ~0ull);

@@ -127,6 +127,14 @@ hipError_t launch_leaf_scan(int width, const TableView *d_tv, int L,
                             const CheckTargets &tg, HitRec *d_hits,
                             uint32_t *d_hit_count, uint32_t hit_cap,
                             hipStream_t stream);
+// SIMBA-RTid scan of threads [g0, g1) of the (size, op) block at `offset` with
+// N trees: *d_best = min(*d_best, g) over threads whose tree matches every
+// sample (t0 = the filter sample's output, the tables' sample).
+hipError_t launch_rtid_scan(int width, const TableView *d_tv, const uint64_t *d_leafvals,
+                            const uint64_t *d_outputs, int n_samples, int size,
+                            uint64_t offset, uint64_t N, uint64_t g0, uint64_t g1,
+                            uint64_t t0, unsigned long long *d_best, int grid,
+                            hipStream_t stream);
 // Full verification of (size, rank) candidates on all samples: unrank and
 // evaluate every sample (leaf values [sample][leaf] staged in LDS), exit at
 // the first mismatching sample.  ok[i] = 1 when candidate i matches all.

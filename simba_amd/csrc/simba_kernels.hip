@@ -681,32 +681,19 @@ __global__ void k_leaf_scan(const TableView *__restrict__ tv, int L, uint64_t lo
 }
 
 // ---------------------------------------------------------------------------
-// Verification: the reference's per-candidate path (numberToFormula +
-// is_solution_rpn, nocache.cu:23-83 / mba.cuh:11-66): unrank the full tree
-// and evaluate it sample by sample, leaving at the first mismatch.  Leaf
-// values of all samples are staged in LDS.
+// The reference's per-candidate path (numberToFormula + is_solution_rpn,
+// nocache.cu:23-83 / mba.cuh:11-66) for one lane: unrank the tree of `size`
+// and `rank` into RPN and evaluate it sample by sample, leaving at the first
+// mismatch.  Leaf values [sample][leaf] and outputs are in LDS.
 // ---------------------------------------------------------------------------
 template <typename word>
-__global__ void __launch_bounds__(256)
-k_verify(const TableView *__restrict__ tv, const uint64_t *__restrict__ leafvals,
-         const uint64_t *__restrict__ outputs, int n_samples,
-         const uint32_t *__restrict__ sizes, const uint64_t *__restrict__ ranks,
-         uint32_t n_cand, uint8_t *__restrict__ ok) {
-  __shared__ word s_leaf[kMaxSamples * kMaxLeaves];
-  __shared__ word s_out[kMaxSamples];
-  const int L = tv->L;
-  for (int i = threadIdx.x; i < n_samples * L; i += blockDim.x)
-    s_leaf[i] = (word)leafvals[i];
-  for (int i = threadIdx.x; i < n_samples; i += blockDim.x) s_out[i] = (word)outputs[i];
-  __syncthreads();
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n_cand) return;
-  const int size = (int)sizes[c];
+__device__ bool tree_matches(const TableView *__restrict__ tv, const word *s_leaf,
+                             const word *s_out, int L, int n_samples, int size, uint64_t rank) {
   uint8_t rpn[kMaxSize];
   struct Item { uint64_t r; int s, pos; };
   Item st[kMaxSize];
   int sp = 0;
-  st[sp++] = {ranks[c], size, 0};
+  st[sp++] = {rank, size, 0};
   while (sp > 0) {
     Item it = st[--sp];
     if (it.s == 1) {
@@ -733,8 +720,7 @@ k_verify(const TableView *__restrict__ tv, const uint64_t *__restrict__ leafvals
     st[sp++] = {a, j, it.pos};
     st[sp++] = {n - a * rs, cs - j, it.pos + j};
   }
-  uint8_t good = 1;
-  for (int s = 0; s < n_samples && good; ++s) {
+  for (int s = 0; s < n_samples; ++s) {
     word vs[kMaxSize];
     int top = -1;
     for (int i = 0; i < size; ++i) {
@@ -751,9 +737,39 @@ k_verify(const TableView *__restrict__ tv, const uint64_t *__restrict__ leafvals
         }
       }
     }
-    good = vs[0] == s_out[s];
+    if (vs[0] != s_out[s]) return false;
   }
-  ok[c] = good;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// SIMBA-RTid order (nocache_rtid.cu:85-95): thread g of a (size, op) block
+// of N trees checks rank offset + ((g * C) mod 2^64) mod N.  Threads
+// [g0, g1) of the block; the smallest g whose tree matches every sample is
+// kept with atomicMin.  The sample-0 value comes from the tables (value_of),
+// the full check only runs on a match.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kRtidC = 2246822507ull;
+
+template <typename word>
+__global__ void __launch_bounds__(256)
+k_rtid_scan(const TableView *__restrict__ tv, const uint64_t *__restrict__ leafvals,
+            const uint64_t *__restrict__ outputs, int n_samples, int size, uint64_t offset,
+            uint64_t N, uint64_t g0, uint64_t g1, uint64_t t0,
+            unsigned long long *__restrict__ best) {
+  __shared__ word s_leaf[kMaxSamples * kMaxLeaves];
+  __shared__ word s_out[kMaxSamples];
+  const int L = tv->L;
+  for (int i = threadIdx.x; i < n_samples * L; i += blockDim.x) s_leaf[i] = (word)leafvals[i];
+  for (int i = threadIdx.x; i < n_samples; i += blockDim.x) s_out[i] = (word)outputs[i];
+  __syncthreads();
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t g = g0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < g1; g += stride) {
+    const uint64_t r = offset + (g * kRtidC) % N;
+    if (value_of<word>(tv, size, r) != (word)t0) continue;
+    if (tree_matches<word>(tv, s_leaf, s_out, L, n_samples, size, r))
+      atomicMin(best, (unsigned long long)g);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1110,6 +1126,20 @@ hipError_t launch_pair_check(int width, const TableView *d_tv, const void *d_bas
     pair_dispatch<uint64_t>(d_tv, base + (p.gen_a ? 0 : off[p.j]), base + (p.gen_b ? 0 : off[p.m]),
                             p, tg, d_hits, d_hit_count, hit_cap, grid, stream);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_rtid_scan(int width, const TableView *d_tv, const uint64_t *d_leafvals,
+                            const uint64_t *d_outputs, int n_samples, int size,
+                            uint64_t offset, uint64_t N, uint64_t g0, uint64_t g1,
+                            uint64_t t0, unsigned long long *d_best, int grid,
+                            hipStream_t stream) {
+  if (width == 32)
+    hipLaunchKernelGGL(k_rtid_scan<uint32_t>, dim3(grid), dim3(256), 0, stream, d_tv,
+                       d_leafvals, d_outputs, n_samples, size, offset, N, g0, g1, t0, d_best);
+  else
+    hipLaunchKernelGGL(k_rtid_scan<uint64_t>, dim3(grid), dim3(256), 0, stream, d_tv,
+                       d_leafvals, d_outputs, n_samples, size, offset, N, g0, g1, t0, d_best);
   return hipGetLastError();
 }
 

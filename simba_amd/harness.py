@@ -13,6 +13,7 @@ std::cout's default 6 significant digits).
 
     python -m simba_amd.harness instances/ --tool simba -o res_simba.csv
     python -m simba_amd.harness instances/ --tool vfb --max-len 20 --timeout 60
+    python -m simba_amd.harness instances/ --tool rtid     (SIMBA-RTid order)
 """
 import argparse
 import glob
@@ -61,6 +62,8 @@ def run(paths, tool="simba", max_len=20, timeout_s=60.0, batch=False, native=Fal
             for p, inst in zip(kept, insts):
                 if tool == "simba":
                     r = ctx.simba(inst, **kw)
+                elif tool == "rtid":
+                    r = ctx.simba_rtid(inst, **kw)
                 elif tool == "vfb":
                     if inst.n_examples > 100:  # full-cache.cu:671-675
                         r = {"status": 0, "expr": "see_the_error", "seconds": 0.0}
@@ -84,7 +87,7 @@ def run(paths, tool="simba", max_len=20, timeout_s=60.0, batch=False, native=Fal
 def main(argv=None):
     ap = argparse.ArgumentParser(description="Run a search tool over a folder of instances.")
     ap.add_argument("folder")
-    ap.add_argument("--tool", choices=["simba", "vfb"], default="simba")
+    ap.add_argument("--tool", choices=["simba", "rtid", "vfb"], default="simba")
     ap.add_argument("--max-len", type=int, default=20)
     ap.add_argument("--timeout", type=float, default=60.0, help="per-instance cap, seconds")
     ap.add_argument("--batch", action="store_true",

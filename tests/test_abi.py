@@ -109,8 +109,9 @@ def test_rpn_to_infix_reference_format():
     assert out.value == b"((v0 + v1) * (-v1))"
 
 
-def test_cli_usage_contract():
-    exe = os.path.join(ROOT, "simba_amd", "bin", "simba_exe")
+@pytest.mark.parametrize("name", ["simba_exe", "simba_rtid_exe"])
+def test_cli_usage_contract(name):
+    exe = os.path.join(ROOT, "simba_amd", "bin", name)
     if not os.path.exists(exe):
         pytest.skip("CLI not built")
     r = subprocess.run([exe], capture_output=True, text=True)
