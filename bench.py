@@ -169,6 +169,33 @@ def run_simba(args):
     return out
 
 
+def run_simba_cap(args):
+    """The same instance swept exhaustively over sizes 1..20 under a time cap
+    (one run): how far the search gets, and its rate over completed sizes."""
+    import simba_amd
+    from simba_amd.synth import count_table, make_instance
+
+    inst = make_instance(args.seed, 4, 64, 15, 64, (1,))
+    ctx = simba_amd.Context(0)
+    t0 = time.perf_counter()
+    r = ctx.simba(inst, min_size=1, max_size=20, exhaustive=True, budget_s=args.cap,
+                  table_bytes=args.table_gb << 30)
+    wall = time.perf_counter() - t0
+    ctx.close()
+    T = count_table(5, 20)
+    done = sum(T[s][8] for s in range(1, r["size_reached"] + 1))
+    return {
+        "metric": "Gcand/s evaluated", "value": round(done / r["seconds"] / 1e9, 3),
+        "unit": "Gcand/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(1e3 * wall, 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "simba_4vars_const1_u64_64samples_sizes1to20_cap%ds" % args.cap,
+                   "status": r["status"], "size_completed": r["size_reached"],
+                   "candidates_completed": done, "search_seconds": round(r["seconds"], 3),
+                   "first_solution": r["expr"], "first_solution_size": r["size"]},
+    }
+
+
 def run_vfb(args):
     import simba_amd
     from simba_amd.synth import make_instance
@@ -362,7 +389,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="simba",
-                    choices=["simba", "vfb", "batch", "learned", "copy"])
+                    choices=["simba", "simba_cap", "vfb", "batch", "learned", "copy"])
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--budget-size", type=int, default=9)
     ap.add_argument("--train", type=int, default=200000)
@@ -375,6 +402,7 @@ def main():
     ap.add_argument("--cache-gb", type=int, default=64)
     ap.add_argument("--seed", type=int, default=15)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cap", type=int, default=60, help="simba_cap: time cap in seconds")
     ap.add_argument("--lanes-b", type=int, default=0, help="B values per lane (4 or 8)")
     ap.add_argument("--wg-per-cu", type=int, default=0, help="workgroups per CU")
     ap.add_argument("--autotune", action="store_true",
@@ -385,7 +413,8 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     out = {"simba": run_simba, "vfb": run_vfb, "batch": run_batch,
-           "learned": run_learned, "copy": run_copy}[args.workload](args)
+           "learned": run_learned, "copy": run_copy,
+           "simba_cap": run_simba_cap}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

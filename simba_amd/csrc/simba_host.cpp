@@ -44,55 +44,104 @@ static uint64_t block_tiles(const BBlock &b, int rb) {
   return ((nY + rows - 1) / rows) * b.ntc;
 }
 
-// The B-side trees of size m > Kt in rank order, as blocks whose values come
-// from at most two tables under the affine map (neg, k) of the unary chain
-// above them; blocks with a child above Kt are generic.  `start` is the rank
-// (at the B side's own size) of this subtree's rank 0.
+// The B-side trees of size m > Kt in rank order, as blocks (see BBlock).
+// `start` and `stride` map this subtree's rank r to the B side's rank
+// start + stride * r; `steps` is the path above it (innermost first).  A
+// binary node with both children tabulated is a 2-D block; one whose child
+// above Kt has a sibling of at most kFixedSiblings trees is split into one
+// path step per sibling tree (that tree's value fixed); anything else is a
+// generic block.
+constexpr uint64_t kFixedSiblings = 64;
+
 static void b_blocks(const CountTable &T, int Kt, const uint64_t *off, int m, uint64_t start,
-                     bool neg, uint64_t k, int rb, std::vector<BBlock> &out) {
-  uint64_t r = start;
+                     uint64_t stride, const std::vector<BStep> &steps, int rb,
+                     std::vector<BBlock> &out) {
+  auto emit = [&](BBlock b, const std::vector<BStep> &st) {
+    b.stride = stride;
+    if ((int)st.size() > kMaxBSteps && !b.generic) {
+      // too deep a path: fall back to the generic evaluation of the range
+      BBlock g{};
+      g.start = b.start;
+      g.stride = stride;
+      g.nA = b.nA * b.nB;
+      g.generic = 1;
+      out.push_back(g);
+      return;
+    }
+    b.nsteps = b.generic ? 0 : (int)st.size();
+    for (int i = 0; i < b.nsteps; ++i) b.steps[i] = st[i];
+    out.push_back(b.generic ? b : shape_block(b, rb));
+  };
+  auto with = [&](BStep first) {
+    std::vector<BStep> v;
+    v.reserve(steps.size() + 1);
+    // merge consecutive unary nodes into one affine map: s*(s'*x + k') + k
+    if (first.kind == 0 && !steps.empty() && steps[0].kind == 0) {
+      const BStep &o = steps[0];
+      BStep mrg;
+      mrg.kind = 0;
+      mrg.op = first.op ^ o.op;
+      mrg.c = (o.op ? 0 - first.c : first.c) + o.c;
+      v.push_back(mrg);
+      v.insert(v.end(), steps.begin() + 1, steps.end());
+    } else {
+      v.push_back(first);
+      v.insert(v.end(), steps.begin(), steps.end());
+    }
+    return v;
+  };
+  uint64_t r = 0;  // rank offset within size m
   for (int op = 0; op < 8; ++op) {
     const uint64_t cnt = T.t[m][op];
     if (cnt == 0) continue;
     if (is_unary(op)) {
-      // s*u(x) + k with u(x) = -x - c: the map becomes (-s, k - s*c)
-      const uint64_t c = op == NOT ? 1 : 0;
-      const bool nneg = !neg;
-      const uint64_t nk = neg ? k + c : k - c;
+      // ~x = -x - 1, neg x = -x
+      const BStep u{0, 1, op == NOT ? ~0ull : 0ull};
+      const std::vector<BStep> st = with(u);
       if (m - 1 <= Kt) {
         BBlock b{};
-        b.start = r;
+        b.start = start + stride * r;
         b.offA = off[m - 1];
         b.nA = T.t[m - 1][8];
         b.nB = 1;
-        b.k = nk;
         b.op = -1;
-        b.neg = nneg;
-        out.push_back(shape_block(b, rb));
+        emit(b, st);
       } else {
-        b_blocks(T, Kt, off, m - 1, r, nneg, nk, rb, out);
+        b_blocks(T, Kt, off, m - 1, start + stride * r, stride, st, rb, out);
       }
     } else {
       const int cs = m - 1;
       const int last = op == SUB ? cs - 1 : cs / 2;
       uint64_t q = r;
       for (int j = 1; j <= last; ++j) {
-        const uint64_t nl = T.t[j][8], nr = T.t[cs - j][8];
-        BBlock b{};
-        b.start = q;
-        if (j <= Kt && cs - j <= Kt) {
-          b.offA = off[j];
-          b.offB = off[cs - j];
+        const int l = j, rr = cs - j;
+        const uint64_t nl = T.t[l][8], nr = T.t[rr][8];
+        const uint64_t base = start + stride * q;
+        if (l <= Kt && rr <= Kt) {
+          BBlock b{};
+          b.start = base;
+          b.offA = off[l];
+          b.offB = off[rr];
           b.nA = nl;
           b.nB = nr;
-          b.k = k;
           b.op = op;
-          b.neg = neg;
-          out.push_back(shape_block(b, rb));
+          emit(b, steps);
+        } else if (l > Kt && rr <= Kt && nr <= kFixedSiblings) {
+          // right child fixed to tree b: rank base + stride*(ra*nr + b)
+          for (uint64_t bi = 0; bi < nr; ++bi)
+            b_blocks(T, Kt, off, l, base + stride * bi, stride * nr,
+                     with(BStep{2, op, off[rr] + bi}), rb, out);
+        } else if (rr > Kt && l <= Kt && nl <= kFixedSiblings) {
+          // left child fixed to tree a: rank base + stride*(a*nr + rb)
+          for (uint64_t ai = 0; ai < nl; ++ai)
+            b_blocks(T, Kt, off, rr, base + stride * ai * nr, stride,
+                     with(BStep{1, op, off[l] + ai}), rb, out);
         } else {
+          BBlock b{};
+          b.start = base;
           b.nA = nl * nr;
           b.generic = 1;
-          out.push_back(b);
+          emit(b, steps);
         }
         q += nl * nr;
       }
@@ -511,13 +560,32 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
       auto it = bcache.find(m * 16 + p.rb);
       if (it == bcache.end()) {
         std::vector<BBlock> v;
-        b_blocks(T, Kt, tv.off, m, 0, false, 0, p.rb, v);
+        b_blocks(T, Kt, tv.off, m, 0, 1, {}, p.rb, v);
         it = bcache.emplace(m * 16 + p.rb, std::move(v)).first;
       }
       blocks = &it->second;
     }
     const uint64_t per_tile = (uint64_t)kPairTA * tb;
     const uint64_t tiles_per_launch = std::max<uint64_t>(1, kPairsPerLaunch / per_tile);
+    // launches go out back to back; the stream is synchronised (timing and
+    // budget check) once a group holds about kPairsPerLaunch pairs
+    uint64_t group_tiles = 0;
+    bool group_open = false;
+    auto close_group = [&]() -> int {
+      if (!group_open) return 0;
+      hipError_t e = hipEventRecord(ctx.ev1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(ctx.ev1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1);
+      if (e != hipSuccess) {
+        ctx.last_error = std::string("pair launch: ") + hipGetErrorString(e);
+        return -1;
+      }
+      ktime += ms * 1e-3;
+      group_open = false;
+      group_tiles = 0;
+      return over_budget() ? 1 : 0;
+    };
     for (const BBlock &bb : *blocks) {
       p.bb = bb;
       p.nTB = p.gen_b ? block_tiles(bb, p.rb) : (p.nB + tb - 1) / tb;
@@ -527,24 +595,24 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
         p.tile_end = std::min(tiles, t0 + tiles_per_launch);
         const uint64_t ntl = p.tile_end - p.tile_begin;
         const int g = (int)std::min<uint64_t>((uint64_t)grid, ntl);
-        hipError_t e = hipEventRecord(ctx.ev0, st);
+        hipError_t e = hipSuccess;
+        if (!group_open) e = hipEventRecord(ctx.ev0, st);
+        group_open = true;
         if (e == hipSuccess)
           e = launch_pair_check(W, d_tv, ctx.tables.p, tv.off, p, tg, d_hits, d_hc, kHitCap, g,
                                 st);
-        if (e == hipSuccess) e = hipEventRecord(ctx.ev1, st);
-        if (e == hipSuccess) e = hipEventSynchronize(ctx.ev1);
-        float ms = 0.f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1);
         if (e != hipSuccess) {
           ctx.last_error = std::string("pair launch: ") + hipGetErrorString(e);
           return -1;
         }
-        ktime += ms * 1e-3;
         ++klaunch;
-        if (over_budget()) return 1;
+        group_tiles += ntl;
+        if (group_tiles >= tiles_per_launch) {
+          if (int rc = close_group()) return rc;
+        }
       }
     }
-    return 0;
+    return close_group();
   };
 
   for (int s = 3; (answer < 0 || o.exhaustive) && s <= max_size; ++s) {
@@ -577,7 +645,9 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
         if (rc < 0) return MBA_ERR_HIP;
         stopped = rc == 1;
       }
-      if (!stopped) {
+      // once the answer is settled (an exhaustive sweep going on to max_size)
+      // the windows' hits no longer matter: their count may exceed the buffer
+      if (!stopped && answer < 0) {
         if (int rc = process_hits(s)) return rc;
       }
     } else {

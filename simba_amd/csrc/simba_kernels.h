@@ -33,21 +33,30 @@ struct CheckTargets {
 };
 
 // A block of B-side trees of size m > Kt whose sample-0 values each lane
-// forms from at most two tables:
-//   v = s * op(VA[ra], VB[rb]) + k   (op >= 0),   v = s * VA[ra] + k   (op < 0)
-// for ra < nA, rb < nB; the tree's rank within size m is start + ra*nB + rb.
-// (s, k) is the affine map x -> s*x + k of the unary chain above the block's
-// root (~x = -x - 1, neg x = -x; s = -1 when neg).  A tile covers 2^logTB
-// columns (over ra when trans, else over rb) by (256*RB) >> logTB rows; ntc
-// tiles span a row band.  generic: ranks [start, start + nA) of size m, each
-// evaluated by a table-assisted unrank (children above Kt).
+// forms from tables: an inner value
+//   v = op(VA[ra], VB[rb])   (op >= 0),   v = VA[ra]   (op < 0)
+// for ra < nA, rb < nB, then the steps of the path from the inner node up to
+// the B side's root, innermost first: an affine map x -> s*x + k (a chain of
+// unary nodes: ~x = -x - 1, neg x = -x; s = -1 when neg), or a binary node
+// whose other child is a fixed tree with sample-0 value V[coff] (kind 1:
+// op(V[coff], x), kind 2: op(x, V[coff])).  The tree's rank within size m is
+// start + stride * (ra * nB + rb).  A tile covers 2^logTB columns (over ra
+// when trans, else over rb) by (256*RB) >> logTB rows; ntc tiles span a row
+// band.  generic: ranks start + stride * i, i < nA, each evaluated by a
+// table-assisted unrank (children above Kt).
+constexpr int kMaxBSteps = 8;
+struct BStep {
+  int32_t kind;  // 0 affine, 1 op(c, x), 2 op(x, c)
+  int32_t op;    // kinds 1-2: the binary operator; kind 0: neg flag
+  uint64_t c;    // kind 0: k; kinds 1-2: table element offset of the fixed tree
+};
 struct BBlock {
-  uint64_t start;
+  uint64_t start, stride;
   uint64_t offA, offB;
   uint64_t nA, nB;
-  uint64_t k;
   uint64_t ntc;
-  int32_t op, neg, generic, trans, logTB, pad;
+  int32_t op, generic, trans, logTB, nsteps, pad;
+  BStep steps[kMaxBSteps];
 };
 
 struct PairLaunch {

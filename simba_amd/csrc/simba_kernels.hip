@@ -368,26 +368,74 @@ __device__ __forceinline__ bool desc_coords(const BBlock &bb, uint64_t bt, int k
   return ra < bb.nA && rb < bb.nB;
 }
 
+// The path steps of a block (innermost first) on one value.
+template <typename word>
+__device__ __forceinline__ word apply_steps(const BBlock &bb, const word *V, word v) {
+  for (int i = 0; i < bb.nsteps; ++i) {
+    const BStep &st = bb.steps[i];
+    if (st.kind == 0) {
+      if (st.op) v = word(0) - v;
+      v = word(v + (word)st.c);
+    } else {
+      const word c = V[st.c];
+      v = st.kind == 1 ? apply_op<word>(st.op, c, v) : apply_op<word>(st.op, v, c);
+    }
+  }
+  return v;
+}
+
 template <typename word>
 __device__ __forceinline__ word desc_value(const BBlock &bb, const word *V, uint64_t ra,
                                            uint64_t rb) {
   word v = V[bb.offA + ra];
   if (bb.op >= 0) v = apply_op<word>(bb.op, v, V[bb.offB + rb]);
-  if (bb.neg) v = word(0) - v;
-  return word(v + (word)bb.k);
+  return apply_steps<word>(bb, V, v);
 }
 
-// Low 32 bits of the RB B values of this lane in a block tile.  Tiles of one
-// row (the block's columns span at least a tile) take the row's value with a
-// uniform load and the columns with contiguous loads; smaller blocks use the
-// general 2-D mapping.  The unary chain: -v = (v ^ ~0) - ~0, so
-// s*v + k = (v ^ sg) + (k - sg) with sg = 0 or ~0.
+__device__ __forceinline__ uint32_t lo_op(int op, uint32_t a, uint32_t b) {
+  switch (op) {
+    case AND: return a & b;
+    case OR: return a | b;
+    case XOR: return a ^ b;
+    case ADD: return a + b;
+    case SUB: return a - b;
+    default: return a * b;
+  }
+}
+
+// The path steps on the low halves of a lane's RB values: an affine step is
+// -v = (v ^ ~0) - ~0, so s*v + k = (v ^ sg) + (k - sg) with sg = 0 or ~0; a
+// fixed sibling's low half is one uniform load.
+template <typename word, int RB>
+__device__ __forceinline__ void steps_lo(const BBlock &bb, const word *V, uint32_t *bl) {
+  for (int i = 0; i < bb.nsteps; ++i) {
+    const BStep &st = bb.steps[i];
+    if (st.kind == 0) {
+      const uint32_t sg = st.op ? 0xffffffffu : 0u;
+      const uint32_t kk = (uint32_t)st.c - sg;
+#pragma unroll
+      for (int k = 0; k < RB; ++k) bl[k] = (bl[k] ^ sg) + kk;
+    } else {
+      const uint32_t c = lo32_at<word>(V, st.c);
+      if (st.kind == 1) {
+#pragma unroll
+        for (int k = 0; k < RB; ++k) bl[k] = lo_op(st.op, c, bl[k]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < RB; ++k) bl[k] = lo_op(st.op, bl[k], c);
+      }
+    }
+  }
+}
+
+// Low 32 bits of the RB B values of this lane in a block tile (inner value,
+// then the path steps).  Tiles of one row (the block's columns span at least
+// a tile) take the row's value with a uniform load and the columns with
+// contiguous loads; smaller blocks use the general 2-D mapping.
 template <typename word, int RB, int OP>
 __device__ __forceinline__ void desc_lo_op(const BBlock &bb, const word *V, uint64_t bt,
                                            uint32_t *bl, bool *ok) {
   constexpr uint32_t tile = kPairThreads * RB;
-  const uint32_t sg = bb.neg ? 0xffffffffu : 0u;
-  const uint32_t kk = (uint32_t)bb.k - sg;
   auto comb = [](uint32_t a, uint32_t b) -> uint32_t {
     return OP == AND ? (a & b) : OP == OR ? (a | b) : OP == XOR ? (a ^ b)
          : OP == ADD ? (a + b) : OP == SUB ? (a - b) : OP == MUL ? (a * b) : a;
@@ -406,20 +454,20 @@ __device__ __forceinline__ void desc_lo_op(const BBlock &bb, const word *V, uint
       const uint32_t off = threadIdx.x + (uint32_t)k * kPairThreads;
       ok[k] = off < rem;
       const uint32_t c = lo32_at<word>(col, ok[k] ? off : 0u);
-      const uint32_t v = OP < 0 ? c : bb.trans ? comb(c, rowv) : comb(rowv, c);
-      bl[k] = (v ^ sg) + kk;
+      bl[k] = OP < 0 ? c : bb.trans ? comb(c, rowv) : comb(rowv, c);
     }
-    return;
-  }
+  } else {
 #pragma unroll
-  for (int k = 0; k < RB; ++k) {
-    uint64_t ra, rb;
-    ok[k] = desc_coords<RB>(bb, bt, k, ra, rb);
-    if (!ok[k]) ra = 0, rb = 0;
-    uint32_t v = lo32_at<word>(V, bb.offA + ra);
-    if (OP >= 0) v = comb(v, lo32_at<word>(V, bb.offB + rb));
-    bl[k] = (v ^ sg) + kk;
+    for (int k = 0; k < RB; ++k) {
+      uint64_t ra, rb;
+      ok[k] = desc_coords<RB>(bb, bt, k, ra, rb);
+      if (!ok[k]) ra = 0, rb = 0;
+      uint32_t v = lo32_at<word>(V, bb.offA + ra);
+      if (OP >= 0) v = comb(v, lo32_at<word>(V, bb.offB + rb));
+      bl[k] = v;
+    }
   }
+  if (bb.nsteps) steps_lo<word, RB>(bb, V, bl);
 }
 
 template <typename word, int RB>
@@ -445,12 +493,12 @@ __device__ __forceinline__ word b_full(const TableView *__restrict__ tv,
   if (BSRC == kBDesc) {
     uint64_t ra, rb;
     desc_coords<RB>(p.bb, bt, k, ra, rb);
-    rank = p.bb.start + ra * p.bb.nB + rb;
+    rank = p.bb.start + p.bb.stride * (ra * p.bb.nB + rb);
     return desc_value<word>(p.bb, (const word *)tv->base, ra, rb);
   }
   const uint64_t local = bt * (kPairThreads * RB) + threadIdx.x + (uint64_t)k * kPairThreads;
   if (BSRC == kBGeneric) {
-    rank = p.bb.start + local;
+    rank = p.bb.start + p.bb.stride * local;
     return value_of<word>(tv, p.m, rank);
   }
   rank = local;
@@ -491,7 +539,7 @@ __device__ __forceinline__ void b_lo(const TableView *__restrict__ tv,
     if (BSRC == kBTable)
       bl[k] = ok[k] ? lo32_at<word>(B, b) : 0u;
     else
-      bl[k] = ok[k] ? lo32(value_of<word>(tv, p.m, p.bb.start + b)) : 0u;
+      bl[k] = ok[k] ? lo32(value_of<word>(tv, p.m, p.bb.start + p.bb.stride * b)) : 0u;
   }
 }
 

@@ -53,6 +53,18 @@ def test_simba_generated_sides_match_oracle(ctx, seed):
         _check(ctx, inst, 1, 8, table_bytes=tb)
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_simba_deep_generated_blocks_match_oracle(ctx, seed):
+    # Targets at the largest size with tables up to Kt = 3..5: B sides of up to
+    # Kt + 6 nodes are formed from table blocks under fixed-sibling and unary
+    # path steps (and generic blocks past them).
+    from simba_amd.synth import make_instance
+    inst = make_instance(7000 + seed, 2, 16, 9, 64 if seed % 2 else 32, (1,))
+    for tb, rb in ((2000, 4), (2000, 8), (60000, 8)):
+        note(f"simba deep blocks seed={seed} table_bytes={tb} lanes_b={rb}")
+        _check(ctx, inst, 1, 9, table_bytes=tb, lanes_b=rb)
+
+
 def test_simba_compat_range_matches_reference_semantics(ctx):
     import simba_amd
     from simba_amd.synth import make_instance
@@ -100,7 +112,7 @@ def test_simba_degenerate_first_sample(ctx):
     _check(ctx, inst, 1, 7)
 
 
-@pytest.mark.parametrize("cfg", [dict(lanes_b=8), dict(wg_per_cu=4), dict(wg_per_cu=16),
+@pytest.mark.parametrize("cfg", [dict(lanes_b=4), dict(wg_per_cu=4), dict(wg_per_cu=16),
                                  dict(table_delta=1), dict(lanes_b=8, table_delta=1)])
 def test_simba_launch_configs_keep_answers(ctx, cfg):
     from simba_amd.synth import make_instance
