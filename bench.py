@@ -398,8 +398,8 @@ def main():
     ap.add_argument("--size", type=int, default=15)
     ap.add_argument("--cpu-size", type=int, default=11)
     ap.add_argument("--table-gb", type=int, default=40)
-    ap.add_argument("--vfb-levels", type=int, default=11)
-    ap.add_argument("--cache-gb", type=int, default=64)
+    ap.add_argument("--vfb-levels", type=int, default=13)
+    ap.add_argument("--cache-gb", type=int, default=96)
     ap.add_argument("--seed", type=int, default=15)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cap", type=int, default=60, help="simba_cap: time cap in seconds")

@@ -86,7 +86,8 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
   uint64_t slots = 1ull << 16;
   // the table also holds the new fingerprints of the last, hash-only level
   // (about 6x the previous level's rows): 8 slots per stored row
-  while (slots < 8 * cap && slots < (1ull << 32)) slots <<= 1;
+  // at most 2^31 slots: slot indices are stored as u32 with ~0 = none
+  while (slots < 8 * cap && slots < (1ull << 31)) slots <<= 1;
   S.table_slots = slots;
   S.rows_capacity = cap;
 

@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(kVfbThreads) k_vfb_flag(VfbLevelArgs a) {
       const uint64_t idx = ch * kVfbChunk + (uint64_t)k * kVfbThreads + threadIdx.x;
       if (idx >= a.total) break;
       const uint32_t s = a.slot_of[idx];
-      const uint8_t kp = (s != ~0u) && (a.minidx[2 * s] == (a.tag | idx));
+      const uint8_t kp = (s != ~0u) && (a.minidx[2 * (uint64_t)s] == (a.tag | idx));
       a.keep[idx] = kp;
       mine += kp;
     }

@@ -67,3 +67,22 @@ def test_vfb_capacity_overflow_is_reported(ctx):
     note("vfb overflow")
     got = ctx.vfb(inst, max_size=9, cache_bytes=200 * (16 * 8 + 9))
     assert got["status"] == simba_amd.OVERFLOW
+
+
+def test_vfb_largest_hash_table_keeps_counts():
+    # a 96 GB row budget asks for the largest hash table (2^31 slots, 32 GB):
+    # slot indices near the top of the u32 range must address their own
+    # companion words (regression: 2 * slot in 32-bit arithmetic)
+    import simba_amd
+    from simba_amd.synth import make_instance
+    inst = make_instance(4100, 2, 16, 9, 64)
+    note("vfb largest table")
+    want = _oracle(inst, 6, exhaustive=True)
+    c = simba_amd.Context(0)
+    try:
+        got = c.vfb(inst, max_size=6, exhaustive=True, cache_bytes=96 << 30)
+    finally:
+        c.close()
+    assert got["table_slots"] == 1 << 31
+    assert got["level_new"][1:6] == want["level_new"][1:6]
+    assert (got["size"], got["rank"]) == (want["size"], want["rank"])
