@@ -242,6 +242,12 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
           finish(i, s);
           continue;
         }
+        // small sizes are swept fused in one round (a round per block
+        // group costs a graph replay and a sync)
+        if (st.T->t[s][8] < kLearnedBlockMin) {
+          order[i].clear();
+          continue;
+        }
         order[i] = binary_blocks(*st.T, s);
         const float *sc = scores + i * 256;
         std::stable_sort(order[i].begin(), order[i].end(),
@@ -311,7 +317,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
           J.kind = 0;
           push(J);
         }
-        if (s >= 3 && !learned && round == 0) {
+        if (s >= 3 && round == 0 && (!learned || T.t[s][8] < kLearnedBlockMin)) {
           for (int j = 1; j <= (s - 1) / 2; ++j) pair_job(j, kAllInst);
         }
         if (learned) {

@@ -315,13 +315,24 @@ __device__ __forceinline__ bool exact_layout(uint32_t mask, bool same, uint32_t 
   return span == 0u && (mask & full) == full;
 }
 
+// Whether a & b or a | b can land in the window [w, w + span] for some b
+// (low halves): every value of the window shares its bits above the span's
+// bit length p with w or with w + span, and a & b lies inside a (a | b
+// contains a), so one of those two high parts must lie inside a (contain a).
+__device__ __forceinline__ bool window_bitops(uint32_t al, uint32_t w, uint32_t span) {
+  const int p = span ? 32 - __builtin_clz(span) : 0;
+  if (p >= 32) return true;
+  const uint32_t ah = al >> p, h0 = w >> p, h1 = (w + span) >> p;
+  return (h0 & ~ah) == 0u || (h1 & ~ah) == 0u || (ah & ~h0) == 0u || (ah & ~h1) == 0u;
+}
+
 // Stages A values [a0, a1) (GEN_A: computed per lane from the tables).
 // Returns, for the exact mode, whether any A value admits & or | hits.
 template <typename word>
 __device__ __forceinline__ bool stage_a(AStage &S, const TableView *__restrict__ tv,
                                         const word *__restrict__ A, bool gen_a, int j,
                                         uint64_t a0, uint64_t a1, bool exact, uint32_t le,
-                                        uint32_t lo) {
+                                        uint32_t lo, uint32_t span) {
   __syncthreads();  // the previous tile's readers are done
   const uint32_t i = threadIdx.x;
   bool bit = false;
@@ -334,8 +345,9 @@ __device__ __forceinline__ bool stage_a(AStage &S, const TableView *__restrict__
       S.q1[i] = make_uint4(le + al, gap, 0u, 0u);
       bit = gap == 0u;
     } else {
+      const bool wb = window_bitops(al, le, span) || window_bitops(al, lo, span);
       S.q0[i] = make_uint4(al, le - al, lo - al, al - le);
-      S.q1[i] = make_uint4(al - lo, al + le, al + lo, 0u);
+      S.q1[i] = make_uint4(al - lo, al + le, al + lo, wb ? 1u : 0u);
     }
   }
   return __syncthreads_or(bit) != 0;
@@ -581,6 +593,8 @@ __device__ __forceinline__ bool exact_tile(const AStage &S, uint32_t na, const u
   return (m >> (threadIdx.x & 63)) & 1ull;
 }
 
+// Window mode: & and | only for the A values whose staged flag says they
+// can reach a window (a wave-uniform branch per A value).
 template <bool SAME, int RB>
 __device__ __forceinline__ uint32_t window_tile(const AStage &S, uint32_t na,
                                                 const uint32_t *bl, uint32_t le, uint32_t lo) {
@@ -588,12 +602,18 @@ __device__ __forceinline__ uint32_t window_tile(const AStage &S, uint32_t na,
   for (uint32_t i = 0; i < na; ++i) {
     const uint4 q0 = S.q0[i];
     const uint4 q1 = S.q1[i];
+    if (uniform(q1.w)) {
+#pragma unroll
+      for (int k = 0; k < RB; ++k) {
+        const uint32_t x_and = q0.x & bl[k], x_or = q0.x | bl[k];
+        acc = min3u(acc, x_and - le, x_and - lo);
+        acc = min3u(acc, x_or - le, x_or - lo);
+      }
+    }
 #pragma unroll
     for (int k = 0; k < RB; ++k) {
       const uint32_t b = bl[k];
-      const uint32_t x_and = q0.x & b, x_or = q0.x | b, x_xor = q0.x ^ b, x_mul = q0.x * b;
-      acc = min3u(acc, x_and - le, x_and - lo);
-      acc = min3u(acc, x_or - le, x_or - lo);
+      const uint32_t x_xor = q0.x ^ b, x_mul = q0.x * b;
       acc = min3u(acc, x_xor - le, x_xor - lo);
       acc = min3u(acc, x_mul - le, x_mul - lo);
       acc = min3u(acc, b - q0.y, b - q0.z);                // a + b
@@ -705,7 +725,7 @@ k_pair_check(const TableView *__restrict__ tv, const word *__restrict__ A,
     if (at != staged) {  // workgroup-uniform: every wave takes the same tiles
       const uint64_t a0 = at * kPairTA;
       const uint64_t a1 = a0 + kPairTA < p.nA ? a0 + kPairTA : p.nA;
-      bitops = stage_a<word>(S, tv, A, p.gen_a, p.j, a0, a1, exact, le, lo);
+      bitops = stage_a<word>(S, tv, A, p.gen_a, p.j, a0, a1, exact, le, lo, (uint32_t)span);
       staged = at;
     }
     pair_tile<word, SAME, BSRC, RB>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o, span, hits,
@@ -840,7 +860,7 @@ __device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J, ui
   const uint64_t a1 = a0 + kPairTA < p.nA ? a0 + kPairTA : p.nA;
   const bool bitops = stage_a<word>(S, tv, A, p.gen_a, p.j, a0, a1,
                                     exact_layout(p.mask, p.j == p.m, (uint32_t)span),
-                                    lo32(lo_e), lo32(lo_o));
+                                    lo32(lo_e), lo32(lo_o), (uint32_t)span);
   if (p.j == p.m) {
     if (p.gen_b)
       pair_tile<word, true, kBStep, kPairRB>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o, span,
