@@ -73,7 +73,8 @@ typedef struct {
   uint64_t table_bytes;     /* SIMBA: value-table budget, 0 = default        */
   uint64_t cache_bytes;     /* VFB: row-store budget, 0 = default            */
   int32_t verbose;          /* progress on stderr                            */
-  int32_t wg_per_cu;        /* launch tuning: workgroups per CU (0: 8)       */
+  int32_t wg_per_cu;        /* launch tuning: workgroups per CU (0: 32       */
+                            /* single-instance, 8 batched)                   */
   int32_t lanes_b;          /* launch tuning: B values per lane, 4 or 8 (0: 8) */
   int32_t table_delta;      /* tabulate Kt - table_delta sizes (0: budget)   */
   int32_t block_log2;       /* sweep one (op, split) block per launch from   */

@@ -2,7 +2,7 @@
 
 Tunes, by timing real sweeps on the device:
   lanes_b     -- B values (candidates' right operands) held per lane: 4 or 8;
-  wg_per_cu   -- workgroups per CU of the persistent launches: 4, 8 or 16;
+  wg_per_cu   -- workgroups per CU of the grid-stride launches: 8 to 48;
   table_delta -- how many of the largest value tables to leave to per-tile
                  generation instead of HBM (0 or 1): the table-build /
                  on-the-fly-evaluation trade-off.
@@ -14,7 +14,7 @@ import itertools
 import json
 import os
 
-SIMBA_SPACE = {"lanes_b": (4, 8), "wg_per_cu": (4, 8, 16), "table_delta": (0, 1)}
+SIMBA_SPACE = {"lanes_b": (4, 8), "wg_per_cu": (8, 16, 32, 48), "table_delta": (0, 1)}
 BATCH_SPACE = {"wg_per_cu": (2, 4, 8, 16)}
 
 

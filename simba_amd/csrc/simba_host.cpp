@@ -470,7 +470,9 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   std::vector<Candidate> cands;
   uint32_t *d_hc = ctx.hit_count.as<uint32_t>();
   HitRec *d_hits = ctx.hits.as<HitRec>();
-  const int grid = ctx.cu_count * (o.wg_per_cu > 0 ? o.wg_per_cu : 8);
+  // 32 workgroups per CU by default: more than stay resident, so tiles are
+  // handed out in small grid-stride shares and no CU waits on a long tail
+  const int grid = ctx.cu_count * (o.wg_per_cu > 0 ? o.wg_per_cu : 32);
   double ktime = 0.0;
   uint32_t klaunch = 0;
   int answer = -1;
