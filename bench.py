@@ -169,6 +169,49 @@ def run_simba(args):
     return out
 
 
+def run_simba_small(args):
+    """Leaves {x, y, 1}, u64, 16 samples, a uniform size-7 target, sizes 1..9:
+    time to the first solution (latency-bound: at most 6.3e6 candidates).
+    The default seed draws a target whose first solution has size 7."""
+    import numpy as np
+
+    import simba_amd
+    from simba_amd.synth import make_instance
+
+    inst = make_instance(args.small_seed, 2, 16, 7, 64, (1,))
+    ctx = simba_amd.Context(0)
+    kw = dict(min_size=1, max_size=9)
+    for _ in range(max(args.warmup, 1)):
+        r = ctx.simba(inst, **kw)
+    reps = max(args.steps, 1) * 20
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = ctx.simba(inst, **kw)
+        times.append(time.perf_counter() - t0)
+    ctx.close()
+    med = float(np.median(times))
+    out = {
+        "metric": "time to first solution", "value": round(1e3 * med, 4), "unit": "ms",
+        "n_gpus": 1, "steps": reps, "warmup": max(args.warmup, 1),
+        "ms_per_step": round(1e3 * med, 4), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "simba_2vars_const1_u64_16samples_size7target_sizes1to9",
+                   "target": inst.original_formula, "first_solution": r["expr"],
+                   "size": r["size"], "rank": r["rank"], "candidates": r["candidates"],
+                   "p95_ms": round(1e3 * float(np.percentile(times, 95)), 4)},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        threads = min(16, os.cpu_count() or 1)
+        cb = orc.simba_search(inst.inputs, inst.outputs, 2, [1], 64, 1, 9, threads)
+        assert (cb["size"], cb["rank"]) == (r["size"], r["rank"])
+        out["cpu_baseline"] = {"value": round(1e3 * cb["seconds"], 4), "unit": "ms",
+                               "cores": threads, "kind": "port",
+                               "sample": "oracle SIMBA search of the same instance, sizes 1..9"}
+    return out
+
+
 def run_simba_cap(args):
     """The same instance swept exhaustively over sizes 1..20 under a time cap
     (one run): how far the search gets, and its rate over completed sizes."""
@@ -399,7 +442,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="simba",
-                    choices=["simba", "simba_cap", "vfb", "batch", "learned", "copy"])
+                    choices=["simba", "simba_small", "simba_cap", "vfb", "batch", "learned",
+                             "copy"])
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--budget-size", type=int, default=9)
     ap.add_argument("--train", type=int, default=200000)
@@ -413,6 +457,8 @@ def main():
     ap.add_argument("--seed", type=int, default=15)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cap", type=int, default=60, help="simba_cap: time cap in seconds")
+    ap.add_argument("--small-seed", type=int, default=27,
+                    help="simba_small: instance seed (27: no equivalent below size 7)")
     ap.add_argument("--lanes-b", type=int, default=0, help="B values per lane (4 or 8)")
     ap.add_argument("--wg-per-cu", type=int, default=0, help="workgroups per CU")
     ap.add_argument("--autotune", action="store_true",
@@ -424,7 +470,7 @@ def main():
         sys.exit(2)
     out = {"simba": run_simba, "vfb": run_vfb, "batch": run_batch,
            "learned": run_learned, "copy": run_copy,
-           "simba_cap": run_simba_cap}[args.workload](args)
+           "simba_cap": run_simba_cap, "simba_small": run_simba_small}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

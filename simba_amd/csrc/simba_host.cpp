@@ -586,6 +586,20 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
       ktime += ms * 1e-3;
       group_open = false;
       group_tiles = 0;
+      if (answer < 0) {
+        // drain a filling hit buffer between groups (rare: many window
+        // hits that other samples reject)
+        uint32_t cnt = 0;
+        e = hipMemcpy(&cnt, d_hc, 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+          ctx.last_error = std::string("hit count: ") + hipGetErrorString(e);
+          return -1;
+        }
+        if (cnt > kHitCap / 2) {
+          if (process_hits(s) != 0) return -1;
+          if (hipMemsetAsync(d_hc, 0, 4, st) != hipSuccess) return -1;
+        }
+      }
       return over_budget() ? 1 : 0;
     };
     for (const BBlock &bb : *blocks) {
