@@ -313,7 +313,7 @@ def run_batch(args):
         from simba_amd import autotune
         tuned, _ = autotune.tune_batch(ctx, insts[:1024], args.budget_size)
         kw.update(tuned)
-    ctx.batch(insts[:64], **kw)
+    ctx.batch(insts, **kw)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         rb = ctx.batch(insts, **kw)

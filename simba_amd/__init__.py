@@ -353,6 +353,14 @@ class MbaInstance:
             lib().mba_instance_free(ctypes.byref(ci))
 
     def to_c(self):
+        """The C view of the instance (cached: an MbaInstance is not changed
+        after construction; build a new one for new data)."""
+        c = getattr(self, "_c", None)
+        if c is None:
+            c = self._c = self._make_c()
+        return c
+
+    def _make_c(self):
         ci = Instance()
         ci.width = self.width
         ci.n_vars = self.n_variables

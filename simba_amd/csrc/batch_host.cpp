@@ -113,7 +113,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     idat[i].leaf_off = leafvals.size();
     idat[i].out_off = outputs.size();
     idat[i].n_samples = (uint32_t)n;
-    idat[i].pad = (uint32_t)st.fs;
+    idat[i].fs = (uint32_t)st.fs;
     leafvals.insert(leafvals.end(), lv.begin(), lv.end());
     outputs.insert(outputs.end(), outv.begin(), outv.end());
     if (max_size < min_size) st.active = false;
@@ -135,27 +135,9 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
                          hipMemcpyHostToDevice, stq));
   MBA_HIP(hipMemcpyAsync(ctx.b_results.p, idat.data(), sizeof(BatchInstData) * count,
                          hipMemcpyHostToDevice, stq));
-  {  // V[1] of every instance: leaf values of its filter sample
-    std::vector<uint64_t> v1;
-    std::vector<uint32_t> v1w;
-    for (size_t i = 0; i < count; ++i) {
-      const uint64_t *src = leafvals.data() + idat[i].leaf_off + (size_t)S[i].fs * S[i].L;
-      for (int l = 0; l < S[i].L; ++l) {
-        v1.push_back(src[l]);
-        v1w.push_back((uint32_t)src[l]);
-      }
-    }
-    // scatter with one copy per instance from a single staging buffer
-    MBA_HIP(ctx.b_misc.ensure(8 * v1.size() + 64));
-    MBA_HIP(hipMemcpyAsync(ctx.b_misc.p, W == 64 ? (void *)v1.data() : (void *)v1w.data(),
-                           wb * v1.size(), hipMemcpyHostToDevice, stq));
-    size_t pos = 0;
-    for (size_t i = 0; i < count; ++i) {
-      MBA_HIP(hipMemcpyAsync((void *)tvs[i].base, ctx.b_misc.as<char>() + pos * wb,
-                             wb * S[i].L, hipMemcpyDeviceToDevice, stq));
-      pos += S[i].L;
-    }
-  }
+  // V[1] of every instance: leaf values of its filter sample (one kernel)
+  MBA_HIP(launch_batch_leaves(W, ctx.b_inst.as<TableView>(), ctx.leafvals.as<uint64_t>(),
+                              ctx.b_results.as<BatchInstData>(), (uint32_t)count, stq));
 
   // ---- per-step graph: upload jobs, reset counters, drain, read hit count --
   // jobs per instance and step: a leaf scan or a table build, plus the pairs
@@ -194,8 +176,6 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   MBA_HIP(hipStreamSynchronize(stq));
   MBA_HIP(hipStreamBeginCapture(stq, hipStreamCaptureModeThreadLocal));
   MBA_HIP(hipMemcpyAsync(d_step, h_step, sizeof(BatchStep), hipMemcpyHostToDevice, stq));
-  MBA_HIP(hipMemcpyAsync(d_jobs, h_jobs, sizeof(BatchJob) * job_cap, hipMemcpyHostToDevice,
-                         stq));
   MBA_HIP(launch_batch_reset(d_head, d_hc, stq));
   MBA_HIP(launch_batch_step(W, d_step, d_jobs, d_tvs, d_head, d_hits, d_hc, kBatchHitCap,
                             grid, stq));
@@ -329,6 +309,10 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
         h_step->total_tiles = tiles;
         h_step->njobs = (uint32_t)nj;
         MBA_HIP(hipEventRecord(ctx.ev0, stq));
+        // this step's jobs only (the graph then uploads the step header,
+        // resets the queue, drains it and reads the hit count back)
+        MBA_HIP(hipMemcpyAsync(d_jobs, h_jobs, sizeof(BatchJob) * nj, hipMemcpyHostToDevice,
+                               stq));
         MBA_HIP(hipGraphLaunch(gexec, stq));
         MBA_HIP(hipEventRecord(ctx.ev1, stq));
         MBA_HIP(hipEventSynchronize(ctx.ev1));

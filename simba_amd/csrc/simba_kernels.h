@@ -59,7 +59,7 @@ struct BBlock {
   BStep steps[kMaxBSteps];
 };
 
-struct PairLaunch {
+struct PairCore {
   int sb, j, m;            // candidate size sb = j + m + 1, j <= m
   uint64_t nA, nB;         // T[j], T[m]
   uint64_t tile_begin, tile_end;  // tile range of this launch
@@ -68,6 +68,9 @@ struct PairLaunch {
   bool gen_a, gen_b;       // side is above Kt: values computed per lane
   uint32_t mask;           // operator instances to check (bit i = inst i)
   int rb;                  // B values per lane: 4 or 8 (tile = 256 * rb B values)
+};
+
+struct PairLaunch : PairCore {
   BBlock bb;               // the B block (gen_b, per-instance launches)
 };
 
@@ -87,7 +90,7 @@ constexpr int kBuildTile = 1024;  // table values per build tile
 struct BatchJob {
   uint64_t tile0, ntiles;
   uint64_t lo_e, lo_o, span;  // windows (kinds 1 and 2)
-  PairLaunch p;
+  PairCore p;                 // B sides above Kt are stepped per tile (no blocks)
   uint32_t inst;
   uint32_t kind;
 };
@@ -103,7 +106,7 @@ struct BatchInstData {
   uint64_t leaf_off;   // [sample][leaf] values of this instance in the batch array
   uint64_t out_off;    // outputs of this instance
   uint32_t n_samples;
-  uint32_t pad;
+  uint32_t fs;         // filter sample (the tables' sample)
 };
 
 struct VerifyCand {
@@ -113,6 +116,10 @@ struct VerifyCand {
 
 hipError_t launch_batch_reset(unsigned long long *d_head, uint32_t *d_hit_count,
                               hipStream_t stream);
+// V[1] (filter-sample leaf values) of every instance of a batch, one launch;
+hipError_t launch_batch_leaves(int width, const TableView *d_tvs, const uint64_t *d_leafvals,
+                               const BatchInstData *d_idat, uint32_t count,
+                               hipStream_t stream);
 hipError_t launch_batch_step(int width, const BatchStep *d_step, const BatchJob *d_jobs,
                              const TableView *d_tvs, unsigned long long *d_head,
                              HitRec *d_hits, uint32_t *d_hit_count, uint32_t hit_cap,

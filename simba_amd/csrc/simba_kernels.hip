@@ -850,7 +850,8 @@ template <typename word>
 __device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J, uint64_t lt,
                            HitRec *hits, uint32_t *hc, uint32_t cap) {
   const word *base = (const word *)tv->base;
-  const PairLaunch &p = J.p;
+  PairLaunch p;
+  static_cast<PairCore &>(p) = J.p;  // p.bb is not read by the table/stepped sources
   const word *A = base + (p.gen_a ? 0 : tv->off[p.j]);
   const word *B = base + (p.gen_b ? 0 : tv->off[p.m]);
   const word lo_e = (word)J.lo_e, lo_o = (word)J.lo_o, span = (word)J.span;
@@ -926,6 +927,22 @@ k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jo
                            1u << 16);
       }
     }
+  }
+}
+
+// V[1] of every instance of a batch: the leaf values of its filter sample
+// (workgroup i, lane l < L: tables_i[l] = leafvals[leaf_off_i + fs_i*L + l]).
+template <typename word>
+__global__ void k_batch_leaves(const TableView *__restrict__ tvs,
+                               const uint64_t *__restrict__ leafvals,
+                               const BatchInstData *__restrict__ idat, uint32_t count) {
+  const uint32_t i = blockIdx.x;
+  if (i >= count) return;
+  const TableView &tv = tvs[i];
+  const uint32_t l = threadIdx.x;
+  if (l < (uint32_t)tv.L) {
+    const BatchInstData &d = idat[i];
+    ((word *)tv.base)[l] = (word)leafvals[d.leaf_off + (uint64_t)d.fs * tv.L + l];
   }
 }
 
@@ -1221,6 +1238,19 @@ hipError_t launch_leaf_scan(int width, const TableView *d_tv, int L,
   else
     hipLaunchKernelGGL(k_leaf_scan<uint64_t>, dim3(1), dim3(64), 0, stream, d_tv, L,
                        tg.lo_even, tg.lo_odd, tg.span, d_hits, d_hit_count, hit_cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_leaves(int width, const TableView *d_tvs, const uint64_t *d_leafvals,
+                               const BatchInstData *d_idat, uint32_t count,
+                               hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  if (width == 32)
+    hipLaunchKernelGGL(k_batch_leaves<uint32_t>, dim3(count), dim3(64), 0, stream, d_tvs,
+                       d_leafvals, d_idat, count);
+  else
+    hipLaunchKernelGGL(k_batch_leaves<uint64_t>, dim3(count), dim3(64), 0, stream, d_tvs,
+                       d_leafvals, d_idat, count);
   return hipGetLastError();
 }
 

@@ -81,6 +81,15 @@ def top1_accuracy(mlp, X, Y):
     return float((z.argmax(axis=1) == Y).mean())
 
 
+def mean_label_position(mlp, X, Y):
+    """Mean position (0 = first) of each instance's solution block in the
+    order the network proposes: how many blocks the learned search visits
+    before the one that holds the solution."""
+    z = mlp.forward(X)
+    true = z[np.arange(len(Y)), Y][:, None]
+    return float((z > true).sum(axis=1).mean())
+
+
 def save(mlp, path):
     w1, b1, w2, b2 = mlp.get_weights()
     np.savez(path, w1=w1, b1=b1, w2=w2, b2=b2)
