@@ -316,7 +316,7 @@ def run_batch(args):
     ctx.batch(insts, **kw)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        rb = ctx.batch(insts, **kw)
+        rb = ctx.batch(insts, verbose=args.verbose, **kw)
     t_batch = (time.perf_counter() - t0) / args.steps
     t0 = time.perf_counter()
     rp = [ctx.simba(x, **kw) for x in insts]
@@ -460,6 +460,7 @@ def main():
     ap.add_argument("--small-seed", type=int, default=27,
                     help="simba_small: instance seed (27: no equivalent below size 7)")
     ap.add_argument("--lanes-b", type=int, default=0, help="B values per lane (4 or 8)")
+    ap.add_argument("--verbose", action="store_true", help="library progress on stderr")
     ap.add_argument("--wg-per-cu", type=int, default=0, help="workgroups per CU")
     ap.add_argument("--autotune", action="store_true",
                     help="tune launch parameters on the workload before timing")

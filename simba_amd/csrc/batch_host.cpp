@@ -205,6 +205,11 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     st.active = false;
   };
   std::vector<std::vector<BlockRef>> order(count);
+  if (o.verbose)
+    std::fprintf(stderr, "batch: %zu instances set up in %.3f ms\n", count,
+                 1e3 * seconds_since(t_start));
+  double t_build = 0, t_gpu = 0, t_resolve = 0;
+  uint64_t n_hits = 0, n_cands = 0;
   for (int s = 0; s <= max_size; ++s) {
     if ((o.cancel && *o.cancel) ||
         (o.time_budget_s > 0 && seconds_since(t_start) > o.time_budget_s)) {
@@ -243,6 +248,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       const size_t re = (size_t)1 << round;
       size_t nj = 0;
       uint64_t tiles = 0;
+      const auto t_b0 = Clock::now();
       for (size_t i = 0; i < count; ++i) {
         InstState &st = S[i];
         if (!st.active) continue;
@@ -305,7 +311,9 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
             pair_job(order[i][k].pair_j, 1u << order[i][k].inst);
         }
       }
+      t_build += seconds_since(t_b0);
       if (nj > 0) {
+        const auto t_g0 = Clock::now();
         h_step->total_tiles = tiles;
         h_step->njobs = (uint32_t)nj;
         MBA_HIP(hipEventRecord(ctx.ev0, stq));
@@ -320,11 +328,14 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
         MBA_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
         ktime += ms * 1e-3;
         ++klaunch;
+        t_gpu += seconds_since(t_g0);
+        const auto t_r0 = Clock::now();
         const uint32_t nh = *h_hc;
         if (nh > kBatchHitCap) {
           ctx.last_error = "batch: sample hit buffer overflow";
           return MBA_ERR_LIMIT;
         }
+        n_hits += nh;
         if (nh > 0) {
           hrec.resize(nh);
           MBA_HIP(hipMemcpy(hrec.data(), d_hits, sizeof(HitRec) * nh, hipMemcpyDeviceToHost));
@@ -340,6 +351,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
             resolve_hit(*st.T, W, st.t0, sb, rX, h.v & M, min_size, max_size, cands);
             for (const Candidate &c : cands) vc.push_back({i, (uint32_t)c.size, c.rank});
           }
+          n_cands += vc.size();
           if (!vc.empty()) {
             const uint32_t ncand = (uint32_t)vc.size();
             MBA_HIP(ctx.vranks.ensure(sizeof(VerifyCand) * ncand));
@@ -361,6 +373,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
               }
           }
         }
+        t_resolve += seconds_since(t_r0);
       }
       if (!learned) break;
       // learned order: any verified solution of this (minimal) size settles
@@ -385,6 +398,12 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     }
   }
   const double total = seconds_since(t_start);
+  if (o.verbose)
+    std::fprintf(stderr,
+                 "batch: jobs %.3f ms, device %.3f ms, hits %.3f ms (%llu hits, %llu "
+                 "candidates), total %.3f ms\n",
+                 1e3 * t_build, 1e3 * t_gpu, 1e3 * t_resolve, (unsigned long long)n_hits,
+                 (unsigned long long)n_cands, 1e3 * total);
   for (size_t i = 0; i < count; ++i) {
     InstState &st = S[i];
     mba_result &r = results[i];

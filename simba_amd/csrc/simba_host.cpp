@@ -180,10 +180,13 @@ void resolve_hit(const CountTable &T, int W, uint64_t t0, int sb, uint64_t rX,
   const int64_t ko = as_signed((t0 + v) & M, W);
   const bool want_e = ke >= -U && ke <= U, want_o = ko >= -U && ko <= U;
   if (!want_e && !want_o) return;
-  // cost[k + off] = min rank increment of a chain of length i reaching k.
+  // cost[k + off] = min rank increment of a chain of length i reaching k
+  // (fixed arrays: this runs once per window hit)
   const int off = U + 2, width = 2 * U + 5;
   const uint64_t INF = ~0ull;
-  std::vector<uint64_t> cost(width, INF), next(width, INF);
+  uint64_t cost_a[2 * kMaxSize + 8], next_a[2 * kMaxSize + 8];
+  uint64_t *cost = cost_a, *next = next_a;
+  std::fill(cost, cost + width, INF);
   cost[off] = 0;
   bool done_e = !want_e, done_o = !want_o;
   for (int i = 0; sb + i <= max_size && !(done_e && done_o); ++i) {
@@ -199,7 +202,7 @@ void resolve_hit(const CountTable &T, int W, uint64_t t0, int sb, uint64_t rX,
       }
     }
     if (size + 1 > max_size) break;
-    std::fill(next.begin(), next.end(), INF);
+    std::fill(next, next + width, INF);
     const uint64_t neg_off = T.op_offset(size + 1, NEG);
     for (int k = -U - 1; k <= U + 1; ++k) {
       const uint64_t c = cost[k + off];
