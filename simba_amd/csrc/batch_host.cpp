@@ -10,6 +10,7 @@
 // instance's answer is the one mba_simba_solve returns on it: the same
 // tables, windows, hit resolution and verification, per instance.
 #include <algorithm>
+#include <thread>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -19,6 +20,9 @@
 #include "simba_host.h"
 
 namespace mba {
+
+// host threads that resolve a step's window hits when there are many
+constexpr size_t kResolveThreads = 8;
 
 namespace {
 
@@ -192,7 +196,6 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   } gguard{graph, gexec};
 
   std::vector<HitRec> hrec;
-  std::vector<Candidate> cands;
   std::vector<VerifyCand> vc;
   double ktime = 0.0;
   uint32_t klaunch = 0;
@@ -210,6 +213,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
                  1e3 * seconds_since(t_start));
   double t_build = 0, t_gpu = 0, t_resolve = 0;
   uint64_t n_hits = 0, n_cands = 0;
+  std::vector<uint32_t> hits_of(o.verbose ? count : 0, 0);
   for (int s = 0; s <= max_size; ++s) {
     if ((o.cancel && *o.cancel) ||
         (o.time_budget_s > 0 && seconds_since(t_start) > o.time_budget_s)) {
@@ -340,16 +344,33 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
           hrec.resize(nh);
           MBA_HIP(hipMemcpy(hrec.data(), d_hits, sizeof(HitRec) * nh, hipMemcpyDeviceToHost));
           vc.clear();
-          for (const HitRec &h0 : hrec) {
-            HitRec h = h0;
-            const uint32_t i = h.inst >> 8;
-            h.inst &= 0xff;
-            const InstState &st = S[i];
-            const int sb = (int)(h.pair >> 16), j = (int)(h.pair & 0xffff);
-            const uint64_t rX = base_rank(*st.T, sb, j, sb - 1 - j, h);
-            cands.clear();
-            resolve_hit(*st.T, W, st.t0, sb, rX, h.v & M, min_size, max_size, cands);
-            for (const Candidate &c : cands) vc.push_back({i, (uint32_t)c.size, c.rank});
+          if (!hits_of.empty())
+            for (const HitRec &h0 : hrec) hits_of[h0.inst >> 8]++;
+          // resolve the hits to candidate trees, in host threads when many
+          auto resolve = [&](size_t lo, size_t hi, std::vector<VerifyCand> &out) {
+            std::vector<Candidate> cs;
+            for (size_t k = lo; k < hi; ++k) {
+              HitRec h = hrec[k];
+              const uint32_t i = h.inst >> 8;
+              h.inst &= 0xff;
+              const InstState &st = S[i];
+              const int sb = (int)(h.pair >> 16), j = (int)(h.pair & 0xffff);
+              const uint64_t rX = base_rank(*st.T, sb, j, sb - 1 - j, h);
+              cs.clear();
+              resolve_hit(*st.T, W, st.t0, sb, rX, h.v & M, min_size, max_size, cs);
+              for (const Candidate &c : cs) out.push_back({i, (uint32_t)c.size, c.rank});
+            }
+          };
+          const size_t nthr = nh >= 16384 ? std::min<size_t>(kResolveThreads, nh / 8192) : 1;
+          if (nthr <= 1) {
+            resolve(0, nh, vc);
+          } else {
+            std::vector<std::vector<VerifyCand>> parts(nthr);
+            std::vector<std::thread> th;
+            for (size_t t = 0; t < nthr; ++t)
+              th.emplace_back(resolve, nh * t / nthr, nh * (t + 1) / nthr, std::ref(parts[t]));
+            for (auto &x : th) x.join();
+            for (auto &p : parts) vc.insert(vc.end(), p.begin(), p.end());
           }
           n_cands += vc.size();
           if (!vc.empty()) {
@@ -404,6 +425,17 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
                  "candidates), total %.3f ms\n",
                  1e3 * t_build, 1e3 * t_gpu, 1e3 * t_resolve, (unsigned long long)n_hits,
                  (unsigned long long)n_cands, 1e3 * total);
+  if (o.verbose && count > 0) {
+    std::vector<uint32_t> idx(count);
+    for (size_t i = 0; i < count; ++i) idx[i] = (uint32_t)i;
+    std::partial_sort(idx.begin(), idx.begin() + std::min<size_t>(8, count), idx.end(),
+                      [&](uint32_t a, uint32_t b) { return hits_of[a] > hits_of[b]; });
+    for (size_t k = 0; k < std::min<size_t>(8, count); ++k) {
+      const uint32_t i = idx[k];
+      std::fprintf(stderr, "batch: instance %u: %u hits, L=%d n=%d, answer size %d (%s)\n", i,
+                   hits_of[i], S[i].L, S[i].n, results[i].size, results[i].expr);
+    }
+  }
   for (size_t i = 0; i < count; ++i) {
     InstState &st = S[i];
     mba_result &r = results[i];

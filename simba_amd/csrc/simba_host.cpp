@@ -223,9 +223,16 @@ int pick_filter_sample(const std::vector<uint64_t> &lv, const std::vector<uint64
     const uint64_t d = std::min(x & M, (0 - x) & M);
     return d <= 2 || x == (1ull << (W - 1));
   };
+  // an output with few or many set bits is the value of many trees (the
+  // bitwise operators drift toward 0 and ~0): prefer balanced ones
+  auto balanced = [&](uint64_t x) {
+    const int pc = __builtin_popcountll(x & M);
+    return pc >= W / 4 && pc <= 3 * W / 4;
+  };
   int best = 0, best_score = -1;
   for (int s = 0; s < n; ++s) {
     int score = weak(outv[s]) ? 0 : 2 * L;
+    if (balanced(outv[s])) score += 2 * L;
     for (int a = 0; a < L; ++a) {
       const uint64_t x = lv[(size_t)s * L + a];
       if (weak(x)) continue;
