@@ -308,7 +308,8 @@ def run_batch(args):
     insts = sample_instances(args.seed, args.count, (5, 17), (2, 5), (16, 64))
     ctx = simba_amd.Context(0)
     kw = dict(max_size=args.budget_size)
-    tuned = {}
+    tuned = {"wg_per_cu": args.wg_per_cu} if args.wg_per_cu else {}
+    kw.update(tuned)
     if args.autotune:
         from simba_amd import autotune
         tuned, _ = autotune.tune_batch(ctx, insts[:1024], args.budget_size)

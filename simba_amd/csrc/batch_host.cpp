@@ -112,6 +112,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     std::memcpy(tv.T, st.T->t, sizeof tv.T);
     tv.Kt = st.Kt;
     tv.L = L;
+    set_second_sample(tv, lv, outv, L, st.fs, pick_filter_sample(lv, outv, n, L, W, st.fs));
     tv.base = (const void *)(uintptr_t)(pool_elems * wb);  // patched to a pointer below
     pool_elems += e;
     idat[i].leaf_off = leafvals.size();
@@ -212,7 +213,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     std::fprintf(stderr, "batch: %zu instances set up in %.3f ms\n", count,
                  1e3 * seconds_since(t_start));
   double t_build = 0, t_gpu = 0, t_resolve = 0;
-  uint64_t n_hits = 0, n_cands = 0;
+  uint64_t n_hits = 0, n_kept = 0, n_cands = 0;
   std::vector<uint32_t> hits_of(o.verbose ? count : 0, 0);
   for (int s = 0; s <= max_size; ++s) {
     if ((o.cancel && *o.cancel) ||
@@ -340,9 +341,23 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
           return MBA_ERR_LIMIT;
         }
         n_hits += nh;
-        if (nh > 0) {
+        uint32_t nk = nh;
+        const HitRec *src = d_hits;
+        if (nh > 0) {  // keep the hits that also meet each instance's second sample
+          uint32_t *d_fc = (uint32_t *)(ctx.hit_count.as<char>() + 128);
+          MBA_HIP(ctx.hits2.ensure(sizeof(HitRec) * nh));
+          MBA_HIP(hipMemsetAsync(d_fc, 0, 4, stq));
+          MBA_HIP(launch_hit_filter(W, d_tvs, d_hits, nh, max_size, ctx.hits2.as<HitRec>(),
+                                    d_fc, stq));
+          MBA_HIP(hipMemcpyAsync(&nk, d_fc, 4, hipMemcpyDeviceToHost, stq));
+          MBA_HIP(hipStreamSynchronize(stq));
+          src = ctx.hits2.as<HitRec>();
+        }
+        n_kept += nk;
+        if (nk > 0) {
+          const uint32_t nh = nk;
           hrec.resize(nh);
-          MBA_HIP(hipMemcpy(hrec.data(), d_hits, sizeof(HitRec) * nh, hipMemcpyDeviceToHost));
+          MBA_HIP(hipMemcpy(hrec.data(), src, sizeof(HitRec) * nh, hipMemcpyDeviceToHost));
           vc.clear();
           if (!hits_of.empty())
             for (const HitRec &h0 : hrec) hits_of[h0.inst >> 8]++;
@@ -421,10 +436,10 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   const double total = seconds_since(t_start);
   if (o.verbose)
     std::fprintf(stderr,
-                 "batch: jobs %.3f ms, device %.3f ms, hits %.3f ms (%llu hits, %llu "
-                 "candidates), total %.3f ms\n",
+                 "batch: jobs %.3f ms, device %.3f ms, hits %.3f ms (%llu hits, %llu on "
+                 "both samples, %llu candidates), total %.3f ms\n",
                  1e3 * t_build, 1e3 * t_gpu, 1e3 * t_resolve, (unsigned long long)n_hits,
-                 (unsigned long long)n_cands, 1e3 * total);
+                 (unsigned long long)n_kept, (unsigned long long)n_cands, 1e3 * total);
   if (o.verbose && count > 0) {
     std::vector<uint32_t> idx(count);
     for (size_t i = 0; i < count; ++i) idx[i] = (uint32_t)i;

@@ -216,8 +216,17 @@ void resolve_hit(const CountTable &T, int W, uint64_t t0, int sb, uint64_t rX,
   }
 }
 
+void set_second_sample(TableView &tv, const std::vector<uint64_t> &lv,
+                       const std::vector<uint64_t> &outv, int L, int fs, int s2) {
+  tv.t1 = outv[fs];
+  tv.has2 = s2 >= 0;
+  if (!tv.has2) return;
+  for (int l = 0; l < L; ++l) tv.leaf2[l] = lv[(size_t)s2 * L + l];
+  tv.t2 = outv[s2];
+}
+
 int pick_filter_sample(const std::vector<uint64_t> &lv, const std::vector<uint64_t> &outv,
-                       int n, int L, int W) {
+                       int n, int L, int W, int skip) {
   const uint64_t M = wmask(W);
   auto weak = [&](uint64_t x) {
     const uint64_t d = std::min(x & M, (0 - x) & M);
@@ -229,8 +238,9 @@ int pick_filter_sample(const std::vector<uint64_t> &lv, const std::vector<uint64
     const int pc = __builtin_popcountll(x & M);
     return pc >= W / 4 && pc <= 3 * W / 4;
   };
-  int best = 0, best_score = -1;
+  int best = -1, best_score = -1;
   for (int s = 0; s < n; ++s) {
+    if (s == skip) continue;
     int score = weak(outv[s]) ? 0 : 2 * L;
     if (balanced(outv[s])) score += 2 * L;
     for (int a = 0; a < L; ++a) {
@@ -392,6 +402,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   std::memcpy(tv.T, T.t, sizeof tv.T);
   tv.Kt = Kt;
   tv.L = L;
+  set_second_sample(tv, lv, outv, L, fs, pick_filter_sample(lv, outv, n, L, W, fs));
   MBA_HIP(ctx.tables.ensure(std::max<size_t>(elems * wb, 256)));
   tv.base = ctx.tables.p;
   MBA_HIP(ctx.tview.ensure(sizeof(TableView)));
@@ -499,8 +510,20 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
       ctx.last_error = "sample-0 hit buffer overflow";
       return MBA_ERR_LIMIT;
     }
+    const HitRec *src = d_hits;
+    if (tv.has2) {  // keep the hits that also meet the second sample
+      uint32_t *d_fc = (uint32_t *)(ctx.hit_count.as<char>() + 128);
+      MBA_HIP(ctx.hits2.ensure(sizeof(HitRec) * cnt));
+      MBA_HIP(hipMemsetAsync(d_fc, 0, 4, st));
+      MBA_HIP(launch_hit_filter(W, d_tv, d_hits, cnt, max_size, ctx.hits2.as<HitRec>(), d_fc,
+                                st));
+      MBA_HIP(hipMemcpyAsync(&cnt, d_fc, 4, hipMemcpyDeviceToHost, st));
+      MBA_HIP(hipStreamSynchronize(st));
+      if (cnt == 0) return 0;
+      src = ctx.hits2.as<HitRec>();
+    }
     hrec.resize(cnt);
-    MBA_HIP(hipMemcpy(hrec.data(), d_hits, sizeof(HitRec) * cnt, hipMemcpyDeviceToHost));
+    MBA_HIP(hipMemcpy(hrec.data(), src, sizeof(HitRec) * cnt, hipMemcpyDeviceToHost));
     cands.clear();
     for (const HitRec &h : hrec) {
       const int sb = (int)(h.pair >> 16), j = (int)(h.pair & 0xffff);

@@ -13,6 +13,12 @@ struct TableView {
   const void *base;               // uint32_t or uint64_t elements
   int Kt;                         // largest tabulated size
   int L;                          // number of leaves
+  // a second sample that a window hit must also meet before it is recorded
+  // (its leaf values and output); has2 = 0 when the instance has one sample
+  uint64_t leaf2[kMaxLeaves];
+  uint64_t t1, t2;                // outputs of the filter and second samples
+  int has2;
+  int pad;
 };
 
 // One sample-0 hit: the value c = op(A[a], B[b]) (op-instance `inst`) lies in
@@ -77,7 +83,7 @@ struct PairLaunch : PairCore {
 constexpr uint32_t kAllInst = 0x7f;
 
 constexpr int kPairThreads = 256;
-constexpr int kPairRB = 4;                      // B values held per lane
+constexpr int kPairRB = 8;                      // B values per lane (batched kernel)
 constexpr int kPairTB = kPairThreads * kPairRB; // B values per tile
 constexpr int kPairTA = 256;                    // A values per tile
 
@@ -143,6 +149,11 @@ hipError_t launch_leaf_scan(int width, const TableView *d_tv, int L,
                             const CheckTargets &tg, HitRec *d_hits,
                             uint32_t *d_hit_count, uint32_t hit_cap,
                             hipStream_t stream);
+// Keeps the hits in[0, n) that also meet each instance's second sample
+// (tvs[hit.inst >> 8]); kept hits are appended to d_out / *d_out_count.
+hipError_t launch_hit_filter(int width, const TableView *d_tvs, const HitRec *d_in, uint32_t n,
+                             int max_size, HitRec *d_out, uint32_t *d_out_count,
+                             hipStream_t stream);
 // SIMBA-RTid scan of threads [g0, g1) of the (size, op) block at `offset` with
 // N trees: *d_best = min(*d_best, g) over threads whose tree matches every
 // sample (t0 = the filter sample's output, the tables' sample).

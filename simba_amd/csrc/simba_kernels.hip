@@ -94,6 +94,69 @@ __device__ word value_of(const TableView *__restrict__ tv, int m, uint64_t r) {
 }
 
 // ---------------------------------------------------------------------------
+// Unrank (size, rank) into reference RPN tokens (numberToFormula,
+// nocache.cu:23-83) with a per-lane stack.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void unrank_rpn(const TableView *__restrict__ tv, int size,
+                                           uint64_t rank, uint8_t *rpn) {
+  struct Item { uint64_t r; int s, pos; };
+  Item st[kMaxSize];
+  int sp = 0;
+  st[sp++] = {rank, size, 0};
+  while (sp > 0) {
+    Item it = st[--sp];
+    if (it.s == 1) {
+      rpn[it.pos] = (uint8_t)it.r;
+      continue;
+    }
+    uint64_t n = it.r;
+    int op = 0;
+    while (n >= tv->T[it.s][op]) n -= tv->T[it.s][op++];
+    const int cs = it.s - 1;
+    rpn[it.pos + cs] = (uint8_t)(kTokOp + op);
+    if (is_unary(op)) {
+      st[sp++] = {n, cs, it.pos};
+      continue;
+    }
+    int j = 1;
+    for (;; ++j) {
+      const uint64_t blk = tv->T[j][8] * tv->T[cs - j][8];
+      if (n < blk) break;
+      n -= blk;
+    }
+    const uint64_t rs = tv->T[cs - j][8];
+    const uint64_t a = n / rs;
+    st[sp++] = {a, j, it.pos};
+    st[sp++] = {n - a * rs, cs - j, it.pos + j};
+  }
+}
+
+// Value of the tree (size, rank) on one sample's leaf values.
+template <typename word>
+__device__ word eval_tree(const TableView *__restrict__ tv, int size, uint64_t rank,
+                          const uint64_t *leaves) {
+  uint8_t rpn[kMaxSize];
+  unrank_rpn(tv, size, rank, rpn);
+  word vs[kMaxSize];
+  int top = -1;
+  for (int i = 0; i < size; ++i) {
+    const uint8_t t = rpn[i];
+    if (t < kTokOp) {
+      vs[++top] = (word)leaves[t];
+    } else {
+      const int op = t - kTokOp;
+      if (is_unary(op)) {
+        vs[top] = apply_op<word>(op, vs[top], word(0));
+      } else {
+        const word rv = vs[top--];
+        vs[top] = apply_op<word>(op, vs[top], rv);
+      }
+    }
+  }
+  return vs[0];
+}
+
+// ---------------------------------------------------------------------------
 // Table build: V[m][r] for r in [0, T[m]).  One lane per rank; the op block
 // and split are located by prefix sums over the count table (the reference's
 // unrank order, nocache.cu:52-79), the children are read from V[<m].
@@ -758,36 +821,7 @@ template <typename word>
 __device__ bool tree_matches(const TableView *__restrict__ tv, const word *s_leaf,
                              const word *s_out, int L, int n_samples, int size, uint64_t rank) {
   uint8_t rpn[kMaxSize];
-  struct Item { uint64_t r; int s, pos; };
-  Item st[kMaxSize];
-  int sp = 0;
-  st[sp++] = {rank, size, 0};
-  while (sp > 0) {
-    Item it = st[--sp];
-    if (it.s == 1) {
-      rpn[it.pos] = (uint8_t)it.r;
-      continue;
-    }
-    uint64_t n = it.r;
-    int op = 0;
-    while (n >= tv->T[it.s][op]) n -= tv->T[it.s][op++];
-    const int cs = it.s - 1;
-    rpn[it.pos + cs] = (uint8_t)(kTokOp + op);
-    if (is_unary(op)) {
-      st[sp++] = {n, cs, it.pos};
-      continue;
-    }
-    int j = 1;
-    for (;; ++j) {
-      const uint64_t blk = tv->T[j][8] * tv->T[cs - j][8];
-      if (n < blk) break;
-      n -= blk;
-    }
-    const uint64_t rs = tv->T[cs - j][8];
-    const uint64_t a = n / rs;
-    st[sp++] = {a, j, it.pos};
-    st[sp++] = {n - a * rs, cs - j, it.pos + j};
-  }
+  unrank_rpn(tv, size, rank, rpn);
   for (int s = 0; s < n_samples; ++s) {
     word vs[kMaxSize];
     int top = -1;
@@ -808,6 +842,39 @@ __device__ bool tree_matches(const TableView *__restrict__ tv, const word *s_lea
     if (vs[0] != s_out[s]) return false;
   }
   return true;
+}
+
+// ---------------------------------------------------------------------------
+// Second-sample filter of recorded window hits: a hit X = inst(A_a, B_b) of
+// the split {j, m} at size sb is kept when it also meets the second sample
+// with the same chain offset (even chain: t - x equal on both samples, odd
+// chain: t + x); the trees' values there are evaluated from their ranks.
+// Kept hits are appended to `out` (order is irrelevant: hits only lower
+// per-size minimum ranks).  tvs[h.inst >> 8] is the hit's instance.
+// ---------------------------------------------------------------------------
+template <typename word>
+__global__ void __launch_bounds__(256)
+k_hit_filter(const TableView *__restrict__ tvs, const HitRec *__restrict__ in, uint32_t n,
+             int max_size, HitRec *__restrict__ out, uint32_t *__restrict__ out_count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const HitRec h = in[i];
+  const TableView *tv = tvs + (h.inst >> 8);
+  bool keep = true;
+  const uint32_t inst = h.inst & 0xff;
+  if (tv->has2 && inst < 7) {
+    const int sb = (int)(h.pair >> 16), j = (int)(h.pair & 0xffff), m = sb - 1 - j;
+    const word a2 = eval_tree<word>(tv, j, h.a, tv->leaf2);
+    const word b2 = eval_tree<word>(tv, m, h.b, tv->leaf2);
+    const word c = (word)h.v, c2 = inst_value<word>((int)inst, a2, b2);
+    const word t1 = (word)tv->t1, t2 = (word)tv->t2;
+    const word U = word(max_size - sb);
+    const word span = word(2) * U;
+    const bool even = word(c - (t1 - U)) <= span && word(t1 - c) == word(t2 - c2);
+    const bool odd = word(c - (word(0) - t1 - U)) <= span && word(t1 + c) == word(t2 + c2);
+    keep = even || odd;
+  }
+  if (keep) out[atomicAdd(out_count, 1u)] = h;
 }
 
 // ---------------------------------------------------------------------------
@@ -1225,6 +1292,20 @@ hipError_t launch_rtid_scan(int width, const TableView *d_tv, const uint64_t *d_
   else
     hipLaunchKernelGGL(k_rtid_scan<uint64_t>, dim3(grid), dim3(256), 0, stream, d_tv,
                        d_leafvals, d_outputs, n_samples, size, offset, N, g0, g1, t0, d_best);
+  return hipGetLastError();
+}
+
+hipError_t launch_hit_filter(int width, const TableView *d_tvs, const HitRec *d_in, uint32_t n,
+                             int max_size, HitRec *d_out, uint32_t *d_out_count,
+                             hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  const dim3 g((n + 255) / 256);
+  if (width == 32)
+    hipLaunchKernelGGL(k_hit_filter<uint32_t>, g, dim3(256), 0, stream, d_tvs, d_in, n,
+                       max_size, d_out, d_out_count);
+  else
+    hipLaunchKernelGGL(k_hit_filter<uint64_t>, g, dim3(256), 0, stream, d_tvs, d_in, n,
+                       max_size, d_out, d_out_count);
   return hipGetLastError();
 }
 
