@@ -13,6 +13,7 @@ the full result record.  There is no CPU fallback: without the built library
 or without a gfx950 device every entry point raises.
 """
 import ctypes
+from collections.abc import Sequence
 import os
 
 import numpy as np
@@ -60,6 +61,32 @@ class Result(ctypes.Structure):
                 "kernel_seconds": self.kernel_seconds,
                 "kernel_launches": self.kernel_launches,
                 "candidates": int(self.candidates), "size_reached": self.size_reached}
+
+
+def _result_array(n):
+    """n result structs in an uninitialised buffer (the library writes every
+    field of every result)."""
+    return (Result * n).from_buffer(np.empty(n * ctypes.sizeof(Result), np.uint8))
+
+
+class BatchResults(Sequence):
+    """Results of a batched call, one per instance in input order: the C
+    result structs, each turned into a dict (Result.as_dict) when read."""
+
+    def __init__(self, res):
+        self._res = res
+        self._d = [None] * len(res)
+
+    def __len__(self):
+        return len(self._res)
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self[k] for k in range(*i.indices(len(self)))]
+        d = self._d[i]
+        if d is None:
+            d = self._d[i] = self._res[i].as_dict()
+        return d
 
 
 class VfbStats(ctypes.Structure):
@@ -191,17 +218,14 @@ class Context:
         return d
 
     def batch(self, insts, **kw):
+        """Solves the instances in one batched call; returns a BatchResults
+        sequence of result dicts (mba_batch_solve)."""
         n = len(insts)
-        arr = (Instance * n)()
-        keep = []
-        for i, x in enumerate(insts):
-            ci, k = x.to_c()
-            arr[i] = ci
-            keep.append(k)
+        arr = pack_instances(insts)
         o = make_opts(**kw)
-        res = (Result * n)()
+        res = _result_array(n)
         self._check(lib().mba_batch_solve(self._h, arr, n, ctypes.byref(o), res))
-        return [r.as_dict() for r in res]
+        return BatchResults(res)
 
 
 def _fptr(a):
@@ -293,17 +317,12 @@ class Mlp:
 
     def batch_solve(self, insts, **kw):
         n = len(insts)
-        arr = (Instance * n)()
-        keep = []
-        for i, x in enumerate(insts):
-            ci, k = x.to_c()
-            arr[i] = ci
-            keep.append(k)
+        arr = pack_instances(insts)
         o = make_opts(**kw)
-        res = (Result * n)()
+        res = _result_array(n)
         self.ctx._check(lib().mba_learned_batch_solve(self.ctx._h, self._h, arr, n,
                                                       ctypes.byref(o), res))
-        return [r.as_dict() for r in res]
+        return BatchResults(res)
 
 
 def make_opts(min_size=1, max_size=9, compat=False, exhaustive=False, budget_s=0.0,
@@ -358,7 +377,13 @@ class MbaInstance:
         c = getattr(self, "_c", None)
         if c is None:
             c = self._c = self._make_c()
+            self._cb = bytes(c[0])
         return c
+
+    def to_c_bytes(self):
+        """The bytes of the C view (its arrays stay referenced by to_c)."""
+        self.to_c()
+        return self._cb
 
     def _make_c(self):
         ci = Instance()
@@ -374,6 +399,11 @@ class MbaInstance:
         ci.outputs = out.ctypes.data_as(P)
         ci.consts = cst.ctypes.data_as(P)
         return ci, (inp, out, cst)
+
+
+def pack_instances(insts):
+    """ctypes array of the instances' C views, for the batched calls."""
+    return (Instance * len(insts)).from_buffer_copy(b"".join(x.to_c_bytes() for x in insts))
 
 
 _default_ctx = None

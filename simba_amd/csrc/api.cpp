@@ -51,7 +51,8 @@ int mba_ctx_create(int device, mba_ctx **out) {
   c->c.hbm_bytes = prop.totalGlobalMem;
   if (hipSetDevice(device) != hipSuccess ||
       hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&c->c.ev0) != hipSuccess || hipEventCreate(&c->c.ev1) != hipSuccess) {
+      hipEventCreate(&c->c.ev0) != hipSuccess || hipEventCreate(&c->c.ev1) != hipSuccess ||
+      hipHostMalloc(&c->c.pinned, Ctx::kPinnedBytes, hipHostMallocDefault) != hipSuccess) {
     delete c;
     return MBA_ERR_HIP;
   }
@@ -63,7 +64,7 @@ void mba_ctx_destroy(mba_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->c.device);
   (void)hipStreamSynchronize(c->c.stream);
-  DevBuf *bufs[] = {&c->c.tables, &c->c.tview, &c->c.hits, &c->c.hit_count,
+  DevBuf *bufs[] = {&c->c.tables, &c->c.tview, &c->c.hits, &c->c.surv, &c->c.hit_count,
                     &c->c.leafvals, &c->c.outputs, &c->c.vsizes, &c->c.vranks,
                     &c->c.vok, &c->c.vfb_rows, &c->c.vfb_left, &c->c.vfb_right,
                     &c->c.vfb_keys, &c->c.vfb_idx, &c->c.vfb_flags, &c->c.vfb_misc,
@@ -71,6 +72,7 @@ void mba_ctx_destroy(mba_ctx *c) {
                     &c->c.b_queue, &c->c.b_results, &c->c.b_misc, &c->c.mlp_w,
                     &c->c.mlp_misc};
   for (DevBuf *b : bufs) b->release();
+  if (c->c.pinned) (void)hipHostFree(c->c.pinned);
   (void)hipEventDestroy(c->c.ev0);
   (void)hipEventDestroy(c->c.ev1);
   (void)hipStreamDestroy(c->c.stream);

@@ -5,12 +5,14 @@
 // table builds, fused pair checks, leaf scans) in a device work queue; one
 // persistent kernel per step drains it (workgroups pull tiles from an atomic
 // counter), so small and large instances share the GPU.  The per-step launch
-// sequence -- job upload, counter reset, persistent kernel, hit-count read
-// back -- is captured once into a hipGraph and replayed per step.  Each
-// instance's answer is the one mba_simba_solve returns on it: the same
-// tables, windows, hit resolution and verification, per instance.
+// sequence -- step header upload, counter reset, persistent kernel, hit-count
+// read back -- is captured once into a hipGraph and replayed per step (the
+// step's jobs are copied in just before).  Window hits are resolved on the
+// device (k_hit_resolve: every sample, chain DP, per-slot minimum rank) and
+// the lowered slots' trees verified.  Each instance's answer is the one
+// mba_simba_solve returns on it: the same tables, windows and candidate
+// order per instance.
 #include <algorithm>
-#include <thread>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -20,9 +22,6 @@
 #include "simba_host.h"
 
 namespace mba {
-
-// host threads that resolve a step's window hits when there are many
-constexpr size_t kResolveThreads = 8;
 
 namespace {
 
@@ -101,7 +100,10 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     }
     st.fs = pick_filter_sample(lv, outv, n, L, W);
     st.t0 = outv[st.fs];
-    st.Kt = choose_kt(*st.T, std::max(max_size, 1), W, per_budget);
+    // V[max_size - 2] is only ever the B side of leaves: formed per tile
+    st.Kt = std::max(1, std::min(choose_kt(*st.T, std::max(max_size, 1), W, per_budget),
+                                 max_size - 3) -
+                            std::max(0, o.table_delta));
     TableView &tv = tvs[i];
     std::memset(&tv, 0, sizeof tv);
     uint64_t e = 0;
@@ -112,7 +114,6 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     std::memcpy(tv.T, st.T->t, sizeof tv.T);
     tv.Kt = st.Kt;
     tv.L = L;
-    set_second_sample(tv, lv, outv, L, st.fs, pick_filter_sample(lv, outv, n, L, W, st.fs));
     tv.base = (const void *)(uintptr_t)(pool_elems * wb);  // patched to a pointer below
     pool_elems += e;
     idat[i].leaf_off = leafvals.size();
@@ -173,6 +174,17 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   uint32_t *d_hc = ctx.hit_count.as<uint32_t>();
   HitRec *d_hits = ctx.hits.as<HitRec>();
   const TableView *d_tvs = ctx.b_inst.as<TableView>();
+  // device hit resolution: per-(instance, size) minimum rank, slot flags and
+  // the list of slots a resolve launch lowered
+  const size_t nslots = count * (kMaxSize + 1);
+  MBA_HIP(ctx.b_misc.ensure(nslots * 16 + 256));
+  unsigned long long *d_found = (unsigned long long *)(ctx.b_misc.as<char>() + 256);
+  uint32_t *d_flags = (uint32_t *)(d_found + nslots);
+  uint32_t *d_touched = d_flags + nslots;
+  uint32_t *d_ntouched = ctx.b_misc.as<uint32_t>();
+  uint32_t *h_nt = h_hc + 4;
+  MBA_HIP(hipMemsetAsync(d_found, 0xff, nslots * 8, stq));
+  MBA_HIP(hipMemsetAsync(d_flags, 0, nslots * 4, stq));
   const int mult = o.wg_per_cu > 0 ? o.wg_per_cu : 8;  // workgroups per CU
   const int grid = ctx.cu_count * mult;
 
@@ -198,6 +210,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
 
   std::vector<HitRec> hrec;
   std::vector<VerifyCand> vc;
+  std::vector<uint8_t> okv;
   double ktime = 0.0;
   uint32_t klaunch = 0;
   bool budget_hit = false;
@@ -334,6 +347,9 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
         ktime += ms * 1e-3;
         ++klaunch;
         t_gpu += seconds_since(t_g0);
+        if (o.verbose)
+          std::fprintf(stderr, "batch: size %d round %d: %zu jobs, %llu tiles, %.3f ms\n", s,
+                       round, nj, (unsigned long long)tiles, ms);
         const auto t_r0 = Clock::now();
         const uint32_t nh = *h_hc;
         if (nh > kBatchHitCap) {
@@ -341,72 +357,49 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
           return MBA_ERR_LIMIT;
         }
         n_hits += nh;
-        uint32_t nk = nh;
-        const HitRec *src = d_hits;
-        if (nh > 0) {  // keep the hits that also meet each instance's second sample
-          uint32_t *d_fc = (uint32_t *)(ctx.hit_count.as<char>() + 128);
-          MBA_HIP(ctx.hits2.ensure(sizeof(HitRec) * nh));
-          MBA_HIP(hipMemsetAsync(d_fc, 0, 4, stq));
-          MBA_HIP(launch_hit_filter(W, d_tvs, d_hits, nh, max_size, ctx.hits2.as<HitRec>(),
-                                    d_fc, stq));
-          MBA_HIP(hipMemcpyAsync(&nk, d_fc, 4, hipMemcpyDeviceToHost, stq));
-          MBA_HIP(hipStreamSynchronize(stq));
-          src = ctx.hits2.as<HitRec>();
-        }
-        n_kept += nk;
-        if (nk > 0) {
-          const uint32_t nh = nk;
+        if (nh > 0 && !hits_of.empty()) {  // verbose: hits per instance
           hrec.resize(nh);
-          MBA_HIP(hipMemcpy(hrec.data(), src, sizeof(HitRec) * nh, hipMemcpyDeviceToHost));
-          vc.clear();
-          if (!hits_of.empty())
-            for (const HitRec &h0 : hrec) hits_of[h0.inst >> 8]++;
-          // resolve the hits to candidate trees, in host threads when many
-          auto resolve = [&](size_t lo, size_t hi, std::vector<VerifyCand> &out) {
-            std::vector<Candidate> cs;
-            for (size_t k = lo; k < hi; ++k) {
-              HitRec h = hrec[k];
-              const uint32_t i = h.inst >> 8;
-              h.inst &= 0xff;
-              const InstState &st = S[i];
-              const int sb = (int)(h.pair >> 16), j = (int)(h.pair & 0xffff);
-              const uint64_t rX = base_rank(*st.T, sb, j, sb - 1 - j, h);
-              cs.clear();
-              resolve_hit(*st.T, W, st.t0, sb, rX, h.v & M, min_size, max_size, cs);
-              for (const Candidate &c : cs) out.push_back({i, (uint32_t)c.size, c.rank});
-            }
-          };
-          const size_t nthr = nh >= 16384 ? std::min<size_t>(kResolveThreads, nh / 8192) : 1;
-          if (nthr <= 1) {
-            resolve(0, nh, vc);
-          } else {
-            std::vector<std::vector<VerifyCand>> parts(nthr);
-            std::vector<std::thread> th;
-            for (size_t t = 0; t < nthr; ++t)
-              th.emplace_back(resolve, nh * t / nthr, nh * (t + 1) / nthr, std::ref(parts[t]));
-            for (auto &x : th) x.join();
-            for (auto &p : parts) vc.insert(vc.end(), p.begin(), p.end());
-          }
-          n_cands += vc.size();
-          if (!vc.empty()) {
-            const uint32_t ncand = (uint32_t)vc.size();
-            MBA_HIP(ctx.vranks.ensure(sizeof(VerifyCand) * ncand));
-            MBA_HIP(ctx.vok.ensure(ncand));
-            MBA_HIP(hipMemcpyAsync(ctx.vranks.p, vc.data(), sizeof(VerifyCand) * ncand,
-                                   hipMemcpyHostToDevice, stq));
+          MBA_HIP(hipMemcpy(hrec.data(), d_hits, sizeof(HitRec) * nh, hipMemcpyDeviceToHost));
+          for (const HitRec &h0 : hrec) hits_of[h0.inst >> 8]++;
+        }
+        if (nh > 0) {
+          // resolve every hit on the device (screen on two samples, check on
+          // every sample, per-slot minimum), then verify the lowered slots'
+          // trees
+          MBA_HIP(ctx.surv.ensure(sizeof(SurvRec) * 2 * (size_t)nh));
+          MBA_HIP(hipMemsetAsync(d_ntouched, 0, 8, stq));  // touched and candidate counts
+          MBA_HIP(launch_hit_resolve(W, d_tvs, ctx.leafvals.as<uint64_t>(),
+                                     ctx.outputs.as<uint64_t>(),
+                                     ctx.b_results.as<BatchInstData>(), d_hits, d_hc, nh,
+                                     (nh + 63) / 64, std::min<uint32_t>(2 * nh, 8192),
+                                     min_size, max_size, d_found, ctx.surv.as<SurvRec>(),
+                                     d_ntouched + 1, d_flags, d_touched, d_ntouched, stq));
+          MBA_HIP(hipMemcpyAsync(h_nt, d_ntouched, 4, hipMemcpyDeviceToHost, stq));
+          MBA_HIP(hipStreamSynchronize(stq));
+          const uint32_t nt = *h_nt;
+          n_kept += nt;
+          if (nt > 0) {
+            MBA_HIP(ctx.vranks.ensure(sizeof(VerifyCand) * nt));
+            MBA_HIP(ctx.vok.ensure(nt));
+            MBA_HIP(launch_found_gather(d_touched, d_ntouched, nt, d_found, d_flags,
+                                        ctx.vranks.as<VerifyCand>(), stq));
             MBA_HIP(launch_verify_batch(W, d_tvs, ctx.leafvals.as<uint64_t>(),
                                         ctx.outputs.as<uint64_t>(),
                                         ctx.b_results.as<BatchInstData>(),
-                                        ctx.vranks.as<VerifyCand>(), ncand,
+                                        ctx.vranks.as<VerifyCand>(), nt,
                                         ctx.vok.as<uint8_t>(), stq));
-            std::vector<uint8_t> okv(ncand);
-            MBA_HIP(hipMemcpyAsync(okv.data(), ctx.vok.p, ncand, hipMemcpyDeviceToHost, stq));
+            vc.resize(nt);
+            okv.resize(nt);
+            MBA_HIP(hipMemcpyAsync(vc.data(), ctx.vranks.p, sizeof(VerifyCand) * nt,
+                                   hipMemcpyDeviceToHost, stq));
+            MBA_HIP(hipMemcpyAsync(okv.data(), ctx.vok.p, nt, hipMemcpyDeviceToHost, stq));
             MBA_HIP(hipStreamSynchronize(stq));
-            for (uint32_t c = 0; c < ncand; ++c)
-              if (okv[c]) {
-                auto &f = S[vc[c].inst].found[vc[c].size];
-                f = std::min(f, vc[c].rank);
-              }
+            for (uint32_t c = 0; c < nt; ++c) {
+              if (!okv[c]) continue;
+              ++n_cands;
+              auto &f = S[vc[c].inst].found[vc[c].size];
+              f = std::min(f, vc[c].rank);
+            }
           }
         }
         t_resolve += seconds_since(t_r0);
@@ -436,8 +429,8 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   const double total = seconds_since(t_start);
   if (o.verbose)
     std::fprintf(stderr,
-                 "batch: jobs %.3f ms, device %.3f ms, hits %.3f ms (%llu hits, %llu on "
-                 "both samples, %llu candidates), total %.3f ms\n",
+                 "batch: jobs %.3f ms, device %.3f ms, hits %.3f ms (%llu hits, %llu lowered "
+                 "slots, %llu verified), total %.3f ms\n",
                  1e3 * t_build, 1e3 * t_gpu, 1e3 * t_resolve, (unsigned long long)n_hits,
                  (unsigned long long)n_kept, (unsigned long long)n_cands, 1e3 * total);
   if (o.verbose && count > 0) {

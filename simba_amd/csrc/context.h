@@ -43,12 +43,15 @@ struct Ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string last_error;
   // SIMBA
-  DevBuf tables, tview, hits, hits2, hit_count, leafvals, outputs, vsizes, vranks, vok;
+  DevBuf tables, tview, hits, surv, hit_count, leafvals, outputs, vsizes, vranks, vok;
   // VFB
   DevBuf vfb_rows, vfb_left, vfb_right, vfb_keys, vfb_idx, vfb_flags, vfb_misc,
       vfb_counts, vfb_target;
   // batched / learned
   DevBuf b_inst, b_data, b_queue, b_results, b_misc, mlp_w, mlp_misc;
+  // pinned host staging for the per-step read backs of the search drivers
+  void *pinned = nullptr;
+  static constexpr size_t kPinnedBytes = 4096;
 };
 
 using Clock = std::chrono::steady_clock;

@@ -18,6 +18,8 @@
 namespace mba {
 
 constexpr uint32_t kHitCap = 1u << 20;
+// value tables up to this many entries in all are built by one workgroup
+constexpr uint64_t kSmallTableValues = 1u << 16;
 constexpr uint64_t kDefaultTableBytes = 40ull << 30;
 // Work per launch, in A x B value pairs, so that budgets are honoured.
 constexpr uint64_t kPairsPerLaunch = 1ull << 36;
@@ -148,81 +150,6 @@ static void b_blocks(const CountTable &T, int Kt, const uint64_t *off, int m, ui
     }
     r += cnt;
   }
-}
-
-static inline int64_t as_signed(uint64_t x, int W) {
-  return W == 64 ? (int64_t)x : (int64_t)(int32_t)(uint32_t)x;
-}
-
-// Rank at size sb of the binary-rooted tree X = inst(A[a], B[b]) of the
-// split {j, m} (see HitRec), or of leaf `a` when inst == 7.
-uint64_t base_rank(const CountTable &T, int sb, int j, int m, const HitRec &h) {
-  static const int kInstOp[7] = {AND, OR, XOR, ADD, MUL, SUB, SUB};
-  if (h.inst == 7) return h.a;
-  const int op = kInstOp[h.inst];
-  if (h.inst == 6)  // (B - A): '-' with left child of size m
-    return T.op_offset(sb, SUB) + T.split_offset(sb, m) + h.b * T.t[j][8] + h.a;
-  return T.op_offset(sb, op) + T.split_offset(sb, j) + h.a * T.t[m][8] + h.b;
-}
-
-// Turns one window hit into candidate trees.  X has size sb and value v on
-// sample 0.  A chain of u unary operators over X computes (-1)^u * x + k; the
-// target is met by the even map x + (t0 - v) or the odd map -x + (t0 + v).
-// For each map the smallest chain length whose size lies in [min, max] is
-// found, and among chains of that length the one with the smallest rank:
-// a unary node at size s adds op_offset(s, op) to the rank of its child
-// (nocache.cu:52-63), 0 for '~' and T[s][0..3] for neg.
-void resolve_hit(const CountTable &T, int W, uint64_t t0, int sb, uint64_t rX,
-                 uint64_t v, int min_size, int max_size, std::vector<Candidate> &out) {
-  const uint64_t M = wmask(W);
-  const int U = max_size - sb;
-  const int64_t ke = as_signed((t0 - v) & M, W);
-  const int64_t ko = as_signed((t0 + v) & M, W);
-  const bool want_e = ke >= -U && ke <= U, want_o = ko >= -U && ko <= U;
-  if (!want_e && !want_o) return;
-  // cost[k + off] = min rank increment of a chain of length i reaching k
-  // (fixed arrays: this runs once per window hit)
-  const int off = U + 2, width = 2 * U + 5;
-  const uint64_t INF = ~0ull;
-  uint64_t cost_a[2 * kMaxSize + 8], next_a[2 * kMaxSize + 8];
-  uint64_t *cost = cost_a, *next = next_a;
-  std::fill(cost, cost + width, INF);
-  cost[off] = 0;
-  bool done_e = !want_e, done_o = !want_o;
-  for (int i = 0; sb + i <= max_size && !(done_e && done_o); ++i) {
-    const int size = sb + i;
-    if (size >= min_size) {
-      if (!done_e && (i % 2 == 0) && cost[ke + off] != INF) {
-        out.push_back({size, rX + cost[ke + off]});
-        done_e = true;
-      }
-      if (!done_o && (i % 2 == 1) && cost[ko + off] != INF) {
-        out.push_back({size, rX + cost[ko + off]});
-        done_o = true;
-      }
-    }
-    if (size + 1 > max_size) break;
-    std::fill(next, next + width, INF);
-    const uint64_t neg_off = T.op_offset(size + 1, NEG);
-    for (int k = -U - 1; k <= U + 1; ++k) {
-      const uint64_t c = cost[k + off];
-      if (c == INF) continue;
-      const int kn = -k - 1, kg = -k;  // '~': -x-1 ; neg: -x
-      if (kn >= -U - 1 && kn <= U + 1) next[kn + off] = std::min(next[kn + off], c);
-      if (kg >= -U - 1 && kg <= U + 1)
-        next[kg + off] = std::min(next[kg + off], c + neg_off);
-    }
-    std::swap(cost, next);
-  }
-}
-
-void set_second_sample(TableView &tv, const std::vector<uint64_t> &lv,
-                       const std::vector<uint64_t> &outv, int L, int fs, int s2) {
-  tv.t1 = outv[fs];
-  tv.has2 = s2 >= 0;
-  if (!tv.has2) return;
-  for (int l = 0; l < L; ++l) tv.leaf2[l] = lv[(size_t)s2 * L + l];
-  tv.t2 = outv[s2];
 }
 
 int pick_filter_sample(const std::vector<uint64_t> &lv, const std::vector<uint64_t> &outv,
@@ -391,7 +318,11 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
 
   // ---- value tables V[1..Kt] -------------------------------------------
   const uint64_t budget = o.table_bytes ? o.table_bytes : kDefaultTableBytes;
-  const int Kt = std::max(1, choose_kt(T, max_size, W, budget) - std::max(0, o.table_delta));
+  // V[max_size - 2] is only ever the B side of leaves (one read per value),
+  // cheaper to form per tile than to build; RTid reads every size directly
+  const int kt_cap = rtid ? max_size : max_size - 3;
+  const int Kt = std::max(1, std::min(choose_kt(T, max_size, W, budget), kt_cap) -
+                                 std::max(0, o.table_delta));
   TableView tv;
   std::memset(&tv, 0, sizeof tv);
   uint64_t elems = 0;
@@ -402,7 +333,6 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   std::memcpy(tv.T, T.t, sizeof tv.T);
   tv.Kt = Kt;
   tv.L = L;
-  set_second_sample(tv, lv, outv, L, fs, pick_filter_sample(lv, outv, n, L, W, fs));
   MBA_HIP(ctx.tables.ensure(std::max<size_t>(elems * wb, 256)));
   tv.base = ctx.tables.p;
   MBA_HIP(ctx.tview.ensure(sizeof(TableView)));
@@ -427,7 +357,12 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     else
       MBA_HIP(hipMemcpyAsync(ctx.tables.p, v1w.data(), 4 * L, hipMemcpyHostToDevice, st));
   }
-  for (int m = 2; m <= Kt; ++m)
+  // the small sizes in one workgroup (one launch), the rest per block
+  int m_small = 1;
+  for (uint64_t acc = 0; m_small + 1 <= Kt && acc + T.t[m_small + 1][8] <= kSmallTableValues;)
+    acc += T.t[++m_small][8];
+  if (m_small >= 2) MBA_HIP(launch_table_small(W, d_tv, 2, m_small, st));
+  for (int m = m_small + 1; m <= Kt; ++m)
     MBA_HIP(launch_table_build(W, ctx.tables.p, tv.off, T, m, st));
 
   if (rtid) {
@@ -487,8 +422,6 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
 
   // ---- size sweep --------------------------------------------------------
   std::vector<uint64_t> found(kMaxSize + 2, ~0ull);
-  std::vector<HitRec> hrec;
-  std::vector<Candidate> cands;
   uint32_t *d_hc = ctx.hit_count.as<uint32_t>();
   HitRec *d_hits = ctx.hits.as<HitRec>();
   // 32 workgroups per CU by default: more than stay resident, so tiles are
@@ -500,56 +433,59 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   int reached = 0;
   int status = MBA_NOT_FOUND;
 
-  // Resolve the hits of one step, verify the candidates on all samples.
-  auto process_hits = [&](int sb_step) -> int {
-    uint32_t cnt = 0;
-    MBA_HIP(hipMemcpyAsync(&cnt, d_hc, 4, hipMemcpyDeviceToHost, st));
+  // Device hit resolution (k_hit_resolve): the per-size minimum chain rank
+  // of the hits that meet every sample, flags and the list of lowered slots;
+  // the instance's sample layout for it at byte 128 of the header.
+  constexpr size_t kSlots = kMaxSize + 1;
+  MBA_HIP(ctx.b_misc.ensure(kSlots * 16 + 256));
+  uint32_t *d_nt = ctx.b_misc.as<uint32_t>();
+  BatchInstData *d_idat = (BatchInstData *)(ctx.b_misc.as<char>() + 128);
+  unsigned long long *d_found = (unsigned long long *)(ctx.b_misc.as<char>() + 256);
+  uint32_t *d_flags = (uint32_t *)(d_found + kSlots);
+  uint32_t *d_touched = d_flags + kSlots;
+  const BatchInstData idat1{0, 0, (uint32_t)n, (uint32_t)fs};
+  MBA_HIP(hipMemcpyAsync(d_idat, &idat1, sizeof idat1, hipMemcpyHostToDevice, st));
+  MBA_HIP(hipMemsetAsync(d_found, 0xff, kSlots * 8, st));
+  MBA_HIP(hipMemsetAsync(d_flags, 0, kSlots * 4, st));
+  MBA_HIP(ctx.surv.ensure(sizeof(SurvRec) * 2 * (size_t)kHitCap));
+  MBA_HIP(ctx.vranks.ensure(sizeof(VerifyCand) * kSlots));
+  MBA_HIP(ctx.vok.ensure(kSlots));
+  // read back area: hit count, lowered slot count, candidates, verdicts
+  static_assert(16 + kSlots * (sizeof(VerifyCand) + 1) <= Ctx::kPinnedBytes, "pinned area");
+  uint32_t *h_counts = (uint32_t *)ctx.pinned;
+  VerifyCand *h_vc = (VerifyCand *)((char *)ctx.pinned + 16);
+  uint8_t *h_ok = (uint8_t *)(h_vc + kSlots);
+
+  // Resolve the hits of one step on the device and verify the lowered slots'
+  // trees; the kernels read the counts on the device, so one sync suffices.
+  auto process_hits = [&]() -> int {
+    MBA_HIP(hipMemsetAsync(d_nt, 0, 8, st));  // touched and candidate counts
+    MBA_HIP(launch_hit_resolve(W, d_tv, ctx.leafvals.as<uint64_t>(), ctx.outputs.as<uint64_t>(),
+                               d_idat, d_hits, d_hc, kHitCap, 512, 512, min_size, max_size,
+                               d_found, ctx.surv.as<SurvRec>(), d_nt + 1, d_flags, d_touched,
+                               d_nt, st));
+    MBA_HIP(launch_found_gather(d_touched, d_nt, kSlots, d_found, d_flags,
+                                ctx.vranks.as<VerifyCand>(), st));
+    MBA_HIP(launch_verify_batch(W, d_tv, ctx.leafvals.as<uint64_t>(), ctx.outputs.as<uint64_t>(),
+                                d_idat, ctx.vranks.as<VerifyCand>(), kSlots,
+                                ctx.vok.as<uint8_t>(), st, d_nt));
+    MBA_HIP(hipMemcpyAsync(h_counts, d_hc, 4, hipMemcpyDeviceToHost, st));
+    MBA_HIP(hipMemcpyAsync(h_counts + 1, d_nt, 4, hipMemcpyDeviceToHost, st));
+    MBA_HIP(hipMemcpyAsync(h_vc, ctx.vranks.p, sizeof(VerifyCand) * kSlots,
+                           hipMemcpyDeviceToHost, st));
+    MBA_HIP(hipMemcpyAsync(h_ok, ctx.vok.p, kSlots, hipMemcpyDeviceToHost, st));
     MBA_HIP(hipStreamSynchronize(st));
-    if (cnt == 0) return 0;
-    if (cnt > kHitCap) {
+    if (h_counts[0] > kHitCap) {
       ctx.last_error = "sample-0 hit buffer overflow";
       return MBA_ERR_LIMIT;
     }
-    const HitRec *src = d_hits;
-    if (tv.has2) {  // keep the hits that also meet the second sample
-      uint32_t *d_fc = (uint32_t *)(ctx.hit_count.as<char>() + 128);
-      MBA_HIP(ctx.hits2.ensure(sizeof(HitRec) * cnt));
-      MBA_HIP(hipMemsetAsync(d_fc, 0, 4, st));
-      MBA_HIP(launch_hit_filter(W, d_tv, d_hits, cnt, max_size, ctx.hits2.as<HitRec>(), d_fc,
-                                st));
-      MBA_HIP(hipMemcpyAsync(&cnt, d_fc, 4, hipMemcpyDeviceToHost, st));
-      MBA_HIP(hipStreamSynchronize(st));
-      if (cnt == 0) return 0;
-      src = ctx.hits2.as<HitRec>();
+    const uint32_t nt = h_counts[1];
+    if (nt > kSlots) {
+      ctx.last_error = "hit resolution: slot list overflow";
+      return MBA_ERR_LIMIT;
     }
-    hrec.resize(cnt);
-    MBA_HIP(hipMemcpy(hrec.data(), src, sizeof(HitRec) * cnt, hipMemcpyDeviceToHost));
-    cands.clear();
-    for (const HitRec &h : hrec) {
-      const int sb = (int)(h.pair >> 16), j = (int)(h.pair & 0xffff);
-      const int m = sb - 1 - j;
-      const uint64_t rX = base_rank(T, sb, j, m, h);
-      resolve_hit(T, W, t0, sb, rX, h.v & M, min_size, max_size, cands);
-    }
-    (void)sb_step;
-    if (cands.empty()) return 0;
-    const uint32_t nc_ = (uint32_t)cands.size();
-    MBA_HIP(ctx.vsizes.ensure(4ull * nc_));
-    MBA_HIP(ctx.vranks.ensure(8ull * nc_));
-    MBA_HIP(ctx.vok.ensure(nc_));
-    std::vector<uint32_t> sz(nc_);
-    std::vector<uint64_t> rk(nc_);
-    for (uint32_t i = 0; i < nc_; ++i) sz[i] = (uint32_t)cands[i].size, rk[i] = cands[i].rank;
-    MBA_HIP(hipMemcpyAsync(ctx.vsizes.p, sz.data(), 4ull * nc_, hipMemcpyHostToDevice, st));
-    MBA_HIP(hipMemcpyAsync(ctx.vranks.p, rk.data(), 8ull * nc_, hipMemcpyHostToDevice, st));
-    MBA_HIP(launch_verify(W, d_tv, ctx.leafvals.as<uint64_t>(), ctx.outputs.as<uint64_t>(),
-                          n, ctx.vsizes.as<uint32_t>(), ctx.vranks.as<uint64_t>(), nc_,
-                          ctx.vok.as<uint8_t>(), st));
-    std::vector<uint8_t> okv(nc_);
-    MBA_HIP(hipMemcpyAsync(okv.data(), ctx.vok.p, nc_, hipMemcpyDeviceToHost, st));
-    MBA_HIP(hipStreamSynchronize(st));
-    for (uint32_t i = 0; i < nc_; ++i)
-      if (okv[i]) found[sz[i]] = std::min(found[sz[i]], rk[i]);
+    for (uint32_t i = 0; i < nt; ++i)
+      if (h_ok[i]) found[h_vc[i].size] = std::min(found[h_vc[i].size], h_vc[i].rank);
     return 0;
   };
 
@@ -562,7 +498,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   // leaf (sizes 2..max).
   MBA_HIP(hipMemsetAsync(d_hc, 0, 4, st));
   MBA_HIP(launch_leaf_scan(W, d_tv, L, targets_for(t0, 1, max_size, W), d_hits, d_hc, kHitCap, st));
-  if (int rc = process_hits(1)) return rc;
+  if (int rc = process_hits()) return rc;
   for (int s = 1; s <= std::min(2, max_size); ++s) {
     reached = s;
     if (s >= min_size && found[s] != ~0ull) {
@@ -629,7 +565,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
           return -1;
         }
         if (cnt > kHitCap / 2) {
-          if (process_hits(s) != 0) return -1;
+          if (process_hits() != 0) return -1;
           if (hipMemsetAsync(d_hc, 0, 4, st) != hipSuccess) return -1;
         }
       }
@@ -697,7 +633,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
       // once the answer is settled (an exhaustive sweep going on to max_size)
       // the windows' hits no longer matter: their count may exceed the buffer
       if (!stopped && answer < 0) {
-        if (int rc = process_hits(s)) return rc;
+        if (int rc = process_hits()) return rc;
       }
     } else {
       // one (op, split) block at a time, in rank order (plain) or in the
@@ -717,7 +653,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
           stopped = true;
           break;
         }
-        if (int rc2 = process_hits(s)) return rc2;
+        if (int rc2 = process_hits()) return rc2;
         MBA_HIP(hipMemsetAsync(d_hc, 0, 4, st));
         done[bi] = 1;
         if (found[s] == ~0ull || s < min_size) continue;

@@ -9,26 +9,10 @@ namespace mba {
 
 inline uint64_t wmask(int W) { return W == 64 ? ~0ull : 0xffffffffull; }
 
-struct Candidate {
-  int size;
-  uint64_t rank;
-};
-
-// Rank, at size sb, of the binary-rooted tree recorded by a pair hit (or of
-// the leaf of a leaf hit).
-uint64_t base_rank(const CountTable &T, int sb, int j, int m, const HitRec &h);
-// Candidate trees (chains of ~/neg over the hit tree) that can meet the
-// target; appended to `out`.
-void resolve_hit(const CountTable &T, int W, uint64_t t0, int sb, uint64_t rX, uint64_t v,
-                 int min_size, int max_size, std::vector<Candidate> &out);
 // The sample whose values reject the most candidates (see simba_host.cpp);
 // `skip` excludes one sample (-1: none).  Returns -1 when none is left.
 int pick_filter_sample(const std::vector<uint64_t> &lv, const std::vector<uint64_t> &outv,
                        int n, int L, int W, int skip = -1);
-// Fills the filter-sample output and second-sample fields of a TableView
-// (sample s2 < 0: none).
-void set_second_sample(TableView &tv, const std::vector<uint64_t> &lv,
-                       const std::vector<uint64_t> &outv, int L, int fs, int s2);
 int choose_kt(const CountTable &T, int max_size, int W, uint64_t budget);
 // A binary (op, left size) block of size s and the pair launch covering it.
 struct BlockRef {

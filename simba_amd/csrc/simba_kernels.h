@@ -13,17 +13,11 @@ struct TableView {
   const void *base;               // uint32_t or uint64_t elements
   int Kt;                         // largest tabulated size
   int L;                          // number of leaves
-  // a second sample that a window hit must also meet before it is recorded
-  // (its leaf values and output); has2 = 0 when the instance has one sample
-  uint64_t leaf2[kMaxLeaves];
-  uint64_t t1, t2;                // outputs of the filter and second samples
-  int has2;
-  int pad;
 };
 
-// One sample-0 hit: the value c = op(A[a], B[b]) (op-instance `inst`) lies in
-// one of the two target windows.  Resolved to (size, rank) on the host and
-// verified on all samples by the verify kernel.
+// One filter-sample hit: the value c = op(A[a], B[b]) (op-instance `inst`)
+// lies in one of the two target windows.  Resolved to (size, rank) on the
+// device (k_hit_resolve) and verified on all samples by the verify kernel.
 struct HitRec {
   uint64_t a, b, v;
   uint32_t inst;  // 0..6: & | ^ + * (A-B) (B-A); 7: leaf/scan hit
@@ -130,15 +124,20 @@ hipError_t launch_batch_step(int width, const BatchStep *d_step, const BatchJob 
                              const TableView *d_tvs, unsigned long long *d_head,
                              HitRec *d_hits, uint32_t *d_hit_count, uint32_t hit_cap,
                              int grid, hipStream_t stream);
+// One wave per candidate; with d_n, only candidates [0, *d_n) (n_cand is then
+// the bound the grid covers).
 hipError_t launch_verify_batch(int width, const TableView *d_tvs, const uint64_t *d_leafvals,
                                const uint64_t *d_outputs, const BatchInstData *d_idat,
                                const VerifyCand *d_cand, uint32_t n_cand, uint8_t *d_ok,
-                               hipStream_t stream);
+                               hipStream_t stream, const uint32_t *d_n = nullptr);
 
 // All launches are asynchronous on `stream`.
 // V[m] from V[<m], one launch per (op, left size) block of size m.
 hipError_t launch_table_build(int width, void *d_base, const uint64_t *off, const CountTable &T,
                               int m, hipStream_t stream);
+// V[m_lo..m_hi] by one workgroup, size after size (small tables: one launch).
+hipError_t launch_table_small(int width, const TableView *d_tv, int m_lo, int m_hi,
+                              hipStream_t stream);
 // d_base/off: the table buffer and the host copy of TableView::off.
 hipError_t launch_pair_check(int width, const TableView *d_tv, const void *d_base,
                              const uint64_t *off, const PairLaunch &p,
@@ -149,11 +148,34 @@ hipError_t launch_leaf_scan(int width, const TableView *d_tv, int L,
                             const CheckTargets &tg, HitRec *d_hits,
                             uint32_t *d_hit_count, uint32_t hit_cap,
                             hipStream_t stream);
-// Keeps the hits in[0, n) that also meet each instance's second sample
-// (tvs[hit.inst >> 8]); kept hits are appended to d_out / *d_out_count.
-hipError_t launch_hit_filter(int width, const TableView *d_tvs, const HitRec *d_in, uint32_t n,
-                             int max_size, HitRec *d_out, uint32_t *d_out_count,
-                             hipStream_t stream);
+// Candidate of a window hit that holds on two samples, awaiting the check on
+// every sample: slot = instance * (kMaxSize + 1) + size, bit 31 = odd chain.
+struct SurvRec {
+  uint32_t hit, slot;
+  uint64_t rank;
+};
+
+// Device resolution of window hits (d_tvs[hit.inst >> 8] is a hit's
+// instance, d_idat its samples): each of the hits [0, min(*d_n_hits, cap))
+// whose chain candidate holds on every sample lowers
+// d_found[inst * (kMaxSize + 1) + size] to the candidate's rank; lowered slots
+// are appended once to d_touched (count in *d_n_touched; d_flags all zero
+// between launches).  d_surv holds up to 2 * cap candidates, *d_n_surv
+// (zeroed by the caller, like *d_n_touched) counts them.  Both kernels are
+// grid-stride over device-side counts: screen_blocks waves of 64 hits,
+// confirm_blocks waves of one candidate.
+hipError_t launch_hit_resolve(int width, const TableView *d_tvs, const uint64_t *d_leafvals,
+                              const uint64_t *d_outputs, const BatchInstData *d_idat,
+                              const HitRec *d_hits, const uint32_t *d_n_hits, uint32_t cap,
+                              uint32_t screen_blocks, uint32_t confirm_blocks, int min_size,
+                              int max_size, unsigned long long *d_found, SurvRec *d_surv,
+                              uint32_t *d_n_surv, uint32_t *d_flags, uint32_t *d_touched,
+                              uint32_t *d_n_touched, hipStream_t stream);
+// (instance, size, found rank) of the listed slots [0, *d_n) (at most max_n);
+// clears their flags.
+hipError_t launch_found_gather(const uint32_t *d_touched, const uint32_t *d_n, uint32_t max_n,
+                               const unsigned long long *d_found, uint32_t *d_flags,
+                               VerifyCand *d_out, hipStream_t stream);
 // SIMBA-RTid scan of threads [g0, g1) of the (size, op) block at `offset` with
 // N trees: *d_best = min(*d_best, g) over threads whose tree matches every
 // sample (t0 = the filter sample's output, the tables' sample).

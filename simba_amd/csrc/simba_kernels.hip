@@ -214,6 +214,21 @@ k_table_block(const word *__restrict__ VA, const word *__restrict__ VB, word *__
   }
 }
 
+// Value tables of the small sizes m_lo..m_hi by one workgroup, size after
+// size: each value from its root op and the smaller sizes' tables
+// (build_value); the barrier between sizes orders the workgroup's stores
+// before the next size reads them.
+template <typename word>
+__global__ void __launch_bounds__(1024)
+k_table_small(const TableView *__restrict__ tv, int m_lo, int m_hi) {
+  for (int m = m_lo; m <= m_hi; ++m) {
+    word *out = (word *)tv->base + tv->off[m];
+    const uint64_t n = tv->T[m][8];
+    for (uint64_t r = threadIdx.x; r < n; r += blockDim.x) out[r] = build_value<word>(tv, m, r);
+    __syncthreads();
+  }
+}
+
 template <typename word>
 __device__ __forceinline__ bool in_window(word c, word lo_e, word lo_o, word span) {
   return (word)(c - lo_e) <= span || (word)(c - lo_o) <= span;
@@ -845,36 +860,262 @@ __device__ bool tree_matches(const TableView *__restrict__ tv, const word *s_lea
 }
 
 // ---------------------------------------------------------------------------
-// Second-sample filter of recorded window hits: a hit X = inst(A_a, B_b) of
-// the split {j, m} at size sb is kept when it also meets the second sample
-// with the same chain offset (even chain: t - x equal on both samples, odd
-// chain: t + x); the trees' values there are evaluated from their ranks.
-// Kept hits are appended to `out` (order is irrelevant: hits only lower
-// per-size minimum ranks).  tvs[h.inst >> 8] is the hit's instance.
+// Window-hit resolution on the device.  A hit X of size sb with filter-sample
+// value x meets an even chain when t_s - x_s is the same offset k on every
+// sample s (a chain of unary nodes then computes x + k), an odd chain when
+// t_s + x_s is (-x + k); |k| <= U = max_size - sb.  For each parity, the
+// shortest chain whose size lies in [min_size, max_size] and, among those,
+// the smallest rank (chain_rank) give the hit's candidate (size, rank); a
+// candidate that holds on every sample lowers found[inst][size].
+// k_hit_screen (a lane per hit) keeps the candidates that also hold on a
+// second sample, k_hit_confirm (a wave per candidate) checks every sample.
 // ---------------------------------------------------------------------------
+constexpr int kVerifyDepth = kMaxSize / 2 + 2;  // max RPN stack depth
+
 template <typename word>
-__global__ void __launch_bounds__(256)
-k_hit_filter(const TableView *__restrict__ tvs, const HitRec *__restrict__ in, uint32_t n,
-             int max_size, HitRec *__restrict__ out, uint32_t *__restrict__ out_count) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const HitRec h = in[i];
-  const TableView *tv = tvs + (h.inst >> 8);
-  bool keep = true;
-  const uint32_t inst = h.inst & 0xff;
-  if (tv->has2 && inst < 7) {
-    const int sb = (int)(h.pair >> 16), j = (int)(h.pair & 0xffff), m = sb - 1 - j;
-    const word a2 = eval_tree<word>(tv, j, h.a, tv->leaf2);
-    const word b2 = eval_tree<word>(tv, m, h.b, tv->leaf2);
-    const word c = (word)h.v, c2 = inst_value<word>((int)inst, a2, b2);
-    const word t1 = (word)tv->t1, t2 = (word)tv->t2;
-    const word U = word(max_size - sb);
-    const word span = word(2) * U;
-    const bool even = word(c - (t1 - U)) <= span && word(t1 - c) == word(t2 - c2);
-    const bool odd = word(c - (word(0) - t1 - U)) <= span && word(t1 + c) == word(t2 + c2);
-    keep = even || odd;
+__device__ word eval_rpn_lds(const uint8_t *rpn, int size, const uint64_t *leaves,
+                             word (*stk)[64], int lane) {
+  int top = -1;
+  for (int i = 0; i < size; ++i) {
+    const uint8_t t = rpn[i];
+    if (t < kTokOp) {
+      stk[++top][lane] = (word)leaves[t];
+    } else {
+      const int op = t - kTokOp;
+      if (is_unary(op)) {
+        stk[top][lane] = apply_op<word>(op, stk[top][lane], word(0));
+      } else {
+        const word rv = stk[top--][lane];
+        stk[top][lane] = apply_op<word>(op, stk[top][lane], rv);
+      }
+    }
   }
-  if (keep) out[atomicAdd(out_count, 1u)] = h;
+  return stk[0][lane];
+}
+
+__device__ __forceinline__ uint64_t op_offset_dev(const TableView *tv, int s, int op) {
+  uint64_t r = 0;
+  for (int o = 0; o < op; ++o) r += tv->T[s][o];
+  return r;
+}
+
+__device__ __forceinline__ uint64_t split_offset_dev(const TableView *tv, int s, int j) {
+  uint64_t r = 0;
+  for (int jj = 1; jj < j; ++jj) r += tv->T[jj][8] * tv->T[s - 1 - jj][8];
+  return r;
+}
+
+template <typename word>
+__device__ __forceinline__ int64_t signed_of(word x) {
+  return sizeof(word) == 4 ? (int64_t)(int32_t)(uint32_t)x : (int64_t)(uint64_t)x;
+}
+
+// Rank of the binary-rooted tree of a pair hit at its size sb (order of
+// nocache.cu:52-79), or the leaf index of a leaf hit.
+__device__ __forceinline__ uint64_t hit_base_rank(const TableView *tv, const HitRec &h) {
+  const uint32_t inst = h.inst & 0xffu;
+  const int sb = (int)(h.pair >> 16), j = (int)(h.pair & 0xffffu), m = sb - 1 - j;
+  if (inst == 7) return h.a;
+  if (inst == 6)  // (B - A): '-' with left child of size m
+    return op_offset_dev(tv, sb, SUB) + split_offset_dev(tv, sb, m) + h.b * tv->T[j][8] + h.a;
+  const int op = inst == 0 ? AND : inst == 1 ? OR : inst == 2 ? XOR
+               : inst == 3 ? ADD : inst == 4 ? MUL : SUB;
+  return op_offset_dev(tv, sb, op) + split_offset_dev(tv, sb, j) + h.a * tv->T[m][8] + h.b;
+}
+
+// Shortest chain of unary nodes over a tree of size sb mapping x to x + k
+// (odd = false) or to -x + k (odd = true), of length u >= u_min with
+// sb + u <= max_size, and its smallest rank increment (a '~' node adds 0 to
+// its child's rank, a neg node at size s adds T[s][0..3], nocache.cu:52-63).
+// Number the nodes 1 (innermost) .. u: the chain computes (-1)^u x + c with
+// c = -sum over its '~' nodes i of (-1)^(u-i); all '~' gives c = 0 (u even)
+// or -1 (u odd), and each '~' replaced by neg moves c by +1 at the positions
+// i = u, u-2, ... and by -1 at i = u-1, u-3, ...  So |k| <= u/2 (even) and
+// -(u+1)/2 <= k <= (u-1)/2 (odd) are reachable, and the cheapest choice
+// replaces the innermost positions of the needed class (T[s] grows with s).
+// Returns u, or -1 when no chain fits.
+__device__ __forceinline__ int chain_rank(const TableView *tv, int sb, int64_t k, bool odd,
+                                          int u_min, int max_size, uint64_t &add) {
+  const int64_t ak = k >= 0 ? k : -k;
+  const int64_t need = odd ? (k >= 0 ? 2 * k + 1 : 2 * ak - 1) : 2 * ak;
+  const int64_t lo = odd ? (u_min | 1) : u_min + (u_min & 1);
+  const int64_t u = need > lo ? need : lo;
+  if (sb + u > max_size) return -1;
+  const int64_t d = k - (odd ? -1 : 0);
+  int i = ((d >= 0) == odd) ? 1 : 2;
+  uint64_t c = 0;
+  for (int64_t cnt = d >= 0 ? d : -d; cnt > 0; --cnt, i += 2)
+    c += op_offset_dev(tv, sb + i, NEG);
+  add = c;
+  return (int)u;
+}
+
+// Copies *src into the workgroup's LDS copy (every lane takes part).
+__device__ __forceinline__ void stage_view(TableView *dst, const TableView *src) {
+  const uint4 *s = (const uint4 *)src;
+  uint4 *d = (uint4 *)dst;
+  for (uint32_t i = threadIdx.x; i < sizeof(TableView) / 16; i += blockDim.x) d[i] = s[i];
+}
+
+// Screen of recorded hits, one lane per hit: the filter-sample windows and
+// one more sample.  Each parity that holds on both gives a candidate (size,
+// rank) from the filter-sample offset alone; candidates that can still lower
+// their slot are listed (hit index, slot | parity << 31, rank) for the check
+// on every sample.  A wave whose hits share one instance reads its table view
+// from LDS.
+template <typename word>
+__device__ void screen_one(const TableView *tv, const uint64_t *__restrict__ leafvals,
+                           const uint64_t *__restrict__ outputs, const BatchInstData &d,
+                           const HitRec &h, uint32_t hi, int min_size, int max_size,
+                           const unsigned long long *__restrict__ found,
+                           SurvRec *__restrict__ surv, uint32_t *__restrict__ n_surv) {
+  const uint32_t I = h.inst >> 8, inst = h.inst & 0xffu;
+  const int sb = (int)(h.pair >> 16), j = (int)(h.pair & 0xffffu), m = sb - 1 - j;
+  const uint64_t *out = outputs + d.out_off;
+  const word x0 = (word)h.v, t0 = (word)out[d.fs];
+  const word ke = word(t0 - x0), ko = word(t0 + x0);
+  const word wU = (word)(max_size - sb), span = word(2) * wU;
+  bool even = word(ke + wU) <= span, odd = word(ko + wU) <= span;
+  if (!even && !odd) return;
+  if (d.n_samples > 1) {
+    const uint32_t s2 = d.fs + 1 < d.n_samples ? d.fs + 1 : 0;
+    const uint64_t *ls = leafvals + d.leaf_off + (size_t)s2 * tv->L;
+    const word x = inst < 7 ? inst_value<word>((int)inst, eval_tree<word>(tv, j, h.a, ls),
+                                                eval_tree<word>(tv, m, h.b, ls))
+                            : (word)ls[h.a];
+    const word t = (word)out[s2];
+    even = even && word(t - x) == ke;
+    odd = odd && word(t + x) == ko;
+    if (!even && !odd) return;
+  }
+  const uint64_t rX = hit_base_rank(tv, h);
+  const int u_min = min_size > sb ? min_size - sb : 0;
+  for (int par = 0; par < 2; ++par) {
+    if (!(par ? odd : even)) continue;
+    uint64_t add = 0;
+    const int u = chain_rank(tv, sb, signed_of<word>(par ? ko : ke), par != 0, u_min, max_size,
+                             add);
+    if (u < 0) continue;
+    const uint32_t slot = I * (kMaxSize + 1) + (uint32_t)(sb + u);
+    const uint64_t rank = rX + add;
+    if (rank >= found[slot]) continue;
+    SurvRec e;
+    e.hit = hi;
+    e.slot = slot | ((uint32_t)par << 31);
+    e.rank = rank;
+    surv[atomicAdd(n_surv, 1u)] = e;
+  }
+}
+
+// Hits [0, min(*n_hits, cap)), 64 per wave, grid-stride.
+template <typename word>
+__global__ void __launch_bounds__(64)
+k_hit_screen(const TableView *__restrict__ tvs, const uint64_t *__restrict__ leafvals,
+             const uint64_t *__restrict__ outputs, const BatchInstData *__restrict__ idat,
+             const HitRec *__restrict__ hits, const uint32_t *__restrict__ n_hits,
+             uint32_t cap, int min_size, int max_size,
+             const unsigned long long *__restrict__ found, SurvRec *__restrict__ surv,
+             uint32_t *__restrict__ n_surv) {
+  __shared__ __align__(16) TableView s_tv;
+  const uint32_t n = min(*n_hits, cap);
+  for (uint32_t base = blockIdx.x * 64u; base < n; base += gridDim.x * 64u) {
+    const uint32_t hi = base + threadIdx.x;
+    const bool valid = hi < n;
+    HitRec h{};
+    if (valid) h = hits[hi];
+    const uint32_t I = h.inst >> 8;
+    const uint32_t I0 = __builtin_amdgcn_readfirstlane(I);  // lane 0 always holds a hit
+    const bool shared = __ballot(valid && I != I0) == 0ull;  // wave-uniform
+    if (shared) {
+      stage_view(&s_tv, tvs + I0);
+      __syncthreads();
+    }
+    if (valid)
+      screen_one<word>(shared ? &s_tv : tvs + I, leafvals, outputs, idat[I], h, hi, min_size,
+                       max_size, found, surv, n_surv);
+    __syncthreads();  // s_tv is restaged by the next round
+  }
+}
+
+// Check of the listed candidates on every sample, one wave per candidate
+// (grid-stride over *n_surv): the instance's table view is staged in LDS,
+// lane 0 unranks the hit's two subtrees, each lane evaluates its samples
+// (RPN stacks in LDS).  A candidate that holds lowers found[slot]; a slot
+// lowered in this launch is listed once in `touched` (its flag is cleared by
+// k_found_gather).  Candidates that can no longer lower their slot are
+// skipped.
+template <typename word>
+__global__ void __launch_bounds__(64)
+k_hit_confirm(const TableView *__restrict__ tvs, const uint64_t *__restrict__ leafvals,
+              const uint64_t *__restrict__ outputs, const BatchInstData *__restrict__ idat,
+              const HitRec *__restrict__ hits, const SurvRec *__restrict__ surv,
+              const uint32_t *__restrict__ n_surv, unsigned long long *__restrict__ found,
+              uint32_t *__restrict__ flags, uint32_t *__restrict__ touched,
+              uint32_t *__restrict__ n_touched) {
+  __shared__ __align__(16) TableView s_tv;
+  __shared__ uint8_t s_ra[kMaxSize], s_rb[kMaxSize];
+  __shared__ word s_stack[kVerifyDepth][64];
+  const int lane = (int)threadIdx.x;
+  const uint32_t ns = *n_surv;
+  for (uint32_t e = blockIdx.x; e < ns; e += gridDim.x) {  // workgroup-uniform
+    const SurvRec c = surv[e];
+    const uint32_t slot = c.slot & 0x7fffffffu;
+    const bool odd = (c.slot >> 31) != 0u;
+    const unsigned long long f = found[slot];
+    if (__ballot(c.rank >= f) != 0ull) continue;
+    const HitRec g = hits[c.hit];
+    const uint32_t I = g.inst >> 8, inst = g.inst & 0xffu;
+    const BatchInstData d = idat[I];
+    stage_view(&s_tv, tvs + I);
+    __syncthreads();
+    const int L = s_tv.L;
+    const int sb = (int)(g.pair >> 16), j = (int)(g.pair & 0xffffu), m = sb - 1 - j;
+    const uint64_t *lv = leafvals + d.leaf_off, *out = outputs + d.out_off;
+    const word x0 = (word)g.v, t0 = (word)out[d.fs];
+    const word k = odd ? word(t0 + x0) : word(t0 - x0);
+    if (lane == 0 && inst < 7) {
+      unrank_rpn(&s_tv, j, g.a, s_ra);
+      unrank_rpn(&s_tv, m, g.b, s_rb);
+    }
+    __syncthreads();
+    bool bad = false;
+    for (uint32_t s = (uint32_t)lane; s < d.n_samples; s += 64) {
+      const uint64_t *ls = lv + (size_t)s * L;
+      word x;
+      if (inst < 7) {
+        const word xa = eval_rpn_lds<word>(s_ra, j, ls, s_stack, lane);
+        const word xb = eval_rpn_lds<word>(s_rb, m, ls, s_stack, lane);
+        x = inst_value<word>((int)inst, xa, xb);
+      } else {
+        x = (word)ls[g.a];
+      }
+      const word t = (word)out[s];
+      bad = bad || (odd ? word(t + x) : word(t - x)) != k;
+    }
+    if (__ballot(bad) == 0ull && lane == 0) {
+      atomicMin(found + slot, (unsigned long long)c.rank);
+      if (atomicExch(flags + slot, 1u) == 0u) touched[atomicAdd(n_touched, 1u)] = slot;
+    }
+    __syncthreads();  // s_tv, s_ra, s_rb and the stacks are reused
+  }
+}
+
+// Candidates (instance, size, minimum rank) of the slots k_hit_confirm
+// listed, [0, *n_touched); clears their flags for the next launch.
+__global__ void __launch_bounds__(256)
+k_found_gather(const uint32_t *__restrict__ touched, const uint32_t *__restrict__ n_touched,
+               const unsigned long long *__restrict__ found, uint32_t *__restrict__ flags,
+               VerifyCand *__restrict__ out) {
+  const uint32_t n = *n_touched;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t slot = touched[i];
+    flags[slot] = 0u;
+    VerifyCand c;
+    c.inst = slot / (kMaxSize + 1);
+    c.size = slot % (kMaxSize + 1);
+    c.rank = found[slot];
+    out[i] = c;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -909,13 +1150,16 @@ k_rtid_scan(const TableView *__restrict__ tv, const uint64_t *__restrict__ leafv
 
 // ---------------------------------------------------------------------------
 // Batched search: one persistent kernel per size step over the jobs of every
-// active instance.  Workgroups pull tile indices from a device counter; a
-// binary search over the jobs' first tiles locates the job.  Every workgroup
-// leaves once the counter passes the step's total, so the grid drains.
+// active instance.  Workgroups pull runs of up to 32 consecutive tiles from a
+// device counter; a binary search over the jobs' first tiles locates a run's
+// first job, and the run walks forward from there, keeping the job and its
+// staged A tile while they do not change.  Every workgroup leaves once the
+// counter passes the step's total, so the grid drains.
 // ---------------------------------------------------------------------------
 template <typename word>
 __device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J, uint64_t lt,
-                           HitRec *hits, uint32_t *hc, uint32_t cap) {
+                           uint64_t &staged, bool &bitops, HitRec *hits, uint32_t *hc,
+                           uint32_t cap) {
   const word *base = (const word *)tv->base;
   PairLaunch p;
   static_cast<PairCore &>(p) = J.p;  // p.bb is not read by the table/stepped sources
@@ -924,11 +1168,14 @@ __device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J, ui
   const word lo_e = (word)J.lo_e, lo_o = (word)J.lo_o, span = (word)J.span;
   const uint32_t tag = J.inst << 8;
   const uint64_t at = p.nTA == 1 ? 0 : lt % p.nTA, bt = p.nTA == 1 ? lt : lt / p.nTA;
-  const uint64_t a0 = at * kPairTA;
-  const uint64_t a1 = a0 + kPairTA < p.nA ? a0 + kPairTA : p.nA;
-  const bool bitops = stage_a<word>(S, tv, A, p.gen_a, p.j, a0, a1,
-                                    exact_layout(p.mask, p.j == p.m, (uint32_t)span),
-                                    lo32(lo_e), lo32(lo_o), (uint32_t)span);
+  if (at != staged) {  // workgroup-uniform; the caller resets `staged` per job
+    const uint64_t a0 = at * kPairTA;
+    const uint64_t a1 = a0 + kPairTA < p.nA ? a0 + kPairTA : p.nA;
+    bitops = stage_a<word>(S, tv, A, p.gen_a, p.j, a0, a1,
+                           exact_layout(p.mask, p.j == p.m, (uint32_t)span), lo32(lo_e),
+                           lo32(lo_o), (uint32_t)span);
+    staged = at;
+  }
   if (p.j == p.m) {
     if (p.gen_b)
       pair_tile<word, true, kBStep, kPairRB>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o, span,
@@ -951,47 +1198,58 @@ k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jo
              const TableView *__restrict__ tvs, unsigned long long *head, HitRec *hits,
              uint32_t *hit_count, uint32_t hit_cap) {
   __shared__ unsigned long long s_t;
-  __shared__ int s_job;
   __shared__ AStage S;
   const uint64_t total = step->total_tiles;
   const int njobs = (int)step->njobs;
+  // tiles per counter grab: up to 32, fewer when the step has few tiles per
+  // workgroup, so the tail stays short
+  const uint64_t per = total / ((uint64_t)gridDim.x * 8u);
+  const uint64_t K = per < 1 ? 1 : (per > 32 ? 32 : per);
+  int jb = 0, jcur = -1;  // workgroup-uniform job cursor
+  BatchJob J;
+  uint64_t staged = ~0ull;  // A tile of job jcur staged in S
+  bool bitops = false;
   for (;;) {
-    if (threadIdx.x == 0) {
-      const unsigned long long t = atomicAdd(head, 1ull);
-      int lo = 0, hi = njobs - 1;
-      if (t < total) {
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (jobs[mid].tile0 <= t) lo = mid;
-          else hi = mid - 1;
-        }
+    if (threadIdx.x == 0) s_t = atomicAdd(head, (unsigned long long)K);
+    __syncthreads();
+    const uint64_t t0 = s_t;
+    __syncthreads();
+    if (t0 >= total) break;
+    const uint64_t t1 = t0 + K < total ? t0 + K : total;
+    if (!(jobs[jb].tile0 <= t0 && (jb + 1 == njobs || jobs[jb + 1].tile0 > t0))) {
+      int lo = 0, hi = njobs - 1;  // the last job whose first tile is <= t0
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile0 <= t0) lo = mid;
+        else hi = mid - 1;
       }
-      s_t = t;
-      s_job = lo;
+      jb = lo;
     }
-    __syncthreads();
-    const uint64_t t = s_t;
-    const int jb = s_job;
-    __syncthreads();
-    if (t >= total) break;
-    const BatchJob J = jobs[jb];
-    const TableView *tv = tvs + J.inst;
-    const uint64_t lt = t - J.tile0;
-    if (J.kind == 0) {  // table build: V[m] ranks [lt*kBuildTile, +kBuildTile)
-      word *out = (word *)tv->base + tv->off[J.p.m];
-      const uint64_t r0 = lt * kBuildTile;
-      const uint64_t r1 = r0 + kBuildTile < J.p.nB ? r0 + kBuildTile : J.p.nB;
-      for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x)
-        out[r] = build_value<word>(tv, J.p.m, r);
-    } else if (J.kind == 1) {
-      batch_pair<word>(S, tv, J, lt, hits, hit_count, hit_cap);
-    } else {  // leaves against the windows of base size 1
-      const word *V = (const word *)tv->base + tv->off[1];
-      if ((int)threadIdx.x < tv->L) {
-        const word c = V[threadIdx.x];
-        if (in_window(c, (word)J.lo_e, (word)J.lo_o, (word)J.span))
-          record_hit<word>(hits, hit_count, hit_cap, threadIdx.x, 0, c, 7 | (J.inst << 8),
-                           1u << 16);
+    for (uint64_t t = t0; t < t1; ++t) {
+      while (jb + 1 < njobs && jobs[jb + 1].tile0 <= t) ++jb;
+      if (jb != jcur) {
+        J = jobs[jb];
+        jcur = jb;
+        staged = ~0ull;
+      }
+      const TableView *tv = tvs + J.inst;
+      const uint64_t lt = t - J.tile0;
+      if (J.kind == 0) {  // table build: V[m] ranks [lt*kBuildTile, +kBuildTile)
+        word *out = (word *)tv->base + tv->off[J.p.m];
+        const uint64_t r0 = lt * kBuildTile;
+        const uint64_t r1 = r0 + kBuildTile < J.p.nB ? r0 + kBuildTile : J.p.nB;
+        for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x)
+          out[r] = build_value<word>(tv, J.p.m, r);
+      } else if (J.kind == 1) {
+        batch_pair<word>(S, tv, J, lt, staged, bitops, hits, hit_count, hit_cap);
+      } else {  // leaves against the windows of base size 1
+        const word *V = (const word *)tv->base + tv->off[1];
+        if ((int)threadIdx.x < tv->L) {
+          const word c = V[threadIdx.x];
+          if (in_window(c, (word)J.lo_e, (word)J.lo_o, (word)J.span))
+            record_hit<word>(hits, hit_count, hit_cap, threadIdx.x, 0, c, 7 | (J.inst << 8),
+                             1u << 16);
+        }
       }
     }
   }
@@ -1095,7 +1353,6 @@ k_verify_batch(const TableView *__restrict__ tvs, const uint64_t *__restrict__ l
 // staged in LDS, and a wave ballot ends the check at the first chunk of
 // samples with a mismatch.
 // ---------------------------------------------------------------------------
-constexpr int kVerifyDepth = kMaxSize / 2 + 2;  // max RPN stack depth
 
 template <typename word>
 __device__ void verify_wave(const TableView *__restrict__ tv, const uint64_t *__restrict__ lv,
@@ -1191,7 +1448,9 @@ __global__ void __launch_bounds__(64)
 k_verify_batch_wave(const TableView *__restrict__ tvs, const uint64_t *__restrict__ leafvals,
                     const uint64_t *__restrict__ outputs,
                     const BatchInstData *__restrict__ idat,
-                    const VerifyCand *__restrict__ cand, uint8_t *__restrict__ ok) {
+                    const VerifyCand *__restrict__ cand, const uint32_t *__restrict__ n_dev,
+                    uint8_t *__restrict__ ok) {
+  if (n_dev && blockIdx.x >= *n_dev) return;  // device-side candidate count
   const VerifyCand vc = cand[blockIdx.x];
   const BatchInstData d = idat[vc.inst];
   verify_wave<word>(tvs + vc.inst, leafvals + d.leaf_off, outputs + d.out_off,
@@ -1224,6 +1483,15 @@ static void table_blocks(word *base, const uint64_t *off, const CountTable &T, i
       r += nA * nB;
     }
   }
+}
+
+hipError_t launch_table_small(int width, const TableView *d_tv, int m_lo, int m_hi,
+                              hipStream_t stream) {
+  if (width == 32)
+    hipLaunchKernelGGL(k_table_small<uint32_t>, dim3(1), dim3(1024), 0, stream, d_tv, m_lo, m_hi);
+  else
+    hipLaunchKernelGGL(k_table_small<uint64_t>, dim3(1), dim3(1024), 0, stream, d_tv, m_lo, m_hi);
+  return hipGetLastError();
 }
 
 hipError_t launch_table_build(int width, void *d_base, const uint64_t *off, const CountTable &T,
@@ -1295,17 +1563,39 @@ hipError_t launch_rtid_scan(int width, const TableView *d_tv, const uint64_t *d_
   return hipGetLastError();
 }
 
-hipError_t launch_hit_filter(int width, const TableView *d_tvs, const HitRec *d_in, uint32_t n,
-                             int max_size, HitRec *d_out, uint32_t *d_out_count,
-                             hipStream_t stream) {
-  if (n == 0) return hipSuccess;
-  const dim3 g((n + 255) / 256);
-  if (width == 32)
-    hipLaunchKernelGGL(k_hit_filter<uint32_t>, g, dim3(256), 0, stream, d_tvs, d_in, n,
-                       max_size, d_out, d_out_count);
-  else
-    hipLaunchKernelGGL(k_hit_filter<uint64_t>, g, dim3(256), 0, stream, d_tvs, d_in, n,
-                       max_size, d_out, d_out_count);
+hipError_t launch_hit_resolve(int width, const TableView *d_tvs, const uint64_t *d_leafvals,
+                              const uint64_t *d_outputs, const BatchInstData *d_idat,
+                              const HitRec *d_hits, const uint32_t *d_n_hits, uint32_t cap,
+                              uint32_t screen_blocks, uint32_t confirm_blocks, int min_size,
+                              int max_size, unsigned long long *d_found, SurvRec *d_surv,
+                              uint32_t *d_n_surv, uint32_t *d_flags, uint32_t *d_touched,
+                              uint32_t *d_n_touched, hipStream_t stream) {
+  const dim3 gs(screen_blocks ? screen_blocks : 1), gc(confirm_blocks ? confirm_blocks : 1);
+  if (width == 32) {
+    hipLaunchKernelGGL(k_hit_screen<uint32_t>, gs, dim3(64), 0, stream, d_tvs, d_leafvals,
+                       d_outputs, d_idat, d_hits, d_n_hits, cap, min_size, max_size, d_found,
+                       d_surv, d_n_surv);
+    hipLaunchKernelGGL(k_hit_confirm<uint32_t>, gc, dim3(64), 0, stream, d_tvs, d_leafvals,
+                       d_outputs, d_idat, d_hits, d_surv, d_n_surv, d_found, d_flags, d_touched,
+                       d_n_touched);
+  } else {
+    hipLaunchKernelGGL(k_hit_screen<uint64_t>, gs, dim3(64), 0, stream, d_tvs, d_leafvals,
+                       d_outputs, d_idat, d_hits, d_n_hits, cap, min_size, max_size, d_found,
+                       d_surv, d_n_surv);
+    hipLaunchKernelGGL(k_hit_confirm<uint64_t>, gc, dim3(64), 0, stream, d_tvs, d_leafvals,
+                       d_outputs, d_idat, d_hits, d_surv, d_n_surv, d_found, d_flags, d_touched,
+                       d_n_touched);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_found_gather(const uint32_t *d_touched, const uint32_t *d_n, uint32_t max_n,
+                               const unsigned long long *d_found, uint32_t *d_flags,
+                               VerifyCand *d_out, hipStream_t stream) {
+  if (max_n == 0) return hipSuccess;
+  const uint32_t blocks = (max_n + 255) / 256;
+  hipLaunchKernelGGL(k_found_gather, dim3(blocks < 1024u ? blocks : 1024u), dim3(256), 0, stream,
+                     d_touched, d_n, d_found, d_flags, d_out);
   return hipGetLastError();
 }
 
@@ -1357,14 +1647,14 @@ hipError_t launch_batch_step(int width, const BatchStep *d_step, const BatchJob 
 hipError_t launch_verify_batch(int width, const TableView *d_tvs, const uint64_t *d_leafvals,
                                const uint64_t *d_outputs, const BatchInstData *d_idat,
                                const VerifyCand *d_cand, uint32_t n_cand, uint8_t *d_ok,
-                               hipStream_t stream) {
+                               hipStream_t stream, const uint32_t *d_n) {
   if (n_cand == 0) return hipSuccess;
   if (width == 32)
     hipLaunchKernelGGL(k_verify_batch_wave<uint32_t>, dim3(n_cand), dim3(64), 0, stream, d_tvs,
-                       d_leafvals, d_outputs, d_idat, d_cand, d_ok);
+                       d_leafvals, d_outputs, d_idat, d_cand, d_n, d_ok);
   else
     hipLaunchKernelGGL(k_verify_batch_wave<uint64_t>, dim3(n_cand), dim3(64), 0, stream, d_tvs,
-                       d_leafvals, d_outputs, d_idat, d_cand, d_ok);
+                       d_leafvals, d_outputs, d_idat, d_cand, d_n, d_ok);
   return hipGetLastError();
 }
 
