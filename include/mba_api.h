@@ -79,7 +79,12 @@ typedef struct {
   int32_t table_delta;      /* tabulate Kt - table_delta sizes (0: budget)   */
   int32_t block_log2;       /* sweep one (op, split) block per launch from   */
                             /* 2^block_log2 trees per size (0: defaults)    */
-  int32_t reserved[3];
+  int32_t hit_cap;          /* SIMBA per-instance: window-hit buffer, in     */
+                            /* hits (0: 2^22, above the hits of one tile); a */
+                            /* launch group that overflows it is replayed in */
+                            /* parts (a smaller buffer that one tile         */
+                            /* overflows is an error)                        */
+  int32_t reserved[2];
 } mba_opts;
 
 typedef struct {
@@ -99,7 +104,9 @@ typedef struct {
   uint32_t kernel_launches; /* intervals summed in kernel_seconds (SIMBA:   */
                             /* one per pair-kernel launch; VFB: one per     */
                             /* level; batch: one per graph replay)          */
-  int32_t reserved[4];
+  int32_t hit_replays;      /* SIMBA per-instance: launch groups replayed    */
+                            /* in parts after a hit-buffer overflow         */
+  int32_t reserved[3];
 } mba_result;
 
 /* VFB per-level statistics (PRINT_CACHE_SIZE mode, full-cache.cu:763-765). */
@@ -163,7 +170,11 @@ typedef struct mba_mlp mba_mlp;
 
 /* 256 features of an instance from its hashed sample input/output bits. */
 int mba_features(const mba_instance *inst, float *out);
+/* The same features of `count` instances computed on the context's device
+ * (one workgroup per instance); out [count x 256], equal to mba_features. */
+int mba_features_batch(mba_ctx *ctx, const mba_instance *insts, size_t count, float *out);
 int mba_mlp_create(mba_ctx *ctx, uint64_t seed, mba_mlp **out);
+/* Destroy every network of a context before the context itself. */
 void mba_mlp_destroy(mba_mlp *m);
 /* Host copies of the FP32 master weights: w1 [256 x 512], b1 [512],
  * w2 [512 x 128], b2 [128], row-major (input-major). */

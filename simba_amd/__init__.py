@@ -13,8 +13,9 @@ the full result record.  There is no CPU fallback: without the built library
 or without a gfx950 device every entry point raises.
 """
 import ctypes
-from collections.abc import Sequence
 import os
+import weakref
+from collections.abc import Sequence
 
 import numpy as np
 
@@ -42,7 +43,8 @@ class Opts(ctypes.Structure):
                 ("table_bytes", ctypes.c_uint64), ("cache_bytes", ctypes.c_uint64),
                 ("verbose", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32),
                 ("lanes_b", ctypes.c_int32), ("table_delta", ctypes.c_int32),
-                ("block_log2", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("block_log2", ctypes.c_int32), ("hit_cap", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2)]
 
 
 class Result(ctypes.Structure):
@@ -52,7 +54,7 @@ class Result(ctypes.Structure):
                 ("rpn", ctypes.c_char * (MAX_SIZE + 1)), ("seconds", ctypes.c_double),
                 ("kernel_seconds", ctypes.c_double), ("candidates", ctypes.c_uint64),
                 ("size_reached", ctypes.c_int32), ("kernel_launches", ctypes.c_uint32),
-                ("reserved", ctypes.c_int32 * 4)]
+                ("hit_replays", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
 
     def as_dict(self):
         return {"status": self.status, "size": self.size, "rank": int(self.rank),
@@ -60,7 +62,8 @@ class Result(ctypes.Structure):
                 "rpn": self.rpn.decode(), "seconds": self.seconds,
                 "kernel_seconds": self.kernel_seconds,
                 "kernel_launches": self.kernel_launches,
-                "candidates": int(self.candidates), "size_reached": self.size_reached}
+                "candidates": int(self.candidates), "size_reached": self.size_reached,
+                "hit_replays": self.hit_replays}
 
 
 def _result_array(n):
@@ -102,8 +105,9 @@ EXPORTS = ["mba_ctx_create", "mba_ctx_destroy", "mba_strerror", "mba_ctx_last_er
            "mba_default_opts", "mba_simba_solve", "mba_simba_rtid_solve", "mba_vfb_solve",
            "mba_batch_solve",
            "mba_instance_read_json", "mba_instance_free", "mba_rpn_to_infix",
-           "mba_features", "mba_mlp_create", "mba_mlp_destroy", "mba_mlp_get_weights",
-           "mba_mlp_set_weights", "mba_mlp_forward", "mba_mlp_grads", "mba_mlp_train_step",
+           "mba_features", "mba_features_batch", "mba_mlp_create", "mba_mlp_destroy",
+           "mba_mlp_get_weights", "mba_mlp_set_weights", "mba_mlp_forward", "mba_mlp_grads",
+           "mba_mlp_train_step",
            "mba_learned_solve", "mba_learned_batch_solve"]
 MLP_IN, MLP_HIDDEN, MLP_OUT = 256, 512, 128
 
@@ -138,6 +142,7 @@ def lib():
                                        ctypes.c_size_t]
         fp = P(ctypes.c_float)
         L.mba_features.argtypes = [P(Instance), fp]
+        L.mba_features_batch.argtypes = [ctypes.c_void_p, P(Instance), ctypes.c_size_t, fp]
         L.mba_mlp_create.argtypes = [ctypes.c_void_p, ctypes.c_uint64, P(ctypes.c_void_p)]
         L.mba_mlp_destroy.argtypes = [ctypes.c_void_p]
         L.mba_mlp_get_weights.argtypes = [ctypes.c_void_p, fp, fp, fp, fp]
@@ -164,12 +169,16 @@ class Context:
 
     def __init__(self, device=0):
         self._h = ctypes.c_void_p()
+        self._mlps = weakref.WeakSet()  # networks created on this context
         rc = lib().mba_ctx_create(device, ctypes.byref(self._h))
         if rc != 0:
             raise MbaError(f"mba_ctx_create failed: {lib().mba_strerror(rc).decode()}")
 
     def close(self):
+        """Destroys the context and, first, every network created on it."""
         if self._h:
+            for m in list(self._mlps):
+                m.close()
             lib().mba_ctx_destroy(self._h)
             self._h = ctypes.c_void_p()
 
@@ -242,6 +251,16 @@ def features(inst):
     return out
 
 
+def features_batch(ctx, insts):
+    """Features of many instances computed on the device (count x 256 float32,
+    equal to features() row by row)."""
+    out = np.zeros((len(insts), MLP_IN), dtype=np.float32)
+    if len(insts):
+        ctx._check(lib().mba_features_batch(ctx._h, pack_instances(insts), len(insts),
+                                            _fptr(out)))
+    return out
+
+
 def block_class(op, split):
     """Output class of a root block: op * 16 + min(left size, 15)."""
     return op * 16 + min(max(split, 0), 15)
@@ -256,6 +275,7 @@ class Mlp:
         rc = lib().mba_mlp_create(ctx._h, seed, ctypes.byref(self._h))
         if rc != 0:
             raise MbaError(lib().mba_strerror(rc).decode())
+        ctx._mlps.add(self)
 
     def close(self):
         if self._h:
@@ -327,11 +347,11 @@ class Mlp:
 
 def make_opts(min_size=1, max_size=9, compat=False, exhaustive=False, budget_s=0.0,
               table_bytes=0, cache_bytes=0, verbose=False, cancel=None, wg_per_cu=0,
-              lanes_b=0, table_delta=0, block_log2=0):
+              lanes_b=0, table_delta=0, block_log2=0, hit_cap=0):
     o = Opts()
     lib().mba_default_opts(ctypes.byref(o))
     o.wg_per_cu, o.lanes_b, o.table_delta = wg_per_cu, lanes_b, table_delta
-    o.block_log2 = block_log2
+    o.block_log2, o.hit_cap = block_log2, hit_cap
     o.min_size, o.max_size = min_size, max_size
     o.compat, o.exhaustive = int(compat), int(exhaustive)
     o.time_budget_s = budget_s

@@ -25,7 +25,7 @@ namespace mba {
 
 namespace {
 
-constexpr uint32_t kBatchHitCap = 1u << 22;
+constexpr uint32_t kBatchHitCap = 1u << 24;  // 512 MiB of hit records
 constexpr uint64_t kBatchTableBytes = 32ull << 30;
 
 struct InstState {
@@ -146,11 +146,10 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
                               ctx.b_results.as<BatchInstData>(), (uint32_t)count, stq));
 
   // ---- per-step graph: upload jobs, reset counters, drain, read hit count --
-  // jobs per instance and step: a leaf scan or a table build, plus the pairs
-  // (plain) or at most half of the binary blocks of the size (learned rounds)
+  // jobs per instance and step: a leaf scan or a table build, plus at most
+  // every split of the size
   size_t job_cap = 0;
-  for (size_t i = 0; i < count; ++i)
-    job_cap += 2 + (size_t)std::max(0, scores ? 3 * max_size : (max_size - 1) / 2);
+  for (size_t i = 0; i < count; ++i) job_cap += 2 + (size_t)std::max(0, (max_size - 1) / 2);
   BatchJob *h_jobs = nullptr;
   BatchStep *h_step = nullptr;
   uint32_t *h_hc = nullptr;
@@ -221,7 +220,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     st.reached = z;
     st.active = false;
   };
-  std::vector<std::vector<BlockRef>> order(count);
+  std::vector<std::vector<int>> order(count);  // learned: splits in score order
   if (o.verbose)
     std::fprintf(stderr, "batch: %zu instances set up in %.3f ms\n", count,
                  1e3 * seconds_since(t_start));
@@ -251,19 +250,26 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
           order[i].clear();
           continue;
         }
-        order[i] = binary_blocks(*st.T, s);
+        std::vector<BlockRef> blocks = binary_blocks(*st.T, s);
         const float *sc = scores + i * 256;
-        std::stable_sort(order[i].begin(), order[i].end(),
-                         [&](const BlockRef &x, const BlockRef &y) {
-                           return sc[x.op * 32 + x.split] > sc[y.op * 32 + y.split];
-                         });
+        std::stable_sort(blocks.begin(), blocks.end(), [&](const BlockRef &x, const BlockRef &y) {
+          return sc[x.op * 32 + x.split] > sc[y.op * 32 + y.split];
+        });
+        order[i].clear();
+        for (const BlockRef &b : blocks)
+          if (std::find(order[i].begin(), order[i].end(), b.pair_j) == order[i].end())
+            order[i].push_back(b.pair_j);
         max_blocks = std::max(max_blocks, order[i].size());
       }
     }
-    // learned order: rounds of 1, 1, 2, 4, ... blocks per instance
+    // learned order: rounds of the split of the best block, the next split,
+    // then the rest, each split swept once with every operator instance fused
+    // (a round costs a replay and a sync; sizes without a solution need
+    // every split anyway)
+    static const size_t kRoundEnd[3] = {1, 2, ~(size_t)0};
     for (int round = 0;; ++round) {
-      const size_t rb = round == 0 ? 0 : (size_t)1 << (round - 1);
-      const size_t re = (size_t)1 << round;
+      const size_t rb = round == 0 ? 0 : kRoundEnd[round - 1];
+      const size_t re = kRoundEnd[std::min(round, 2)];
       size_t nj = 0;
       uint64_t tiles = 0;
       const auto t_b0 = Clock::now();
@@ -325,8 +331,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
           for (int j = 1; j <= (s - 1) / 2; ++j) pair_job(j, kAllInst);
         }
         if (learned) {
-          for (size_t k = rb; k < re && k < order[i].size(); ++k)
-            pair_job(order[i][k].pair_j, 1u << order[i][k].inst);
+          for (size_t k = rb; k < re && k < order[i].size(); ++k) pair_job(order[i][k], kAllInst);
         }
       }
       t_build += seconds_since(t_b0);

@@ -49,6 +49,8 @@ struct Ctx {
       vfb_counts, vfb_target;
   // batched / learned
   DevBuf b_inst, b_data, b_queue, b_results, b_misc, mlp_w, mlp_misc;
+  // instance features on the device (packed inputs/outputs, layout, rows)
+  DevBuf f_in, f_out, f_meta, f_x;
   // pinned host staging for the per-step read backs of the search drivers
   void *pinned = nullptr;
   static constexpr size_t kPinnedBytes = 4096;

@@ -7,6 +7,7 @@
 // descending score and stops at the first verified solution of that size,
 // so the solution has the minimal size (the plain search's size); sizes
 // without a solution are swept completely, as in the plain search.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -14,6 +15,7 @@
 
 #include "context.h"
 #include "mlp_kernels.h"
+#include "simba_host.h"
 
 struct mba_mlp {
   mba_ctx *owner = nullptr;
@@ -158,6 +160,39 @@ uint32_t fmix32(uint32_t x) {
 }
 
 }  // namespace
+
+// Features of instances [0, count) on the device into ctx.f_x (count x 256
+// floats, row-major, equal to instance_features): inputs and outputs packed
+// and uploaded, one k_features launch, on ctx.stream.
+int features_dev(Ctx &ctx, const mba_instance *insts, size_t count, float **d_x) {
+  std::vector<uint64_t> in_all, out_all;
+  std::vector<FeatInst> fi(count);
+  for (size_t i = 0; i < count; ++i) {
+    const mba_instance &in = insts[i];
+    if (!in.outputs || (in.n_vars && !in.inputs)) return MBA_ERR_ARG;
+    if (in.width != 32 && in.width != 64) return MBA_ERR_LIMIT;
+    fi[i] = {in_all.size(), out_all.size(), in.n_samples, in.n_vars, in.width, 0u};
+    in_all.insert(in_all.end(), in.inputs, in.inputs + (size_t)in.n_samples * in.n_vars);
+    out_all.insert(out_all.end(), in.outputs, in.outputs + in.n_samples);
+  }
+  MBA_HIP(ctx.f_in.ensure(8 * std::max<size_t>(in_all.size(), 1)));
+  MBA_HIP(ctx.f_out.ensure(8 * std::max<size_t>(out_all.size(), 1)));
+  MBA_HIP(ctx.f_meta.ensure(sizeof(FeatInst) * std::max<size_t>(count, 1)));
+  MBA_HIP(ctx.f_x.ensure(4 * kMlpIn * std::max<size_t>(count, 1)));
+  MBA_HIP(hipMemcpyAsync(ctx.f_in.p, in_all.data(), 8 * in_all.size(), hipMemcpyHostToDevice,
+                         ctx.stream));
+  MBA_HIP(hipMemcpyAsync(ctx.f_out.p, out_all.data(), 8 * out_all.size(), hipMemcpyHostToDevice,
+                         ctx.stream));
+  MBA_HIP(hipMemcpyAsync(ctx.f_meta.p, fi.data(), sizeof(FeatInst) * count,
+                         hipMemcpyHostToDevice, ctx.stream));
+  MBA_HIP(launch_features(ctx.f_in.as<uint64_t>(), ctx.f_out.as<uint64_t>(),
+                          ctx.f_meta.as<FeatInst>(), (uint32_t)count, ctx.f_x.as<float>(),
+                          ctx.stream));
+  // the host vectors leave scope: finish the copies before returning
+  MBA_HIP(hipStreamSynchronize(ctx.stream));
+  *d_x = ctx.f_x.as<float>();
+  return 0;
+}
 
 // 256 features of an instance: for every sample and bit position k, the
 // pattern of (output bit, input bits of up to 5 variables) at k, hashed
@@ -358,12 +393,26 @@ int mba_learned_solve(mba_ctx *c, mba_mlp *m, const mba_instance *inst, const mb
     mba_default_opts(&d);
     opts = &d;
   }
+  // the scores order the blocks of sizes with at least kLearnedBlockMin
+  // trees; below that every size is swept fused and the network is not run
+  const auto t0 = Clock::now();
+  const int L = (int)(inst->n_vars + inst->n_consts);
+  const int top = opts->compat ? opts->max_size - 1 : opts->max_size;
+  CountTable T;
+  const bool uses_scores = L >= 1 && top >= 1 && top <= kMaxSize &&
+                           (!T.build(L, top) || T.t[top][8] >= kLearnedBlockMin);
   float feat[kMlpIn], logits[kMlpOut], scores[8 * 32];
-  if (int rc = mba_features(inst, feat)) return rc;
-  if (int rc = mba_mlp_forward(m, feat, 1, logits)) return rc;
-  block_scores_from_logits(logits, scores);
+  std::fill(scores, scores + 8 * 32, 0.f);
+  if (uses_scores) {
+    if (int rc = mba_features(inst, feat)) return rc;
+    if (int rc = mba_mlp_forward(m, feat, 1, logits)) return rc;
+    block_scores_from_logits(logits, scores);
+  }
+  const double t_inference = seconds_since(t0);
   if (hipSetDevice(c->c.device) != hipSuccess) return MBA_ERR_HIP;
-  return simba_solve(c->c, *inst, *opts, *res, scores);
+  const int rc = simba_solve(c->c, *inst, *opts, *res, scores);
+  res->seconds += t_inference;  // time to solution includes the network
+  return rc;
 }
 
 int mba_learned_batch_solve(mba_ctx *c, mba_mlp *m, const mba_instance *insts, size_t count,
@@ -374,19 +423,42 @@ int mba_learned_batch_solve(mba_ctx *c, mba_mlp *m, const mba_instance *insts, s
     mba_default_opts(&d);
     opts = &d;
   }
-  std::vector<float> feat(count * kMlpIn), logits(count * kMlpOut), scores(count * 8 * 32);
-  for (size_t i = 0; i < count; ++i)
-    if (int rc = mba_features(&insts[i], feat.data() + i * kMlpIn)) return rc;
-  // forward in chunks of 4096 rows
+  if (count == 0) return 0;
+  // features and forward on the network's device, rows in chunks of 4096
+  Ctx &ctx = m->owner->c;
+  MBA_HIP(hipSetDevice(ctx.device));
+  float *d_x = nullptr;
+  if (int rc = features_dev(ctx, insts, count, &d_x)) return rc;
+  std::vector<float> logits(count * kMlpOut), scores(count * 8 * 32);
   for (size_t b = 0; b < count; b += 4096) {
     const int nb = (int)std::min<size_t>(4096, count - b);
-    if (int rc = mba_mlp_forward(m, feat.data() + b * kMlpIn, nb, logits.data() + b * kMlpOut))
-      return rc;
+    const int rows = (nb + 15) / 16 * 16;
+    if (int rc = ensure_cap(ctx, m, rows)) return rc;
+    const Acts a = acts_of(m->acts.as<float>(), m->cap);
+    MBA_HIP(hipMemsetAsync(a.X, 0, (size_t)rows * kMlpIn * 4, ctx.stream));
+    MBA_HIP(hipMemcpyAsync(a.X, d_x + b * kMlpIn, (size_t)nb * kMlpIn * 4,
+                           hipMemcpyDeviceToDevice, ctx.stream));
+    if (int rc = forward_dev(ctx, m, rows)) return rc;
+    MBA_HIP(hipMemcpyAsync(logits.data() + b * kMlpOut, a.Z, (size_t)nb * kMlpOut * 4,
+                           hipMemcpyDeviceToHost, ctx.stream));
   }
+  MBA_HIP(hipStreamSynchronize(ctx.stream));
   for (size_t i = 0; i < count; ++i)
     block_scores_from_logits(logits.data() + i * kMlpOut, scores.data() + i * 256);
   if (hipSetDevice(c->c.device) != hipSuccess) return MBA_ERR_HIP;
   return batch_solve(c->c, insts, count, *opts, results, scores.data());
+}
+
+int mba_features_batch(mba_ctx *c, const mba_instance *insts, size_t count, float *out) {
+  if (!c || (!insts && count) || (!out && count)) return MBA_ERR_ARG;
+  if (count == 0) return 0;
+  Ctx &ctx = c->c;
+  MBA_HIP(hipSetDevice(ctx.device));
+  float *d_x = nullptr;
+  if (int rc = features_dev(ctx, insts, count, &d_x)) return rc;
+  MBA_HIP(hipMemcpyAsync(out, d_x, count * kMlpIn * 4, hipMemcpyDeviceToHost, ctx.stream));
+  MBA_HIP(hipStreamSynchronize(ctx.stream));
+  return 0;
 }
 
 }  // extern "C"

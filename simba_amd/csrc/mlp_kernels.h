@@ -26,6 +26,16 @@ hipError_t launch_gemm(const GemmArgs &g, hipStream_t s);
 hipError_t launch_softmax_ce(const float *Z, const int32_t *labels, int B, int K, float inv_n,
                              float *dZ, float *loss_rows, hipStream_t s);
 hipError_t launch_colsum(const float *D, int rows, int cols, float *out, hipStream_t s);
+// One instance of a feature batch: its [sample][var] inputs and outputs in
+// the packed arrays.
+struct FeatInst {
+  uint64_t in_off, out_off;
+  uint32_t n_samples, n_vars, width, pad;
+};
+// X[i * 256 + b] = the 256 hashed features of instance i (the host
+// instance_features, bit for bit), instances [0, count).
+hipError_t launch_features(const uint64_t *d_inputs, const uint64_t *d_outputs,
+                           const FeatInst *d_fi, uint32_t count, float *d_x, hipStream_t s);
 hipError_t launch_adam(float *w, const float *g, float *m, float *v, int n, float lr, float b1,
                        float b2, float eps, float c1, float c2, hipStream_t s);
 

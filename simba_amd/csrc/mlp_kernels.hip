@@ -119,6 +119,54 @@ hipError_t launch_colsum(const float *D, int rows, int cols, float *out, hipStre
   return hipGetLastError();
 }
 
+// Features of a batch of instances, one workgroup (one wave) per instance,
+// lane k = bit position k: the pattern (output bit, bits of the first five
+// variables) of every sample at bit k, hashed with the bit group and the
+// variable count into one of 256 buckets (a histogram in LDS), scaled by
+// 8 / (samples * width).  Counts stay exact in float, so the result equals
+// the host's running float sums.
+__device__ __forceinline__ uint32_t feat_mix(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x85ebca6bu;
+  x ^= x >> 13;
+  x *= 0xc2b2ae35u;
+  x ^= x >> 16;
+  return x;
+}
+
+__global__ void __launch_bounds__(64)
+k_features(const uint64_t *__restrict__ inputs, const uint64_t *__restrict__ outputs,
+           const FeatInst *__restrict__ fi, uint32_t count, float *__restrict__ x) {
+  __shared__ uint32_t hist[kMlpIn];
+  const uint32_t i = blockIdx.x;
+  if (i >= count) return;
+  const FeatInst f = fi[i];
+  for (int b = threadIdx.x; b < kMlpIn; b += 64) hist[b] = 0u;
+  __syncthreads();
+  const uint32_t k = threadIdx.x;
+  if (k < f.width) {
+    const uint32_t g = k < 8 ? k : k < 16 ? 8u : k < 32 ? 9u : 10u;
+    const uint32_t nv5 = f.n_vars < 5 ? f.n_vars : 5u;
+    for (uint32_t smp = 0; smp < f.n_samples; ++smp) {
+      uint32_t p = (uint32_t)((outputs[f.out_off + smp] >> k) & 1u);
+      const uint64_t *in = inputs + f.in_off + (size_t)smp * f.n_vars;
+      for (uint32_t v = 0; v < nv5; ++v) p |= (uint32_t)((in[v] >> k) & 1u) << (v + 1);
+      atomicAdd(&hist[feat_mix(p * 64u + g + 1024u * f.n_vars) & (kMlpIn - 1)], 1u);
+    }
+  }
+  __syncthreads();
+  const float scale = 8.f / (float)(f.n_samples * f.width);
+  for (int b = threadIdx.x; b < kMlpIn; b += 64) x[(size_t)i * kMlpIn + b] = (float)hist[b] * scale;
+}
+
+hipError_t launch_features(const uint64_t *d_inputs, const uint64_t *d_outputs,
+                           const FeatInst *d_fi, uint32_t count, float *d_x, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_features, dim3(count), dim3(64), 0, s, d_inputs, d_outputs, d_fi, count,
+                     d_x);
+  return hipGetLastError();
+}
+
 hipError_t launch_adam(float *w, const float *g, float *m, float *v, int n, float lr, float b1,
                        float b2, float eps, float c1, float c2, hipStream_t s) {
   hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, s, w, g, m, v, n, lr, b1, b2,

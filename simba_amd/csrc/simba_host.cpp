@@ -7,6 +7,7 @@
 // (size, op) launch with a match, nocache.cu:102-105, which is one of the
 // trees of that size and op block).
 #include <algorithm>
+#include <functional>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -17,7 +18,9 @@
 
 namespace mba {
 
-constexpr uint32_t kHitCap = 1u << 20;
+// default window-hit buffer: more than the 256 x 2048 x 7 hits one tile can
+// record, so replaying an overflowing launch group in parts always succeeds
+constexpr uint32_t kDefaultHitCap = 1u << 22;
 // value tables up to this many entries in all are built by one workgroup
 constexpr uint64_t kSmallTableValues = 1u << 16;
 constexpr uint64_t kDefaultTableBytes = 40ull << 30;
@@ -336,6 +339,9 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   MBA_HIP(ctx.tables.ensure(std::max<size_t>(elems * wb, 256)));
   tv.base = ctx.tables.p;
   MBA_HIP(ctx.tview.ensure(sizeof(TableView)));
+  // hit buffer capacity (opts.hit_cap: smaller buffers exercise the drain
+  // and replay paths in tests)
+  const uint32_t kHitCap = o.hit_cap > 0 ? (uint32_t)o.hit_cap : kDefaultHitCap;
   MBA_HIP(ctx.hits.ensure(sizeof(HitRec) * kHitCap));
   MBA_HIP(ctx.hit_count.ensure(256));
   MBA_HIP(ctx.leafvals.ensure(sizeof(uint64_t) * lv.size()));
@@ -489,6 +495,11 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     return 0;
   };
 
+  // hits recorded in the buffer before the open launch group (known at the
+  // group's start; the buffer is append-only)
+  uint32_t hits_base = 0;
+  int replays = 0;  // launch groups replayed after an overflow
+
   auto over_budget = [&]() {
     if (o.cancel && *o.cancel) return true;
     return o.time_budget_s > 0 && seconds_since(t_start) > o.time_budget_s;
@@ -538,10 +549,49 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     }
     const uint64_t per_tile = (uint64_t)kPairTA * tb;
     const uint64_t tiles_per_launch = std::max<uint64_t>(1, kPairsPerLaunch / per_tile);
+    // Replays a launch's tile range in `pieces` parts, resolving each part's
+    // hits before the next; a part that still overflows the hit buffer is
+    // split again (a single tile that overflows it is an error).
+    std::function<int(const PairLaunch &, int)> replay = [&](const PairLaunch &q,
+                                                              int pieces) -> int {
+      const uint64_t n = q.tile_end - q.tile_begin;
+      const uint64_t step = std::max<uint64_t>(1, (n + pieces - 1) / pieces);
+      for (uint64_t b = q.tile_begin; b < q.tile_end; b += step) {
+        PairLaunch r = q;
+        r.tile_begin = b;
+        r.tile_end = std::min(q.tile_end, b + step);
+        const int g = (int)std::min<uint64_t>((uint64_t)grid, r.tile_end - r.tile_begin);
+        uint32_t cnt = 0;
+        hipError_t e =
+            launch_pair_check(W, d_tv, ctx.tables.p, tv.off, r, tg, d_hits, d_hc, kHitCap, g, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(&cnt, d_hc, 4, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);  // `st` does not wait on null
+        if (e != hipSuccess) {
+          ctx.last_error = std::string("pair replay: ") + hipGetErrorString(e);
+          return -1;
+        }
+        ++klaunch;
+        if (cnt > kHitCap) {
+          if (r.tile_end - r.tile_begin == 1) {
+            ctx.last_error = "hit buffer overflow within one tile";
+            return -1;
+          }
+          if (hipMemsetAsync(d_hc, 0, 4, st) != hipSuccess) return -1;
+          if (int rc = replay(r, 8)) return rc;
+          continue;
+        }
+        if (cnt > 0) {
+          if (process_hits() != 0) return -1;
+          if (hipMemsetAsync(d_hc, 0, 4, st) != hipSuccess) return -1;
+        }
+      }
+      return 0;
+    };
     // launches go out back to back; the stream is synchronised (timing and
     // budget check) once a group holds about kPairsPerLaunch pairs
     uint64_t group_tiles = 0;
     bool group_open = false;
+    std::vector<PairLaunch> group;  // the open group's launches (for a replay)
     auto close_group = [&]() -> int {
       if (!group_open) return 0;
       hipError_t e = hipEventRecord(ctx.ev1, st);
@@ -564,11 +614,28 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
           ctx.last_error = std::string("hit count: ") + hipGetErrorString(e);
           return -1;
         }
-        if (cnt > kHitCap / 2) {
+        if (cnt > kHitCap) {
+          // the group overflowed the buffer: resolve the hits recorded
+          // before it (an intact prefix), then replay its launches in parts
+          if (hipMemsetD32Async((hipDeviceptr_t)d_hc, (int)hits_base, 1, st) != hipSuccess) {
+            ctx.last_error = "hit count reset failed";
+            return -1;
+          }
+          if (hits_base > 0 && process_hits() != 0) return -1;
+          if (hipMemsetAsync(d_hc, 0, 4, st) != hipSuccess) return -1;
+          hits_base = 0;
+          ++replays;
+          for (const PairLaunch &q : group)
+            if (int rc = replay(q, 8)) return rc;
+        } else if (cnt > kHitCap / 2) {
           if (process_hits() != 0) return -1;
           if (hipMemsetAsync(d_hc, 0, 4, st) != hipSuccess) return -1;
+          hits_base = 0;
+        } else {
+          hits_base = cnt;
         }
       }
+      group.clear();
       return over_budget() ? 1 : 0;
     };
     for (const BBlock &bb : *blocks) {
@@ -591,6 +658,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
           return -1;
         }
         ++klaunch;
+        group.push_back(p);
         group_tiles += ntl;
         if (group_tiles >= tiles_per_launch) {
           if (int rc = close_group()) return rc;
@@ -606,6 +674,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
       break;
     }
     MBA_HIP(hipMemsetAsync(d_hc, 0, 4, st));
+    hits_base = 0;
     const CheckTargets tg = targets_for(t0, s, max_size, W);
     bool stopped = false;
     // block-at-a-time only pays where a size's sweep outweighs a launch and
@@ -636,32 +705,56 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
         if (int rc = process_hits()) return rc;
       }
     } else {
-      // one (op, split) block at a time, in rank order (plain) or in the
-      // order of the learned scores; stop once the answer is settled
+      // (op, split) blocks in rank order (plain), or splits in the order of
+      // the learned scores; stop once the answer is settled
       std::vector<BlockRef> blocks = binary_blocks(T, s);
       if (block_scores) {
         std::stable_sort(blocks.begin(), blocks.end(), [&](const BlockRef &x, const BlockRef &y) {
           return block_scores[x.op * 32 + x.split] > block_scores[y.op * 32 + y.split];
         });
       }
-      std::vector<char> done(blocks.size(), 0);
-      for (size_t bi = 0; bi < blocks.size() && !stopped; ++bi) {
-        const BlockRef &b = blocks[bi];
-        const int rc = run_pair(s, b.pair_j, 1u << b.inst, tg);
-        if (rc < 0) return MBA_ERR_HIP;
-        if (rc == 1) {
-          stopped = true;
-          break;
+      if (block_scores) {
+        // learned order: the splits (pairs {j, s-1-j}) in the order of their
+        // best-scored block, in rounds of one split, one split and the rest;
+        // each split is swept once with all operator instances fused (its B
+        // side is formed once), and a round that yields a verified solution
+        // ends the size (any solution of the minimal size is accepted)
+        std::vector<int> js;
+        for (const BlockRef &b : blocks)
+          if (std::find(js.begin(), js.end(), b.pair_j) == js.end()) js.push_back(b.pair_j);
+        for (size_t r0 = 0; r0 < js.size() && !stopped;) {
+          const size_t r1 = r0 < 2 ? r0 + 1 : js.size();
+          for (size_t k = r0; k < r1 && !stopped; ++k) {
+            const int rc = run_pair(s, js[k], kAllInst, tg);
+            if (rc < 0) return MBA_ERR_HIP;
+            stopped = rc == 1;
+          }
+          r0 = r1;
+          if (stopped) break;
+          if (int rc2 = process_hits()) return rc2;
+          MBA_HIP(hipMemsetAsync(d_hc, 0, 4, st));
+          hits_base = 0;
+          if (s >= min_size && found[s] != ~0ull) break;
         }
-        if (int rc2 = process_hits()) return rc2;
-        MBA_HIP(hipMemsetAsync(d_hc, 0, 4, st));
-        done[bi] = 1;
-        if (found[s] == ~0ull || s < min_size) continue;
-        if (block_scores) break;  // any verified tree of the minimal size
-        // rank order: settled when every block below the best rank is done;
-        // the neg block after '^' is complete already (unary chains)
-        const uint64_t prefix = b.end == T.op_offset(s, NEG) ? T.op_offset(s, ADD) : b.end;
-        if (found[s] < prefix) break;
+      } else {
+        // rank order: one block at a time; settled when every block below
+        // the best rank is done (the neg block after '^' is complete
+        // already: unary chains)
+        for (size_t bi = 0; bi < blocks.size() && !stopped; ++bi) {
+          const BlockRef &b = blocks[bi];
+          const int rc = run_pair(s, b.pair_j, 1u << b.inst, tg);
+          if (rc < 0) return MBA_ERR_HIP;
+          if (rc == 1) {
+            stopped = true;
+            break;
+          }
+          if (int rc2 = process_hits()) return rc2;
+          MBA_HIP(hipMemsetAsync(d_hc, 0, 4, st));
+          hits_base = 0;
+          if (found[s] == ~0ull || s < min_size) continue;
+          const uint64_t prefix = b.end == T.op_offset(s, NEG) ? T.op_offset(s, ADD) : b.end;
+          if (found[s] < prefix) break;
+        }
       }
     }
     if (stopped) {
@@ -681,6 +774,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   res.size_reached = reached;
   res.kernel_seconds = ktime;
   res.kernel_launches = klaunch;
+  res.hit_replays = replays;
   if (answer > 0) {
     fill_result(T, answer, found[answer], nv, inst.consts, W, res);
     status = MBA_FOUND;

@@ -702,11 +702,12 @@ __device__ __forceinline__ uint32_t window_tile(const AStage &S, uint32_t na,
   return acc;
 }
 
-// One operator instance (block-at-a-time mode, window form, any span).
-template <int RB>
-__device__ __forceinline__ uint32_t inst_tile(const AStage &S, uint32_t na, int inst,
-                                              const uint32_t *bl, uint32_t le, uint32_t lo,
-                                              uint32_t acc) {
+// One operator instance (block-at-a-time mode, window form, any span); the
+// instance is a template parameter so each loop holds only its own check.
+template <int INST, int RB>
+__device__ __forceinline__ uint32_t inst_tile_t(const AStage &S, uint32_t na,
+                                                const uint32_t *bl, uint32_t le, uint32_t lo,
+                                                uint32_t acc) {
   for (uint32_t i = 0; i < na; ++i) {
     const uint4 q0 = S.q0[i];
     const uint4 q1 = S.q1[i];
@@ -714,19 +715,32 @@ __device__ __forceinline__ uint32_t inst_tile(const AStage &S, uint32_t na, int 
     for (int k = 0; k < RB; ++k) {
       const uint32_t b = bl[k];
       uint32_t d0, d1;
-      switch (inst) {
-        case 0: d0 = (q0.x & b) - le; d1 = (q0.x & b) - lo; break;
-        case 1: d0 = (q0.x | b) - le; d1 = (q0.x | b) - lo; break;
-        case 2: d0 = (q0.x ^ b) - le; d1 = (q0.x ^ b) - lo; break;
-        case 3: d0 = b - q0.y; d1 = b - q0.z; break;
-        case 4: d0 = q0.x * b - le; d1 = q0.x * b - lo; break;
-        case 5: d0 = q0.w - b; d1 = q1.x - b; break;
-        default: d0 = b - q1.y; d1 = b - q1.z; break;
-      }
+      if (INST == 0) d0 = (q0.x & b) - le, d1 = (q0.x & b) - lo;
+      else if (INST == 1) d0 = (q0.x | b) - le, d1 = (q0.x | b) - lo;
+      else if (INST == 2) d0 = (q0.x ^ b) - le, d1 = (q0.x ^ b) - lo;
+      else if (INST == 3) d0 = b - q0.y, d1 = b - q0.z;
+      else if (INST == 4) d0 = q0.x * b - le, d1 = q0.x * b - lo;
+      else if (INST == 5) d0 = q0.w - b, d1 = q1.x - b;
+      else d0 = b - q1.y, d1 = b - q1.z;
       acc = min3u(acc, d0, d1);
     }
   }
   return acc;
+}
+
+template <int RB>
+__device__ __forceinline__ uint32_t inst_tile(const AStage &S, uint32_t na, int inst,
+                                              const uint32_t *bl, uint32_t le, uint32_t lo,
+                                              uint32_t acc) {
+  switch (inst) {  // wave-uniform
+    case 0: return inst_tile_t<0, RB>(S, na, bl, le, lo, acc);
+    case 1: return inst_tile_t<1, RB>(S, na, bl, le, lo, acc);
+    case 2: return inst_tile_t<2, RB>(S, na, bl, le, lo, acc);
+    case 3: return inst_tile_t<3, RB>(S, na, bl, le, lo, acc);
+    case 4: return inst_tile_t<4, RB>(S, na, bl, le, lo, acc);
+    case 5: return inst_tile_t<5, RB>(S, na, bl, le, lo, acc);
+    default: return inst_tile_t<6, RB>(S, na, bl, le, lo, acc);
+  }
 }
 
 // One tile of a pair check (shared by the per-instance and batched kernels).

@@ -11,7 +11,7 @@ import time
 
 import numpy as np
 
-from . import Mlp, block_class, features
+from . import Mlp, block_class, features, features_batch
 from .synth import SplitMix64, make_instance
 
 
@@ -49,10 +49,10 @@ def build_dataset(ctx, count, seed, sizes=(3, 13), n_vars=(2, 4), n_samples=(32,
         insts = sample_instances(seed + done, k, sizes, n_vars, n_samples)
         res = ctx.batch(insts, max_size=max_size)
         lab = label_results(res)
-        for inst, y in zip(insts, lab):
-            if y >= 0:
-                X.append(features(inst))
-                Y.append(y)
+        keep = [i for i, y in enumerate(lab) if y >= 0]
+        if keep:
+            X.extend(features_batch(ctx, [insts[i] for i in keep]))
+            Y.extend(lab[i] for i in keep)
         done += k
         if log:
             log(f"labelled {done}/{count} ({len(Y)} solved) {time.time() - t0:.1f}s")

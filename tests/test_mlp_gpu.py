@@ -120,3 +120,18 @@ def test_trained_network_proposes_solution_blocks_earlier(ctx):
     assert acc_t > acc_u + 0.1
     trained.close()
     untrained.close()
+
+
+def test_device_features_equal_host_features(ctx):
+    """mba_features_batch (one workgroup per instance on the device) gives the
+    host mba_features bit for bit: widths 32/64, 1..6 variables (the first
+    five are hashed), 1..128 samples."""
+    import simba_amd
+    from simba_amd import synth
+    note("device features")
+    insts = [synth.make_instance(700 + i, 1 + i % 6, (1, 5, 32, 64, 100, 128)[i % 6], 3 + i % 5,
+                                 64 if i % 3 else 32) for i in range(24)]
+    dev = simba_amd.features_batch(ctx, insts)
+    host = np.stack([simba_amd.features(x) for x in insts])
+    assert dev.shape == host.shape
+    assert np.array_equal(dev, host)

@@ -123,6 +123,26 @@ def test_simba_launch_configs_keep_answers(ctx, cfg):
         _check(ctx, inst, 1, 8, **cfg)
 
 
+# (instance index in the held-out learned evaluation set, max size, hit
+# buffer): launch groups whose window hits overflow the small buffer are
+# replayed in parts; the answers stay those of the default buffer
+REPLAY_CASES = [(10, 9, 256), (15, 11, 256), (31, 9, 1024), (34, 9, 4096)]
+
+
+@pytest.mark.parametrize("idx,max_size,cap", REPLAY_CASES)
+def test_hit_buffer_overflow_replay_keeps_answers(ctx, idx, max_size, cap):
+    from simba_amd.learn import sample_instances
+    inst = sample_instances(10**6 + 15, 64, (9, 19), (2, 4), (32, 32))[idx]
+    note(f"simba hit replay idx={idx} max={max_size} cap={cap}")
+    base = ctx.simba(inst, max_size=max_size)
+    got = ctx.simba(inst, max_size=max_size, hit_cap=cap)
+    assert got["hit_replays"] >= 1
+    key = lambda r: (r["status"], r["size"], r["rank"], r["expr"])
+    assert key(got) == key(base)
+    if base["status"] == 0:
+        assert verify_rpn_chars(got["rpn"], inst)
+
+
 def test_autotune_picks_a_config_with_identical_answers(ctx, tmp_path, monkeypatch):
     from simba_amd import autotune
     from simba_amd.synth import make_instance
