@@ -126,7 +126,7 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
   uint8_t *d_rpn = (uint8_t *)(d_misc + 64);
   MBA_HIP(hipMemsetAsync(d_misc, 0, 64 * 8, st));
 
-  VfbLevelArgs a;
+  VfbLevelArgs a{};
   std::memset(&a, 0, sizeof a);
   a.rows = ctx.vfb_rows.p;
   a.cap = cap;
@@ -143,6 +143,8 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
   a.right = ctx.vfb_right.as<int32_t>();
   a.row_op = ctx.vfb_flags.as<uint8_t>();
   a.probes = d_probes;
+  unsigned long long *d_new = (unsigned long long *)(d_misc + 5);
+  a.new_keys = d_new;
   MBA_HIP(vfb_launch_seed(W, a, L, st));
 
   std::vector<uint64_t> lstart(kMaxSize + 3, 0);
@@ -253,7 +255,10 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
     a.chunk_count = (uint32_t *)(ctx.b_queue.as<uint64_t>() + nchunks + 1);
     a.out_base = rows;
     const int g = (int)std::min<uint64_t>((uint64_t)grid, nchunks);
+    const bool last = lev == max_len;
+    a.last = last ? 1 : 0;
     MBA_HIP(hipMemsetAsync(d_found, 0xff, 8, st));
+    MBA_HIP(hipMemsetAsync(d_new, 0, 8, st));
     MBA_HIP(hipEventRecord(ctx.ev0, st));
     {
       const uint64_t pgroups = (npairs + kVfbThreads - 1) / kVfbThreads;
@@ -262,8 +267,8 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
                                       (int)pblocks.size(), npairs, gp, st));
     }
     MBA_HIP(vfb_launch_lookup_target(W, a, d_tmin, st));
-    uint64_t cells[4];
-    MBA_HIP(hipMemcpyAsync(cells, d_misc, 32, hipMemcpyDeviceToHost, st));
+    uint64_t cells[6];
+    MBA_HIP(hipMemcpyAsync(cells, d_misc, 48, hipMemcpyDeviceToHost, st));
     MBA_HIP(hipStreamSynchronize(st));
     if ((uint32_t)cells[2]) {
       status = MBA_OVERFLOW;
@@ -286,12 +291,16 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
         break;
       }
     }
-    const bool last = lev == max_len;
-    MBA_HIP(vfb_launch_flag(W, a, g, st));
-    MBA_HIP(vfb_launch_scan(a, nchunks, st));
-    uint64_t kept = 0;
-    MBA_HIP(hipMemcpyAsync(&kept, a.chunk_base + nchunks, 8, hipMemcpyDeviceToHost, st));
-    MBA_HIP(hipStreamSynchronize(st));
+    // kept rows: the flag pass marks each new fingerprint's first candidate
+    // and the scan places them; the hash-only last level only counts the
+    // fingerprints its insert pass added
+    uint64_t kept = cells[5];
+    if (!last) {
+      MBA_HIP(vfb_launch_flag(W, a, g, st));
+      MBA_HIP(vfb_launch_scan(a, nchunks, st));
+      MBA_HIP(hipMemcpyAsync(&kept, a.chunk_base + nchunks, 8, hipMemcpyDeviceToHost, st));
+      MBA_HIP(hipStreamSynchronize(st));
+    }
     bool overflow = false;
     if (!last) {
       if (rows + kept > cap) {

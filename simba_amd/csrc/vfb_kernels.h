@@ -50,6 +50,10 @@ struct VfbLevelArgs {
   int32_t *left, *right;  // child rows per row
   uint8_t *row_op;        // root op per row
   unsigned long long *probes;  // total probe steps (statistics)
+  unsigned long long *new_keys;  // fingerprints this pass inserted (new keys)
+  int last;               // hash-only level: no slot_of, companions of the
+                          // target's fingerprint only
+  int pad;
 };
 
 // The row pairs of a level: the unary block (rows of level L-1, ops ~ and
@@ -65,7 +69,6 @@ struct VfbPairBlock {
   int32_t pad;
 };
 
-hipError_t vfb_launch_insert(int width, const VfbLevelArgs &a, int grid, hipStream_t s);
 hipError_t vfb_launch_insert_pairs(int width, const VfbLevelArgs &a, const VfbPairBlock *d_pb,
                                    int npb, uint64_t npairs, int grid, hipStream_t s);
 hipError_t vfb_launch_flag(int width, const VfbLevelArgs &a, int grid, hipStream_t s);

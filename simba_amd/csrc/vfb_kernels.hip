@@ -113,9 +113,10 @@ __device__ __forceinline__ uint64_t signature_fp(const word *rows, uint64_t cap,
 // Open addressing with linear probing; returns the slot of `key`, or ~0 when
 // the probe sequence exceeds `limit` (table too full: the host reports a
 // capacity overflow), so every lane terminates.
+// `inserted` is set when this call filled an empty slot (the key is new).
 __device__ __forceinline__ uint64_t table_insert(uint64_t *keys, uint64_t mask,
                                                  uint64_t key, uint64_t &probes,
-                                                 uint64_t limit) {
+                                                 uint64_t limit, bool &inserted) {
   uint64_t s = (key ^ (key >> 29)) & mask;
   for (uint64_t step = 0;; ++step) {
     if (step >= limit) return ~0ull;
@@ -126,7 +127,11 @@ __device__ __forceinline__ uint64_t table_insert(uint64_t *keys, uint64_t mask,
       const uint64_t old = atomicCAS((unsigned long long *)&keys[2 * s],
                                      (unsigned long long)kVfbEmpty,
                                      (unsigned long long)key);
-      if (old == kVfbEmpty || old == key) return s;
+      if (old == kVfbEmpty) {
+        inserted = true;
+        return s;
+      }
+      if (old == key) return s;
     }
     s = (s + 1) & mask;
   }
@@ -144,63 +149,33 @@ __device__ __forceinline__ uint64_t table_find(const uint64_t *keys, uint64_t ma
   return ~0ull;
 }
 
-template <typename word>
-__global__ void __launch_bounds__(kVfbThreads) k_vfb_insert(VfbLevelArgs a) {
-  __shared__ VfbBlock s_blk[kVfbMaxBlocks];
-  __shared__ word s_tgt[kMaxSamples];
-  for (int i = threadIdx.x; i < a.nblocks; i += blockDim.x) s_blk[i] = a.blocks[i];
-  for (int i = threadIdx.x; i < a.n; i += blockDim.x) s_tgt[i] = ((const word *)a.target)[i];
-  __syncthreads();
-  const word *rows = (const word *)a.rows;
-  uint64_t probes = 0;
-  const uint64_t nchunks = (a.total + kVfbChunk - 1) / kVfbChunk;
-  for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-    for (int k = 0; k < kVfbChunk / kVfbThreads; ++k) {
-      const uint64_t idx = ch * kVfbChunk + (uint64_t)k * kVfbThreads + threadIdx.x;
-      if (idx >= a.total) break;
-      const Cand c = decode(s_blk, a.nblocks, idx);
-      bool eq;
-      const uint64_t fp = signature_fp<word>(rows, a.cap, a.n, c, s_tgt, eq);
-      if (fp >= kVfbTomb) {  // invalid key: dropped (hash_set.cuh:200-204)
-        a.slot_of[idx] = ~0u;
-        continue;
-      }
-      const uint64_t s = table_insert(a.keys, a.mask, fp, probes, a.max_probes);
-      if (s == ~0ull) {
-        atomicOr(a.fail, 1u);
-        a.slot_of[idx] = ~0u;
-        continue;
-      }
-      if (a.minidx[2 * s] > (a.tag | idx))
-    atomicMin((unsigned long long *)&a.minidx[2 * s], (unsigned long long)(a.tag | idx));
-      a.slot_of[idx] = (uint32_t)s;
-      if (eq) atomicMin(a.found, (unsigned long long)idx);
-    }
-  }
-  if (a.probes) atomicAdd(a.probes, (unsigned long long)probes);
-}
-
 // Pair-major form of the insert pass: one lane takes one (left, right) row
 // pair of a split (or one row for the unary ops) and produces every
 // candidate built on it -- & | ^ + * and both orders of '-' (or ~ and neg) --
 // so each pair of rows is read once for up to seven candidates.  Candidate
 // indices are those of the enumeration order (opbegin[op] + local index).
+// The last level is hash-only: no rows are stored, so only the target's
+// fingerprint needs its first occurrence (the companion) and the level's new
+// fingerprints are counted as the empty slots this pass fills.
 template <typename word>
 __device__ __forceinline__ void insert_one(const VfbLevelArgs &a, uint64_t idx, uint64_t fp,
-                                           bool eq, uint64_t &probes) {
+                                           bool eq, uint64_t tfp, uint64_t &probes,
+                                           uint64_t &fresh) {
   if (fp >= kVfbTomb) {  // invalid key: dropped (hash_set.cuh:200-204)
-    a.slot_of[idx] = ~0u;
+    if (!a.last) a.slot_of[idx] = ~0u;
     return;
   }
-  const uint64_t s = table_insert(a.keys, a.mask, fp, probes, a.max_probes);
+  bool ins = false;
+  const uint64_t s = table_insert(a.keys, a.mask, fp, probes, a.max_probes, ins);
   if (s == ~0ull) {
     atomicOr(a.fail, 1u);
-    a.slot_of[idx] = ~0u;
+    if (!a.last) a.slot_of[idx] = ~0u;
     return;
   }
-  if (a.minidx[2 * s] > (a.tag | idx))
+  fresh += ins ? 1u : 0u;
+  if ((!a.last || fp == tfp) && a.minidx[2 * s] > (a.tag | idx))
     atomicMin((unsigned long long *)&a.minidx[2 * s], (unsigned long long)(a.tag | idx));
-  a.slot_of[idx] = (uint32_t)s;
+  if (!a.last) a.slot_of[idx] = (uint32_t)s;
   if (eq) atomicMin(a.found, (unsigned long long)idx);
 }
 
@@ -209,11 +184,19 @@ __global__ void __launch_bounds__(kVfbThreads)
 k_vfb_insert_pairs(VfbLevelArgs a, const VfbPairBlock *pblocks, int npb, uint64_t npairs) {
   __shared__ VfbPairBlock s_pb[kVfbMaxPairBlocks];
   __shared__ word s_tgt[kMaxSamples];
+  __shared__ uint64_t s_tfp;
   for (int i = threadIdx.x; i < npb; i += blockDim.x) s_pb[i] = pblocks[i];
   for (int i = threadIdx.x; i < a.n; i += blockDim.x) s_tgt[i] = ((const word *)a.target)[i];
   __syncthreads();
+  if (threadIdx.x == 0) {  // the target's fingerprint
+    uint64_t t = 0;
+    for (int i = 0; i < a.n; ++i) t = fp_step<word>(t, s_tgt[i], i);
+    s_tfp = t;
+  }
+  __syncthreads();
+  const uint64_t tfp = s_tfp;
   const word *rows = (const word *)a.rows;
-  uint64_t probes = 0;
+  uint64_t probes = 0, fresh = 0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
     int b = 0;
@@ -232,8 +215,8 @@ k_vfb_insert_pairs(VfbLevelArgs a, const VfbPairBlock *pblocks, int npb, uint64_
         sn = fp_step<word>(sn, vn, i);
         sg = fp_step<word>(sg, vg, i);
       }
-      insert_one<word>(a, B.opbegin[NOT] + local, sn, en, probes);
-      insert_one<word>(a, B.opbegin[NEG] + local, sg, eg, probes);
+      insert_one<word>(a, B.opbegin[NOT] + local, sn, en, tfp, probes, fresh);
+      insert_one<word>(a, B.opbegin[NEG] + local, sg, eg, tfp, probes, fresh);
       continue;
     }
     const uint64_t q = local / B.nr;
@@ -252,15 +235,16 @@ k_vfb_insert_pairs(VfbLevelArgs a, const VfbPairBlock *pblocks, int npb, uint64_
         sd[k] = fp_step<word>(sd[k], v[k], i);
       }
     }
-    insert_one<word>(a, B.opbegin[AND] + local, sd[0], eq[0], probes);
-    insert_one<word>(a, B.opbegin[OR] + local, sd[1], eq[1], probes);
-    insert_one<word>(a, B.opbegin[XOR] + local, sd[2], eq[2], probes);
-    insert_one<word>(a, B.opbegin[ADD] + local, sd[3], eq[3], probes);
-    insert_one<word>(a, B.opbegin[MUL] + local, sd[4], eq[4], probes);
-    insert_one<word>(a, B.opbegin[SUB] + 2 * local, sd[5], eq[5], probes);
-    insert_one<word>(a, B.opbegin[SUB] + 2 * local + 1, sd[6], eq[6], probes);
+    insert_one<word>(a, B.opbegin[AND] + local, sd[0], eq[0], tfp, probes, fresh);
+    insert_one<word>(a, B.opbegin[OR] + local, sd[1], eq[1], tfp, probes, fresh);
+    insert_one<word>(a, B.opbegin[XOR] + local, sd[2], eq[2], tfp, probes, fresh);
+    insert_one<word>(a, B.opbegin[ADD] + local, sd[3], eq[3], tfp, probes, fresh);
+    insert_one<word>(a, B.opbegin[MUL] + local, sd[4], eq[4], tfp, probes, fresh);
+    insert_one<word>(a, B.opbegin[SUB] + 2 * local, sd[5], eq[5], tfp, probes, fresh);
+    insert_one<word>(a, B.opbegin[SUB] + 2 * local + 1, sd[6], eq[6], tfp, probes, fresh);
   }
   if (a.probes) atomicAdd(a.probes, (unsigned long long)probes);
+  if (fresh) atomicAdd(a.new_keys, (unsigned long long)fresh);
 }
 
 // keep = first occurrence in this level of a fingerprint unseen before.
@@ -367,7 +351,8 @@ __global__ void k_vfb_seed(VfbLevelArgs a, int L) {
   for (int i = 0; i < a.n; ++i) seed = fp_step<word>(seed, rows[(uint64_t)i * a.cap + v], i);
   if (seed >= kVfbTomb) return;
   uint64_t probes = 0;
-  const uint64_t s = table_insert(a.keys, a.mask, seed, probes, a.max_probes);
+  bool ins = false;
+  const uint64_t s = table_insert(a.keys, a.mask, seed, probes, a.max_probes, ins);
   if (s == ~0ull) {
     atomicOr(a.fail, 1u);
     return;
@@ -440,14 +425,6 @@ __global__ void k_fill_u64(uint64_t *p, uint64_t n, uint64_t v) {
 }
 
 // --- launchers -------------------------------------------------------------
-hipError_t vfb_launch_insert(int width, const VfbLevelArgs &a, int grid, hipStream_t s) {
-  if (width == 32)
-    hipLaunchKernelGGL(k_vfb_insert<uint32_t>, dim3(grid), dim3(kVfbThreads), 0, s, a);
-  else
-    hipLaunchKernelGGL(k_vfb_insert<uint64_t>, dim3(grid), dim3(kVfbThreads), 0, s, a);
-  return hipGetLastError();
-}
-
 hipError_t vfb_launch_insert_pairs(int width, const VfbLevelArgs &a, const VfbPairBlock *d_pb,
                                    int npb, uint64_t npairs, int grid, hipStream_t s) {
   if (width == 32)
