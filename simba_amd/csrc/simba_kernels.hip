@@ -20,6 +20,17 @@
 
 namespace mba {
 
+// Table memory is device (global) memory.  Its pointer reaches the kernels
+// through TableView as a generic pointer; routing it through the global
+// address space lets the compiler emit global rather than flat accesses
+// (flat loads also count against the LDS/scalar wait counter).
+template <typename T>
+__device__ __forceinline__ T *global_ptr(const void *p) {
+  typedef T __attribute__((address_space(1))) *gp;
+  return (T *)(gp)(uintptr_t)p;
+}
+
+
 template <typename word>
 struct WordOf;
 
@@ -34,7 +45,7 @@ __device__ __forceinline__ word build_value(const TableView *__restrict__ tv, in
 
 template <typename word>
 __device__ word value_of(const TableView *__restrict__ tv, int m, uint64_t r) {
-  const word *V = (const word *)tv->base;
+  const word *V = global_ptr<const word>(tv->base);
   const int Kt = tv->Kt;
   if (m <= Kt) return V[tv->off[m] + r];
   // one level above the tables: the root's children are tabulated
@@ -164,7 +175,7 @@ __device__ word eval_tree(const TableView *__restrict__ tv, int size, uint64_t r
 template <typename word>
 __device__ __forceinline__ word build_value(const TableView *__restrict__ tv, int m,
                                             uint64_t r) {
-  const word *V = (const word *)tv->base;
+  const word *V = global_ptr<const word>(tv->base);
   uint64_t x = r;
   int op = 0;
   while (x >= tv->T[m][op]) x -= tv->T[m][op++];
@@ -222,7 +233,7 @@ template <typename word>
 __global__ void __launch_bounds__(1024)
 k_table_small(const TableView *__restrict__ tv, int m_lo, int m_hi) {
   for (int m = m_lo; m <= m_hi; ++m) {
-    word *out = (word *)tv->base + tv->off[m];
+    word *out = global_ptr<word>(tv->base) + tv->off[m];
     const uint64_t n = tv->T[m][8];
     for (uint64_t r = threadIdx.x; r < n; r += blockDim.x) out[r] = build_value<word>(tv, m, r);
     __syncthreads();
@@ -317,7 +328,7 @@ template <typename word, int RB>
 __device__ __forceinline__ void gen_b_tile(const TableView *__restrict__ tv, int m,
                                            uint64_t first, uint64_t lane_b0, word *bv,
                                            const bool *ok) {
-  const word *V = (const word *)tv->base;
+  const word *V = global_ptr<const word>(tv->base);
   uint64_t x = first;
   int op = 0;
   while (x >= tv->T[m][op]) x -= tv->T[m][op++];
@@ -371,7 +382,8 @@ __device__ __forceinline__ void gen_b_tile(const TableView *__restrict__ tv, int
 
 // Per-A values of a tile, staged in LDS by the workgroup (one A value per
 // lane) and read back as wave-uniform broadcasts inside the loop:
-//   exact mode:  q0 = {a, a ^ t, t - a, a - t},  q1 = {t + a, bit-op gap, -, -}
+//   exact mode:  q0 = {a, a ^ t, t - a, a - t},
+//                q1 = {t + a, bit-op gap, product residue, shift | kind << 8}
 //   window mode: q0 = {a, lo_e - a, lo_o - a, a - lo_e},
 //                q1 = {a - lo_o, a + lo_e, a + lo_o, -}
 // (low halves; t = lo_e in exact mode).
@@ -379,6 +391,29 @@ struct AStage {
   uint4 q0[kPairTA];
   uint4 q1[kPairTA];
 };
+
+// a * b = t (mod 2^32) as a compare: with a = 2^k a' (a' odd), it holds iff
+// 2^k divides t and b = (t >> k) * a'^-1 (mod 2^(32-k)), i.e. (b << k) ==
+// residue << k.  kind 0: no b fits, 1: compare, 2: every b (a = t = 0).
+// The inverse of an odd a' by Newton's iteration x <- x (2 - a' x), which
+// doubles the correct low bits from the 3 of x = a' (a'^2 = 1 mod 8).
+__device__ __forceinline__ uint32_t mul_residue(uint32_t a, uint32_t t, uint32_t &shift_kind) {
+  if (a == 0u) {
+    shift_kind = (t == 0u ? 2u : 0u) << 8;
+    return 0u;
+  }
+  const uint32_t k = (uint32_t)__builtin_ctz(a);
+  if (k != 0u && (t & ((1u << k) - 1u)) != 0u) {
+    shift_kind = 0u;
+    return 0u;
+  }
+  const uint32_t ao = a >> k;
+  uint32_t inv = ao;
+#pragma unroll
+  for (int it = 0; it < 4; ++it) inv *= 2u - ao * inv;
+  shift_kind = k | (1u << 8);
+  return ((t >> k) * inv) << k;
+}
 
 // a & b = t needs t inside a, a | b = t needs a inside t: zero when either
 // instance can hit for some b.
@@ -419,8 +454,10 @@ __device__ __forceinline__ bool stage_a(AStage &S, const TableView *__restrict__
     const uint32_t al = lo32(av);
     if (exact) {
       const uint32_t gap = bitops_gap(al, le);
+      uint32_t shift_kind;
+      const uint32_t res = mul_residue(al, le, shift_kind);
       S.q0[i] = make_uint4(al, al ^ le, le - al, al - le);
-      S.q1[i] = make_uint4(le + al, gap, 0u, 0u);
+      S.q1[i] = make_uint4(le + al, gap, res, shift_kind);
       bit = gap == 0u;
     } else {
       const bool wb = window_bitops(al, le, span) || window_bitops(al, lo, span);
@@ -584,7 +621,7 @@ __device__ __forceinline__ word b_full(const TableView *__restrict__ tv,
     uint64_t ra, rb;
     desc_coords<RB>(p.bb, bt, k, ra, rb);
     rank = p.bb.start + p.bb.stride * (ra * p.bb.nB + rb);
-    return desc_value<word>(p.bb, (const word *)tv->base, ra, rb);
+    return desc_value<word>(p.bb, global_ptr<const word>(tv->base), ra, rb);
   }
   const uint64_t local = bt * (kPairThreads * RB) + threadIdx.x + (uint64_t)k * kPairThreads;
   if (BSRC == kBGeneric) {
@@ -602,7 +639,7 @@ __device__ __forceinline__ void b_lo(const TableView *__restrict__ tv,
                                      const word *__restrict__ B, const PairLaunch &p,
                                      uint64_t bt, uint32_t *bl, bool *ok) {
   if (BSRC == kBDesc) {
-    desc_lo<word, RB>(p.bb, (const word *)tv->base, bt, bl, ok);
+    desc_lo<word, RB>(p.bb, global_ptr<const word>(tv->base), bt, bl, ok);
     return;
   }
   const uint64_t b0 = bt * (kPairThreads * RB) + threadIdx.x;
@@ -635,18 +672,31 @@ __device__ __forceinline__ void b_lo(const TableView *__restrict__ tv,
 
 // ---- the fused checks -------------------------------------------------------
 // Exact mode: each compare's lane mask is or-ed into one wave-wide mask on
-// the scalar unit (one VALU compare + one scalar or per candidate).
+// the scalar unit (one VALU compare + one scalar or per candidate); the
+// product is a compare of b with the A value's residue (mul_residue).
 template <bool SAME, int RB>
-__device__ __forceinline__ void exact_a(uint64_t &m, const uint4 q0, uint32_t q_rsub,
-                                        const uint32_t *bl, uint32_t tl) {
+__device__ __forceinline__ void exact_a(uint64_t &m, const uint4 q0, const uint4 q1,
+                                        const uint32_t *bl) {
 #pragma unroll
   for (int k = 0; k < RB; ++k) {
     const uint32_t b = bl[k];
     m |= __builtin_amdgcn_ballot_w64(b == q0.y);            // a ^ b = t
     m |= __builtin_amdgcn_ballot_w64(b == q0.z);            // a + b = t
     m |= __builtin_amdgcn_ballot_w64(b == q0.w);            // a - b = t
-    if (!SAME) m |= __builtin_amdgcn_ballot_w64(b == q_rsub);  // b - a = t
-    m |= __builtin_amdgcn_ballot_w64(q0.x * b == tl);       // a * b = t
+    if (!SAME) m |= __builtin_amdgcn_ballot_w64(b == q1.x);  // b - a = t
+  }
+  const uint32_t sk = uniform(q1.w), res = q1.z;
+  if ((sk >> 8) == 1u) {                                    // a * b = t
+    const uint32_t sh = sk & 0xffu;
+    if (sh == 0u) {
+#pragma unroll
+      for (int k = 0; k < RB; ++k) m |= __builtin_amdgcn_ballot_w64(bl[k] == res);
+    } else {
+#pragma unroll
+      for (int k = 0; k < RB; ++k) m |= __builtin_amdgcn_ballot_w64((bl[k] << sh) == res);
+    }
+  } else if ((sk >> 8) == 2u) {
+    m = ~0ull;  // a = t = 0: every b; the re-walk checks them exactly
   }
 }
 
@@ -656,7 +706,7 @@ __device__ __forceinline__ bool exact_tile(const AStage &S, uint32_t na, const u
   uint64_t m = 0;
   uint32_t i;
 #pragma unroll 1
-  for (i = 0; i < na; ++i) exact_a<SAME, RB>(m, S.q0[i], SAME ? 0u : S.q1[i].x, bl, tl);
+  for (i = 0; i < na; ++i) exact_a<SAME, RB>(m, S.q0[i], S.q1[i], bl);
   if (bitops) {  // rare: & and | for the A values that admit them
     for (i = 0; i < na; ++i) {
       if (S.q1[i].y != 0u) continue;
@@ -831,7 +881,7 @@ template <typename word>
 __global__ void k_leaf_scan(const TableView *__restrict__ tv, int L, uint64_t lo_e64,
                             uint64_t lo_o64, uint64_t span64, HitRec *hits,
                             uint32_t *hit_count, uint32_t hit_cap) {
-  const word *V = (const word *)tv->base + tv->off[1];
+  const word *V = global_ptr<const word>(tv->base) + tv->off[1];
   const int i = threadIdx.x;
   if (i < L) {
     const word c = V[i];
@@ -1174,7 +1224,7 @@ template <typename word>
 __device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J, uint64_t lt,
                            uint64_t &staged, bool &bitops, HitRec *hits, uint32_t *hc,
                            uint32_t cap) {
-  const word *base = (const word *)tv->base;
+  const word *base = global_ptr<const word>(tv->base);
   PairLaunch p;
   static_cast<PairCore &>(p) = J.p;  // p.bb is not read by the table/stepped sources
   const word *A = base + (p.gen_a ? 0 : tv->off[p.j]);
@@ -1249,7 +1299,7 @@ k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jo
       const TableView *tv = tvs + J.inst;
       const uint64_t lt = t - J.tile0;
       if (J.kind == 0) {  // table build: V[m] ranks [lt*kBuildTile, +kBuildTile)
-        word *out = (word *)tv->base + tv->off[J.p.m];
+        word *out = global_ptr<word>(tv->base) + tv->off[J.p.m];
         const uint64_t r0 = lt * kBuildTile;
         const uint64_t r1 = r0 + kBuildTile < J.p.nB ? r0 + kBuildTile : J.p.nB;
         for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x)
@@ -1257,7 +1307,7 @@ k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jo
       } else if (J.kind == 1) {
         batch_pair<word>(S, tv, J, lt, staged, bitops, hits, hit_count, hit_cap);
       } else {  // leaves against the windows of base size 1
-        const word *V = (const word *)tv->base + tv->off[1];
+        const word *V = global_ptr<const word>(tv->base) + tv->off[1];
         if ((int)threadIdx.x < tv->L) {
           const word c = V[threadIdx.x];
           if (in_window(c, (word)J.lo_e, (word)J.lo_o, (word)J.span))
