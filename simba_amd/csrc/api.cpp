@@ -73,6 +73,9 @@ void mba_ctx_destroy(mba_ctx *c) {
                     &c->c.mlp_misc, &c->c.f_in, &c->c.f_out, &c->c.f_meta, &c->c.f_x};
   for (DevBuf *b : bufs) b->release();
   if (c->c.pinned) (void)hipHostFree(c->c.pinned);
+  if (c->c.b_gexec) (void)hipGraphExecDestroy(c->c.b_gexec);
+  if (c->c.b_graph) (void)hipGraphDestroy(c->c.b_graph);
+  if (c->c.b_pinned) (void)hipHostFree(c->c.b_pinned);
   (void)hipEventDestroy(c->c.ev0);
   (void)hipEventDestroy(c->c.ev1);
   (void)hipStreamDestroy(c->c.stream);

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <thread>
 #include <vector>
 
 #include "context.h"
@@ -31,11 +32,28 @@ constexpr uint64_t kBatchTableBytes = 32ull << 30;
 struct InstState {
   int L = 0, n = 0, fs = 0, Kt = 1;
   uint64_t t0 = 0;
+  uint64_t elems = 0;            // value-table elements V[1..Kt]
   const CountTable *T = nullptr;
-  std::vector<uint64_t> found;
+  uint64_t *found = nullptr;     // verified minimum rank per size
   bool active = true;
   int reached = 0;
 };
+
+// Runs f(lo, hi) over [0, n) in up to kSetupThreads host threads.
+constexpr size_t kSetupThreads = 8;
+template <typename F>
+void parallel_ranges(size_t n, F f) {
+  const size_t nt = n >= 1024 ? kSetupThreads : 1;
+  if (nt == 1) {
+    f((size_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (size_t t = 0; t < nt; ++t) th.emplace_back(f, n * t / nt, n * (t + 1) / nt);
+  for (auto &x : th) x.join();
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 }  // namespace
 
@@ -58,21 +76,16 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   const size_t wb = (size_t)W / 8;
   std::map<int, CountTable> tables_by_L;
   std::vector<InstState> S(count);
-  std::vector<TableView> tvs(count);
-  std::vector<BatchInstData> idat(count);
-  std::vector<uint64_t> leafvals, outputs;
+  std::vector<uint64_t> found_all(count * (kMaxSize + 2), ~0ull);
+  std::vector<uint64_t> leaf_off(count), out_off(count);
   const uint64_t per_budget =
       std::max<uint64_t>((o.table_bytes ? o.table_bytes : kBatchTableBytes) / count, 1 << 16);
-  uint64_t pool_elems = 0;
+  // pass 1 (in order): limits, count tables per leaf count, sample offsets
+  uint64_t nleaf = 0, nout = 0;
   for (size_t i = 0; i < count; ++i) {
     const mba_instance &in = insts[i];
     const int nv = (int)in.n_vars, nc = (int)in.n_consts, n = (int)in.n_samples;
     const int L = nv + nc;
-    mba_result &r = results[i];
-    std::memset(&r, 0, sizeof r);
-    r.status = MBA_NOT_FOUND;
-    r.op = -1;
-    std::snprintf(r.expr, sizeof r.expr, "Not found !");
     if ((int)in.width != W || L < 1 || L > kMaxLeaves || n < 1 || n > kMaxSamples ||
         !in.inputs || !in.outputs || (nc > 0 && !in.consts)) {
       ctx.last_error = "batch: instance outside supported limits or mixed widths";
@@ -91,79 +104,109 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     st.L = L;
     st.n = n;
     st.T = &it->second;
-    st.found.assign(kMaxSize + 2, ~0ull);
-    std::vector<uint64_t> lv((size_t)n * L), outv(n);
-    for (int s = 0; s < n; ++s) {
-      for (int v = 0; v < nv; ++v) lv[(size_t)s * L + v] = in.inputs[(size_t)s * nv + v] & M;
-      for (int c = 0; c < nc; ++c) lv[(size_t)s * L + nv + c] = in.consts[c] & M;
-      outv[s] = in.outputs[s] & M;
-    }
-    st.fs = pick_filter_sample(lv, outv, n, L, W);
-    st.t0 = outv[st.fs];
-    // V[max_size - 2] is only ever the B side of leaves: formed per tile
-    st.Kt = std::max(1, std::min(choose_kt(*st.T, std::max(max_size, 1), W, per_budget),
-                                 max_size - 3) -
-                            std::max(0, o.table_delta));
-    TableView &tv = tvs[i];
-    std::memset(&tv, 0, sizeof tv);
-    uint64_t e = 0;
-    for (int m = 1; m <= st.Kt; ++m) {
-      tv.off[m] = e;
-      e += st.T->t[m][8];
-    }
-    std::memcpy(tv.T, st.T->t, sizeof tv.T);
-    tv.Kt = st.Kt;
-    tv.L = L;
-    tv.base = (const void *)(uintptr_t)(pool_elems * wb);  // patched to a pointer below
-    pool_elems += e;
-    idat[i].leaf_off = leafvals.size();
-    idat[i].out_off = outputs.size();
-    idat[i].n_samples = (uint32_t)n;
-    idat[i].fs = (uint32_t)st.fs;
-    leafvals.insert(leafvals.end(), lv.begin(), lv.end());
-    outputs.insert(outputs.end(), outv.begin(), outv.end());
-    if (max_size < min_size) st.active = false;
+    st.found = found_all.data() + i * (kMaxSize + 2);
+    st.active = max_size >= min_size;
+    leaf_off[i] = nleaf;
+    out_off[i] = nout;
+    nleaf += (uint64_t)n * L;
+    nout += (uint64_t)n;
   }
-
+  // pinned staging (kept in the context, grown when needed): step header at
+  // 0, hit and slot counts at 256, then the jobs of a step, the instances'
+  // table views, leaf values, outputs and sample layouts
+  size_t job_cap = 0;
+  for (size_t i = 0; i < count; ++i) job_cap += 2 + (size_t)std::max(0, (max_size - 1) / 2);
+  const size_t off_jobs = 512;
+  const size_t off_tvs = align256(off_jobs + sizeof(BatchJob) * job_cap);
+  const size_t off_leaf = align256(off_tvs + sizeof(TableView) * count);
+  const size_t off_out = align256(off_leaf + 8 * nleaf);
+  const size_t off_idat = align256(off_out + 8 * nout);
+  const size_t pin_bytes = off_idat + sizeof(BatchInstData) * count;
   hipStream_t stq = ctx.stream;
+  if (ctx.b_pinned_bytes < pin_bytes) {
+    MBA_HIP(hipStreamSynchronize(stq));
+    if (ctx.b_pinned) MBA_HIP(hipHostFree(ctx.b_pinned));
+    ctx.b_pinned = nullptr;
+    ctx.b_pinned_bytes = 0;
+    const size_t want = pin_bytes + pin_bytes / 4;
+    MBA_HIP(hipHostMalloc(&ctx.b_pinned, want, hipHostMallocDefault));
+    ctx.b_pinned_bytes = want;
+  }
+  char *pin = (char *)ctx.b_pinned;
+  BatchStep *h_step = (BatchStep *)pin;
+  uint32_t *h_hc = (uint32_t *)(pin + 256);
+  BatchJob *h_jobs = (BatchJob *)(pin + off_jobs);
+  TableView *tvs = (TableView *)(pin + off_tvs);
+  uint64_t *leafvals = (uint64_t *)(pin + off_leaf);
+  uint64_t *outputs = (uint64_t *)(pin + off_out);
+  BatchInstData *idat = (BatchInstData *)(pin + off_idat);
+  // pass 2 (host threads): samples, filter sample, tables, table view
+  parallel_ranges(count, [&](size_t lo, size_t hi) {
+    for (size_t i = lo; i < hi; ++i) {
+      const mba_instance &in = insts[i];
+      InstState &st = S[i];
+      const int nv = (int)in.n_vars, nc = (int)in.n_consts, n = st.n, L = st.L;
+      mba_result &r = results[i];
+      std::memset(&r, 0, sizeof r);
+      r.status = MBA_NOT_FOUND;
+      r.op = -1;
+      std::snprintf(r.expr, sizeof r.expr, "Not found !");
+      uint64_t *lv = leafvals + leaf_off[i], *outv = outputs + out_off[i];
+      for (int smp = 0; smp < n; ++smp) {
+        for (int v = 0; v < nv; ++v)
+          lv[(size_t)smp * L + v] = in.inputs[(size_t)smp * nv + v] & M;
+        for (int c = 0; c < nc; ++c) lv[(size_t)smp * L + nv + c] = in.consts[c] & M;
+        outv[smp] = in.outputs[smp] & M;
+      }
+      st.fs = pick_filter_sample(lv, outv, n, L, W);
+      st.t0 = outv[st.fs];
+      // V[max_size - 2] is only ever the B side of leaves: formed per tile
+      st.Kt = std::max(1, std::min(choose_kt(*st.T, std::max(max_size, 1), W, per_budget),
+                                   max_size - 3) -
+                              std::max(0, o.table_delta));
+      TableView &tv = tvs[i];
+      std::memset(&tv, 0, sizeof tv);
+      uint64_t e = 0;
+      for (int m = 1; m <= st.Kt; ++m) {
+        tv.off[m] = e;
+        e += st.T->t[m][8];
+      }
+      st.elems = e;
+      std::memcpy(tv.T, st.T->t, sizeof tv.T);
+      tv.Kt = st.Kt;
+      tv.L = L;
+      idat[i].leaf_off = leaf_off[i];
+      idat[i].out_off = out_off[i];
+      idat[i].n_samples = (uint32_t)n;
+      idat[i].fs = (uint32_t)st.fs;
+    }
+  });
+  // pass 3: the value-table pool, one slice per instance
+  uint64_t pool_elems = 0;
+  for (size_t i = 0; i < count; ++i) pool_elems += S[i].elems;
   MBA_HIP(ctx.b_data.ensure(std::max<size_t>(pool_elems * wb, 256)));
-  for (size_t i = 0; i < count; ++i)
-    tvs[i].base = ctx.b_data.as<char>() + (uintptr_t)tvs[i].base;
+  {
+    uint64_t pe = 0;
+    for (size_t i = 0; i < count; ++i) {
+      tvs[i].base = ctx.b_data.as<char>() + pe * wb;
+      pe += S[i].elems;
+    }
+  }
   MBA_HIP(ctx.b_inst.ensure(sizeof(TableView) * count));
-  MBA_HIP(ctx.leafvals.ensure(8 * leafvals.size()));
-  MBA_HIP(ctx.outputs.ensure(8 * outputs.size()));
+  MBA_HIP(ctx.leafvals.ensure(8 * std::max<uint64_t>(nleaf, 1)));
+  MBA_HIP(ctx.outputs.ensure(8 * std::max<uint64_t>(nout, 1)));
   MBA_HIP(ctx.b_results.ensure(sizeof(BatchInstData) * count));
-  MBA_HIP(hipMemcpyAsync(ctx.b_inst.p, tvs.data(), sizeof(TableView) * count,
-                         hipMemcpyHostToDevice, stq));
-  MBA_HIP(hipMemcpyAsync(ctx.leafvals.p, leafvals.data(), 8 * leafvals.size(),
-                         hipMemcpyHostToDevice, stq));
-  MBA_HIP(hipMemcpyAsync(ctx.outputs.p, outputs.data(), 8 * outputs.size(),
-                         hipMemcpyHostToDevice, stq));
-  MBA_HIP(hipMemcpyAsync(ctx.b_results.p, idat.data(), sizeof(BatchInstData) * count,
+  MBA_HIP(hipMemcpyAsync(ctx.b_inst.p, tvs, sizeof(TableView) * count, hipMemcpyHostToDevice,
+                         stq));
+  MBA_HIP(hipMemcpyAsync(ctx.leafvals.p, leafvals, 8 * nleaf, hipMemcpyHostToDevice, stq));
+  MBA_HIP(hipMemcpyAsync(ctx.outputs.p, outputs, 8 * nout, hipMemcpyHostToDevice, stq));
+  MBA_HIP(hipMemcpyAsync(ctx.b_results.p, idat, sizeof(BatchInstData) * count,
                          hipMemcpyHostToDevice, stq));
   // V[1] of every instance: leaf values of its filter sample (one kernel)
   MBA_HIP(launch_batch_leaves(W, ctx.b_inst.as<TableView>(), ctx.leafvals.as<uint64_t>(),
                               ctx.b_results.as<BatchInstData>(), (uint32_t)count, stq));
 
   // ---- per-step graph: upload jobs, reset counters, drain, read hit count --
-  // jobs per instance and step: a leaf scan or a table build, plus at most
-  // every split of the size
-  size_t job_cap = 0;
-  for (size_t i = 0; i < count; ++i) job_cap += 2 + (size_t)std::max(0, (max_size - 1) / 2);
-  BatchJob *h_jobs = nullptr;
-  BatchStep *h_step = nullptr;
-  uint32_t *h_hc = nullptr;
-  MBA_HIP(hipHostMalloc((void **)&h_jobs, sizeof(BatchJob) * job_cap, hipHostMallocDefault));
-  MBA_HIP(hipHostMalloc((void **)&h_step, sizeof(BatchStep) + 64, hipHostMallocDefault));
-  MBA_HIP(hipHostMalloc((void **)&h_hc, 64, hipHostMallocDefault));
-  struct PinnedGuard {
-    void *a, *b, *c;
-    ~PinnedGuard() {
-      (void)hipHostFree(a);
-      (void)hipHostFree(b);
-      (void)hipHostFree(c);
-    }
-  } guard{h_jobs, h_step, h_hc};
   MBA_HIP(ctx.b_queue.ensure(sizeof(BatchJob) * job_cap + 256));
   MBA_HIP(ctx.hits.ensure(sizeof(HitRec) * kBatchHitCap));
   MBA_HIP(ctx.hit_count.ensure(256));
@@ -187,25 +230,30 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   const int mult = o.wg_per_cu > 0 ? o.wg_per_cu : 8;  // workgroups per CU
   const int grid = ctx.cu_count * mult;
 
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  // the step graph names these buffers and the launch shape: it is captured
+  // again only when one of them changed since the previous call
+  const uint64_t key[10] = {(uint64_t)W, (uint64_t)grid, (uint64_t)(uintptr_t)d_step,
+                            (uint64_t)(uintptr_t)d_jobs, (uint64_t)(uintptr_t)d_tvs,
+                            (uint64_t)(uintptr_t)d_head, (uint64_t)(uintptr_t)d_hits,
+                            (uint64_t)(uintptr_t)d_hc, (uint64_t)(uintptr_t)h_step,
+                            (uint64_t)(uintptr_t)h_hc};
   MBA_HIP(hipStreamSynchronize(stq));
-  MBA_HIP(hipStreamBeginCapture(stq, hipStreamCaptureModeThreadLocal));
-  MBA_HIP(hipMemcpyAsync(d_step, h_step, sizeof(BatchStep), hipMemcpyHostToDevice, stq));
-  MBA_HIP(launch_batch_reset(d_head, d_hc, stq));
-  MBA_HIP(launch_batch_step(W, d_step, d_jobs, d_tvs, d_head, d_hits, d_hc, kBatchHitCap,
-                            grid, stq));
-  MBA_HIP(hipMemcpyAsync(h_hc, d_hc, 4, hipMemcpyDeviceToHost, stq));
-  MBA_HIP(hipStreamEndCapture(stq, &graph));
-  MBA_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-  struct GraphGuard {
-    hipGraph_t g;
-    hipGraphExec_t e;
-    ~GraphGuard() {
-      if (e) (void)hipGraphExecDestroy(e);
-      if (g) (void)hipGraphDestroy(g);
-    }
-  } gguard{graph, gexec};
+  if (!ctx.b_gexec || std::memcmp(key, ctx.b_graph_key, sizeof key) != 0) {
+    if (ctx.b_gexec) MBA_HIP(hipGraphExecDestroy(ctx.b_gexec));
+    if (ctx.b_graph) MBA_HIP(hipGraphDestroy(ctx.b_graph));
+    ctx.b_gexec = nullptr;
+    ctx.b_graph = nullptr;
+    MBA_HIP(hipStreamBeginCapture(stq, hipStreamCaptureModeThreadLocal));
+    MBA_HIP(hipMemcpyAsync(d_step, h_step, sizeof(BatchStep), hipMemcpyHostToDevice, stq));
+    MBA_HIP(launch_batch_reset(d_head, d_hc, stq));
+    MBA_HIP(launch_batch_step(W, d_step, d_jobs, d_tvs, d_head, d_hits, d_hc, kBatchHitCap,
+                              grid, stq));
+    MBA_HIP(hipMemcpyAsync(h_hc, d_hc, 4, hipMemcpyDeviceToHost, stq));
+    MBA_HIP(hipStreamEndCapture(stq, &ctx.b_graph));
+    MBA_HIP(hipGraphInstantiate(&ctx.b_gexec, ctx.b_graph, nullptr, nullptr, 0));
+    std::memcpy(ctx.b_graph_key, key, sizeof key);
+  }
+  hipGraphExec_t gexec = ctx.b_gexec;
 
   std::vector<HitRec> hrec;
   std::vector<VerifyCand> vc;

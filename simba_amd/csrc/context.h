@@ -54,6 +54,13 @@ struct Ctx {
   // pinned host staging for the per-step read backs of the search drivers
   void *pinned = nullptr;
   static constexpr size_t kPinnedBytes = 4096;
+  // batched search: pinned job staging and the captured step graph, kept
+  // across calls while the buffers the graph names are unchanged
+  void *b_pinned = nullptr;
+  size_t b_pinned_bytes = 0;
+  hipGraph_t b_graph = nullptr;
+  hipGraphExec_t b_gexec = nullptr;
+  uint64_t b_graph_key[10] = {};
 };
 
 using Clock = std::chrono::steady_clock;

@@ -155,8 +155,8 @@ static void b_blocks(const CountTable &T, int Kt, const uint64_t *off, int m, ui
   }
 }
 
-int pick_filter_sample(const std::vector<uint64_t> &lv, const std::vector<uint64_t> &outv,
-                       int n, int L, int W, int skip) {
+int pick_filter_sample(const uint64_t *lv, const uint64_t *outv, int n, int L, int W,
+                       int skip) {
   const uint64_t M = wmask(W);
   auto weak = [&](uint64_t x) {
     const uint64_t d = std::min(x & M, (0 - x) & M);
@@ -316,7 +316,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   // hit is verified on all samples, so any choice gives the same answer.  A
   // sample whose leaf values are distinct and far from 0/+-1 rejects the most
   // candidates, so it is preferred.
-  const int fs = pick_filter_sample(lv, outv, n, L, W);
+  const int fs = pick_filter_sample(lv.data(), outv.data(), n, L, W);
   const uint64_t t0 = outv[fs];
 
   // ---- value tables V[1..Kt] -------------------------------------------

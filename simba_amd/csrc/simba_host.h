@@ -11,8 +11,8 @@ inline uint64_t wmask(int W) { return W == 64 ? ~0ull : 0xffffffffull; }
 
 // The sample whose values reject the most candidates (see simba_host.cpp);
 // `skip` excludes one sample (-1: none).  Returns -1 when none is left.
-int pick_filter_sample(const std::vector<uint64_t> &lv, const std::vector<uint64_t> &outv,
-                       int n, int L, int W, int skip = -1);
+int pick_filter_sample(const uint64_t *lv, const uint64_t *outv, int n, int L, int W,
+                       int skip = -1);
 int choose_kt(const CountTable &T, int max_size, int W, uint64_t budget);
 // A binary (op, left size) block of size s and the pair launch covering it.
 struct BlockRef {
