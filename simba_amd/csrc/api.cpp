@@ -65,7 +65,7 @@ void mba_ctx_destroy(mba_ctx *c) {
   (void)hipSetDevice(c->c.device);
   (void)hipStreamSynchronize(c->c.stream);
   DevBuf *bufs[] = {&c->c.tables, &c->c.tview, &c->c.hits, &c->c.surv, &c->c.hit_count,
-                    &c->c.leafvals, &c->c.outputs, &c->c.vsizes, &c->c.vranks,
+                    &c->c.leafvals, &c->c.outputs, &c->c.vranks,
                     &c->c.vok, &c->c.vfb_rows, &c->c.vfb_left, &c->c.vfb_right,
                     &c->c.vfb_keys, &c->c.vfb_idx, &c->c.vfb_flags, &c->c.vfb_misc,
                     &c->c.vfb_counts, &c->c.vfb_target, &c->c.b_inst, &c->c.b_data,

@@ -43,7 +43,7 @@ struct Ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string last_error;
   // SIMBA
-  DevBuf tables, tview, hits, surv, hit_count, leafvals, outputs, vsizes, vranks, vok;
+  DevBuf tables, tview, hits, surv, hit_count, leafvals, outputs, vranks, vok;
   // VFB
   DevBuf vfb_rows, vfb_left, vfb_right, vfb_keys, vfb_idx, vfb_flags, vfb_misc,
       vfb_counts, vfb_target;

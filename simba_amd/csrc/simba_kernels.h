@@ -124,8 +124,11 @@ hipError_t launch_batch_step(int width, const BatchStep *d_step, const BatchJob 
                              const TableView *d_tvs, unsigned long long *d_head,
                              HitRec *d_hits, uint32_t *d_hit_count, uint32_t hit_cap,
                              int grid, hipStream_t stream);
-// One wave per candidate; with d_n, only candidates [0, *d_n) (n_cand is then
-// the bound the grid covers).
+// Full verification of (instance, size, rank) candidates on all samples, one
+// wave per candidate (lane per sample, leaf values and RPN stacks in LDS, exit
+// at the first mismatching sample): ok[i] = 1 when candidate i matches all.
+// With d_n, only candidates [0, *d_n) (n_cand is then the bound the grid
+// covers).
 hipError_t launch_verify_batch(int width, const TableView *d_tvs, const uint64_t *d_leafvals,
                                const uint64_t *d_outputs, const BatchInstData *d_idat,
                                const VerifyCand *d_cand, uint32_t n_cand, uint8_t *d_ok,
@@ -184,12 +187,5 @@ hipError_t launch_rtid_scan(int width, const TableView *d_tv, const uint64_t *d_
                             uint64_t offset, uint64_t N, uint64_t g0, uint64_t g1,
                             uint64_t t0, unsigned long long *d_best, int grid,
                             hipStream_t stream);
-// Full verification of (size, rank) candidates on all samples: unrank and
-// evaluate every sample (leaf values [sample][leaf] staged in LDS), exit at
-// the first mismatching sample.  ok[i] = 1 when candidate i matches all.
-hipError_t launch_verify(int width, const TableView *d_tv, const uint64_t *d_leafvals,
-                         const uint64_t *d_outputs, int n_samples,
-                         const uint32_t *d_sizes, const uint64_t *d_ranks,
-                         uint32_t n_cand, uint8_t *d_ok, hipStream_t stream);
 
 }  // namespace mba

@@ -1342,72 +1342,6 @@ __global__ void k_batch_reset(unsigned long long *head, uint32_t *hit_count) {
   }
 }
 
-// Verification of candidates of many instances (global-memory samples).
-template <typename word>
-__global__ void __launch_bounds__(256)
-k_verify_batch(const TableView *__restrict__ tvs, const uint64_t *__restrict__ leafvals,
-               const uint64_t *__restrict__ outputs, const BatchInstData *__restrict__ idat,
-               const VerifyCand *__restrict__ cand, uint32_t n_cand, uint8_t *__restrict__ ok) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n_cand) return;
-  const VerifyCand vc = cand[c];
-  const TableView *tv = tvs + vc.inst;
-  const BatchInstData d = idat[vc.inst];
-  const int L = tv->L;
-  const int size = (int)vc.size;
-  uint8_t rpn[kMaxSize];
-  struct Item { uint64_t r; int s, pos; };
-  Item st[kMaxSize];
-  int sp = 0;
-  st[sp++] = {vc.rank, size, 0};
-  while (sp > 0) {
-    Item it = st[--sp];
-    if (it.s == 1) {
-      rpn[it.pos] = (uint8_t)it.r;
-      continue;
-    }
-    uint64_t n = it.r;
-    int op = 0;
-    while (n >= tv->T[it.s][op]) n -= tv->T[it.s][op++];
-    const int cs = it.s - 1;
-    rpn[it.pos + cs] = (uint8_t)(kTokOp + op);
-    if (is_unary(op)) {
-      st[sp++] = {n, cs, it.pos};
-      continue;
-    }
-    int j = 1;
-    for (;; ++j) {
-      const uint64_t blk = tv->T[j][8] * tv->T[cs - j][8];
-      if (n < blk) break;
-      n -= blk;
-    }
-    const uint64_t rs = tv->T[cs - j][8];
-    const uint64_t a = n / rs;
-    st[sp++] = {a, j, it.pos};
-    st[sp++] = {n - a * rs, cs - j, it.pos + j};
-  }
-  uint8_t good = 1;
-  for (int s = 0; s < (int)d.n_samples && good; ++s) {
-    word vs[kMaxSize];
-    int top = -1;
-    for (int i = 0; i < size; ++i) {
-      const uint8_t t = rpn[i];
-      if (t < kTokOp) {
-        vs[++top] = (word)leafvals[d.leaf_off + (uint64_t)s * L + t];
-      } else {
-        const int op = t - kTokOp;
-        if (is_unary(op)) {
-          vs[top] = apply_op<word>(op, vs[top], word(0));
-        } else {
-          const word rv = vs[top--];
-          vs[top] = apply_op<word>(op, vs[top], rv);
-        }
-      }
-    }
-    good = vs[0] == (word)outputs[d.out_off + s];
-  }
-  ok[c] = good;
-}
 
 // ---------------------------------------------------------------------------
 // Wavefront verification: one wave per candidate, one lane per sample.  Lane
@@ -1497,15 +1431,6 @@ __device__ void verify_wave(const TableView *__restrict__ tv, const uint64_t *__
   if (lane == 0) *ok_out = good ? 1 : 0;
 }
 
-template <typename word>
-__global__ void __launch_bounds__(64)
-k_verify_wave(const TableView *__restrict__ tv, const uint64_t *__restrict__ leafvals,
-              const uint64_t *__restrict__ outputs, int n_samples,
-              const uint32_t *__restrict__ sizes, const uint64_t *__restrict__ ranks,
-              uint8_t *__restrict__ ok) {
-  const uint32_t c = blockIdx.x;
-  verify_wave<word>(tv, leafvals, outputs, n_samples, (int)sizes[c], ranks[c], ok + c);
-}
 
 template <typename word>
 __global__ void __launch_bounds__(64)
@@ -1722,18 +1647,5 @@ hipError_t launch_verify_batch(int width, const TableView *d_tvs, const uint64_t
   return hipGetLastError();
 }
 
-hipError_t launch_verify(int width, const TableView *d_tv, const uint64_t *d_leafvals,
-                         const uint64_t *d_outputs, int n_samples,
-                         const uint32_t *d_sizes, const uint64_t *d_ranks,
-                         uint32_t n_cand, uint8_t *d_ok, hipStream_t stream) {
-  if (n_cand == 0) return hipSuccess;
-  if (width == 32)
-    hipLaunchKernelGGL(k_verify_wave<uint32_t>, dim3(n_cand), dim3(64), 0, stream, d_tv,
-                       d_leafvals, d_outputs, n_samples, d_sizes, d_ranks, d_ok);
-  else
-    hipLaunchKernelGGL(k_verify_wave<uint64_t>, dim3(n_cand), dim3(64), 0, stream, d_tv,
-                       d_leafvals, d_outputs, n_samples, d_sizes, d_ranks, d_ok);
-  return hipGetLastError();
-}
 
 }  // namespace mba
