@@ -352,7 +352,11 @@ __device__ __forceinline__ void gen_b_tile(const TableView *__restrict__ tv, int
     a0 = x / rs;
     r0 = x - a0 * rs;
   }
-  const bool step_ok = un || rs >= (uint64_t)(kPairThreads * RB);
+  // a tile spans at most two rows of the block when its right side has at
+  // least a tile of trees (one carry); otherwise (ra, rb) by a 32-bit
+  // division (r0 + off < rs + tile < 2^32)
+  const bool one_carry = un || rs >= (uint64_t)(kPairThreads * RB);
+  const uint32_t rs32 = (uint32_t)rs;
   const word *VA = V + tv->off[un ? m - 1 : j];
   const word *VB = V + tv->off[un ? 0 : m - 1 - j];
 #pragma unroll
@@ -363,19 +367,28 @@ __device__ __forceinline__ void gen_b_tile(const TableView *__restrict__ tv, int
       continue;
     }
     const uint64_t off = b - first;
-    if (step_ok && off < rem) {
+    if (off < rem) {
       if (un) {
         bv[k] = apply_op<word>(op, VA[x + off], word(0));
       } else {
-        uint64_t rb = r0 + off, ra = a0;
-        if (rb >= rs) {
-          rb -= rs;
-          ra += 1;
+        uint64_t rb, ra;
+        if (one_carry) {
+          rb = r0 + off;
+          ra = a0;
+          if (rb >= rs) {
+            rb -= rs;
+            ra += 1;
+          }
+        } else {
+          const uint32_t t = (uint32_t)(r0 + off);
+          const uint32_t q = t / rs32;
+          rb = t - q * rs32;
+          ra = a0 + q;
         }
         bv[k] = apply_op<word>(op, VA[ra], VB[rb]);
       }
     } else {
-      bv[k] = build_value<word>(tv, m, b);
+      bv[k] = build_value<word>(tv, m, b);  // the tile runs into the next block
     }
   }
 }
