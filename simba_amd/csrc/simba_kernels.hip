@@ -329,45 +329,55 @@ __device__ __forceinline__ void gen_b_tile(const TableView *__restrict__ tv, int
                                            uint64_t first, uint64_t lane_b0, word *bv,
                                            const bool *ok) {
   const word *V = global_ptr<const word>(tv->base);
-  uint64_t x = first;
-  int op = 0;
-  while (x >= tv->T[m][op]) x -= tv->T[m][op++];
-  const bool un = is_unary(op);
-  int j = 0;
-  uint64_t rs = 1, a0 = 0, r0 = 0, rem;
-  if (un) {
-    rem = tv->T[m][op] - x;
-  } else {
-    const int cs = m - 1;
-    j = 1;
-    for (;; ++j) {
-      const uint64_t blk = tv->T[j][8] * tv->T[cs - j][8];
-      if (x < blk) {
-        rem = blk - x;
-        break;
-      }
-      x -= blk;
-    }
-    rs = tv->T[cs - j][8];
-    a0 = x / rs;
-    r0 = x - a0 * rs;
-  }
-  // a tile spans at most two rows of the block when its right side has at
-  // least a tile of trees (one carry); otherwise (ra, rb) by a 32-bit
-  // division (r0 + off < rs + tile < 2^32)
-  const bool one_carry = un || rs >= (uint64_t)(kPairThreads * RB);
-  const uint32_t rs32 = (uint32_t)rs;
-  const word *VA = V + tv->off[un ? m - 1 : j];
-  const word *VB = V + tv->off[un ? 0 : m - 1 - j];
+  const int cs = m - 1;
+  const uint64_t last = first + (uint64_t)(kPairThreads * RB);  // end of the tile
+  bool todo[RB];
 #pragma unroll
   for (int k = 0; k < RB; ++k) {
-    const uint64_t b = lane_b0 + (uint64_t)k * kPairThreads;
-    if (!ok[k]) {
-      bv[k] = word(0);
-      continue;
+    todo[k] = ok[k];
+    bv[k] = word(0);
+  }
+  // the (op, split) block of the tile's first tree and its offset there
+  uint64_t x = first;
+  int op = 0;
+  while (op < 8 && x >= tv->T[m][op]) x -= tv->T[m][op++];
+  int j = 1;
+  if (op < 8 && !is_unary(op)) {
+    for (;; ++j) {
+      const uint64_t blk = tv->T[j][8] * tv->T[cs - j][8];
+      if (x < blk) break;
+      x -= blk;
     }
-    const uint64_t off = b - first;
-    if (off < rem) {
+  }
+  // every block the tile touches, in rank order (workgroup-uniform)
+  uint64_t base = first;
+  while (op < 8 && base < last) {
+    const bool un = is_unary(op);
+    uint64_t rem, rs = 1, a0 = 0, r0 = 0;
+    const word *VA, *VB;
+    if (un) {
+      rem = tv->T[m][op] - x;
+      VA = V + tv->off[m - 1];
+      VB = V;
+    } else {
+      rem = tv->T[j][8] * tv->T[cs - j][8] - x;
+      rs = tv->T[cs - j][8];
+      a0 = x / rs;
+      r0 = x - a0 * rs;
+      VA = V + tv->off[j];
+      VB = V + tv->off[cs - j];
+    }
+    // a tile spans at most two rows of the block when its right side has at
+    // least a tile of trees (one carry); otherwise (ra, rb) by a 32-bit
+    // division (r0 + off < rs + tile < 2^32)
+    const bool one_carry = un || rs >= (uint64_t)(kPairThreads * RB);
+    const uint32_t rs32 = (uint32_t)rs;
+#pragma unroll
+    for (int k = 0; k < RB; ++k) {
+      if (!todo[k]) continue;
+      const uint64_t off = lane_b0 + (uint64_t)k * kPairThreads - base;
+      if (off >= rem) continue;  // a later block
+      todo[k] = false;
       if (un) {
         bv[k] = apply_op<word>(op, VA[x + off], word(0));
       } else {
@@ -387,9 +397,17 @@ __device__ __forceinline__ void gen_b_tile(const TableView *__restrict__ tv, int
         }
         bv[k] = apply_op<word>(op, VA[ra], VB[rb]);
       }
-    } else {
-      bv[k] = build_value<word>(tv, m, b);  // the tile runs into the next block
     }
+    // the next block: the next split of this op, else the next op
+    base += rem;
+    x = 0;
+    if (!un && j < (op == SUB ? m - 2 : (m - 1) / 2)) {
+      ++j;
+      continue;
+    }
+    j = 1;
+    ++op;
+    while (op < 8 && tv->T[m][op] == 0) ++op;
   }
 }
 
