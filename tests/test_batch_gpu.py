@@ -63,3 +63,17 @@ def test_batch_single_and_repeated(ctx):
     b = ctx.batch(insts, max_size=7)
     key = lambda r: (r["status"], r["size"], r["rank"], r["expr"], r["candidates"])
     assert [key(x) for x in a] == [key(x) for x in b]
+
+
+def test_batch_stepped_b_sides_span_blocks(ctx):
+    # two leaves, sizes up to 9: the stepped B sides (size 7, one above the
+    # tables) have (op, split) blocks smaller than a tile, so tiles walk
+    # several blocks; the answers stay the per-instance search's
+    from simba_amd.synth import make_instance
+    insts = [make_instance(4200 + i, 2, 24, 7 + i % 3, 64) for i in range(24)]
+    note("batch stepped blocks")
+    got = ctx.batch(insts, max_size=9)
+    for inst, g in zip(insts, got):
+        p = ctx.simba(inst, max_size=9)
+        assert (g["status"], g["size"], g["rank"], g["expr"]) == \
+            (p["status"], p["size"], p["rank"], p["expr"])
