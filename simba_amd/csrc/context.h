@@ -86,7 +86,8 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &opts,
 int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &opts,
               mba_result &res, mba_vfb_stats *stats);
 // scores: NULL (plain, minimum rank per instance) or count x 256 block
-// scores as for simba_solve (learned order, rounds of 1, 1, 2, 4 ... blocks).
+// scores as for simba_solve (learned order: rounds of the best-scored split,
+// the next split, then the rest).
 int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count,
                 const mba_opts &opts, mba_result *results, const float *scores = nullptr);
 

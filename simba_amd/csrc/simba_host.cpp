@@ -353,7 +353,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
                          hipMemcpyHostToDevice, st));
   MBA_HIP(hipMemcpyAsync(ctx.outputs.p, outv.data(), sizeof(uint64_t) * n,
                          hipMemcpyHostToDevice, st));
-  {  // V[1] = leaf values on sample 0
+  {  // V[1] = leaf values on the filter sample
     std::vector<uint64_t> v1(L);
     std::vector<uint32_t> v1w(L);
     for (int l = 0; l < L; ++l)
@@ -482,7 +482,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     MBA_HIP(hipMemcpyAsync(h_ok, ctx.vok.p, kSlots, hipMemcpyDeviceToHost, st));
     MBA_HIP(hipStreamSynchronize(st));
     if (h_counts[0] > kHitCap) {
-      ctx.last_error = "sample-0 hit buffer overflow";
+      ctx.last_error = "window-hit buffer overflow";
       return MBA_ERR_LIMIT;
     }
     const uint32_t nt = h_counts[1];

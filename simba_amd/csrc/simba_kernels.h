@@ -5,7 +5,7 @@
 namespace mba {
 
 // Device-resident description of the per-size value tables V[1..Kt]
-// (sample-0 value of every tree of that size, in rank order) and of the
+// (filter-sample value of every tree of that size, in rank order) and of the
 // count table.  One instance per solve, read through uniform (scalar) loads.
 struct TableView {
   uint64_t off[kMaxSize + 1];     // element offset of V[m] in the table buffer
@@ -32,13 +32,13 @@ struct CheckTargets {
   uint64_t lo_even, lo_odd, span;
 };
 
-// A block of B-side trees of size m > Kt whose sample-0 values each lane
+// A block of B-side trees of size m > Kt whose filter-sample values each lane
 // forms from tables: an inner value
 //   v = op(VA[ra], VB[rb])   (op >= 0),   v = VA[ra]   (op < 0)
 // for ra < nA, rb < nB, then the steps of the path from the inner node up to
 // the B side's root, innermost first: an affine map x -> s*x + k (a chain of
 // unary nodes: ~x = -x - 1, neg x = -x; s = -1 when neg), or a binary node
-// whose other child is a fixed tree with sample-0 value V[coff] (kind 1:
+// whose other child is a fixed tree with filter-sample value V[coff] (kind 1:
 // op(V[coff], x), kind 2: op(x, V[coff])).  The tree's rank within size m is
 // start + stride * (ra * nB + rb).  A tile covers 2^logTB columns (over ra
 // when trans, else over rb) by (256*RB) >> logTB rows; ntc tiles span a row

@@ -6,16 +6,16 @@
 // size level is instead swept as outer products.  Every binary-rooted tree of
 // size s is op(X, Y) with X of size j and Y of size m = s-1-j; for an
 // unordered split {j, m} the seven operator instances & | ^ + * (X-Y) (Y-X)
-// are fused and evaluated on sample 0 from the value tables V[j], V[m]
-// (value of every tree of that size on sample 0, in rank order).  A lane
-// holds RB (4 or 8) values of the B side in VGPRs; the A side streams through
-// wave-uniform scalar loads, so the inner loop is pure integer VALU work with
-// no divergence.  Unary-rooted trees are the composition of a chain of ~/neg
-// with a binary-rooted tree or a leaf; such a chain maps x to +-x + k with
-// |k| <= chain length, so two windowed compares per value cover every unary
-// tree of every larger size.  Hits on sample 0 are rare; they are appended to
-// a hit buffer, resolved to (size, rank) by the host and verified on all
-// samples by launch_verify.
+// are fused and evaluated on one filter sample from the value tables V[j],
+// V[m] (value of every tree of that size on that sample, in rank order).  A
+// lane holds RB (4 or 8) values of the B side in VGPRs; the A side is staged
+// in LDS and read as wave-uniform broadcasts, so the inner loop is integer
+// VALU work with no divergence.  Unary-rooted trees are the composition of a
+// chain of ~/neg with a binary-rooted tree or a leaf; such a chain maps x to
+// +-x + k with |k| <= chain length, so two windowed compares per value cover
+// every unary tree of every larger size.  Hits are rare; they are appended
+// to a hit buffer, resolved to (size, rank) on the device (k_hit_screen,
+// k_hit_confirm) and verified on all samples (verify_wave).
 #include "simba_kernels.h"
 
 namespace mba {
@@ -35,9 +35,10 @@ template <typename word>
 struct WordOf;
 
 // ---------------------------------------------------------------------------
-// Table-assisted value of the tree (m, r) on sample 0: unrank top-down in the
-// reference's order, but stop at any subtree of size <= Kt and read its value
-// from V[size]; then evaluate the remaining skeleton in postfix order.
+// Table-assisted value of the tree (m, r) on the filter sample: unrank
+// top-down in the reference's order, but stop at any subtree of size <= Kt and
+// read its value from V[size]; then evaluate the remaining skeleton in
+// postfix order.
 // ---------------------------------------------------------------------------
 template <typename word>
 __device__ __forceinline__ word build_value(const TableView *__restrict__ tv, int m,
@@ -1217,7 +1218,7 @@ k_found_gather(const uint32_t *__restrict__ touched, const uint32_t *__restrict_
 // SIMBA-RTid order (nocache_rtid.cu:85-95): thread g of a (size, op) block
 // of N trees checks rank offset + ((g * C) mod 2^64) mod N.  Threads
 // [g0, g1) of the block; the smallest g whose tree matches every sample is
-// kept with atomicMin.  The sample-0 value comes from the tables (value_of),
+// kept with atomicMin.  The filter-sample value comes from the tables (value_of),
 // the full check only runs on a match.
 // ---------------------------------------------------------------------------
 constexpr uint64_t kRtidC = 2246822507ull;
