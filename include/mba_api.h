@@ -79,11 +79,13 @@ typedef struct {
   int32_t table_delta;      /* tabulate Kt - table_delta sizes (0: budget)   */
   int32_t block_log2;       /* sweep one (op, split) block per launch from   */
                             /* 2^block_log2 trees per size (0: defaults)    */
-  int32_t hit_cap;          /* SIMBA per-instance: window-hit buffer, in     */
-                            /* hits (0: 2^22, above the hits of one tile); a */
-                            /* launch group that overflows it is replayed in */
-                            /* parts (a smaller buffer that one tile         */
-                            /* overflows is an error)                        */
+  int32_t hit_cap;          /* window-hit buffer, in hits (0: per-instance   */
+                            /* 2^22, batched 2^24; both above the hits of    */
+                            /* one tile); a launch group (batched: a step)   */
+                            /* that overflows it is replayed in parts, the   */
+                            /* batched step by halves of its jobs, then of   */
+                            /* one job's tiles (a smaller buffer that one    */
+                            /* tile overflows is an error)                   */
   int32_t reserved[2];
 } mba_opts;
 

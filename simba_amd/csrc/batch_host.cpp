@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <thread>
 #include <vector>
@@ -208,7 +209,9 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
 
   // ---- per-step graph: upload jobs, reset counters, drain, read hit count --
   MBA_HIP(ctx.b_queue.ensure(sizeof(BatchJob) * job_cap + 256));
-  MBA_HIP(ctx.hits.ensure(sizeof(HitRec) * kBatchHitCap));
+  // hit buffer (opts.hit_cap; smaller buffers exercise the split replays)
+  const uint32_t hit_cap = o.hit_cap > 0 ? (uint32_t)o.hit_cap : kBatchHitCap;
+  MBA_HIP(ctx.hits.ensure(sizeof(HitRec) * hit_cap));
   MBA_HIP(ctx.hit_count.ensure(256));
   BatchJob *d_jobs = (BatchJob *)(ctx.b_queue.as<char>() + 256);
   BatchStep *d_step = ctx.b_queue.as<BatchStep>();
@@ -232,7 +235,8 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
 
   // the step graph names these buffers and the launch shape: it is captured
   // again only when one of them changed since the previous call
-  const uint64_t key[10] = {(uint64_t)W, (uint64_t)grid, (uint64_t)(uintptr_t)d_step,
+  const uint64_t key[11] = {(uint64_t)W, (uint64_t)grid, (uint64_t)hit_cap,
+                            (uint64_t)(uintptr_t)d_step,
                             (uint64_t)(uintptr_t)d_jobs, (uint64_t)(uintptr_t)d_tvs,
                             (uint64_t)(uintptr_t)d_head, (uint64_t)(uintptr_t)d_hits,
                             (uint64_t)(uintptr_t)d_hc, (uint64_t)(uintptr_t)h_step,
@@ -246,8 +250,8 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     MBA_HIP(hipStreamBeginCapture(stq, hipStreamCaptureModeThreadLocal));
     MBA_HIP(hipMemcpyAsync(d_step, h_step, sizeof(BatchStep), hipMemcpyHostToDevice, stq));
     MBA_HIP(launch_batch_reset(d_head, d_hc, stq));
-    MBA_HIP(launch_batch_step(W, d_step, d_jobs, d_tvs, d_head, d_hits, d_hc, kBatchHitCap,
-                              grid, stq));
+    MBA_HIP(launch_batch_step(W, d_step, d_jobs, d_tvs, d_head, d_hits, d_hc, hit_cap, grid,
+                              stq));
     MBA_HIP(hipMemcpyAsync(h_hc, d_hc, 4, hipMemcpyDeviceToHost, stq));
     MBA_HIP(hipStreamEndCapture(stq, &ctx.b_graph));
     MBA_HIP(hipGraphInstantiate(&ctx.b_gexec, ctx.b_graph, nullptr, nullptr, 0));
@@ -274,6 +278,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
                  1e3 * seconds_since(t_start));
   double t_build = 0, t_gpu = 0, t_resolve = 0;
   uint64_t n_hits = 0, n_kept = 0, n_cands = 0;
+  int replays = 0;  // steps run again in parts after a hit buffer overflow
   std::vector<uint32_t> hits_of(o.verbose ? count : 0, 0);
   for (int s = 0; s <= max_size; ++s) {
     if ((o.cancel && *o.cancel) ||
@@ -383,18 +388,26 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
         }
       }
       t_build += seconds_since(t_b0);
-      if (nj > 0) {
+      // Runs jobs [j0, j1) of this step (one graph replay) and resolves their
+      // hits; a replay whose hits overflow the buffer is run again as two
+      // halves of its jobs (jobs are independent within a step, table builds
+      // idempotent), so only one job that overflows it alone is an error.
+      std::function<int(size_t, size_t)> run_jobs = [&](size_t j0, size_t j1) -> int {
+        const uint64_t tbase = h_jobs[j0].tile0;
+        const uint64_t tend = h_jobs[j1 - 1].tile0 + h_jobs[j1 - 1].ntiles;
+        for (size_t k = j0; k < j1; ++k) h_jobs[k].tile0 -= tbase;
         const auto t_g0 = Clock::now();
-        h_step->total_tiles = tiles;
-        h_step->njobs = (uint32_t)nj;
+        h_step->total_tiles = tend - tbase;
+        h_step->njobs = (uint32_t)(j1 - j0);
         MBA_HIP(hipEventRecord(ctx.ev0, stq));
-        // this step's jobs only (the graph then uploads the step header,
-        // resets the queue, drains it and reads the hit count back)
-        MBA_HIP(hipMemcpyAsync(d_jobs, h_jobs, sizeof(BatchJob) * nj, hipMemcpyHostToDevice,
-                               stq));
+        // these jobs only (the graph then uploads the step header, resets the
+        // queue, drains it and reads the hit count back)
+        MBA_HIP(hipMemcpyAsync(d_jobs, h_jobs + j0, sizeof(BatchJob) * (j1 - j0),
+                               hipMemcpyHostToDevice, stq));
         MBA_HIP(hipGraphLaunch(gexec, stq));
         MBA_HIP(hipEventRecord(ctx.ev1, stq));
         MBA_HIP(hipEventSynchronize(ctx.ev1));
+        for (size_t k = j0; k < j1; ++k) h_jobs[k].tile0 += tbase;
         float ms = 0.f;
         MBA_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
         ktime += ms * 1e-3;
@@ -402,12 +415,35 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
         t_gpu += seconds_since(t_g0);
         if (o.verbose)
           std::fprintf(stderr, "batch: size %d round %d: %zu jobs, %llu tiles, %.3f ms\n", s,
-                       round, nj, (unsigned long long)tiles, ms);
+                       round, j1 - j0, (unsigned long long)(tend - tbase), ms);
         const auto t_r0 = Clock::now();
         const uint32_t nh = *h_hc;
-        if (nh > kBatchHitCap) {
-          ctx.last_error = "batch: sample hit buffer overflow";
-          return MBA_ERR_LIMIT;
+        if (nh > hit_cap) {
+          ++replays;
+          if (j1 - j0 == 1) {
+            // one job: its tiles in two parts (a tile holds at most
+            // 256 * 2048 * 7 hits, well under the buffer)
+            BatchJob &J = h_jobs[j0];
+            const BatchJob keep = J;
+            if (keep.ntiles < 2) {
+              ctx.last_error = "batch: one tile's window hits overflow the hit buffer";
+              return MBA_ERR_LIMIT;
+            }
+            const uint64_t half = keep.ntiles / 2;
+            J.ntiles = half;
+            int rc = run_jobs(j0, j1);
+            if (rc == 0) {
+              J = keep;
+              J.p.tile_begin += half;
+              J.ntiles = keep.ntiles - half;
+              rc = run_jobs(j0, j1);
+            }
+            J = keep;
+            return rc;
+          }
+          const size_t jm = j0 + (j1 - j0) / 2;
+          if (int rc = run_jobs(j0, jm)) return rc;
+          return run_jobs(jm, j1);
         }
         n_hits += nh;
         if (nh > 0 && !hits_of.empty()) {  // verbose: hits per instance
@@ -456,6 +492,10 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
           }
         }
         t_resolve += seconds_since(t_r0);
+        return 0;
+      };
+      if (nj > 0) {
+        if (int rc = run_jobs(0, nj)) return rc;
       }
       if (!learned) break;
       // learned order: any verified solution of this (minimal) size settles
@@ -506,6 +546,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     r.size_reached = st.reached;
     r.kernel_seconds = ktime;
     r.kernel_launches = klaunch;
+    r.hit_replays = replays;
     if (st.active) {
       r.status = budget_hit ? MBA_BUDGET : MBA_NOT_FOUND;
       r.seconds = total;

@@ -60,7 +60,7 @@ struct Ctx {
   size_t b_pinned_bytes = 0;
   hipGraph_t b_graph = nullptr;
   hipGraphExec_t b_gexec = nullptr;
-  uint64_t b_graph_key[10] = {};
+  uint64_t b_graph_key[11] = {};
 };
 
 using Clock = std::chrono::steady_clock;

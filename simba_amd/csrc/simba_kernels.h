@@ -86,7 +86,9 @@ constexpr int kBuildTile = 1024;  // table values per build tile
 
 // One unit of a batch step: kind 0 builds V[p.m] of instance `inst` (p.nB
 // values), kind 1 checks the pair {p.j, p.m} at size p.sb, kind 2 scans the
-// leaves.  Jobs own the global tile range [tile0, tile0 + ntiles).
+// leaves.  Jobs own the global tile range [tile0, tile0 + ntiles): global
+// tile t is the job's tile p.tile_begin + (t - tile0) (p.tile_begin is 0
+// unless the host runs a job's tiles in parts).
 struct BatchJob {
   uint64_t tile0, ntiles;
   uint64_t lo_e, lo_o, span;  // windows (kinds 1 and 2)

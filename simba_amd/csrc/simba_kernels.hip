@@ -1329,7 +1329,7 @@ k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jo
         staged = ~0ull;
       }
       const TableView *tv = tvs + J.inst;
-      const uint64_t lt = t - J.tile0;
+      const uint64_t lt = t - J.tile0 + J.p.tile_begin;
       if (J.kind == 0) {  // table build: V[m] ranks [lt*kBuildTile, +kBuildTile)
         word *out = global_ptr<word>(tv->base) + tv->off[J.p.m];
         const uint64_t r0 = lt * kBuildTile;

@@ -77,3 +77,21 @@ def test_batch_stepped_b_sides_span_blocks(ctx):
         p = ctx.simba(inst, max_size=9)
         assert (g["status"], g["size"], g["rank"], g["expr"]) == \
             (p["status"], p["size"], p["rank"], p["expr"])
+
+
+# (max size, hit buffer): a step whose window hits overflow the small buffer
+# is run again with its jobs (and then one job's tiles) in parts; the answers
+# stay those of the default buffer
+BATCH_REPLAY_CASES = [(9, 4096), (11, 16384), (11, 65536)]
+
+
+@pytest.mark.parametrize("max_size,cap", BATCH_REPLAY_CASES)
+def test_batch_hit_buffer_overflow_split_keeps_answers(ctx, max_size, cap):
+    from simba_amd.learn import sample_instances
+    insts = sample_instances(10**6 + 15, 64, (9, 19), (2, 4), (32, 32))
+    note(f"batch hit split max={max_size} cap={cap}")
+    base = ctx.batch(insts, max_size=max_size)
+    got = ctx.batch(insts, max_size=max_size, hit_cap=cap)
+    assert got[0]["hit_replays"] >= 1 and base[0]["hit_replays"] == 0
+    key = lambda r: (r["status"], r["size"], r["rank"], r["expr"])
+    assert [key(r) for r in got] == [key(r) for r in base]
