@@ -12,6 +12,13 @@
  * variables), which reduce to the reference exactly for W=32 and no
  * constants.
  *
+ * Pinning: every function is checked by tests/ against brute force, closed
+ * forms or golden vectors, except two, whose parity is unpinned beyond
+ * invariants (no reference test or vector exists): the combine step of the
+ * VFB fingerprint (orc_fingerprint; its fmix32 is pinned by the published
+ * MurmurHash3_x86_32 vectors) and the probe hash orc_fmix64, which only
+ * places keys in the hash set and changes no answer or count.
+ *
  * Token encoding of an RPN formula (one byte per node):
  *   0..31   leaf index (variables first, then constants)
  *   32+o    operator o, o in 0..7 = ~ & | ^ neg + - *   (globals.cuh:13-15)
