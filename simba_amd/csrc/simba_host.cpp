@@ -26,6 +26,9 @@ constexpr uint64_t kSmallTableValues = 1u << 16;
 constexpr uint64_t kDefaultTableBytes = 40ull << 30;
 // Work per launch, in A x B value pairs, so that budgets are honoured.
 constexpr uint64_t kPairsPerLaunch = 1ull << 36;
+// trees of the small sizes swept in one fused step (launched back to back,
+// their hits resolved in one sync)
+constexpr uint64_t kFuseTrees = 1ull << 22;
 
 // Tile geometry of a B block of nA x nB values (see BBlock): columns run
 // over the larger dimension, 2^logTB of them, and the tile's remaining
@@ -428,7 +431,6 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
 
   // ---- size sweep --------------------------------------------------------
   std::vector<uint64_t> found(kMaxSize + 2, ~0ull);
-  uint32_t *d_hc = ctx.hit_count.as<uint32_t>();
   HitRec *d_hits = ctx.hits.as<HitRec>();
   // 32 workgroups per CU by default: more than stay resident, so tiles are
   // handed out in small grid-stride shares and no CU waits on a long tail
@@ -442,9 +444,17 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   // Device hit resolution (k_hit_resolve): the per-size minimum chain rank
   // of the hits that meet every sample, flags and the list of lowered slots;
   // the instance's sample layout for it at byte 128 of the header.
+  // The read-back area (the pinned layout: hit count, lowered slot count,
+  // candidate count, candidates, verdicts) follows, so one copy returns it.
   constexpr size_t kSlots = kMaxSize + 1;
-  MBA_HIP(ctx.b_misc.ensure(kSlots * 16 + 256));
-  uint32_t *d_nt = ctx.b_misc.as<uint32_t>();
+  constexpr size_t kReadOff = (256 + kSlots * 16 + 255) / 256 * 256;
+  constexpr size_t kReadBytes = 16 + kSlots * (sizeof(VerifyCand) + 1);
+  MBA_HIP(ctx.b_misc.ensure(kReadOff + kReadBytes));
+  char *d_read = ctx.b_misc.as<char>() + kReadOff;
+  uint32_t *d_hc = (uint32_t *)d_read;
+  uint32_t *d_nt = d_hc + 1;  // lowered slots, candidates
+  VerifyCand *d_vc = (VerifyCand *)(d_read + 16);
+  uint8_t *d_ok = (uint8_t *)(d_vc + kSlots);
   BatchInstData *d_idat = (BatchInstData *)(ctx.b_misc.as<char>() + 128);
   unsigned long long *d_found = (unsigned long long *)(ctx.b_misc.as<char>() + 256);
   uint32_t *d_flags = (uint32_t *)(d_found + kSlots);
@@ -454,34 +464,29 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   MBA_HIP(hipMemsetAsync(d_found, 0xff, kSlots * 8, st));
   MBA_HIP(hipMemsetAsync(d_flags, 0, kSlots * 4, st));
   MBA_HIP(ctx.surv.ensure(sizeof(SurvRec) * 2 * (size_t)kHitCap));
-  MBA_HIP(ctx.vranks.ensure(sizeof(VerifyCand) * kSlots));
-  MBA_HIP(ctx.vok.ensure(kSlots));
-  // read back area: hit count, lowered slot count, candidates, verdicts
-  static_assert(16 + kSlots * (sizeof(VerifyCand) + 1) <= Ctx::kPinnedBytes, "pinned area");
+  static_assert(kReadBytes <= Ctx::kPinnedBytes, "pinned area");
   uint32_t *h_counts = (uint32_t *)ctx.pinned;
   VerifyCand *h_vc = (VerifyCand *)((char *)ctx.pinned + 16);
   uint8_t *h_ok = (uint8_t *)(h_vc + kSlots);
 
   // Resolve the hits of one step on the device and verify the lowered slots'
   // trees; the kernels read the counts on the device, so one sync suffices.
-  auto process_hits = [&]() -> int {
-    MBA_HIP(hipMemsetAsync(d_nt, 0, 8, st));  // touched and candidate counts
+  // The hit buffer is empty afterwards (its count back to zero).
+  // allow_overflow: a count above the buffer is the caller's to handle (the
+  // first kHitCap hits are resolved).
+  auto process_hits = [&](bool allow_overflow = false) -> int {
     MBA_HIP(launch_hit_resolve(W, d_tv, ctx.leafvals.as<uint64_t>(), ctx.outputs.as<uint64_t>(),
                                d_idat, d_hits, d_hc, kHitCap, 512, 512, min_size, max_size,
                                d_found, ctx.surv.as<SurvRec>(), d_nt + 1, d_flags, d_touched,
                                d_nt, st));
-    MBA_HIP(launch_found_gather(d_touched, d_nt, kSlots, d_found, d_flags,
-                                ctx.vranks.as<VerifyCand>(), st));
+    MBA_HIP(launch_found_gather(d_touched, d_nt, kSlots, d_found, d_flags, d_vc, st));
     MBA_HIP(launch_verify_batch(W, d_tv, ctx.leafvals.as<uint64_t>(), ctx.outputs.as<uint64_t>(),
-                                d_idat, ctx.vranks.as<VerifyCand>(), kSlots,
-                                ctx.vok.as<uint8_t>(), st, d_nt));
-    MBA_HIP(hipMemcpyAsync(h_counts, d_hc, 4, hipMemcpyDeviceToHost, st));
-    MBA_HIP(hipMemcpyAsync(h_counts + 1, d_nt, 4, hipMemcpyDeviceToHost, st));
-    MBA_HIP(hipMemcpyAsync(h_vc, ctx.vranks.p, sizeof(VerifyCand) * kSlots,
-                           hipMemcpyDeviceToHost, st));
-    MBA_HIP(hipMemcpyAsync(h_ok, ctx.vok.p, kSlots, hipMemcpyDeviceToHost, st));
+                                d_idat, d_vc, kSlots, d_ok, st, d_nt));
+    MBA_HIP(hipMemcpyAsync(ctx.pinned, d_read, kReadBytes, hipMemcpyDeviceToHost, st));
+    // the hit, lowered slot and candidate counts start again from zero
+    MBA_HIP(hipMemsetAsync(d_read, 0, 12, st));
     MBA_HIP(hipStreamSynchronize(st));
-    if (h_counts[0] > kHitCap) {
+    if (h_counts[0] > kHitCap && !allow_overflow) {
       ctx.last_error = "window-hit buffer overflow";
       return MBA_ERR_LIMIT;
     }
@@ -495,9 +500,6 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     return 0;
   };
 
-  // hits recorded in the buffer before the open launch group (known at the
-  // group's start; the buffer is append-only)
-  uint32_t hits_base = 0;
   int replays = 0;  // launch groups replayed after an overflow
 
   auto over_budget = [&]() {
@@ -505,24 +507,137 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     return o.time_budget_s > 0 && seconds_since(t_start) > o.time_budget_s;
   };
 
+  // block-at-a-time only pays where a size's sweep outweighs a launch and a
+  // sync per block; below that the fused sweep (minimum rank, also a valid
+  // answer for the learned order) is used
+  const uint64_t block_min = o.block_log2 > 0 ? (1ull << std::min(o.block_log2, 63))
+                             : block_scores ? kLearnedBlockMin : kBlockModeMin;
+  // Small sizes are swept in fused steps: sizes s0..last (below kFuseTrees
+  // trees in all, each below the block threshold) are launched back to back
+  // and their hits resolved in one sync; the answer is then the smallest size
+  // of the step with a verified tree, as size by size.  Returns s0 - 1 when
+  // s0 is not fused (exhaustive sweeps go size by size).
+  auto fused_last = [&](int s0) -> int {
+    int e = s0 - 1;
+    if (o.exhaustive) return e;
+    for (uint64_t acc = 0; e < max_size && T.t[e + 1][8] < block_min &&
+                           acc + T.t[e + 1][8] <= kFuseTrees;)
+      acc += T.t[++e][8];
+    return e;
+  };
+
   // Step 1: leaves (size 1) and, through unary chains, every unary tree over a
-  // leaf (sizes 2..max).
-  MBA_HIP(hipMemsetAsync(d_hc, 0, 4, st));
+  // leaf (sizes 2..max).  Their hits join the first fused step when there is
+  // one.
+  MBA_HIP(hipMemsetAsync(d_read, 0, 12, st));  // hit, lowered slot, candidate counts
   MBA_HIP(launch_leaf_scan(W, d_tv, L, targets_for(t0, 1, max_size, W), d_hits, d_hc, kHitCap, st));
-  if (int rc = process_hits()) return rc;
-  for (int s = 1; s <= std::min(2, max_size); ++s) {
-    reached = s;
-    if (s >= min_size && found[s] != ~0ull) {
-      answer = s;
-      break;
+  bool leaf_pending = max_size >= 3 && fused_last(3) >= 3;
+  if (!leaf_pending) {
+    if (int rc = process_hits()) return rc;
+    for (int s = 1; s <= std::min(2, max_size); ++s) {
+      reached = s;
+      if (s >= min_size && found[s] != ~0ull) {
+        answer = s;
+        break;
+      }
     }
   }
 
+  // The open launch group: pair launches sent back to back since the last
+  // sync (all of one size's targets), kept for a replay in parts when their
+  // hits overflow the buffer.
+  bool group_open = false;
+  uint64_t group_tiles = 0;
+  std::vector<std::pair<PairLaunch, CheckTargets>> group;
+  // Replays a launch's tile range in `pieces` parts, resolving each part's
+  // hits before the next; a part that still overflows the hit buffer is
+  // split again (a single tile that overflows it is an error).
+  std::function<int(const PairLaunch &, const CheckTargets &, int)> replay =
+      [&](const PairLaunch &q, const CheckTargets &tg, int pieces) -> int {
+    const uint64_t n = q.tile_end - q.tile_begin;
+    const uint64_t step = std::max<uint64_t>(1, (n + pieces - 1) / pieces);
+    for (uint64_t b = q.tile_begin; b < q.tile_end; b += step) {
+      PairLaunch r = q;
+      r.tile_begin = b;
+      r.tile_end = std::min(q.tile_end, b + step);
+      const int g = (int)std::min<uint64_t>((uint64_t)grid, r.tile_end - r.tile_begin);
+      uint32_t cnt = 0;
+      hipError_t e =
+          launch_pair_check(W, d_tv, ctx.tables.p, tv.off, r, tg, d_hits, d_hc, kHitCap, g, st);
+      if (e == hipSuccess) e = hipMemcpyAsync(&cnt, d_hc, 4, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);  // `st` does not wait on null
+      if (e != hipSuccess) {
+        ctx.last_error = std::string("pair replay: ") + hipGetErrorString(e);
+        return -1;
+      }
+      ++klaunch;
+      if (cnt > kHitCap) {
+        if (r.tile_end - r.tile_begin == 1) {
+          ctx.last_error = "hit buffer overflow within one tile";
+          return -1;
+        }
+        if (hipMemsetAsync(d_hc, 0, 4, st) != hipSuccess) return -1;
+        if (int rc = replay(r, tg, 8)) return rc;
+        continue;
+      }
+      if (cnt > 0 && process_hits() != 0) return -1;
+    }
+    return 0;
+  };
+  // Closes the open group: waits for it (timing, budget check) and, while the
+  // answer is open, watches the hit buffer.  resolve: resolve every recorded
+  // hit in the same sync (end of a fused step); otherwise a filling buffer is
+  // drained between groups.  A group that overflowed the buffer is replayed
+  // in parts after the recorded hits (everything before the group and part
+  // of it) are resolved: resolving any subset of the hits only lowers slots
+  // to verified ranks.  Returns 1 when over the budget.
+  auto close_group = [&](bool resolve) -> int {
+    if (!group_open && !resolve) return 0;
+    hipError_t e = group_open ? hipEventRecord(ctx.ev1, st) : hipSuccess;
+    uint32_t cnt = 0;
+    if (e == hipSuccess && resolve) {
+      if (process_hits(true) != 0) return -1;
+      cnt = *(const uint32_t *)ctx.pinned;
+    } else if (e == hipSuccess && group_open) {
+      e = hipEventSynchronize(ctx.ev1);
+      // drain a filling hit buffer between groups (rare: many window hits
+      // that other samples reject)
+      if (e == hipSuccess && answer < 0) e = hipMemcpy(&cnt, d_hc, 4, hipMemcpyDeviceToHost);
+    }
+    float ms = 0.f;
+    if (e == hipSuccess && group_open) e = hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1);
+    if (e != hipSuccess) {
+      ctx.last_error = std::string("pair launch: ") + hipGetErrorString(e);
+      return -1;
+    }
+    ktime += ms * 1e-3;
+    group_open = false;
+    group_tiles = 0;
+    if (resolve || answer < 0) {
+      if (cnt > kHitCap) {
+        if (group.empty()) {
+          ctx.last_error = "window-hit buffer overflow";
+          return -1;
+        }
+        if (!resolve && process_hits(true) != 0) return -1;
+        ++replays;
+        for (const auto &q : group)
+          if (int rc = replay(q.first, q.second, 8)) return rc;
+      } else if (!resolve && cnt > kHitCap / 2) {
+        if (process_hits() != 0) return -1;
+      }
+    }
+    group.clear();
+    return over_budget() ? 1 : 0;
+  };
+
   // Runs the pair {j, m} (j <= m) of size s for the operator instances in
   // `mask`, in launches of bounded work (B sides above Kt one block at a
-  // time).  Returns 1 when the budget stopped it.
+  // time).  keep_open: the last launch group stays open (closed by the
+  // caller).  Returns 1 when the budget stopped it.
   std::map<int, std::vector<BBlock>> bcache;  // B blocks per size above Kt
-  auto run_pair = [&](int s, int j, uint32_t mask, const CheckTargets &tg) -> int {
+  auto run_pair = [&](int s, int j, uint32_t mask, const CheckTargets &tg,
+                      bool keep_open = false) -> int {
     const int m = s - 1 - j;
     PairLaunch p{};
     p.sb = s;
@@ -549,95 +664,6 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     }
     const uint64_t per_tile = (uint64_t)kPairTA * tb;
     const uint64_t tiles_per_launch = std::max<uint64_t>(1, kPairsPerLaunch / per_tile);
-    // Replays a launch's tile range in `pieces` parts, resolving each part's
-    // hits before the next; a part that still overflows the hit buffer is
-    // split again (a single tile that overflows it is an error).
-    std::function<int(const PairLaunch &, int)> replay = [&](const PairLaunch &q,
-                                                              int pieces) -> int {
-      const uint64_t n = q.tile_end - q.tile_begin;
-      const uint64_t step = std::max<uint64_t>(1, (n + pieces - 1) / pieces);
-      for (uint64_t b = q.tile_begin; b < q.tile_end; b += step) {
-        PairLaunch r = q;
-        r.tile_begin = b;
-        r.tile_end = std::min(q.tile_end, b + step);
-        const int g = (int)std::min<uint64_t>((uint64_t)grid, r.tile_end - r.tile_begin);
-        uint32_t cnt = 0;
-        hipError_t e =
-            launch_pair_check(W, d_tv, ctx.tables.p, tv.off, r, tg, d_hits, d_hc, kHitCap, g, st);
-        if (e == hipSuccess) e = hipMemcpyAsync(&cnt, d_hc, 4, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);  // `st` does not wait on null
-        if (e != hipSuccess) {
-          ctx.last_error = std::string("pair replay: ") + hipGetErrorString(e);
-          return -1;
-        }
-        ++klaunch;
-        if (cnt > kHitCap) {
-          if (r.tile_end - r.tile_begin == 1) {
-            ctx.last_error = "hit buffer overflow within one tile";
-            return -1;
-          }
-          if (hipMemsetAsync(d_hc, 0, 4, st) != hipSuccess) return -1;
-          if (int rc = replay(r, 8)) return rc;
-          continue;
-        }
-        if (cnt > 0) {
-          if (process_hits() != 0) return -1;
-          if (hipMemsetAsync(d_hc, 0, 4, st) != hipSuccess) return -1;
-        }
-      }
-      return 0;
-    };
-    // launches go out back to back; the stream is synchronised (timing and
-    // budget check) once a group holds about kPairsPerLaunch pairs
-    uint64_t group_tiles = 0;
-    bool group_open = false;
-    std::vector<PairLaunch> group;  // the open group's launches (for a replay)
-    auto close_group = [&]() -> int {
-      if (!group_open) return 0;
-      hipError_t e = hipEventRecord(ctx.ev1, st);
-      if (e == hipSuccess) e = hipEventSynchronize(ctx.ev1);
-      float ms = 0.f;
-      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1);
-      if (e != hipSuccess) {
-        ctx.last_error = std::string("pair launch: ") + hipGetErrorString(e);
-        return -1;
-      }
-      ktime += ms * 1e-3;
-      group_open = false;
-      group_tiles = 0;
-      if (answer < 0) {
-        // drain a filling hit buffer between groups (rare: many window
-        // hits that other samples reject)
-        uint32_t cnt = 0;
-        e = hipMemcpy(&cnt, d_hc, 4, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) {
-          ctx.last_error = std::string("hit count: ") + hipGetErrorString(e);
-          return -1;
-        }
-        if (cnt > kHitCap) {
-          // the group overflowed the buffer: resolve the hits recorded
-          // before it (an intact prefix), then replay its launches in parts
-          if (hipMemsetD32Async((hipDeviceptr_t)d_hc, (int)hits_base, 1, st) != hipSuccess) {
-            ctx.last_error = "hit count reset failed";
-            return -1;
-          }
-          if (hits_base > 0 && process_hits() != 0) return -1;
-          if (hipMemsetAsync(d_hc, 0, 4, st) != hipSuccess) return -1;
-          hits_base = 0;
-          ++replays;
-          for (const PairLaunch &q : group)
-            if (int rc = replay(q, 8)) return rc;
-        } else if (cnt > kHitCap / 2) {
-          if (process_hits() != 0) return -1;
-          if (hipMemsetAsync(d_hc, 0, 4, st) != hipSuccess) return -1;
-          hits_base = 0;
-        } else {
-          hits_base = cnt;
-        }
-      }
-      group.clear();
-      return over_budget() ? 1 : 0;
-    };
     for (const BBlock &bb : *blocks) {
       p.bb = bb;
       p.nTB = p.gen_b ? block_tiles(bb, p.rb) : (p.nB + tb - 1) / tb;
@@ -658,14 +684,14 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
           return -1;
         }
         ++klaunch;
-        group.push_back(p);
+        group.emplace_back(p, tg);
         group_tiles += ntl;
         if (group_tiles >= tiles_per_launch) {
-          if (int rc = close_group()) return rc;
+          if (int rc = close_group(false)) return rc;
         }
       }
     }
-    return close_group();
+    return keep_open ? 0 : close_group(false);
   };
 
   for (int s = 3; (answer < 0 || o.exhaustive) && s <= max_size; ++s) {
@@ -673,15 +699,44 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
       status = MBA_BUDGET;
       break;
     }
+    const int last = fused_last(s);
+    if (last >= s) {
+      // fused step: every split of sizes s..last, all seven operator
+      // instances, one launch group (the hit buffer is empty, or holds the
+      // leaves' hits)
+      int rc = 0;
+      for (int z = s; z <= last && rc == 0; ++z) {
+        const CheckTargets tgz = targets_for(t0, z, max_size, W);
+        for (int j = 1; j <= (z - 1) / 2 && rc == 0; ++j) rc = run_pair(z, j, kAllInst, tgz, true);
+      }
+      if (rc < 0) return MBA_ERR_HIP;
+      if (rc == 1) {
+        status = MBA_BUDGET;  // stopped before the step's hits were resolved
+        break;
+      }
+      rc = close_group(true);  // resolves the hits; 1 when over the budget after it
+      if (rc < 0) return MBA_ERR_HIP;
+      for (int z = leaf_pending ? 1 : s; z <= last; ++z) {
+        reached = z;
+        if (o.verbose && z >= 3)
+          std::fprintf(stderr, "Length %-2d | candidates %-20llu | %.3f s\n", z,
+                       (unsigned long long)T.t[z][8], seconds_since(t_start));
+        if (z >= min_size && found[z] != ~0ull) {
+          answer = z;
+          break;
+        }
+      }
+      leaf_pending = false;
+      if (answer < 0 && rc == 1) {
+        status = MBA_BUDGET;
+        break;
+      }
+      s = last;
+      continue;
+    }
     MBA_HIP(hipMemsetAsync(d_hc, 0, 4, st));
-    hits_base = 0;
     const CheckTargets tg = targets_for(t0, s, max_size, W);
     bool stopped = false;
-    // block-at-a-time only pays where a size's sweep outweighs a launch and
-    // a sync per block; below that the fused sweep (minimum rank, also a
-    // valid answer for the learned order) is used
-    const uint64_t block_min = o.block_log2 > 0 ? (1ull << std::min(o.block_log2, 63))
-                               : block_scores ? kLearnedBlockMin : kBlockModeMin;
     const bool by_block = !o.exhaustive && T.t[s][8] >= block_min;
     if (!o.exhaustive && s >= min_size && found[s] != ~0ull &&
         (block_scores || found[s] < T.t[s][NOT])) {
@@ -693,16 +748,19 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
       break;
     }
     if (!by_block) {
-      // every split of this size, all seven operator instances fused
+      // every split of this size, all seven operator instances fused, in one
+      // launch group (unless large) whose hits are resolved in its closing
+      // sync; once the answer is settled (an exhaustive sweep going on to
+      // max_size) the windows' hits no longer matter
       for (int j = 1; j <= (s - 1) / 2 && !stopped; ++j) {
-        const int rc = run_pair(s, j, kAllInst, tg);
+        const int rc = run_pair(s, j, kAllInst, tg, true);
         if (rc < 0) return MBA_ERR_HIP;
         stopped = rc == 1;
       }
-      // once the answer is settled (an exhaustive sweep going on to max_size)
-      // the windows' hits no longer matter: their count may exceed the buffer
-      if (!stopped && answer < 0) {
-        if (int rc = process_hits()) return rc;
+      if (!stopped) {
+        const int rc = close_group(answer < 0);
+        if (rc < 0) return MBA_ERR_HIP;
+        stopped = rc == 1;
       }
     } else {
       // (op, split) blocks in rank order (plain), or splits in the order of
@@ -732,8 +790,6 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
           r0 = r1;
           if (stopped) break;
           if (int rc2 = process_hits()) return rc2;
-          MBA_HIP(hipMemsetAsync(d_hc, 0, 4, st));
-          hits_base = 0;
           if (s >= min_size && found[s] != ~0ull) break;
         }
       } else {
@@ -749,8 +805,6 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
             break;
           }
           if (int rc2 = process_hits()) return rc2;
-          MBA_HIP(hipMemsetAsync(d_hc, 0, 4, st));
-          hits_base = 0;
           if (found[s] == ~0ull || s < min_size) continue;
           const uint64_t prefix = b.end == T.op_offset(s, NEG) ? T.op_offset(s, ADD) : b.end;
           if (found[s] < prefix) break;

@@ -422,6 +422,7 @@ __device__ __forceinline__ void gen_b_tile(const TableView *__restrict__ tv, int
 struct AStage {
   uint4 q0[kPairTA];
   uint4 q1[kPairTA];
+  uint64_t full[kPairTA];  // the full-width A values (exact re-walk of a tile)
 };
 
 // a * b = t (mod 2^32) as a compare: with a = 2^k a' (a' odd), it holds iff
@@ -483,6 +484,7 @@ __device__ __forceinline__ bool stage_a(AStage &S, const TableView *__restrict__
   bool bit = false;
   if (a0 + i < a1) {
     const word av = gen_a ? value_of<word>(tv, j, a0 + i) : A[a0 + i];
+    S.full[i] = (uint64_t)av;
     const uint32_t al = lo32(av);
     if (exact) {
       const uint32_t gap = bitops_gap(al, le);
@@ -861,20 +863,39 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
     hit = acc <= sp;
   }
   hit = hit && any_ok;
-  // Rare: some lane passed the prefilter.  Re-walk the tile, exact test.
+  // Rare: some lane passed the prefilter.  Re-walk the tile, exact test,
+  // with the A values from LDS: per B value, count its hits, reserve their
+  // records with one atomic and write them.
   if (__any(hit)) {
     if (hit) {
+      const int ninst = SAME ? 6 : 7;
       for (int k = 0; k < RB; ++k) {
         if (!ok[k]) continue;
         uint64_t b;
         const word bv = b_full<word, BSRC, RB>(tv, B, p, bt, k, b);
-        for (uint64_t a = a0; a < a1; ++a) {
-          const word av = p.gen_a ? value_of<word>(tv, p.j, a) : A[a];
-          for (int inst = 0; inst < (SAME ? 6 : 7); ++inst) {
-            if (!(p.mask >> inst & 1u)) continue;
+        uint32_t n = 0;
+        for (uint32_t i = 0; i < na; ++i) {
+          const word av = (word)S.full[i];
+          for (int inst = 0; inst < ninst; ++inst)
+            n += (p.mask >> inst & 1u) && in_window(inst_value<word>(inst, av, bv), lo_e, lo_o, span);
+        }
+        if (n == 0) continue;
+        uint32_t r = atomicAdd(hit_count, n);
+        for (uint32_t i = 0; i < na && r < hit_cap; ++i) {
+          const word av = (word)S.full[i];
+          for (int inst = 0; inst < ninst; ++inst) {
             const word c = inst_value<word>(inst, av, bv);
-            if (in_window(c, lo_e, lo_o, span))
-              record_hit<word>(hits, hit_count, hit_cap, a, b, c, inst | tag, pair);
+            if (!((p.mask >> inst & 1u) && in_window(c, lo_e, lo_o, span))) continue;
+            if (r < hit_cap) {
+              HitRec h;
+              h.a = a0 + i;
+              h.b = b;
+              h.v = (uint64_t)c;
+              h.inst = (uint32_t)(inst | tag);
+              h.pair = pair;
+              hits[r] = h;
+            }
+            ++r;
           }
         }
       }
