@@ -704,6 +704,21 @@ __device__ __forceinline__ void b_lo(const TableView *__restrict__ tv,
   }
 }
 
+// Whether lo32(op(a, b)) lies in a window's low half for some operator
+// instance (a necessary condition of the exact test, as in the tile loops).
+__device__ __forceinline__ bool pair_may_hit(uint32_t a, uint32_t b, uint32_t le, uint32_t lo,
+                                             uint32_t span) {
+  const uint32_t x_and = a & b, x_or = a | b, x_xor = a ^ b, x_add = a + b, x_mul = a * b;
+  const uint32_t x_ab = a - b, x_ba = b - a;
+  uint32_t m = min3u(x_and - le, x_and - lo, x_or - le);
+  m = min3u(m, x_or - lo, x_xor - le);
+  m = min3u(m, x_xor - lo, x_add - le);
+  m = min3u(m, x_add - lo, x_mul - le);
+  m = min3u(m, x_mul - lo, x_ab - le);
+  m = min3u(m, x_ab - lo, x_ba - le);
+  return min(m, x_ba - lo) <= span;
+}
+
 // ---- the fused checks -------------------------------------------------------
 // Exact mode: each compare's lane mask is or-ed into one wave-wide mask on
 // the scalar unit (one VALU compare + one scalar or per candidate); the
@@ -864,7 +879,8 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
   }
   hit = hit && any_ok;
   // Rare: some lane passed the prefilter.  Re-walk the tile, exact test,
-  // with the A values from LDS: per B value, count its hits, reserve their
+  // with the A values from LDS: per B value, count its hits (the pairs that
+  // pass the low-half test of every instance checked in full), reserve their
   // records with one atomic and write them.
   if (__any(hit)) {
     if (hit) {
@@ -873,9 +889,11 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
         if (!ok[k]) continue;
         uint64_t b;
         const word bv = b_full<word, BSRC, RB>(tv, B, p, bt, k, b);
+        const uint32_t b32 = lo32(bv);
         uint32_t n = 0;
         for (uint32_t i = 0; i < na; ++i) {
           const word av = (word)S.full[i];
+          if (!pair_may_hit(lo32(av), b32, le, lo, sp)) continue;
           for (int inst = 0; inst < ninst; ++inst)
             n += (p.mask >> inst & 1u) && in_window(inst_value<word>(inst, av, bv), lo_e, lo_o, span);
         }
@@ -883,6 +901,7 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
         uint32_t r = atomicAdd(hit_count, n);
         for (uint32_t i = 0; i < na && r < hit_cap; ++i) {
           const word av = (word)S.full[i];
+          if (!pair_may_hit(lo32(av), b32, le, lo, sp)) continue;
           for (int inst = 0; inst < ninst; ++inst) {
             const word c = inst_value<word>(inst, av, bv);
             if (!((p.mask >> inst & 1u) && in_window(c, lo_e, lo_o, span))) continue;
