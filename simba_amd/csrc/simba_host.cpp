@@ -236,12 +236,18 @@ std::vector<BlockRef> binary_blocks(const CountTable &T, int s) {
 }
 
 CheckTargets targets_for(uint64_t t0, int sb, int max_size, int W) {
+  // A chain of u <= U unary nodes over a tree of value v computes
+  // (-1)^u v + k; even u reach |k| <= u/2, odd u reach
+  // -(u+1)/2 <= k <= (u-1)/2 (each '~' adds -1 to the sign-adjusted
+  // constant, each neg 0).  So v = t0 - k lies in [t0 - U/2, t0 + U/2] and,
+  // for odd chains, v = k - t0 in [-t0 - (U+1)/2, -t0 + (U-1)/2]: both
+  // inside windows of span U (one value wider than needed for one parity).
   const uint64_t M = wmask(W);
   const uint64_t U = (uint64_t)(max_size - sb);
   CheckTargets tg;
-  tg.lo_even = (t0 - U) & M;
-  tg.lo_odd = ((0 - t0) - U) & M;
-  tg.span = 2 * U;
+  tg.lo_even = (t0 - U / 2) & M;
+  tg.lo_odd = ((0 - t0) - (U + 1) / 2) & M;
+  tg.span = U;
   return tg;
 }
 

@@ -25,9 +25,10 @@ struct HitRec {
 };
 
 // Targets for the fused checks.  A value c is recorded when
-//   (c - lo_even) mod 2^W <= span   (c = t0 + k,  |k| <= U: even unary chains)
-// or (c - lo_odd)  mod 2^W <= span   (c = -t0 + k: odd unary chains)
-// with span = 2U.  U = 0 reduces the first window to c == t0.
+//   (c - lo_even) mod 2^W <= span   (c = t0 + k, |k| <= U/2: even unary chains)
+// or (c - lo_odd)  mod 2^W <= span   (c = -t0 + k, -(U+1)/2 <= k <= (U-1)/2:
+// odd unary chains) with span = U, the number of unary nodes that still fit
+// above the candidate.  U = 0 reduces the first window to c == t0.
 struct CheckTargets {
   uint64_t lo_even, lo_odd, span;
 };

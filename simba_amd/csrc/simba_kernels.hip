@@ -11,9 +11,10 @@
 // lane holds RB (4 or 8) values of the B side in VGPRs; the A side is staged
 // in LDS and read as wave-uniform broadcasts, so the inner loop is integer
 // VALU work with no divergence.  Unary-rooted trees are the composition of a
-// chain of ~/neg with a binary-rooted tree or a leaf; such a chain maps x to
-// +-x + k with |k| <= chain length, so two windowed compares per value cover
-// every unary tree of every larger size.  Hits are rare; they are appended
+// chain of ~/neg with a binary-rooted tree or a leaf; a chain of u nodes maps
+// x to (-1)^u x + k with |k| <= (u+1)/2, so two windowed compares per value
+// (span U = the nodes that still fit) cover every unary tree of every larger
+// size.  Hits are rare; they are appended
 // to a hit buffer, resolved to (size, rank) on the device (k_hit_screen,
 // k_hit_confirm) and verified on all samples (verify_wave).
 #include "simba_kernels.h"

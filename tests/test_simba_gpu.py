@@ -126,7 +126,7 @@ def test_simba_launch_configs_keep_answers(ctx, cfg):
 # (instance index in the held-out learned evaluation set, max size, hit
 # buffer): launch groups whose window hits overflow the small buffer are
 # replayed in parts; the answers stay those of the default buffer
-REPLAY_CASES = [(10, 9, 256), (15, 11, 256), (31, 9, 1024), (34, 9, 4096)]
+REPLAY_CASES = [(19, 9, 256), (15, 11, 256), (31, 9, 1024), (34, 9, 4096), (46, 11, 64)]
 
 
 @pytest.mark.parametrize("idx,max_size,cap", REPLAY_CASES)
