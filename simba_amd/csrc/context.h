@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <functional>
 #include <string>
 
 #include "../../include/mba_api.h"
@@ -80,9 +81,12 @@ inline double seconds_since(Clock::time_point t0) {
 // block_scores: NULL for the plain (rank-order, minimum-rank) search, else
 // per (op, left size) scores at [op * 32 + split] that order the binary
 // blocks of each size (learned proposal; any solution of minimal size).
+// score_init: when given, fills block_scores the first time the search
+// orders a size by them (a search settled below that runs no network).
 // rtid: the SIMBA-RTid order instead (nocache_rtid.cu:85-95).
 int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &opts,
-                mba_result &res, const float *block_scores = nullptr, bool rtid = false);
+                mba_result &res, const float *block_scores = nullptr, bool rtid = false,
+                const std::function<int()> *score_init = nullptr);
 int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &opts,
               mba_result &res, mba_vfb_stats *stats);
 // scores: NULL (plain, minimum rank per instance) or count x 256 block

@@ -394,25 +394,20 @@ int mba_learned_solve(mba_ctx *c, mba_mlp *m, const mba_instance *inst, const mb
     opts = &d;
   }
   // the scores order the blocks of sizes with at least kLearnedBlockMin
-  // trees; below that every size is swept fused and the network is not run
-  const auto t0 = Clock::now();
-  const int L = (int)(inst->n_vars + inst->n_consts);
-  const int top = opts->compat ? opts->max_size - 1 : opts->max_size;
-  CountTable T;
-  const bool uses_scores = L >= 1 && top >= 1 && top <= kMaxSize &&
-                           (!T.build(L, top) || T.t[top][8] >= kLearnedBlockMin);
-  float feat[kMlpIn], logits[kMlpOut], scores[8 * 32];
+  // trees; the network runs when the search first reaches such a size (its
+  // time is inside the search's, so time to solution includes it), and not
+  // at all for a search settled below
+  float scores[8 * 32];
   std::fill(scores, scores + 8 * 32, 0.f);
-  if (uses_scores) {
+  const std::function<int()> score_init = [&]() -> int {
+    float feat[kMlpIn], logits[kMlpOut];
     if (int rc = mba_features(inst, feat)) return rc;
     if (int rc = mba_mlp_forward(m, feat, 1, logits)) return rc;
     block_scores_from_logits(logits, scores);
-  }
-  const double t_inference = seconds_since(t0);
+    return hipSetDevice(c->c.device) == hipSuccess ? 0 : MBA_ERR_HIP;
+  };
   if (hipSetDevice(c->c.device) != hipSuccess) return MBA_ERR_HIP;
-  const int rc = simba_solve(c->c, *inst, *opts, *res, scores);
-  res->seconds += t_inference;  // time to solution includes the network
-  return rc;
+  return simba_solve(c->c, *inst, *opts, *res, scores, false, &score_init);
 }
 
 int mba_learned_batch_solve(mba_ctx *c, mba_mlp *m, const mba_instance *insts, size_t count,

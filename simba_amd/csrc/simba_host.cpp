@@ -279,7 +279,7 @@ void fill_result(const CountTable &T, int size, uint64_t rank, int n_vars,
 }
 
 int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result &res,
-                const float *block_scores, bool rtid) {
+                const float *block_scores, bool rtid, const std::function<int()> *score_init) {
   const auto t_start = Clock::now();
   std::memset(&res, 0, sizeof res);
   res.status = MBA_NOT_FOUND;
@@ -772,6 +772,10 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
       // (op, split) blocks in rank order (plain), or splits in the order of
       // the learned scores; stop once the answer is settled
       std::vector<BlockRef> blocks = binary_blocks(T, s);
+      if (block_scores && score_init) {
+        if (int rc = (*score_init)()) return rc;
+        score_init = nullptr;
+      }
       if (block_scores) {
         std::stable_sort(blocks.begin(), blocks.end(), [&](const BlockRef &x, const BlockRef &y) {
           return block_scores[x.op * 32 + x.split] > block_scores[y.op * 32 + y.split];
