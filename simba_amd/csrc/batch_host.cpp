@@ -345,6 +345,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
           J.p.m = s - 1 - j;
           J.p.nA = T.t[j][8];
           J.p.nB = T.t[J.p.m][8];
+          J.p.ta = kPairTA;
           J.p.nTA = (J.p.nA + kPairTA - 1) / kPairTA;
           J.p.gen_a = j > st.Kt;
           J.p.gen_b = J.p.m > st.Kt;

@@ -26,6 +26,10 @@ constexpr uint64_t kSmallTableValues = 1u << 16;
 constexpr uint64_t kDefaultTableBytes = 40ull << 30;
 // Work per launch, in A x B value pairs, so that budgets are honoured.
 constexpr uint64_t kPairsPerLaunch = 1ull << 36;
+// pair launches of fewer tiles than kSmallLaunchTiles use A tiles down to
+// kMinTA values
+constexpr uint64_t kSmallLaunchTiles = 128;
+constexpr uint32_t kMinTA = 16;
 // trees of the small sizes swept in one fused step (launched back to back,
 // their hits resolved in one sync)
 constexpr uint64_t kFuseTrees = 1ull << 22;
@@ -651,7 +655,6 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     p.m = m;
     p.nA = T.t[j][8];
     p.nB = T.t[m][8];
-    p.nTA = (p.nA + kPairTA - 1) / kPairTA;
     p.rb = o.lanes_b == 4 ? 4 : 8;
     const uint64_t tb = (uint64_t)kPairThreads * p.rb;
     p.gen_a = j > Kt;
@@ -673,6 +676,11 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     for (const BBlock &bb : *blocks) {
       p.bb = bb;
       p.nTB = p.gen_b ? block_tiles(bb, p.rb) : (p.nB + tb - 1) / tb;
+      // a launch of few tiles gets narrower A tiles, so that more
+      // workgroups share it (latency of the small sizes)
+      p.ta = kPairTA;
+      while (p.ta > kMinTA && (p.nA + p.ta - 1) / p.ta * p.nTB < kSmallLaunchTiles) p.ta /= 2;
+      p.nTA = (p.nA + p.ta - 1) / p.ta;
       const uint64_t tiles = p.nTA * p.nTB;
       for (uint64_t t0 = 0; t0 < tiles; t0 += tiles_per_launch) {
         p.tile_begin = t0;

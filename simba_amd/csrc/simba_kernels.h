@@ -69,6 +69,7 @@ struct PairCore {
   bool gen_a, gen_b;       // side is above Kt: values computed per lane
   uint32_t mask;           // operator instances to check (bit i = inst i)
   int rb;                  // B values per lane: 4 or 8 (tile = 256 * rb B values)
+  uint32_t ta;             // A values per tile (<= kPairTA; nTA = ceil(nA / ta))
 };
 
 struct PairLaunch : PairCore {

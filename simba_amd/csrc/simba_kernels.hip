@@ -856,8 +856,8 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
                                           uint32_t *__restrict__ hit_count, uint32_t hit_cap,
                                           uint32_t tag) {
   const uint32_t pair = ((uint32_t)p.sb << 16) | (uint32_t)p.j;
-  const uint64_t a0 = at * kPairTA;
-  const uint64_t a1 = a0 + kPairTA < p.nA ? a0 + kPairTA : p.nA;
+  const uint64_t a0 = at * p.ta;
+  const uint64_t a1 = a0 + p.ta < p.nA ? a0 + p.ta : p.nA;
   const uint32_t na = (uint32_t)(a1 - a0);
   uint32_t bl[RB];
   bool ok[RB];
@@ -938,8 +938,8 @@ k_pair_check(const TableView *__restrict__ tv, const word *__restrict__ A,
   for (uint64_t t = p.tile_begin + blockIdx.x; t < p.tile_end; t += gridDim.x) {
     const uint64_t at = p.nTA == 1 ? 0 : t % p.nTA, bt = p.nTA == 1 ? t : t / p.nTA;
     if (at != staged) {  // workgroup-uniform: every wave takes the same tiles
-      const uint64_t a0 = at * kPairTA;
-      const uint64_t a1 = a0 + kPairTA < p.nA ? a0 + kPairTA : p.nA;
+      const uint64_t a0 = at * p.ta;
+      const uint64_t a1 = a0 + p.ta < p.nA ? a0 + p.ta : p.nA;
       bitops = stage_a<word>(S, tv, A, p.gen_a, p.j, a0, a1, exact, le, lo, (uint32_t)span);
       staged = at;
     }
@@ -1306,8 +1306,8 @@ __device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J, ui
   const uint32_t tag = J.inst << 8;
   const uint64_t at = p.nTA == 1 ? 0 : lt % p.nTA, bt = p.nTA == 1 ? lt : lt / p.nTA;
   if (at != staged) {  // workgroup-uniform; the caller resets `staged` per job
-    const uint64_t a0 = at * kPairTA;
-    const uint64_t a1 = a0 + kPairTA < p.nA ? a0 + kPairTA : p.nA;
+    const uint64_t a0 = at * p.ta;
+    const uint64_t a1 = a0 + p.ta < p.nA ? a0 + p.ta : p.nA;
     bitops = stage_a<word>(S, tv, A, p.gen_a, p.j, a0, a1,
                            exact_layout(p.mask, p.j == p.m, (uint32_t)span), lo32(lo_e),
                            lo32(lo_o), (uint32_t)span);
@@ -1598,6 +1598,9 @@ hipError_t launch_pair_check(int width, const TableView *d_tv, const void *d_bas
                              const CheckTargets &tg, HitRec *d_hits,
                              uint32_t *d_hit_count, uint32_t hit_cap, int grid,
                              hipStream_t stream) {
+  // the A tiles must cover [0, nA) as the kernel maps them
+  if (p.ta == 0 || p.ta > (uint32_t)kPairTA || p.nTA != (p.nA + p.ta - 1) / p.ta)
+    return hipErrorInvalidValue;
   // sides above Kt have no table: their pointer is unused by the kernel
   if (width == 32) {
     const uint32_t *base = (const uint32_t *)d_base;
