@@ -28,7 +28,11 @@ namespace mba {
 namespace {
 
 constexpr uint32_t kBatchHitCap = 1u << 24;  // 512 MiB of hit records
-constexpr uint64_t kBatchTableBytes = 32ull << 30;
+// default value-table budget of a batch, split evenly over its instances
+// (at most 2/5 of the device memory): 4096 instances of up to 4 leaves keep
+// V[8] in tables at a size budget of 11, so the last size's B sides are
+// formed from tabulated children
+constexpr uint64_t kBatchTableBytes = 96ull << 30;
 
 struct InstState {
   int L = 0, n = 0, fs = 0, Kt = 1;
@@ -79,8 +83,10 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   std::vector<InstState> S(count);
   std::vector<uint64_t> found_all(count * (kMaxSize + 2), ~0ull);
   std::vector<uint64_t> leaf_off(count), out_off(count);
-  const uint64_t per_budget =
-      std::max<uint64_t>((o.table_bytes ? o.table_bytes : kBatchTableBytes) / count, 1 << 16);
+  const uint64_t table_budget =
+      o.table_bytes ? (uint64_t)o.table_bytes
+                    : std::min<uint64_t>(kBatchTableBytes, ctx.hbm_bytes / 5 * 2);
+  const uint64_t per_budget = std::max<uint64_t>(table_budget / count, 1 << 16);
   // pass 1 (in order): limits, count tables per leaf count, sample offsets
   uint64_t nleaf = 0, nout = 0;
   for (size_t i = 0; i < count; ++i) {
