@@ -175,8 +175,30 @@ __device__ word eval_tree(const TableView *__restrict__ tv, int size, uint64_t r
 // unrank order, nocache.cu:52-79), the children are read from V[<m].
 // ---------------------------------------------------------------------------
 template <typename word>
+__device__ __forceinline__ word build_value32(const TableView *__restrict__ tv, int m,
+                                              uint32_t r) {
+  // every count below is at most T[m][8] < 2^32: 32-bit arithmetic is exact
+  const word *V = global_ptr<const word>(tv->base);
+  uint32_t x = r;
+  int op = 0;
+  while (x >= (uint32_t)tv->T[m][op]) x -= (uint32_t)tv->T[m][op++];
+  if (is_unary(op)) return apply_op<word>(op, V[tv->off[m - 1] + x], word(0));
+  const int cs = m - 1;
+  int j = 1;
+  for (;; ++j) {
+    const uint32_t blk = (uint32_t)tv->T[j][8] * (uint32_t)tv->T[cs - j][8];
+    if (x < blk) break;
+    x -= blk;
+  }
+  const uint32_t rs = (uint32_t)tv->T[cs - j][8];
+  const uint32_t a = x / rs;
+  return apply_op<word>(op, V[tv->off[j] + a], V[tv->off[cs - j] + (x - a * rs)]);
+}
+
+template <typename word>
 __device__ __forceinline__ word build_value(const TableView *__restrict__ tv, int m,
                                             uint64_t r) {
+  if (tv->T[m][8] <= 0xffffffffull) return build_value32<word>(tv, m, (uint32_t)r);
   const word *V = global_ptr<const word>(tv->base);
   uint64_t x = r;
   int op = 0;
