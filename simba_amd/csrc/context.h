@@ -53,8 +53,9 @@ struct Ctx {
   // instance features on the device (packed inputs/outputs, layout, rows)
   DevBuf f_in, f_out, f_meta, f_x;
   // pinned host staging for the per-step read backs of the search drivers
+  // (first 4 KiB) and the job lists of fused steps (the rest)
   void *pinned = nullptr;
-  static constexpr size_t kPinnedBytes = 4096;
+  static constexpr size_t kPinnedBytes = 64 << 10;
   // batched search: pinned job staging and the captured step graph, kept
   // across calls while the buffers the graph names are unchanged
   void *b_pinned = nullptr;

@@ -463,6 +463,8 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   char *d_read = ctx.b_misc.as<char>() + kReadOff;
   uint32_t *d_hc = (uint32_t *)d_read;
   uint32_t *d_nt = d_hc + 1;  // lowered slots, candidates
+  // work-queue head of the fused steps' persistent launches
+  unsigned long long *d_head = (unsigned long long *)ctx.b_misc.p;
   VerifyCand *d_vc = (VerifyCand *)(d_read + 16);
   uint8_t *d_ok = (uint8_t *)(d_vc + kSlots);
   BatchInstData *d_idat = (BatchInstData *)(ctx.b_misc.as<char>() + 128);
@@ -534,6 +536,70 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
                            acc + T.t[e + 1][8] <= kFuseTrees;)
       acc += T.t[++e][8];
     return e;
+  };
+
+  // A fused step as one persistent launch (k_batch_step over the step's
+  // splits as jobs, narrow A tiles for the small ones).  Returns 1 when its
+  // hits were resolved, 0 when the caller must run the splits one launch at
+  // a time (too many jobs for the staging area, or hits that overflowed the
+  // buffer: those resolved from the truncated buffer stand, as they only
+  // lower slots to verified ranks), < 0 on an error.
+  auto fused_persistent = [&](int s0, int s1) -> int {
+    constexpr size_t kStageOff = 4096;
+    constexpr size_t kMaxJobs = (Ctx::kPinnedBytes - kStageOff - 256) / sizeof(BatchJob);
+    auto *h_step = (BatchStep *)((char *)ctx.pinned + kStageOff);
+    auto *h_jobs = (BatchJob *)((char *)ctx.pinned + kStageOff + 256);
+    size_t nj = 0;
+    uint64_t tiles = 0;
+    for (int z = s0; z <= s1; ++z) {
+      const CheckTargets tgz = targets_for(t0, z, max_size, W);
+      for (int j = 1; j <= (z - 1) / 2; ++j) {
+        if (nj == kMaxJobs) return 0;
+        BatchJob &J = h_jobs[nj++];
+        std::memset(&J, 0, sizeof J);
+        J.p.sb = z;
+        J.p.j = j;
+        J.p.m = z - 1 - j;
+        J.p.nA = T.t[j][8];
+        J.p.nB = T.t[J.p.m][8];
+        J.p.gen_a = j > Kt;
+        J.p.gen_b = J.p.m > Kt;
+        J.p.mask = kAllInst;
+        J.p.rb = kPairRB;
+        const uint64_t ntb = (J.p.nB + kPairTB - 1) / kPairTB;
+        J.p.ta = kPairTA;
+        while (J.p.ta > kMinTA && (J.p.nA + J.p.ta - 1) / J.p.ta * ntb < kSmallLaunchTiles)
+          J.p.ta /= 2;
+        J.p.nTA = (J.p.nA + J.p.ta - 1) / J.p.ta;
+        J.ntiles = J.p.nTA * ntb;
+        J.lo_e = tgz.lo_even;
+        J.lo_o = tgz.lo_odd;
+        J.span = tgz.span;
+        J.kind = 1;
+        J.tile0 = tiles;
+        tiles += J.ntiles;
+      }
+    }
+    if (nj == 0) return 0;
+    h_step->total_tiles = tiles;
+    h_step->njobs = (uint32_t)nj;
+    h_step->pad = 0;
+    MBA_HIP(ctx.b_queue.ensure(256 + sizeof(BatchJob) * kMaxJobs));
+    MBA_HIP(hipMemcpyAsync(ctx.b_queue.p, h_step, 256 + sizeof(BatchJob) * nj,
+                           hipMemcpyHostToDevice, st));
+    MBA_HIP(hipMemsetAsync(d_head, 0, 8, st));
+    MBA_HIP(hipEventRecord(ctx.ev0, st));
+    const int g = (int)std::min<uint64_t>((uint64_t)ctx.cu_count * 8, tiles);
+    MBA_HIP(launch_batch_step(W, ctx.b_queue.as<BatchStep>(),
+                              (const BatchJob *)(ctx.b_queue.as<char>() + 256), d_tv, d_head,
+                              d_hits, d_hc, kHitCap, g, st));
+    MBA_HIP(hipEventRecord(ctx.ev1, st));
+    ++klaunch;
+    if (int rc = process_hits(true)) return rc < 0 ? rc : -1;  // syncs
+    float ms = 0.f;
+    MBA_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
+    ktime += ms * 1e-3;
+    return *(const uint32_t *)ctx.pinned > kHitCap ? 0 : 1;
   };
 
   // Step 1: leaves (size 1) and, through unary chains, every unary tree over a
@@ -716,19 +782,28 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     const int last = fused_last(s);
     if (last >= s) {
       // fused step: every split of sizes s..last, all seven operator
-      // instances, one launch group (the hit buffer is empty, or holds the
-      // leaves' hits)
+      // instances (the hit buffer is empty, or holds the leaves' hits).
+      // The splits become the jobs of one persistent launch, so they run
+      // side by side; a step whose hits overflow the buffer (or with more
+      // jobs than the staging holds) runs as one launch per split instead.
       int rc = 0;
-      for (int z = s; z <= last && rc == 0; ++z) {
-        const CheckTargets tgz = targets_for(t0, z, max_size, W);
-        for (int j = 1; j <= (z - 1) / 2 && rc == 0; ++j) rc = run_pair(z, j, kAllInst, tgz, true);
+      const int prc = fused_persistent(s, last);
+      if (prc < 0) return MBA_ERR_HIP;
+      if (prc == 0) {
+        for (int z = s; z <= last && rc == 0; ++z) {
+          const CheckTargets tgz = targets_for(t0, z, max_size, W);
+          for (int j = 1; j <= (z - 1) / 2 && rc == 0; ++j)
+            rc = run_pair(z, j, kAllInst, tgz, true);
+        }
       }
       if (rc < 0) return MBA_ERR_HIP;
       if (rc == 1) {
         status = MBA_BUDGET;  // stopped before the step's hits were resolved
         break;
       }
-      rc = close_group(true);  // resolves the hits; 1 when over the budget after it
+      // resolve the hits (done already by a persistent launch); 1 when over
+      // the budget after it
+      rc = prc == 1 ? (over_budget() ? 1 : 0) : close_group(true);
       if (rc < 0) return MBA_ERR_HIP;
       for (int z = leaf_pending ? 1 : s; z <= last; ++z) {
         reached = z;
