@@ -595,7 +595,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
                               d_hits, d_hc, kHitCap, g, st));
     MBA_HIP(hipEventRecord(ctx.ev1, st));
     ++klaunch;
-    if (int rc = process_hits(true)) return rc < 0 ? rc : -1;  // syncs
+    if (int rc = process_hits(true)) return rc;  // syncs; rc < 0
     float ms = 0.f;
     MBA_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
     ktime += ms * 1e-3;
@@ -788,7 +788,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
       // jobs than the staging holds) runs as one launch per split instead.
       int rc = 0;
       const int prc = fused_persistent(s, last);
-      if (prc < 0) return MBA_ERR_HIP;
+      if (prc < 0) return prc;
       if (prc == 0) {
         for (int z = s; z <= last && rc == 0; ++z) {
           const CheckTargets tgz = targets_for(t0, z, max_size, W);
