@@ -455,7 +455,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   // of the hits that meet every sample, flags and the list of lowered slots;
   // the instance's sample layout for it at byte 128 of the header.
   // The read-back area (the pinned layout: hit count, lowered slot count,
-  // candidate count, candidates, verdicts) follows, so one copy returns it.
+  // candidate count, candidates, their check results) follows, so one copy returns it.
   constexpr size_t kSlots = kMaxSize + 1;
   constexpr size_t kReadOff = (256 + kSlots * 16 + 255) / 256 * 256;
   constexpr size_t kReadBytes = 16 + kSlots * (sizeof(VerifyCand) + 1);
