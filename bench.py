@@ -99,7 +99,8 @@ def run_simba(args):
     T, C = naive_ops_per_size(L, S, 64)
     cands = sum(T[s][8] for s in range(1, S + 1))
     ctx = simba_amd.Context(0)
-    kw = dict(min_size=1, max_size=S, exhaustive=True, table_bytes=args.table_gb << 30)
+    kw = dict(min_size=1, max_size=args.window_max, stop_size=S, exhaustive=True,
+              table_bytes=args.table_gb << 30)
     tuned = {k: v for k, v in (("lanes_b", args.lanes_b), ("wg_per_cu", args.wg_per_cu)) if v}
     kw.update(tuned)
     if args.autotune:
@@ -456,6 +457,9 @@ def main():
     ap.add_argument("--label-size", type=int, default=9)
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--size", type=int, default=15)
+    ap.add_argument("--window-max", type=int, default=20,
+                    help="simba: the search's max size (unary-chain windows of size "
+                         "max - s); the sweep ends after --size")
     ap.add_argument("--cpu-size", type=int, default=11)
     ap.add_argument("--table-gb", type=int, default=40)
     ap.add_argument("--vfb-levels", type=int, default=13)

@@ -86,7 +86,12 @@ typedef struct {
                             /* batched step by halves of its jobs, then of   */
                             /* one job's tiles (a smaller buffer that one    */
                             /* tile overflows is an error)                   */
-  int32_t reserved[2];
+  int32_t stop_size;        /* SIMBA: end the sweep after this size (0:     */
+                            /* max_size); the unary-chain windows stay those */
+                            /* of max_size, so sizes 1..stop_size are swept  */
+                            /* exactly as in a search to max_size that is    */
+                            /* interrupted there (benchmarks of one shape)   */
+  int32_t reserved[1];
 } mba_opts;
 
 typedef struct {

@@ -44,7 +44,7 @@ class Opts(ctypes.Structure):
                 ("verbose", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32),
                 ("lanes_b", ctypes.c_int32), ("table_delta", ctypes.c_int32),
                 ("block_log2", ctypes.c_int32), ("hit_cap", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 2)]
+                ("stop_size", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1)]
 
 
 class Result(ctypes.Structure):
@@ -347,11 +347,11 @@ class Mlp:
 
 def make_opts(min_size=1, max_size=9, compat=False, exhaustive=False, budget_s=0.0,
               table_bytes=0, cache_bytes=0, verbose=False, cancel=None, wg_per_cu=0,
-              lanes_b=0, table_delta=0, block_log2=0, hit_cap=0):
+              lanes_b=0, table_delta=0, block_log2=0, hit_cap=0, stop_size=0):
     o = Opts()
     lib().mba_default_opts(ctypes.byref(o))
     o.wg_per_cu, o.lanes_b, o.table_delta = wg_per_cu, lanes_b, table_delta
-    o.block_log2, o.hit_cap = block_log2, hit_cap
+    o.block_log2, o.hit_cap, o.stop_size = block_log2, hit_cap, stop_size
     o.min_size, o.max_size = min_size, max_size
     o.compat, o.exhaustive = int(compat), int(exhaustive)
     o.time_budget_s = budget_s

@@ -36,15 +36,21 @@ constexpr uint64_t kFuseTrees = 1ull << 22;
 
 // Tile geometry of a B block of nA x nB values (see BBlock): columns run
 // over the larger dimension, 2^logTB of them, and the tile's remaining
-// (256*rb) >> logTB lanes take successive rows.
+// (256*rb) >> logTB lanes take successive rows.  When the columns span a
+// whole tile, a tile takes up to kBandRows rows of its columns (the column
+// values are read once for all of them).
+constexpr uint64_t kBandRows = 8;
+
 static BBlock shape_block(BBlock b, int rb) {
   const uint64_t tile = (uint64_t)kPairThreads * rb;
   b.trans = b.nB >= b.nA ? 0 : 1;
-  const uint64_t nX = b.trans ? b.nA : b.nB;
+  const uint64_t nX = b.trans ? b.nA : b.nB, nY = b.trans ? b.nB : b.nA;
   int lg = 0;
   while ((1ull << lg) < nX && (1ull << lg) < tile) ++lg;
   b.logTB = lg;
   b.ntc = (nX + (1ull << lg) - 1) >> lg;
+  b.ntc_div = div_magic(b.ntc <= 0xffffffffull ? (uint32_t)b.ntc : 1u);
+  b.rows_pt = (1ull << lg) == tile ? (int32_t)std::min(nY, kBandRows) : 1;
   return b;
 }
 
@@ -52,7 +58,7 @@ static uint64_t block_tiles(const BBlock &b, int rb) {
   const uint64_t tile = (uint64_t)kPairThreads * rb;
   if (b.generic) return (b.nA + tile - 1) / tile;
   const uint64_t nY = b.trans ? b.nB : b.nA;
-  const uint64_t rows = tile >> b.logTB;
+  const uint64_t rows = (1ull << b.logTB) == tile ? (uint64_t)b.rows_pt : tile >> b.logTB;
   return ((nY + rows - 1) / rows) * b.ntc;
 }
 
@@ -440,6 +446,8 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   }
 
   // ---- size sweep --------------------------------------------------------
+  // sizes min..sweep_last are swept; the windows are those of max_size
+  const int sweep_last = o.stop_size > 0 ? std::min(o.stop_size, max_size) : max_size;
   std::vector<uint64_t> found(kMaxSize + 2, ~0ull);
   HitRec *d_hits = ctx.hits.as<HitRec>();
   // 32 workgroups per CU by default: more than stay resident, so tiles are
@@ -532,7 +540,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   auto fused_last = [&](int s0) -> int {
     int e = s0 - 1;
     if (o.exhaustive) return e;
-    for (uint64_t acc = 0; e < max_size && T.t[e + 1][8] < block_min &&
+    for (uint64_t acc = 0; e < sweep_last && T.t[e + 1][8] < block_min &&
                            acc + T.t[e + 1][8] <= kFuseTrees;)
       acc += T.t[++e][8];
     return e;
@@ -607,10 +615,10 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   // one.
   MBA_HIP(hipMemsetAsync(d_read, 0, 12, st));  // hit, lowered slot, candidate counts
   MBA_HIP(launch_leaf_scan(W, d_tv, L, targets_for(t0, 1, max_size, W), d_hits, d_hc, kHitCap, st));
-  bool leaf_pending = max_size >= 3 && fused_last(3) >= 3;
+  bool leaf_pending = sweep_last >= 3 && fused_last(3) >= 3;
   if (!leaf_pending) {
     if (int rc = process_hits()) return rc;
-    for (int s = 1; s <= std::min(2, max_size); ++s) {
+    for (int s = 1; s <= std::min(2, sweep_last); ++s) {
       reached = s;
       if (s >= min_size && found[s] != ~0ull) {
         answer = s;
@@ -774,7 +782,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     return keep_open ? 0 : close_group(false);
   };
 
-  for (int s = 3; (answer < 0 || o.exhaustive) && s <= max_size; ++s) {
+  for (int s = 3; (answer < 0 || o.exhaustive) && s <= sweep_last; ++s) {
     if (over_budget()) {
       status = MBA_BUDGET;
       break;

@@ -1,6 +1,7 @@
 // simba_kernels.h -- launch interface of the SIMBA gfx950 kernels.
 #pragma once
 #include "mba_common.h"
+#include "pair_math.h"
 
 namespace mba {
 
@@ -43,8 +44,9 @@ struct CheckTargets {
 // op(V[coff], x), kind 2: op(x, V[coff])).  The tree's rank within size m is
 // start + stride * (ra * nB + rb).  A tile covers 2^logTB columns (over ra
 // when trans, else over rb) by (256*RB) >> logTB rows; ntc tiles span a row
-// band.  generic: ranks start + stride * i, i < nA, each evaluated by a
-// table-assisted unrank (children above Kt).
+// band (ntc_div: the division by ntc).  When the columns span a whole tile
+// (2^logTB = 256*RB) a tile covers rows_pt consecutive rows of its columns.  generic: ranks start + stride * i,
+// i < nA, each evaluated by a table-assisted unrank (children above Kt).
 constexpr int kMaxBSteps = 8;
 struct BStep {
   int32_t kind;  // 0 affine, 1 op(c, x), 2 op(x, c)
@@ -56,7 +58,9 @@ struct BBlock {
   uint64_t offA, offB;
   uint64_t nA, nB;
   uint64_t ntc;
-  int32_t op, generic, trans, logTB, nsteps, pad;
+  int32_t op, generic, trans, logTB, nsteps, rows_pt;
+  DivMagic ntc_div;
+  uint32_t pad2;
   BStep steps[kMaxBSteps];
 };
 

@@ -18,6 +18,7 @@
 // to a hit buffer, resolved to (size, rank) on the device (k_hit_screen,
 // k_hit_confirm) and verified on all samples (verify_wave).
 #include "simba_kernels.h"
+#include "pair_math.h"
 
 namespace mba {
 
@@ -437,10 +438,14 @@ __device__ __forceinline__ void gen_b_tile(const TableView *__restrict__ tv, int
 
 // Per-A values of a tile, staged in LDS by the workgroup (one A value per
 // lane) and read back as wave-uniform broadcasts inside the loop:
-//   exact mode:  q0 = {a, a ^ t, t - a, a - t},
-//                q1 = {t + a, bit-op gap, product residue, shift | kind << 8}
-//   window mode: q0 = {a, lo_e - a, lo_o - a, a - lo_e},
-//                q1 = {a - lo_o, a + lo_e, a + lo_o, -}
+//   exact mode:   q0 = {a, a ^ t, t - a, a - t},
+//                 q1 = {t + a, bit-op gap, product residue, shift | kind << 8}
+//   window mode, every instance (pair_math.h):
+//                 q0 = {a, ~a, fold(t - a) - h, fold(t + a) - h},
+//                 q1 = {-, -, -, & | possible}
+//   window mode, some instances:
+//                 q0 = {a, lo_e - a, lo_o - a, a - lo_e},
+//                 q1 = {a - lo_o, a + lo_e, a + lo_o, & | possible}
 // (low halves; t = lo_e in exact mode).
 struct AStage {
   uint4 q0[kPairTA];
@@ -477,30 +482,24 @@ __device__ __forceinline__ uint32_t bitops_gap(uint32_t al, uint32_t tl) {
   return min(tl & ~al, al & ~tl);
 }
 
-// The exact-mode layout is staged when every operator instance is checked
-// against a single target (span 0); otherwise the window layout.
-__device__ __forceinline__ bool exact_layout(uint32_t mask, bool same, uint32_t span) {
+// Staged layout of a launch: the exact-mode layout when every operator
+// instance is checked against a single target (span 0), the folded window
+// layout when every instance is checked against windows, else the
+// per-instance window layout.
+enum : int { kStageExact = 0, kStageWindow = 1, kStageInst = 2 };
+__device__ __forceinline__ int stage_mode(uint32_t mask, bool same, uint32_t span) {
   const uint32_t full = same ? 0x3fu : 0x7fu;
-  return span == 0u && (mask & full) == full;
+  if ((mask & full) != full) return kStageInst;
+  return span == 0u ? kStageExact : kStageWindow;
 }
 
-// Whether a & b or a | b can land in the window [w, w + span] for some b
-// (low halves): every value of the window shares its bits above the span's
-// bit length p with w or with w + span, and a & b lies inside a (a | b
-// contains a), so one of those two high parts must lie inside a (contain a).
-__device__ __forceinline__ bool window_bitops(uint32_t al, uint32_t w, uint32_t span) {
-  const int p = span ? 32 - __builtin_clz(span) : 0;
-  if (p >= 32) return true;
-  const uint32_t ah = al >> p, h0 = w >> p, h1 = (w + span) >> p;
-  return (h0 & ~ah) == 0u || (h1 & ~ah) == 0u || (ah & ~h0) == 0u || (ah & ~h1) == 0u;
-}
-
-// Stages A values [a0, a1) (GEN_A: computed per lane from the tables).
-// Returns, for the exact mode, whether any A value admits & or | hits.
+// Stages A values [a0, a1) (GEN_A: computed per lane from the tables) in
+// the layout `mode` (stage_mode).  Returns, for the exact mode, whether any A
+// value admits & or | hits.
 template <typename word>
 __device__ __forceinline__ bool stage_a(AStage &S, const TableView *__restrict__ tv,
                                         const word *__restrict__ A, bool gen_a, int j,
-                                        uint64_t a0, uint64_t a1, bool exact, uint32_t le,
+                                        uint64_t a0, uint64_t a1, int mode, uint32_t le,
                                         uint32_t lo, uint32_t span) {
   __syncthreads();  // the previous tile's readers are done
   const uint32_t i = threadIdx.x;
@@ -509,13 +508,20 @@ __device__ __forceinline__ bool stage_a(AStage &S, const TableView *__restrict__
     const word av = gen_a ? value_of<word>(tv, j, a0 + i) : A[a0 + i];
     S.full[i] = (uint64_t)av;
     const uint32_t al = lo32(av);
-    if (exact) {
+    if (mode == kStageExact) {
       const uint32_t gap = bitops_gap(al, le);
       uint32_t shift_kind;
       const uint32_t res = mul_residue(al, le, shift_kind);
       S.q0[i] = make_uint4(al, al ^ le, le - al, al - le);
       S.q1[i] = make_uint4(le + al, gap, res, shift_kind);
       bit = gap == 0u;
+    } else if (mode == kStageWindow) {
+      const WinPre w = win_pre(le, lo, span);
+      uint32_t f1, f2;
+      win_stage(w, al, f1, f2);
+      const bool wb = window_bitops(al, le, span) || window_bitops(al, lo, span);
+      S.q0[i] = make_uint4(al, ~al, f1, f2);
+      S.q1[i] = make_uint4(0u, 0u, 0u, wb ? 1u : 0u);
     } else {
       const bool wb = window_bitops(al, le, span) || window_bitops(al, lo, span);
       S.q0[i] = make_uint4(al, le - al, lo - al, al - le);
@@ -537,11 +543,19 @@ __device__ __forceinline__ uint32_t lo32_at<uint64_t>(const uint64_t *p, uint64_
   return ((const uint32_t *)p)[2 * i];  // little-endian low half
 }
 
+// Row band of block tile bt: bt / ntc, by the invariant-divisor multiply
+// when both fit 32 bits (the usual case), else a 64-bit division.
+__device__ __forceinline__ uint64_t block_row(const BBlock &bb, uint64_t bt) {
+  if (bb.ntc == 1) return bt;
+  if ((bt >> 32) == 0 && bb.ntc <= 0xffffffffull) return div_u32((uint32_t)bt, bb.ntc_div);
+  return bt / bb.ntc;
+}
+
 // Coordinates of the k-th B value of this lane in block tile bt.
 template <int RB>
 __device__ __forceinline__ bool desc_coords(const BBlock &bb, uint64_t bt, int k, uint64_t &ra,
                                             uint64_t &rb) {
-  const uint64_t tr = bb.ntc == 1 ? bt : bt / bb.ntc;
+  const uint64_t tr = block_row(bb, bt);
   const uint64_t tc = bt - tr * bb.ntc;
   const uint32_t idx = threadIdx.x + (uint32_t)k * kPairThreads;
   const uint32_t TB = 1u << bb.logTB;
@@ -612,44 +626,24 @@ __device__ __forceinline__ void steps_lo(const BBlock &bb, const word *V, uint32
   }
 }
 
-// Low 32 bits of the RB B values of this lane in a block tile (inner value,
-// then the path steps).  Tiles of one row (the block's columns span at least
-// a tile) take the row's value with a uniform load and the columns with
-// contiguous loads; smaller blocks use the general 2-D mapping.
+// Low 32 bits of the RB B values of this lane in a block tile of the general
+// 2-D mapping (blocks whose columns span less than a tile; row-band tiles go
+// through desc_band_tile): inner value, then the path steps.
 template <typename word, int RB, int OP>
 __device__ __forceinline__ void desc_lo_op(const BBlock &bb, const word *V, uint64_t bt,
                                            uint32_t *bl, bool *ok) {
-  constexpr uint32_t tile = kPairThreads * RB;
   auto comb = [](uint32_t a, uint32_t b) -> uint32_t {
     return OP == AND ? (a & b) : OP == OR ? (a | b) : OP == XOR ? (a ^ b)
          : OP == ADD ? (a + b) : OP == SUB ? (a - b) : OP == MUL ? (a * b) : a;
   };
-  if ((1u << bb.logTB) == tile) {
-    const uint64_t tr = bb.ntc == 1 ? bt : bt / bb.ntc;
-    const uint64_t x0 = (bt - tr * bb.ntc) * tile;
-    const uint64_t nX = bb.trans ? bb.nA : bb.nB;
-    const uint64_t left = nX - x0;
-    const uint32_t rem = left > tile ? tile : (uint32_t)left;
-    // columns run over ra (trans) or rb; the row is the other child
-    const word *col = V + (bb.trans ? bb.offA : bb.offB) + x0;
-    const uint32_t rowv = OP >= 0 ? lo32_at<word>(V, (bb.trans ? bb.offB : bb.offA) + tr) : 0u;
 #pragma unroll
-    for (int k = 0; k < RB; ++k) {
-      const uint32_t off = threadIdx.x + (uint32_t)k * kPairThreads;
-      ok[k] = off < rem;
-      const uint32_t c = lo32_at<word>(col, ok[k] ? off : 0u);
-      bl[k] = OP < 0 ? c : bb.trans ? comb(c, rowv) : comb(rowv, c);
-    }
-  } else {
-#pragma unroll
-    for (int k = 0; k < RB; ++k) {
-      uint64_t ra, rb;
-      ok[k] = desc_coords<RB>(bb, bt, k, ra, rb);
-      if (!ok[k]) ra = 0, rb = 0;
-      uint32_t v = lo32_at<word>(V, bb.offA + ra);
-      if (OP >= 0) v = comb(v, lo32_at<word>(V, bb.offB + rb));
-      bl[k] = v;
-    }
+  for (int k = 0; k < RB; ++k) {
+    uint64_t ra, rb;
+    ok[k] = desc_coords<RB>(bb, bt, k, ra, rb);
+    if (!ok[k]) ra = 0, rb = 0;
+    uint32_t v = lo32_at<word>(V, bb.offA + ra);
+    if (OP >= 0) v = comb(v, lo32_at<word>(V, bb.offB + rb));
+    bl[k] = v;
   }
   if (bb.nsteps) steps_lo<word, RB>(bb, V, bl);
 }
@@ -793,33 +787,27 @@ __device__ __forceinline__ bool exact_tile(const AStage &S, uint32_t na, const u
   return (m >> (threadIdx.x & 63)) & 1ull;
 }
 
-// Window mode: & and | only for the A values whose staged flag says they
-// can reach a window (a wave-uniform branch per A value).
-template <bool SAME, int RB>
+// Window mode, every instance: the folded prefilter of pair_math.h (ten
+// VALU instructions per pair: two xor-adds, a product and its two
+// distances, two distances of fold(b), three min3), & and | only for the A
+// values whose staged flag says they can reach a window (a wave-uniform
+// branch per A value).  The result is compared with w.psp.  (SAME pairs need
+// no separate B - A check: its windows are those of the additive tests.)
+template <int RB>
 __device__ __forceinline__ uint32_t window_tile(const AStage &S, uint32_t na,
-                                                const uint32_t *bl, uint32_t le, uint32_t lo) {
+                                                const uint32_t *bl, const WinPre &w) {
+  uint32_t fb[RB];
+#pragma unroll
+  for (int k = 0; k < RB; ++k) fb[k] = fold32(bl[k]);
   uint32_t acc = 0xffffffffu;
   for (uint32_t i = 0; i < na; ++i) {
     const uint4 q0 = S.q0[i];
-    const uint4 q1 = S.q1[i];
-    if (uniform(q1.w)) {
+    if (uniform(S.q1[i].w)) {
 #pragma unroll
-      for (int k = 0; k < RB; ++k) {
-        const uint32_t x_and = q0.x & bl[k], x_or = q0.x | bl[k];
-        acc = min3u(acc, x_and - le, x_and - lo);
-        acc = min3u(acc, x_or - le, x_or - lo);
-      }
+      for (int k = 0; k < RB; ++k) acc = win_pair_bitops(w, q0.x, bl[k], acc);
     }
 #pragma unroll
-    for (int k = 0; k < RB; ++k) {
-      const uint32_t b = bl[k];
-      const uint32_t x_xor = q0.x ^ b, x_mul = q0.x * b;
-      acc = min3u(acc, x_xor - le, x_xor - lo);
-      acc = min3u(acc, x_mul - le, x_mul - lo);
-      acc = min3u(acc, b - q0.y, b - q0.z);                // a + b
-      acc = min3u(acc, q0.w - b, q1.x - b);                // a - b
-      if (!SAME) acc = min3u(acc, b - q1.y, b - q1.z);     // b - a
-    }
+    for (int k = 0; k < RB; ++k) acc = win_pair(w, q0.x, q0.y, q0.z, q0.w, bl[k], fb[k], acc);
   }
   return acc;
 }
@@ -865,6 +853,169 @@ __device__ __forceinline__ uint32_t inst_tile(const AStage &S, uint32_t na, int 
   }
 }
 
+// Low-half prefilter of a lane's RB B values against the staged A values of
+// a tile: whether some pair of this lane may hit (exact, folded-window or
+// per-instance window test, by the launch's layout).
+template <bool SAME, int RB>
+__device__ __forceinline__ bool pair_prefilter(const AStage &S, uint32_t na, const uint32_t *bl,
+                                               const bool *ok, uint32_t mask, bool bitops,
+                                               uint32_t le, uint32_t lo, uint32_t sp) {
+  bool any_ok = false;
+#pragma unroll
+  for (int k = 0; k < RB; ++k) any_ok |= ok[k];
+  const uint32_t full = SAME ? 0x3fu : 0x7fu;
+  const int mode = stage_mode(mask, SAME, sp);
+  bool hit;
+  if (mode == kStageExact) {
+    hit = exact_tile<SAME, RB>(S, na, bl, le, bitops);
+  } else if (mode == kStageWindow) {
+    const WinPre w = win_pre(le, lo, sp);
+    hit = window_tile<RB>(S, na, bl, w) <= w.psp;
+  } else {
+    uint32_t acc = 0xffffffffu;
+    for (int inst = 0; inst < 7; ++inst)
+      if ((mask & full) >> inst & 1u) acc = inst_tile<RB>(S, na, inst, bl, le, lo, acc);
+    hit = acc <= sp;
+  }
+  return hit && any_ok;
+}
+
+// Rare: some lane passed the prefilter.  Its tile is re-walked with the exact
+// test, the A values from LDS (full width): per B value, count its hits (the
+// pairs that pass the low-half test of every instance checked in full),
+// reserve their records with one atomic and write them.  bfull(k, rank)
+// returns the full k-th B value of this lane and its rank.
+template <typename word, bool SAME, int RB, typename BFull>
+__device__ __forceinline__ void pair_rewalk(const AStage &S, uint32_t na, uint64_t a0,
+                                            const bool *ok, uint32_t mask, word lo_e, word lo_o,
+                                            word span, uint32_t pair, uint32_t tag,
+                                            HitRec *__restrict__ hits,
+                                            uint32_t *__restrict__ hit_count, uint32_t hit_cap,
+                                            BFull bfull) {
+  const uint32_t le = lo32(lo_e), lo = lo32(lo_o), sp = (uint32_t)span;
+  const int ninst = SAME ? 6 : 7;
+  for (int k = 0; k < RB; ++k) {
+    if (!ok[k]) continue;
+    uint64_t b;
+    const word bv = bfull(k, b);
+    const uint32_t b32 = lo32(bv);
+    uint32_t n = 0;
+    for (uint32_t i = 0; i < na; ++i) {
+      const word av = (word)S.full[i];
+      if (!pair_may_hit(lo32(av), b32, le, lo, sp)) continue;
+      for (int inst = 0; inst < ninst; ++inst)
+        n += (mask >> inst & 1u) && in_window(inst_value<word>(inst, av, bv), lo_e, lo_o, span);
+    }
+    if (n == 0) continue;
+    uint32_t r = atomicAdd(hit_count, n);
+    for (uint32_t i = 0; i < na && r < hit_cap; ++i) {
+      const word av = (word)S.full[i];
+      if (!pair_may_hit(lo32(av), b32, le, lo, sp)) continue;
+      for (int inst = 0; inst < ninst; ++inst) {
+        const word c = inst_value<word>(inst, av, bv);
+        if (!((mask >> inst & 1u) && in_window(c, lo_e, lo_o, span))) continue;
+        if (r < hit_cap) {
+          HitRec h;
+          h.a = a0 + i;
+          h.b = b;
+          h.v = (uint64_t)c;
+          h.inst = (uint32_t)(inst | tag);
+          h.pair = pair;
+          hits[r] = h;
+        }
+        ++r;
+      }
+    }
+  }
+}
+
+// Row-band tile of a B block (its columns span at least a tile): the tile's
+// column values are loaded once (low halves) and combined with each of its
+// rows (up to bb.rows_pt block rows, one uniform load each), so a column
+// value read from memory serves every row of the tile.
+template <typename word, bool SAME, int RB>
+__device__ __forceinline__ void desc_band_tile(const TableView *__restrict__ tv,
+                                               const AStage &S, const PairLaunch &p,
+                                               uint32_t na, uint64_t a0, uint64_t bt,
+                                               bool bitops, word lo_e, word lo_o, word span,
+                                               HitRec *__restrict__ hits,
+                                               uint32_t *__restrict__ hit_count,
+                                               uint32_t hit_cap, uint32_t tag) {
+  constexpr uint32_t tile = kPairThreads * RB;
+  const BBlock &bb = p.bb;
+  const word *V = global_ptr<const word>(tv->base);
+  const uint64_t tr = block_row(bb, bt);
+  const uint64_t tc = bt - tr * bb.ntc;
+  const uint64_t x0 = tc * tile;
+  const uint64_t nX = bb.trans ? bb.nA : bb.nB, nY = bb.trans ? bb.nB : bb.nA;
+  const uint64_t left = nX - x0;
+  const uint32_t rem = left > tile ? tile : (uint32_t)left;
+  const word *col = V + (bb.trans ? bb.offA : bb.offB) + x0;
+  uint32_t cl[RB], bl[RB];
+  bool ok[RB];
+#pragma unroll
+  for (int k = 0; k < RB; ++k) {
+    const uint32_t off = threadIdx.x + (uint32_t)k * kPairThreads;
+    ok[k] = off < rem;
+    cl[k] = lo32_at<word>(col, ok[k] ? off : 0u);
+  }
+  const uint64_t y0 = tr * (uint64_t)bb.rows_pt;
+  const uint32_t nr = (uint32_t)(nY - y0 < (uint64_t)bb.rows_pt ? nY - y0 : bb.rows_pt);
+  const uint32_t pair = ((uint32_t)p.sb << 16) | (uint32_t)p.j;
+  const uint32_t le = lo32(lo_e), lo = lo32(lo_o), sp = (uint32_t)span;
+  for (uint32_t r = 0; r < nr; ++r) {  // workgroup-uniform
+    const uint64_t y = y0 + r;
+    const uint32_t rowv = bb.op < 0 ? 0u : lo32_at<word>(V, (bb.trans ? bb.offB : bb.offA) + y);
+    switch (bb.op) {  // workgroup-uniform; only '-' depends on the order
+      case AND:
+#pragma unroll
+        for (int k = 0; k < RB; ++k) bl[k] = cl[k] & rowv;
+        break;
+      case OR:
+#pragma unroll
+        for (int k = 0; k < RB; ++k) bl[k] = cl[k] | rowv;
+        break;
+      case XOR:
+#pragma unroll
+        for (int k = 0; k < RB; ++k) bl[k] = cl[k] ^ rowv;
+        break;
+      case ADD:
+#pragma unroll
+        for (int k = 0; k < RB; ++k) bl[k] = cl[k] + rowv;
+        break;
+      case MUL:
+#pragma unroll
+        for (int k = 0; k < RB; ++k) bl[k] = cl[k] * rowv;
+        break;
+      case SUB:
+        if (bb.trans) {
+#pragma unroll
+          for (int k = 0; k < RB; ++k) bl[k] = cl[k] - rowv;
+        } else {
+#pragma unroll
+          for (int k = 0; k < RB; ++k) bl[k] = rowv - cl[k];
+        }
+        break;
+      default:
+#pragma unroll
+        for (int k = 0; k < RB; ++k) bl[k] = cl[k];
+        break;
+    }
+    if (bb.nsteps) steps_lo<word, RB>(bb, V, bl);
+    const bool hit = pair_prefilter<SAME, RB>(S, na, bl, ok, p.mask, bitops, le, lo, sp);
+    if (__any(hit) && hit) {
+      pair_rewalk<word, SAME, RB>(
+          S, na, a0, ok, p.mask, lo_e, lo_o, span, pair, tag, hits, hit_count, hit_cap,
+          [&](int k, uint64_t &rank) -> word {
+            const uint64_t x = x0 + threadIdx.x + (uint64_t)k * kPairThreads;
+            const uint64_t ra = bb.trans ? x : y, rb = bb.trans ? y : x;
+            rank = bb.start + bb.stride * (ra * bb.nB + rb);
+            return desc_value<word>(bb, V, ra, rb);
+          });
+    }
+  }
+}
+
 // One tile of a pair check (shared by the per-instance and batched kernels).
 // The A values of the tile must be staged in S (stage_a).  p.mask selects the
 // operator instances (bit i: instance i, see HitRec); `tag` is or-ed into
@@ -877,71 +1028,25 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
                                           word lo_o, word span, HitRec *__restrict__ hits,
                                           uint32_t *__restrict__ hit_count, uint32_t hit_cap,
                                           uint32_t tag) {
-  const uint32_t pair = ((uint32_t)p.sb << 16) | (uint32_t)p.j;
   const uint64_t a0 = at * p.ta;
   const uint64_t a1 = a0 + p.ta < p.nA ? a0 + p.ta : p.nA;
   const uint32_t na = (uint32_t)(a1 - a0);
+  if (BSRC == kBDesc && (1u << p.bb.logTB) == (uint32_t)(kPairThreads * RB)) {
+    desc_band_tile<word, SAME, RB>(tv, S, p, na, a0, bt, bitops, lo_e, lo_o, span, hits,
+                                   hit_count, hit_cap, tag);
+    return;
+  }
   uint32_t bl[RB];
   bool ok[RB];
   b_lo<word, BSRC, RB>(tv, B, p, bt, bl, ok);
-  bool any_ok = false;
-#pragma unroll
-  for (int k = 0; k < RB; ++k) any_ok |= ok[k];
-  const uint32_t le = lo32(lo_e), lo = lo32(lo_o), sp = (uint32_t)span;
-  const uint32_t full = SAME ? 0x3fu : 0x7fu;
-  bool hit;
-  if (exact_layout(p.mask, SAME, sp)) {
-    hit = exact_tile<SAME, RB>(S, na, bl, le, bitops);
-  } else if ((p.mask & full) == full) {
-    hit = window_tile<SAME, RB>(S, na, bl, le, lo) <= sp;
-  } else {
-    uint32_t acc = 0xffffffffu;
-    for (int inst = 0; inst < 7; ++inst)
-      if ((p.mask & full) >> inst & 1u) acc = inst_tile<RB>(S, na, inst, bl, le, lo, acc);
-    hit = acc <= sp;
-  }
-  hit = hit && any_ok;
-  // Rare: some lane passed the prefilter.  Re-walk the tile, exact test,
-  // with the A values from LDS: per B value, count its hits (the pairs that
-  // pass the low-half test of every instance checked in full), reserve their
-  // records with one atomic and write them.
-  if (__any(hit)) {
-    if (hit) {
-      const int ninst = SAME ? 6 : 7;
-      for (int k = 0; k < RB; ++k) {
-        if (!ok[k]) continue;
-        uint64_t b;
-        const word bv = b_full<word, BSRC, RB>(tv, B, p, bt, k, b);
-        const uint32_t b32 = lo32(bv);
-        uint32_t n = 0;
-        for (uint32_t i = 0; i < na; ++i) {
-          const word av = (word)S.full[i];
-          if (!pair_may_hit(lo32(av), b32, le, lo, sp)) continue;
-          for (int inst = 0; inst < ninst; ++inst)
-            n += (p.mask >> inst & 1u) && in_window(inst_value<word>(inst, av, bv), lo_e, lo_o, span);
-        }
-        if (n == 0) continue;
-        uint32_t r = atomicAdd(hit_count, n);
-        for (uint32_t i = 0; i < na && r < hit_cap; ++i) {
-          const word av = (word)S.full[i];
-          if (!pair_may_hit(lo32(av), b32, le, lo, sp)) continue;
-          for (int inst = 0; inst < ninst; ++inst) {
-            const word c = inst_value<word>(inst, av, bv);
-            if (!((p.mask >> inst & 1u) && in_window(c, lo_e, lo_o, span))) continue;
-            if (r < hit_cap) {
-              HitRec h;
-              h.a = a0 + i;
-              h.b = b;
-              h.v = (uint64_t)c;
-              h.inst = (uint32_t)(inst | tag);
-              h.pair = pair;
-              hits[r] = h;
-            }
-            ++r;
-          }
-        }
-      }
-    }
+  const bool hit = pair_prefilter<SAME, RB>(S, na, bl, ok, p.mask, bitops, lo32(lo_e),
+                                            lo32(lo_o), (uint32_t)span);
+  if (__any(hit) && hit) {
+    const uint32_t pair = ((uint32_t)p.sb << 16) | (uint32_t)p.j;
+    pair_rewalk<word, SAME, RB>(S, na, a0, ok, p.mask, lo_e, lo_o, span, pair, tag, hits,
+                                hit_count, hit_cap, [&](int k, uint64_t &rank) -> word {
+                                  return b_full<word, BSRC, RB>(tv, B, p, bt, k, rank);
+                                });
   }
 }
 
@@ -953,7 +1058,7 @@ k_pair_check(const TableView *__restrict__ tv, const word *__restrict__ A,
              uint32_t hit_cap) {
   __shared__ AStage S;
   const word lo_e = (word)lo_e64, lo_o = (word)lo_o64, span = (word)span64;
-  const bool exact = exact_layout(p.mask, SAME, (uint32_t)span);
+  const int mode = stage_mode(p.mask, SAME, (uint32_t)span);
   const uint32_t le = lo32(lo_e), lo = lo32(lo_o);
   uint64_t staged = ~0ull;
   bool bitops = false;
@@ -962,7 +1067,7 @@ k_pair_check(const TableView *__restrict__ tv, const word *__restrict__ A,
     if (at != staged) {  // workgroup-uniform: every wave takes the same tiles
       const uint64_t a0 = at * p.ta;
       const uint64_t a1 = a0 + p.ta < p.nA ? a0 + p.ta : p.nA;
-      bitops = stage_a<word>(S, tv, A, p.gen_a, p.j, a0, a1, exact, le, lo, (uint32_t)span);
+      bitops = stage_a<word>(S, tv, A, p.gen_a, p.j, a0, a1, mode, le, lo, (uint32_t)span);
       staged = at;
     }
     pair_tile<word, SAME, BSRC, RB>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o, span, hits,
@@ -1331,7 +1436,7 @@ __device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J, ui
     const uint64_t a0 = at * p.ta;
     const uint64_t a1 = a0 + p.ta < p.nA ? a0 + p.ta : p.nA;
     bitops = stage_a<word>(S, tv, A, p.gen_a, p.j, a0, a1,
-                           exact_layout(p.mask, p.j == p.m, (uint32_t)span), lo32(lo_e),
+                           stage_mode(p.mask, p.j == p.m, (uint32_t)span), lo32(lo_e),
                            lo32(lo_o), (uint32_t)span);
     staged = at;
   }
