@@ -158,6 +158,16 @@ int mba_simba_rtid_solve(mba_ctx *ctx, const mba_instance *inst, const mba_opts 
 int mba_vfb_solve(mba_ctx *ctx, const mba_instance *inst, const mba_opts *opts,
                   mba_result *res, mba_vfb_stats *stats);
 
+/* VFB candidate fingerprints as the device computes them for dedup:
+ * makeUnqChk (full-cache.cu:126-134, with simpleHash :113-120) of each of
+ * `count` signatures values[r * n_words .. + n_words) (each word taken mod
+ * 2^width); out[r] receives the 64-bit fingerprint.  Width 32 is the
+ * reference's; width 64 uses the 64-bit MurmurHash3 finalizer and a 64-bit
+ * sum.  The reference computes them inside processOperator and exports no
+ * entry point: this one exposes the kernels' own routine for checks. */
+int mba_vfb_fingerprints(mba_ctx *ctx, uint32_t width, const uint64_t *values,
+                         uint32_t n_words, size_t count, uint64_t *out);
+
 /* Batched SIMBA: `count` instances solved by one persistent kernel with a
  * device work queue.  Each result equals mba_simba_solve's on that instance
  * within the per-instance budget `opts->time_budget_s` (0 = none). */

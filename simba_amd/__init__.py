@@ -103,7 +103,7 @@ class VfbStats(ctypes.Structure):
 # Every symbol the C ABI exports (include/mba_api.h).
 EXPORTS = ["mba_ctx_create", "mba_ctx_destroy", "mba_strerror", "mba_ctx_last_error",
            "mba_default_opts", "mba_simba_solve", "mba_simba_rtid_solve", "mba_vfb_solve",
-           "mba_batch_solve",
+           "mba_batch_solve", "mba_vfb_fingerprints",
            "mba_instance_read_json", "mba_instance_free", "mba_rpn_to_infix",
            "mba_features", "mba_features_batch", "mba_mlp_create", "mba_mlp_destroy",
            "mba_mlp_get_weights", "mba_mlp_set_weights", "mba_mlp_forward", "mba_mlp_grads",
@@ -135,6 +135,9 @@ def lib():
                                     P(VfbStats)]
         L.mba_batch_solve.argtypes = [ctypes.c_void_p, P(Instance), ctypes.c_size_t,
                                       P(Opts), P(Result)]
+        L.mba_vfb_fingerprints.argtypes = [ctypes.c_void_p, ctypes.c_uint32,
+                                           P(ctypes.c_uint64), ctypes.c_uint32,
+                                           ctypes.c_size_t, P(ctypes.c_uint64)]
         L.mba_instance_read_json.argtypes = [ctypes.c_char_p, P(Instance)]
         L.mba_instance_free.argtypes = [P(Instance)]
         L.mba_rpn_to_infix.argtypes = [ctypes.c_char_p, ctypes.c_uint32, P(ctypes.c_uint64),
@@ -225,6 +228,18 @@ class Context:
         d["level_bytes"] = list(st.level_bytes)
         d["table_slots"] = int(st.table_slots)
         return d
+
+    def vfb_fingerprints(self, values, width):
+        """Device fingerprints (the VFB dedup key) of the rows of `values`
+        ([count][n] words), as numpy uint64 (mba_vfb_fingerprints)."""
+        v = np.ascontiguousarray(np.asarray(values, dtype=np.uint64))
+        if v.ndim != 2:
+            raise ValueError("values must be [count][n_words]")
+        out = np.zeros(v.shape[0], np.uint64)
+        P = ctypes.POINTER(ctypes.c_uint64)
+        self._check(lib().mba_vfb_fingerprints(self._h, width, v.ctypes.data_as(P), v.shape[1],
+                                               v.shape[0], out.ctypes.data_as(P)))
+        return out
 
     def batch(self, insts, **kw):
         """Solves the instances in one batched call; returns a BatchResults

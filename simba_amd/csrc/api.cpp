@@ -121,6 +121,13 @@ int mba_vfb_solve(mba_ctx *c, const mba_instance *inst, const mba_opts *opts,
   return vfb_solve(c->c, *inst, *opts, *res, stats);
 }
 
+int mba_vfb_fingerprints(mba_ctx *c, uint32_t width, const uint64_t *values,
+                         uint32_t n_words, size_t count, uint64_t *out) {
+  if (!c) return MBA_ERR_ARG;
+  if (hipSetDevice(c->c.device) != hipSuccess) return MBA_ERR_HIP;
+  return vfb_fingerprints(c->c, (int)width, values, (int)n_words, count, out);
+}
+
 int mba_batch_solve(mba_ctx *c, const mba_instance *insts, size_t count,
                     const mba_opts *opts, mba_result *results) {
   if (!c || (!insts && count) || (!results && count)) return MBA_ERR_ARG;

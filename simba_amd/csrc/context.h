@@ -90,6 +90,9 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &opts,
                 const std::function<int()> *score_init = nullptr);
 int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &opts,
               mba_result &res, mba_vfb_stats *stats);
+// Device fingerprints of `count` signatures of n words (see mba_api.h).
+int vfb_fingerprints(Ctx &ctx, int W, const uint64_t *values, int n, size_t count,
+                     uint64_t *out);
 // scores: NULL (plain, minimum rank per instance) or count x 256 block
 // scores as for simba_solve (learned order: rounds of the best-scored split,
 // the next split, then the rest).

@@ -418,4 +418,23 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
   return status;
 }
 
+int vfb_fingerprints(Ctx &ctx, int W, const uint64_t *values, int n, size_t count,
+                     uint64_t *out) {
+  if ((W != 32 && W != 64) || n < 1 || n > kMaxSamples || (count && (!values || !out))) {
+    ctx.last_error = "fingerprints: width 32 or 64, 1..128 words per signature";
+    return MBA_ERR_ARG;
+  }
+  if (count == 0) return 0;
+  const size_t in_bytes = count * (size_t)n * 8;
+  MBA_HIP(ctx.f_in.ensure(in_bytes));
+  MBA_HIP(ctx.f_out.ensure(count * 8));
+  hipStream_t st = ctx.stream;
+  MBA_HIP(hipMemcpyAsync(ctx.f_in.p, values, in_bytes, hipMemcpyHostToDevice, st));
+  MBA_HIP(vfb_launch_fingerprints(W, ctx.f_in.as<uint64_t>(), n, count, ctx.f_out.as<uint64_t>(),
+                                  st));
+  MBA_HIP(hipMemcpyAsync(out, ctx.f_out.p, count * 8, hipMemcpyDeviceToHost, st));
+  MBA_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
 }  // namespace mba

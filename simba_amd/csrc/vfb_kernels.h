@@ -85,5 +85,9 @@ hipError_t vfb_launch_rpn(const int32_t *left, const int32_t *right, const uint8
                           int L, int32_t root_op, int32_t root_l, int32_t root_r,
                           uint8_t *out, int32_t *out_len, int cap, hipStream_t s);
 hipError_t vfb_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s);
+// out[r] = fingerprint of the n words vals[r * n .. r * n + n) (the insert
+// pass's makeUnqChk), r < count.
+hipError_t vfb_launch_fingerprints(int width, const uint64_t *d_vals, int n, uint64_t count,
+                                   uint64_t *d_out, hipStream_t s);
 
 }  // namespace mba

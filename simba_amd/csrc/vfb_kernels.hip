@@ -418,6 +418,20 @@ __global__ void k_vfb_rpn(const int32_t *left, const int32_t *right, const uint8
   *out_len = n;
 }
 
+// Fingerprints of `count` signatures of n words (vals [count][n]), one lane
+// per signature, by the fp_step of the insert pass.
+template <typename word>
+__global__ void __launch_bounds__(256)
+k_vfb_fingerprints(const uint64_t *__restrict__ vals, int n, uint64_t count,
+                   uint64_t *__restrict__ out) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < count; r += stride) {
+    uint64_t seed = 0;
+    for (int i = 0; i < n; ++i) seed = fp_step<word>(seed, (word)vals[r * n + i], i);
+    out[r] = seed;
+  }
+}
+
 __global__ void k_fill_u64(uint64_t *p, uint64_t n, uint64_t v) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
@@ -477,6 +491,20 @@ hipError_t vfb_launch_rpn(const int32_t *left, const int32_t *right, const uint8
                           uint8_t *out, int32_t *out_len, int cap, hipStream_t s) {
   hipLaunchKernelGGL(k_vfb_rpn, dim3(1), dim3(64), 0, s, left, right, op, L, root_op,
                      root_l, root_r, out, out_len, cap);
+  return hipGetLastError();
+}
+
+hipError_t vfb_launch_fingerprints(int width, const uint64_t *d_vals, int n, uint64_t count,
+                                   uint64_t *d_out, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  const uint64_t blocks = (count + 255) / 256;
+  const dim3 grid((unsigned)(blocks < 4096 ? blocks : 4096));
+  if (width == 32)
+    hipLaunchKernelGGL(k_vfb_fingerprints<uint32_t>, grid, dim3(256), 0, s, d_vals, n, count,
+                       d_out);
+  else
+    hipLaunchKernelGGL(k_vfb_fingerprints<uint64_t>, grid, dim3(256), 0, s, d_vals, n, count,
+                       d_out);
   return hipGetLastError();
 }
 

@@ -34,3 +34,51 @@ def verify_rpn_chars(rpn_chars, inst):
         if orc.evaluate(toks, lv, inst.width) != int(inst.outputs[i]):
             return False
     return True
+
+
+def eval_vfb_infix(expr, leafvals, W):
+    """Value of a VFB-format expression (full-cache.cu:483-509: leaves `v<i>`
+    or decimal constants, unary `~(X)` / `-(X)`, binary `(L)op(R)`) on one
+    sample's leaf values, mod 2^W.  Constant leaves are read from the text."""
+    M = (1 << W) - 1
+    ops = {"&": lambda a, b: a & b, "|": lambda a, b: a | b, "^": lambda a, b: a ^ b,
+           "+": lambda a, b: a + b, "-": lambda a, b: a - b, "*": lambda a, b: a * b}
+
+    def expect(i, ch):
+        assert expr[i] == ch, (expr, i, ch)
+        return i + 1
+
+    def parse(i):
+        c = expr[i]
+        if c in "~-" and expr[i + 1] == "(":
+            x, i = parse(i + 2)
+            i = expect(i, ")")
+            return ((~x) if c == "~" else -x) & M, i
+        if c == "(":
+            a, i = parse(i + 1)
+            i = expect(i, ")")
+            op = expr[i]
+            i = expect(i + 1, "(")
+            b, i = parse(i)
+            i = expect(i, ")")
+            return ops[op](a, b) & M, i
+        j = i + 1
+        while j < len(expr) and (expr[j].isdigit()):
+            j += 1
+        tok = expr[i:j]
+        if tok[0] == "v":
+            return leafvals[int(tok[1:])] & M, j
+        return int(tok) & M, j
+
+    v, end = parse(0)
+    assert end == len(expr), expr
+    return v
+
+
+def verify_vfb_expr(expr, inst):
+    """Evaluates a VFB result expression on every sample of the instance."""
+    for i in range(inst.n_examples):
+        lv = [int(inst.inputs[i * inst.n_variables + v]) for v in range(inst.n_variables)]
+        if eval_vfb_infix(expr, lv, inst.width) != int(inst.outputs[i]):
+            return False
+    return True

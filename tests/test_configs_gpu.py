@@ -1,0 +1,163 @@
+"""GPU parity at the shapes of the benchmark configurations, against the CPU
+oracle:
+
+  - largest single-instance SIMBA shape: leaves {a, b, c, d, 1}, u64 words,
+    64 samples (searched to size 9 here, fused and block-at-a-time orders);
+  - VFB: vars {x, y, z}, constants {1, -1}, u64 words, 32 samples, levels
+    to 8-9 against the oracle, and the full 13-level sweep checked by
+    properties that hold at any size (per-level candidate counts from the
+    kept-row counts, the returned expression re-evaluated on every sample,
+    the levels the oracle reaches equal to it);
+  - the device's VFB fingerprint (makeUnqChk) against the oracle's.
+"""
+import random
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.gpu_util import note, oracle_simba, verify_rpn_chars, verify_vfb_expr
+
+pytestmark = pytest.mark.gpu
+M64 = (1 << 64) - 1
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import simba_amd
+    c = simba_amd.Context(0)
+    yield c
+    c.close()
+
+
+# (seed, target size): sizes 5..9 have a solution within the searched sizes;
+# the size-15 targets are the configuration's own (found at a small size or
+# not at all)
+C3_CASES = [(100 + i, 5 + i % 5) for i in range(8)] + [(200, 15), (201, 15)]
+
+
+@pytest.mark.parametrize("seed,tsize", C3_CASES)
+@pytest.mark.parametrize("order", ["fused", "block"])
+def test_simba_4vars_const1_64samples(ctx, seed, tsize, order):
+    from simba_amd.synth import make_instance
+    inst = make_instance(seed, 4, 64, tsize, 64, (1,))
+    note(f"c3 shape seed={seed} tsize={tsize} order={order}")
+    want = oracle_simba(inst, 1, 9)
+    kw = dict(min_size=1, max_size=9)
+    if order == "block":
+        kw["block_log2"] = 14  # (op, split) blocks from sizes of 2^14 trees
+    got = ctx.simba(inst, **kw)
+    assert got["status"] == want["status"]
+    if want["status"] == 0:
+        assert (got["size"], got["rank"], got["expr"]) == (want["size"], want["rank"],
+                                                            want["expr"])
+        assert verify_rpn_chars(got["rpn"], inst)
+
+
+def test_simba_4vars_const1_windows_of_size_20(ctx):
+    # sizes 1..9 swept with the windows of a search to size 20 (the
+    # benchmark's form): the answer is the oracle's minimum-rank solution
+    from simba_amd.synth import make_instance
+    for seed in (300, 301, 302):
+        inst = make_instance(seed, 4, 64, 7, 64, (1,))
+        note(f"c3 windows20 seed={seed}")
+        want = oracle_simba(inst, 1, 9)
+        got = ctx.simba(inst, min_size=1, max_size=20, stop_size=9, exhaustive=True)
+        assert got["size_reached"] == 9
+        assert (got["status"], got["size"], got["rank"]) == (want["status"], want["size"],
+                                                             want["rank"])
+
+
+def _vfb_inst(seed, tsize, W=64):
+    from simba_amd.synth import make_instance
+    m = (1 << W) - 1
+    return make_instance(seed, 3, 32, tsize, W, (1, m))
+
+
+def _vfb_oracle(inst, lv, exhaustive):
+    return orc.vfb_search(inst.inputs, inst.outputs, 3, [int(c) for c in inst.consts],
+                          inst.width, lv, exhaustive=exhaustive)
+
+
+# targets of size 11 (the configuration's) and smaller ones that have a
+# solution within the checked levels
+C2_CASES = [(500 + i, 11, 8) for i in range(3)] + [(510 + i, 4 + i, 8) for i in range(5)] + \
+           [(520, 11, 9), (521, 7, 9)]
+
+
+@pytest.mark.parametrize("seed,tsize,lv", C2_CASES)
+def test_vfb_3vars_consts_32samples(ctx, seed, tsize, lv):
+    inst = _vfb_inst(seed, tsize)
+    note(f"c2 shape seed={seed} tsize={tsize} levels={lv}")
+    want = _vfb_oracle(inst, lv, False)
+    got = ctx.vfb(inst, max_size=lv)
+    assert got["status"] == want["status"]
+    if want["status"] == 0:
+        assert (got["size"], got["rank"], got["expr"]) == (want["size"], want["rank"],
+                                                            want["expr"])
+        assert verify_vfb_expr(got["expr"], inst)
+    ex_want = _vfb_oracle(inst, lv, True)
+    ex_got = ctx.vfb(inst, max_size=lv, exhaustive=True)
+    assert ex_got["level_new"][1:lv + 1] == ex_want["level_new"][1:lv + 1]
+    assert ex_got["level_cand"][1:lv + 1] == ex_want["level_cand"][1:lv + 1]
+
+
+def test_vfb_3vars_consts_32samples_w32(ctx):
+    inst = _vfb_inst(530, 6, W=32)
+    note("c2 shape w32")
+    want = _vfb_oracle(inst, 8, True)
+    got = ctx.vfb(inst, max_size=8, exhaustive=True)
+    assert got["level_new"][1:9] == want["level_new"][1:9]
+    assert got["level_cand"][1:9] == want["level_cand"][1:9]
+    assert (got["status"], got["size"], got["rank"]) == (want["status"], want["size"],
+                                                         want["rank"])
+
+
+def _pair_counts(new, lev):
+    """Candidates of level `lev` from the kept rows of the levels below:
+    ~ and neg over level lev-1, & | ^ + * over the splits i <= (lev-1)/2 and
+    '-' in both orders (full-cache.cu:568-663)."""
+    s = sum(new[i] * new[lev - 1 - i] for i in range(1, (lev - 1) // 2 + 1))
+    return 2 * new[lev - 1] + 7 * s
+
+
+@pytest.mark.parametrize("seed", [540, 541])
+def test_vfb_13_levels_properties(seed):
+    import simba_amd
+    inst = _vfb_inst(seed, 11)
+    note(f"c2 13 levels seed={seed}")
+    c = simba_amd.Context(0)
+    try:
+        got = c.vfb(inst, max_size=13, exhaustive=True, cache_bytes=96 << 30)
+        first = c.vfb(inst, max_size=13, cache_bytes=96 << 30)
+    finally:
+        c.close()
+    assert got["status"] in (0, 1) and got["size_reached"] == 13
+    new, cand = got["level_new"], got["level_cand"]
+    for lev in range(2, 14):
+        assert cand[lev] == _pair_counts(new, lev), lev
+    assert 0 < new[13] <= cand[13]
+    want = _vfb_oracle(inst, 9, True)
+    assert new[1:10] == want["level_new"][1:10]
+    if first["status"] == 0:
+        assert verify_vfb_expr(first["expr"], inst)
+        assert first["size"] == got["size"]
+        if want["status"] == 0:
+            assert (first["size"], first["rank"], first["expr"]) == (want["size"], want["rank"],
+                                                                      want["expr"])
+
+
+@pytest.mark.parametrize("W", [32, 64])
+def test_device_fingerprint_matches_oracle(ctx, W):
+    # the insert pass's fingerprint routine (makeUnqChk, full-cache.cu:126-134)
+    # on random signatures and on words whose simpleHash sum wraps
+    rng = random.Random(W)
+    m = (1 << W) - 1
+    edge = [0, 1, m, m - 1, 1 << (W - 1), 0x9e3779b9 & m, 0x61c88647]
+    for n in (1, 2, 3, 7, 32, 64, 100):
+        rows = [[rng.getrandbits(W) for _ in range(n)] for _ in range(200)]
+        rows += [[edge[(r + i) % len(edge)] for i in range(n)] for r in range(len(edge))]
+        note(f"device fingerprints W={W} n={n}")
+        got = ctx.vfb_fingerprints(np.array(rows, dtype=np.uint64), W)
+        want = [orc.fingerprint(r, W) for r in rows]
+        assert [int(x) for x in got] == want

@@ -200,6 +200,96 @@ def test_fmix32_matches_published_murmur3_vectors():
         assert _murmur3_x86_32(data, seed, f) == want, (data, seed)
 
 
+def _murmur3_x64_128(data, seed, fmix64):
+    """MurmurHash3_x64_128 (Appleby's published algorithm) with the oracle's
+    64-bit finalizer plugged in; returns (h1, h2)."""
+    M = (1 << 64) - 1
+    rotl = lambda x, r: ((x << r) | (x >> (64 - r))) & M
+    c1, c2 = 0x87c37b91114253d5, 0x4cf5ad432745937f
+    h1 = h2 = seed
+    n = len(data)
+    for i in range(0, n - n % 16, 16):
+        k1 = int.from_bytes(data[i:i + 8], "little")
+        k2 = int.from_bytes(data[i + 8:i + 16], "little")
+        h1 ^= rotl(k1 * c1 & M, 31) * c2 & M
+        h1 = (rotl(h1, 27) + h2) * 5 + 0x52dce729 & M
+        h2 ^= rotl(k2 * c2 & M, 33) * c1 & M
+        h2 = (rotl(h2, 31) + h1) * 5 + 0x38495ab5 & M
+    tail = data[n - n % 16:]
+    if len(tail) > 8:
+        h2 ^= rotl(int.from_bytes(tail[8:], "little") * c2 & M, 33) * c1 & M
+    if tail:
+        h1 ^= rotl(int.from_bytes(tail[:8], "little") * c1 & M, 31) * c2 & M
+    h1 ^= n
+    h2 ^= n
+    h1 = (h1 + h2) & M
+    h2 = (h2 + h1) & M
+    h1, h2 = fmix64(h1), fmix64(h2)
+    h1 = (h1 + h2) & M
+    h2 = (h2 + h1) & M
+    return h1, h2
+
+
+def test_fmix64_matches_published_murmur3_x64_128_vectors():
+    # fmix64 is the warpcore MurmurHash finalizer (hashers.cuh:130-139) and
+    # the W=64 fingerprint's hash; MurmurHash3_x64_128 ends with it, so the
+    # published 128-bit vectors pin its shifts and constants: the mmh3
+    # package's documented hash64("foo") / hash128("foo", 42), and Guava's
+    # murmur3_128 digest of the quick-brown-fox sentence
+    f = orc.lib().orc_fmix64
+    assert _murmur3_x64_128(b"", 0, f) == (0, 0)
+    assert _murmur3_x64_128(b"foo", 0, f) == (16316970633193145697, 9128664383759220103)
+    h1, h2 = _murmur3_x64_128(b"foo", 42, f)
+    assert (h2 << 64) | h1 == 215966891540331383248189432718888555506
+    h1, h2 = _murmur3_x64_128(b"The quick brown fox jumps over the lazy dog", 0, f)
+    assert (h1.to_bytes(8, "little") + h2.to_bytes(8, "little")).hex() == \
+        "6c1b07bc7bbc4be347939ac4a93c437a"
+
+
+def test_make_unq_chk_one_sample_wraps_mod_2_32():
+    # makeUnqChk (full-cache.cu:126-134) with one sample: seed 0, so the
+    # fingerprint is simpleHash(x) + 0x9e3779b9 computed in 32-bit unsigned
+    # arithmetic (both operands are unsigned int), not rotated (index 0)
+    f32 = orc.lib().orc_fmix32
+    rng = random.Random(5)
+    xs = [0, 1, 0xffffffff] + [rng.getrandbits(32) for _ in range(200)]
+    wrapped = [x for x in xs if f32(x) + 0x9e3779b9 >= 1 << 32]
+    assert len(wrapped) > 10  # the sum overflows for most inputs
+    for x in xs:
+        fp = orc.fingerprint([x], 32)
+        assert fp == (f32(x) + 0x9e3779b9) & 0xffffffff
+        assert fp < 1 << 32
+
+
+def test_make_unq_chk_rotates_odd_indices_only():
+    # Index 1 is rotated by 32 bits: with s1 = fp([x0]) < 2^32 the second
+    # term v = simpleHash(x1) + 0x9e3779b9 (mod 2^32) + (s1 << 6) + (s1 >> 2)
+    # is below 2^39, so s1 ^ fp([x0, x1]) = rotl64(v, 32) has a low half
+    # below 2^7 and the low half of v on top.  Index 2 is not rotated.
+    f32 = orc.lib().orc_fmix32
+    M = (1 << 64) - 1
+    rng = random.Random(6)
+    for _ in range(300):
+        x0, x1, x2 = (rng.getrandbits(32) for _ in range(3))
+        s1 = orc.fingerprint([x0], 32)
+        s2 = orc.fingerprint([x0, x1], 32)
+        v = ((f32(x1) + 0x9e3779b9) & 0xffffffff) + (s1 << 6) + (s1 >> 2)
+        assert v < 1 << 39
+        d = s1 ^ s2
+        assert d & 0xffffffff == v >> 32 and d >> 32 == v & 0xffffffff
+        s3 = orc.fingerprint([x0, x1, x2], 32)
+        w = (((f32(x2) + 0x9e3779b9) & 0xffffffff) + (s2 << 6) + (s2 >> 2)) & M
+        assert s2 ^ s3 == w
+
+
+def test_fingerprint_w64_one_sample():
+    # the 64-bit extension: fmix64(x) + 0x9e3779b9 in 64-bit arithmetic
+    f64 = orc.lib().orc_fmix64
+    rng = random.Random(7)
+    for x in [0, 1, (1 << 64) - 1] + [rng.getrandbits(64) for _ in range(100)]:
+        assert orc.fingerprint([x], 64) == (f64(x) + 0x9e3779b9) & ((1 << 64) - 1)
+
+
 def test_fingerprint_depends_on_order_and_values():
     a = [1, 2, 3, 4]
     assert orc.fingerprint(a, 32) == orc.fingerprint(list(a), 32)
