@@ -142,6 +142,21 @@ void mba_default_opts(mba_opts *o);
 int mba_simba_solve(mba_ctx *ctx, const mba_instance *inst, const mba_opts *opts,
                     mba_result *res);
 
+/* SIMBA plain search that allocates no device memory.  No counterpart in
+ * the reference: it is for callers that cannot allocate during a call
+ * (serving loops, harnesses that forbid hipMalloc).  Same answers as
+ * mba_simba_solve; every device buffer of the search is a region of a static
+ * 64 MiB device array (one per device, kept until the process ends, located
+ * when a context is created on that device): value tables up to about
+ * 55.8 MiB (opts->table_bytes: 0 or at most that; larger searches form more
+ * subtree values per candidate instead) and at most 2^17 window-hit records
+ * (opts->hit_cap: 0 or at most 131072; a tile with more hits is
+ * MBA_ERR_LIMIT).  Larger requests fail with MBA_ERR_LIMIT.  Calls that
+ * overlap on one device share the array and race: serialise them.  Host-side
+ * bookkeeping uses the C++ heap. */
+int mba_simba_solve_noalloc(mba_ctx *ctx, const mba_instance *inst, const mba_opts *opts,
+                            mba_result *res);
+
 /* SIMBA-RTid ablation (nocache_rtid.cu, the same driver as nocache.cu with
  * the kernel of :85-95): in each (size, op) block of N trees, thread g of
  * the ceil(N/1024)*1024 launched checks rank ((g * 2246822507) mod 2^64)

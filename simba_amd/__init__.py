@@ -102,7 +102,8 @@ class VfbStats(ctypes.Structure):
 
 # Every symbol the C ABI exports (include/mba_api.h).
 EXPORTS = ["mba_ctx_create", "mba_ctx_destroy", "mba_strerror", "mba_ctx_last_error",
-           "mba_default_opts", "mba_simba_solve", "mba_simba_rtid_solve", "mba_vfb_solve",
+           "mba_default_opts", "mba_simba_solve", "mba_simba_solve_noalloc",
+           "mba_simba_rtid_solve", "mba_vfb_solve",
            "mba_batch_solve", "mba_vfb_fingerprints",
            "mba_instance_read_json", "mba_instance_free", "mba_rpn_to_infix",
            "mba_features", "mba_features_batch", "mba_mlp_create", "mba_mlp_destroy",
@@ -131,6 +132,8 @@ def lib():
         L.mba_default_opts.argtypes = [P(Opts)]
         L.mba_simba_solve.argtypes = [ctypes.c_void_p, P(Instance), P(Opts), P(Result)]
         L.mba_simba_rtid_solve.argtypes = [ctypes.c_void_p, P(Instance), P(Opts), P(Result)]
+        L.mba_simba_solve_noalloc.argtypes = [ctypes.c_void_p, P(Instance), P(Opts),
+                                              P(Result)]
         L.mba_vfb_solve.argtypes = [ctypes.c_void_p, P(Instance), P(Opts), P(Result),
                                     P(VfbStats)]
         L.mba_batch_solve.argtypes = [ctypes.c_void_p, P(Instance), ctypes.c_size_t,
@@ -203,6 +206,15 @@ class Context:
         r = Result()
         self._check(lib().mba_simba_solve(self._h, ctypes.byref(ci), ctypes.byref(o),
                                           ctypes.byref(r)))
+        return r.as_dict()
+
+    def simba_noalloc(self, inst, **kw):
+        """SIMBA search without device allocation (mba_simba_solve_noalloc)."""
+        ci, keep = inst.to_c()
+        o = make_opts(**kw)
+        r = Result()
+        self._check(lib().mba_simba_solve_noalloc(self._h, ctypes.byref(ci), ctypes.byref(o),
+                                                  ctypes.byref(r)))
         return r.as_dict()
 
     def simba_rtid(self, inst, **kw):

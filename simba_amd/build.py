@@ -27,7 +27,7 @@ HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops"]
 
 LIB_SOURCES = [
-    "simba_kernels.hip", "vfb_kernels.hip", "mlp_kernels.hip",
+    "simba_kernels.hip", "vfb_kernels.hip", "mlp_kernels.hip", "noalloc.hip",
     "simba_host.cpp", "vfb_host.cpp", "batch_host.cpp", "mlp_host.cpp", "api.cpp", "expr.cpp",
     "instance_io.cpp",
 ]

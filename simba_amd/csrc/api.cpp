@@ -56,6 +56,11 @@ int mba_ctx_create(int device, mba_ctx **out) {
     delete c;
     return MBA_ERR_HIP;
   }
+  // the no-alloc search's scratch array: its address on this device
+  if (int rc = noalloc_setup(device)) {
+    mba_ctx_destroy(c);
+    return rc;
+  }
   *out = c;
   return 0;
 }
@@ -95,6 +100,18 @@ int mba_simba_solve(mba_ctx *c, const mba_instance *inst, const mba_opts *opts,
   }
   if (hipSetDevice(c->c.device) != hipSuccess) return MBA_ERR_HIP;
   return simba_solve(c->c, *inst, *opts, *res);
+}
+
+int mba_simba_solve_noalloc(mba_ctx *c, const mba_instance *inst, const mba_opts *opts,
+                            mba_result *res) {
+  if (!c || !inst || !res) return MBA_ERR_ARG;
+  mba_opts d;
+  if (!opts) {
+    mba_default_opts(&d);
+    opts = &d;
+  }
+  if (hipSetDevice(c->c.device) != hipSuccess) return MBA_ERR_HIP;
+  return simba_solve_noalloc(c->c, *inst, *opts, *res);
 }
 
 int mba_simba_rtid_solve(mba_ctx *c, const mba_instance *inst, const mba_opts *opts,

@@ -11,12 +11,16 @@
 
 namespace mba {
 
-// Grow-only device allocation owned by a context.
+// Grow-only device allocation owned by a context.  A fixed buffer is a
+// view of memory owned elsewhere: it never allocates, and asking it for more
+// than it holds fails.
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  bool fixed = false;
   hipError_t ensure(size_t want) {
     if (want <= bytes) return hipSuccess;
+    if (fixed) return hipErrorOutOfMemory;
     if (p) {
       hipError_t e = hipFree(p);
       if (e != hipSuccess) return e;
@@ -28,7 +32,7 @@ struct DevBuf {
     return e;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && !fixed) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
   }
@@ -90,6 +94,12 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &opts,
                 const std::function<int()> *score_init = nullptr);
 int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &opts,
               mba_result &res, mba_vfb_stats *stats);
+// The SIMBA search with every device buffer carved from the library's
+// static scratch array (see noalloc.hip); noalloc_setup finds the array's
+// address on the current device (once per device, at context creation).
+int noalloc_setup(int device);
+int simba_solve_noalloc(Ctx &ctx, const mba_instance &inst, const mba_opts &opts,
+                        mba_result &res);
 // Device fingerprints of `count` signatures of n words (see mba_api.h).
 int vfb_fingerprints(Ctx &ctx, int W, const uint64_t *values, int n, size_t count,
                      uint64_t *out);
