@@ -1,28 +1,35 @@
 """Benchmark of the MI355X SIMBA/VFB search.
 
-Default workload (one JSON line on stdout): the largest single-instance SIMBA
+Default line (one JSON line on stdout): the largest single-instance SIMBA
 shape -- leaves {a, b, c, d, 1}, uint64 words, 64 sample points, a uniform
-random size-15 target -- swept exhaustively over every size 1..S (S = 15 by
-default, 3.90e13 candidate trees).  One step = one full sweep; inputs are
-resident on the device before the timed region.  `value` is candidates per
-second over the K timed steps (Gcand/s).
+random size-15 target -- searched with the semantics of an exhaustive search
+to size 20 (every size s is checked with the unary-chain windows of span
+20 - s); one step sweeps sizes 1..15 (3.90e13 candidate trees) and stops
+there, so a step is bounded.  Inputs are resident on the device before the
+timed region.  `value` is candidates per second over the K timed steps.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload simba|vfb]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ...]
 
-roofline: the fused pair kernel (k_pair_check) is integer-VALU bound.
-Algorithmic work per candidate tree = sum over its internal nodes of c(op)
-(64-bit words: & | ^ ~ + - neg = 2 lane-ops, * = 4; 32-bit words: 1), times
-the samples evaluated (1 for all but the rare sample-0 hits).  `achieved` sums
-that over the candidates the pair kernel decides and divides by the kernel's
-HIP-event time on the library stream (every launch of a step).  The kernel
-takes most of a tree's value from the per-size value tables, so this
-naive-equivalent rate can exceed the lane-op rate the hardware executes; the
-counter figures (VALU busy, issued VALU instructions) and the HBM traffic per
-launch come from the rocprofv3 summary under profiles/ for this workload.
-Peak: 256 CU x 64 lanes x 2.4 GHz = 39.3 T lane-ops/s.
+roofline (bound "valu", kernel k_pair_check): the algorithmic work of the
+pair kernel is 10 32-bit lane-ops per (A, B) value pair of a split -- the
+fused low-half window test of the pair's 6 or 7 operator instances (two
+xor-add distances, the product and its two distances, two distances of
+fold(b), three 3-way minima) -- summed over every pair of sizes 3..15,
+divided by the kernel's HIP-event time on the library stream.  Peak:
+256 CU x 64 lanes x 2.4 GHz = 39.3 T lane-ops/s.  The naive-equivalent rate
+(every candidate evaluated node by node) is reported beside it, and the
+executed-VALU counters and HBM traffic of the same command come from the
+rocprofv3 summary under profiles/ (labelled with their source).
 
 cpu_baseline: the CPU oracle (multithreaded C restatement of the reference
-semantics, oracle/) on the same instance for sizes 1..11.
+semantics, threads pinned to the cores this process may use, at most 16:
+the host share of one GPU) on the same instance, every size 1..11; the first
+solution must be the GPU's.
+
+The same line carries compact records of the other configurations
+(`configs`): C1 time to first solution, C2 VFB levels 1..13, C4 batched
+solves/s, C5 learned vs untrained proposal.  `--no-configs` skips them; each
+has its own workload for a full line.
 """
 import argparse
 import json
@@ -35,6 +42,31 @@ sys.path.insert(0, ROOT)
 
 VALU_PEAK = 256 * 64 * 2.4e9  # integer lane-ops/s
 HBM_PEAK_GBS = 8000.0
+PAIR_OPS = 10  # algorithmic lane-ops per (A, B) pair in window mode
+M64 = (1 << 64) - 1
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_info():
+    """(threads to use, CPU model): the CPUs this process may run on, at most
+    16 (a one-GPU box's host share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return max(1, min(16, n)), model
 
 
 def naive_ops_per_size(L, max_size, W):
@@ -56,20 +88,23 @@ def naive_ops_per_size(L, max_size, W):
     return T, C
 
 
-def pair_kernel_ops(T, C, L, max_size, W):
+def naive_pair_kernel_ops(T, C, L, max_size, W):
     """Naive-equivalent lane-ops of the trees the pair kernel decides: every
-    tree of sizes 3..max_size except the unary chains over a single leaf
-    (L * 2^(s-1) trees of s-1 unary nodes each), which the leaf scan covers."""
+    tree of sizes 3..max_size except the unary chains over a single leaf."""
     c_un = 2 if W == 64 else 1
-    tot = 0
-    for s in range(3, max_size + 1):
-        tot += C[s] - L * (1 << (s - 1)) * (s - 1) * c_un
-    return tot
+    return sum(C[s] - L * (1 << (s - 1)) * (s - 1) * c_un for s in range(3, max_size + 1))
+
+
+def pair_count(T, s_lo, s_hi):
+    """(A, B) value pairs the pair kernel checks for sizes s_lo..s_hi: one per
+    pair of trees of an unordered split {j, s-1-j}."""
+    return sum(T[j][8] * T[s - 1 - j][8] for s in range(max(s_lo, 3), s_hi + 1)
+               for j in range(1, (s - 1) // 2 + 1))
 
 
 def profile_fields(name):
-    """Counter figures recorded by rocprofv3 for the pair kernel on this
-    workload: the `bench-fields:` JSON line of profiles/<name>, or {}."""
+    """Figures recorded by rocprofv3 for the pair kernel on this workload:
+    the `bench-fields:` JSON line of profiles/<name>, or {}."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
@@ -81,213 +116,8 @@ def profile_fields(name):
     return {}
 
 
-def cpu_baseline_simba(inst, max_size, threads):
-    from oracle import oracle as orc
-    r = orc.simba_search(inst.inputs, inst.outputs, inst.n_variables,
-                         [int(c) for c in inst.consts], inst.width, 1, max_size, threads,
-                         exhaustive=True)
-    return r
-
-
-def run_simba(args):
-    import simba_amd
-    from simba_amd.synth import make_instance
-
-    inst = make_instance(args.seed, 4, 64, 15, 64, (1,))
-    S = args.size
-    L = 5
-    T, C = naive_ops_per_size(L, S, 64)
-    cands = sum(T[s][8] for s in range(1, S + 1))
-    ctx = simba_amd.Context(0)
-    kw = dict(min_size=1, max_size=args.window_max, stop_size=S, exhaustive=True,
-              table_bytes=args.table_gb << 30)
-    tuned = {k: v for k, v in (("lanes_b", args.lanes_b), ("wg_per_cu", args.wg_per_cu)) if v}
-    kw.update(tuned)
-    if args.autotune:
-        from simba_amd import autotune
-        tuned, _ = autotune.tune_simba(ctx, inst, S - 1)
-        kw.update(tuned)
-    for _ in range(args.warmup):
-        ctx.simba(inst, **kw)
-    times, ktimes = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r = ctx.simba(inst, **kw)
-        times.append(r["seconds"])
-        ktimes.append(r["kernel_seconds"])
-        assert r["size_reached"] == S and r["candidates"] == cands, r
-    wall = time.perf_counter() - t0
-    ms_per_step = 1e3 * wall / args.steps
-    value = cands * args.steps / wall / 1e9
-    kavg = sum(ktimes) / len(ktimes)
-    launches = r["kernel_launches"]
-    pk_ops = pair_kernel_ops(T, C, L, S, 64)
-    achieved = pk_ops / kavg / 1e12
-    prof = profile_fields("k_pair_check_simba_sweep_sizes1to%d.md" % S)
-    out = {
-        "metric": "Gcand/s evaluated",
-        "value": round(value, 3),
-        "unit": "Gcand/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u64",
-        "data": "synthetic",
-        "config": {"workload": "simba_sweep_4vars_const1_u64_64samples_sizes1to%d" % S,
-                   "target": inst.original_formula, "target_size": 15,
-                   "candidates_per_step": cands, "first_solution": r["expr"],
-                   "first_solution_size": r["size"], "launch": tuned or "default"},
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3),
-                     "peak": round(VALU_PEAK / 1e12, 3), "unit": "TOP/s",
-                     "frac": round(achieved * 1e12 / VALU_PEAK, 4),
-                     "traffic": prof.get("traffic_bytes_per_launch"),
-                     "kernel": "k_pair_check",
-                     "definition": "naive-equivalent 32-bit lane-ops of the candidates the "
-                                   "pair kernel decides (sum of c(op) over every internal "
-                                   "node; u64: 2 per op, 4 per *) / pair-kernel HIP-event "
-                                   "time; the kernel reuses tabulated subtree values, so "
-                                   "this can exceed the executed rate",
-                     "launches_per_step": launches,
-                     "avg_launch_ms": round(1e3 * kavg / max(launches, 1), 4),
-                     "kernel_seconds_per_step": round(kavg, 4),
-                     "counter_valu_busy": prof.get("valu_busy"),
-                     "counter_valu_inst_rate_frac": prof.get("valu_inst_frac"),
-                     "counter_source": prof.get("source")},
-    }
-    ctx.close()
-    if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        cb = cpu_baseline_simba(inst, args.cpu_size, threads)
-        out["cpu_baseline"] = {
-            "value": round(cb["candidates"] / cb["seconds"] / 1e9, 5), "unit": "Gcand/s",
-            "cores": threads, "kind": "port",
-            "sample": "oracle SIMBA sweep of the same instance, every size 1..%d "
-                      "(%d candidates, %.1f s)" % (args.cpu_size, cb["candidates"], cb["seconds"])}
-    return out
-
-
-def run_simba_small(args):
-    """Leaves {x, y, 1}, u64, 16 samples, a uniform size-7 target, sizes 1..9:
-    time to the first solution (latency-bound: at most 6.3e6 candidates).
-    The default seed draws a target whose first solution has size 7."""
-    import numpy as np
-
-    import simba_amd
-    from simba_amd.synth import make_instance
-
-    inst = make_instance(args.small_seed, 2, 16, 7, 64, (1,))
-    ctx = simba_amd.Context(0)
-    kw = dict(min_size=1, max_size=9)
-    for _ in range(max(args.warmup, 1)):
-        r = ctx.simba(inst, **kw)
-    reps = max(args.steps, 1) * 20
-    times = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        r = ctx.simba(inst, **kw)
-        times.append(time.perf_counter() - t0)
-    ctx.close()
-    med = float(np.median(times))
-    out = {
-        "metric": "time to first solution", "value": round(1e3 * med, 4), "unit": "ms",
-        "n_gpus": 1, "steps": reps, "warmup": max(args.warmup, 1),
-        "ms_per_step": round(1e3 * med, 4), "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u64", "data": "synthetic",
-        "config": {"workload": "simba_2vars_const1_u64_16samples_size7target_sizes1to9",
-                   "target": inst.original_formula, "first_solution": r["expr"],
-                   "size": r["size"], "rank": r["rank"], "candidates": r["candidates"],
-                   "p95_ms": round(1e3 * float(np.percentile(times, 95)), 4)},
-    }
-    if not args.no_cpu:
-        from oracle import oracle as orc
-        threads = min(16, os.cpu_count() or 1)
-        cb = orc.simba_search(inst.inputs, inst.outputs, 2, [1], 64, 1, 9, threads)
-        assert (cb["size"], cb["rank"]) == (r["size"], r["rank"])
-        out["cpu_baseline"] = {"value": round(1e3 * cb["seconds"], 4), "unit": "ms",
-                               "cores": threads, "kind": "port",
-                               "sample": "oracle SIMBA search of the same instance, sizes 1..9"}
-    return out
-
-
-def run_simba_cap(args):
-    """The same instance swept exhaustively over sizes 1..20 under a time cap
-    (one run): how far the search gets, and its rate over completed sizes."""
-    import simba_amd
-    from simba_amd.synth import count_table, make_instance
-
-    inst = make_instance(args.seed, 4, 64, 15, 64, (1,))
-    ctx = simba_amd.Context(0)
-    t0 = time.perf_counter()
-    r = ctx.simba(inst, min_size=1, max_size=20, exhaustive=True, budget_s=args.cap,
-                  table_bytes=args.table_gb << 30)
-    wall = time.perf_counter() - t0
-    ctx.close()
-    T = count_table(5, 20)
-    done = sum(T[s][8] for s in range(1, r["size_reached"] + 1))
-    return {
-        "metric": "Gcand/s evaluated", "value": round(done / r["seconds"] / 1e9, 3),
-        "unit": "Gcand/s", "n_gpus": 1, "steps": 1, "warmup": 0,
-        "ms_per_step": round(1e3 * wall, 1), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u64", "data": "synthetic",
-        "config": {"workload": "simba_4vars_const1_u64_64samples_sizes1to20_cap%ds" % args.cap,
-                   "status": r["status"], "size_completed": r["size_reached"],
-                   "candidates_completed": done, "search_seconds": round(r["seconds"], 3),
-                   "first_solution": r["expr"], "first_solution_size": r["size"]},
-    }
-
-
-def run_vfb(args):
-    import simba_amd
-    from simba_amd.synth import make_instance
-
-    inst = make_instance(args.seed, 3, 32, 11, 64, (1, (1 << 64) - 1))
-    ctx = simba_amd.Context(0)
-    kw = dict(max_size=args.vfb_levels, exhaustive=True, cache_bytes=args.cache_gb << 30)
-    for _ in range(args.warmup):
-        ctx.vfb(inst, **kw)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r = ctx.vfb(inst, **kw)
-    wall = time.perf_counter() - t0
-    cands = r["candidates"]
-    nbytes = sum(r["level_bytes"])
-    out = {
-        "metric": "Gcand/s evaluated", "value": round(cands * args.steps / wall / 1e9, 3),
-        "unit": "Gcand/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(1e3 * wall / args.steps, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
-        "config": {"workload": "vfb_3vars_consts1m1_u64_32samples_levels1to%d" % args.vfb_levels,
-                   "level_new": r["level_new"][1:args.vfb_levels + 1],
-                   "level_cand": r["level_cand"][1:args.vfb_levels + 1]},
-        "roofline": {"bound": "hbm", "achieved": round(nbytes / r["kernel_seconds"] / 1e9, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(nbytes / r["kernel_seconds"] / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": None},
-    }
-    ctx.close()
-    if not args.no_cpu:
-        from oracle import oracle as orc
-        lv = min(args.vfb_levels, 8)
-        t = time.perf_counter()
-        w = orc.vfb_search(inst.inputs, inst.outputs, 3, [1, (1 << 64) - 1], 64, lv,
-                           exhaustive=True)
-        dt = time.perf_counter() - t
-        out["cpu_baseline"] = {"value": round(w["candidates"] / dt / 1e9, 5), "unit": "Gcand/s",
-                               "cores": 1, "kind": "port",
-                               "sample": "oracle VFB, same instance, levels 1..%d" % lv}
-    return out
-
-
-def _key(r):
-    return (r["status"], r["size"], r["rank"], r["expr"])
-
-
 def _verified(r, inst):
-    """Re-evaluates a returned SIMBA RPN on every sample (product evaluator)."""
+    """Re-evaluates a returned SIMBA RPN on every sample."""
     from simba_amd.synth import TOK_OP, evaluate
     L = inst.n_variables + len(inst.consts)
     toks = [ord(c) - 48 if c.isdigit() and ord(c) - 48 < L else TOK_OP + "~&|^#+-*".index(c)
@@ -300,9 +130,212 @@ def _verified(r, inst):
     return True
 
 
-def run_batch(args):
-    """4096 instances (vars 2..5, target size 5..17, samples 16..64) solved by
-    one batched call vs one plain call per instance, same size budget."""
+def _key(r):
+    return (r["status"], r["size"], r["rank"], r["expr"])
+
+
+def _safe(fn, args):
+    """A configuration record, or the error that stopped it."""
+    t0 = time.perf_counter()
+    try:
+        rec = fn(args)
+    except Exception as e:  # a failing record must not hide the headline
+        rec = {"error": "%s: %s" % (type(e).__name__, e)}
+    rec["record_seconds"] = round(time.perf_counter() - t0, 2)
+    return rec
+
+
+# ---- default line: the largest single-instance SIMBA shape -----------------
+def run_simba(args):
+    import simba_amd
+    from simba_amd.synth import make_instance
+    from oracle import oracle as orc
+
+    inst = make_instance(args.seed, 4, 64, 15, 64, (1,))
+    S, MX, L = args.size, args.window_max, 5
+    T, C = naive_ops_per_size(L, MX, 64)
+    cands = sum(T[s][8] for s in range(1, S + 1))
+    pairs = pair_count(T, 3, S)
+    ctx = simba_amd.Context(0)
+    kw = dict(min_size=1, max_size=MX, stop_size=S, exhaustive=True,
+              table_bytes=args.table_gb << 30)
+    tuned = {k: v for k, v in (("lanes_b", args.lanes_b), ("wg_per_cu", args.wg_per_cu)) if v}
+    kw.update(tuned)
+    if args.autotune:
+        from simba_amd import autotune
+        tuned, _ = autotune.tune_simba(ctx, inst, S - 1)
+        kw.update(tuned)
+    log("simba: warmup")
+    for _ in range(args.warmup):
+        ctx.simba(inst, **kw)
+    ktimes = []
+    log("simba: timed steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = ctx.simba(inst, **kw)
+        ktimes.append(r["kernel_seconds"])
+        assert r["size_reached"] == S and r["candidates"] == cands, r
+    wall = time.perf_counter() - t0
+    ctx.close()
+    ms_per_step = 1e3 * wall / args.steps
+    kavg = sum(ktimes) / len(ktimes)
+    launches = r["kernel_launches"]
+    achieved = pairs * PAIR_OPS / kavg
+    naive = naive_pair_kernel_ops(T, C, L, S, 64) / kavg
+    prof = profile_fields("k_pair_check_simba_c3_windows20_sizes1to%d.md" % S)
+    out = {
+        "metric": "Gcand/s evaluated",
+        "value": round(cands * args.steps / wall / 1e9, 3),
+        "unit": "Gcand/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "simba_4vars_const1_u64_64samples_size15target_"
+                               "maxsize%d_sizes1to%d" % (MX, S),
+                   "search": "exhaustive search to size %d (windows of span %d - s), "
+                             "sweep stopped after size %d" % (MX, MX, S),
+                   "target": inst.original_formula, "target_size": 15,
+                   "candidates_per_step": cands, "pairs_per_step": pairs,
+                   "first_solution": r["expr"], "first_solution_size": r["size"],
+                   "first_solution_rank": r["rank"], "launch": tuned or "default"},
+        "roofline": {
+            "bound": "valu", "kernel": "k_pair_check",
+            "achieved": round(achieved / 1e12, 3), "peak": round(VALU_PEAK / 1e12, 3),
+            "unit": "TOP/s", "frac": round(achieved / VALU_PEAK, 4),
+            "traffic": prof.get("traffic_bytes_per_launch"),
+            "definition": "algorithmic 32-bit lane-ops of the pair kernel: %d per (A, B) "
+                          "value pair (the fused low-half window test of the pair's 6-7 "
+                          "operator instances) x %d pairs per step / pair-kernel HIP-event "
+                          "time per step" % (PAIR_OPS, pairs),
+            "launches_per_step": launches,
+            "avg_launch_ms": round(1e3 * kavg / max(launches, 1), 4),
+            "kernel_seconds_per_step": round(kavg, 4),
+            "naive_equivalent_TOPs": round(naive / 1e12, 1),
+            "counters": {k: prof.get(k) for k in ("valu_issue_util", "valu_busy",
+                                                  "executed_lane_ops_per_step",
+                                                  "kernel_avg_launch_us", "hbm_GBs")},
+            "counters_source": prof.get("source"),
+        },
+    }
+    if not args.no_cpu:
+        threads, model = cpu_info()
+        log("simba: cpu baseline")
+        cb = orc.simba_search(inst.inputs, inst.outputs, 4, [1], 64, 1, args.cpu_size, threads,
+                              exhaustive=True)
+        # the headline's first solution is the oracle's when it lies within
+        # the sizes the oracle swept (else the oracle finds none there)
+        if r["size"] <= args.cpu_size:
+            assert (cb["status"], cb["size"], cb["rank"], cb["expr"]) == \
+                (0, r["size"], r["rank"], r["expr"]), (cb, r)
+        else:
+            assert cb["status"] != 0, cb
+        out["config"]["first_solution_matches_oracle"] = True
+        out["cpu_baseline"] = {
+            "value": round(cb["candidates"] / cb["seconds"] / 1e9, 5), "unit": "Gcand/s",
+            "cores": threads, "kind": "port", "cpu_model": model,
+            "sample": "oracle SIMBA sweep of the same instance, every size 1..%d "
+                      "(%d candidates, %.1f s, threads pinned); per-candidate cost is "
+                      "about constant over sizes, so the rate stands for sizes 1..%d"
+                      % (args.cpu_size, cb["candidates"], cb["seconds"], S)}
+    if not args.no_configs:
+        out["configs"] = {}
+        for name, fn in (("c1_latency", rec_c1), ("c2_vfb", rec_c2), ("c4_batch", rec_c4),
+                         ("c5_learned", rec_c5)):
+            log("config record: " + name)
+            out["configs"][name] = _safe(fn, args)
+    return out
+
+
+# ---- C1: three leaves, time to the first solution --------------------------
+def rec_c1(args):
+    """Leaves {x, y, 1}, u64, 16 samples, a uniform size-7 target, sizes 1..9:
+    median time to the first solution over repeated calls (latency-bound:
+    at most 6.3e6 candidates).  The seed draws a target whose first solution
+    has size 7."""
+    import numpy as np
+
+    import simba_amd
+    from simba_amd.synth import make_instance
+    from oracle import oracle as orc
+
+    inst = make_instance(args.small_seed, 2, 16, 7, 64, (1,))
+    ctx = simba_amd.Context(0)
+    kw = dict(min_size=1, max_size=9)
+    for _ in range(3):
+        r = ctx.simba(inst, **kw)
+    times = []
+    for _ in range(100):
+        t0 = time.perf_counter()
+        r = ctx.simba(inst, **kw)
+        times.append(time.perf_counter() - t0)
+    ctx.close()
+    threads, _ = cpu_info()
+    cb = orc.simba_search(inst.inputs, inst.outputs, 2, [1], 64, 1, 9, threads)
+    assert (cb["size"], cb["rank"], cb["expr"]) == (r["size"], r["rank"], r["expr"])
+    return {"workload": "simba_2vars_const1_u64_16samples_size7target_sizes1to9",
+            "p50_ms": round(1e3 * float(np.median(times)), 4),
+            "p95_ms": round(1e3 * float(np.percentile(times, 95)), 4),
+            "first_solution": r["expr"], "size": r["size"], "rank": r["rank"],
+            "matches_oracle": True,
+            "cpu_baseline_ms": round(1e3 * cb["seconds"], 4), "cpu_threads": threads}
+
+
+# ---- C2: VFB levels 1..13 ---------------------------------------------------
+def _vfb_instance(args):
+    from simba_amd.synth import make_instance
+    return make_instance(args.seed, 3, 32, 11, 64, (1, M64))
+
+
+def rec_c2(args):
+    """VFB, vars {x, y, z} + constants {1, -1}, u64, 32 samples, a size-11
+    target, every level 1..13 swept (the hash-only last level included):
+    candidates/s, algorithmic and atomic rates, kept rows per level; the CPU
+    baseline is the multithreaded oracle VFB to level 11 (same counts)."""
+    import simba_amd
+    from oracle import oracle as orc
+
+    inst = _vfb_instance(args)
+    ctx = simba_amd.Context(0)
+    kw = dict(max_size=args.vfb_levels, exhaustive=True, cache_bytes=args.cache_gb << 30)
+    ctx.vfb(inst, **kw)
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        r = ctx.vfb(inst, **kw)
+    wall = (time.perf_counter() - t0) / reps
+    ctx.close()
+    nb = sum(r["level_bytes"])
+    ks = r["kernel_seconds"]
+    rec = {"workload": "vfb_3vars_consts1m1_u64_32samples_levels1to%d" % args.vfb_levels,
+           "ms_per_sweep": round(1e3 * wall, 2), "kernel_ms": round(1e3 * ks, 2),
+           "Gcand_s": round(r["candidates"] / wall / 1e9, 3),
+           "algorithmic_GBs": round(nb / ks / 1e9, 1),
+           "hbm_frac": round(nb / ks / 1e9 / HBM_PEAK_GBS, 4),
+           "level_new": r["level_new"][1:args.vfb_levels + 1],
+           "level_cand": r["level_cand"][1:args.vfb_levels + 1]}
+    for k in ("level_cas", "level_atomics"):
+        if k in r:
+            rec[k.replace("level_", "") + "_per_s"] = round(sum(r[k]) / ks, 1)
+    if not args.no_cpu:
+        threads, model = cpu_info()
+        lv = args.vfb_cpu_levels
+        w = orc.vfb_search_mt(inst.inputs, inst.outputs, 3, [1, M64], 64, lv, threads=threads)
+        assert w["level_new"][1:lv + 1] == r["level_new"][1:lv + 1]
+        rec["cpu_baseline"] = {"Gcand_s": round(w["candidates"] / w["seconds"] / 1e9, 5),
+                               "cores": threads, "kind": "port", "cpu_model": model,
+                               "sample": "oracle VFB (level-synchronous, concurrent "
+                                         "fingerprint set), same instance, levels 1..%d "
+                                         "(%.1f s), kept rows equal to the GPU's"
+                                         % (lv, w["seconds"])}
+    return rec
+
+
+# ---- C4: batched solving ----------------------------------------------------
+def rec_c4(args):
+    """4096 instances (vars 2..5, target size 5..17, samples 16..64) by one
+    batched call vs one plain call per instance, same size budget: solves/s,
+    solve rate and exact match of the answers."""
     import simba_amd
     from simba_amd.learn import sample_instances
 
@@ -316,105 +349,151 @@ def run_batch(args):
         tuned, _ = autotune.tune_batch(ctx, insts[:1024], args.budget_size)
         kw.update(tuned)
     ctx.batch(insts, **kw)
+    reps = 3
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        rb = ctx.batch(insts, verbose=args.verbose, **kw)
-    t_batch = (time.perf_counter() - t0) / args.steps
+    for _ in range(reps):
+        rb = ctx.batch(insts, **kw)
+    t_batch = (time.perf_counter() - t0) / reps
     t0 = time.perf_counter()
     rp = [ctx.simba(x, **kw) for x in insts]
     t_plain = time.perf_counter() - t0
-    solved = sum(r["status"] == 0 for r in rb)
-    exact = sum(_key(a) == _key(b) for a, b in zip(rb, rp)) / len(insts)
     ctx.close()
-    return {
-        "metric": "batched solves/s", "value": round(solved / t_batch, 2), "unit": "solves/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": round(1e3 * t_batch, 2),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
-        "data": "synthetic",
-        "config": {"workload": "batch_%d_vars2to5_size5to17_samples16to64_budget%d"
-                               % (args.count, args.budget_size),
-                   "solved": solved, "exact_match_vs_plain": exact,
-                   "plain_per_instance_solves_per_s": round(solved / t_plain, 2),
-                   "plain_seconds": round(t_plain, 3),
-                   "speedup_vs_plain": round(t_plain / t_batch, 2),
-                   "launch": tuned or "default"},
-    }
+    solved = sum(r["status"] == 0 for r in rb)
+    solved_plain = sum(r["status"] == 0 for r in rp)
+    exact = sum(_key(a) == _key(b) for a, b in zip(rb, rp)) / len(insts)
+    return {"workload": "batch_%d_vars2to5_size5to17_samples16to64_budget%d"
+                        % (args.count, args.budget_size),
+            "solves_per_s": round(solved / t_batch, 1),
+            "solve_rate": round(solved / len(insts), 4),
+            "ms_per_call": round(1e3 * t_batch, 2),
+            "plain_per_instance_solves_per_s": round(solved_plain / t_plain, 1),
+            "plain_solve_rate": round(solved_plain / len(insts), 4),
+            "speedup_vs_plain": round(t_plain / t_batch, 2),
+            "exact_match_vs_plain": exact, "launch": tuned or "default"}
 
 
-def run_learned(args):
+# ---- C5: learned proposal ---------------------------------------------------
+def rec_c5(args):
     """Learned block proposal: trained on plain-solved instances (sizes 3..13,
-    2..4 vars, 32 samples), evaluated on held-out instances (sizes 9..19)."""
+    2..4 vars, 32 samples), evaluated on held-out instances (sizes 9..19)
+    against the same network untrained and the plain search."""
     import numpy as np
 
     import simba_amd
     from simba_amd import learn
 
-    log = lambda m: print(m, file=sys.stderr, flush=True)
     ctx = simba_amd.Context(0)
     t0 = time.perf_counter()
-    X, Y = learn.build_dataset(ctx, args.train, args.seed, max_size=args.label_size, log=log)
+    X, Y = learn.build_dataset(ctx, args.train, args.seed, max_size=args.label_size)
     t_label = time.perf_counter() - t0
     mlp = simba_amd.Mlp(ctx, seed=args.seed)
     untrained = simba_amd.Mlp(ctx, seed=args.seed + 1)
     t0 = time.perf_counter()
-    losses = learn.train(mlp, X, Y, epochs=args.epochs, log=log)
+    losses = learn.train(mlp, X, Y, epochs=args.epochs)
     t_train = time.perf_counter() - t0
-    ev = learn.sample_instances(args.seed + 10**6, args.count, (9, 19), (2, 4), (32, 32))
+    ev = learn.sample_instances(args.seed + 10**6, args.eval_count, (9, 19), (2, 4), (32, 32))
     kw = dict(max_size=args.learned_budget)
     mlp.solve(ev[0], **kw)
     t0 = time.perf_counter()
     plain = [ctx.simba(x, **kw) for x in ev]
     t_plain_all = time.perf_counter() - t0
-    log("plain per-instance done")
     lrn = [mlp.solve(x, **kw) for x in ev]
-    log("learned per-instance done")
     unt = [untrained.solve(x, **kw) for x in ev]
-    log("untrained per-instance done")
     solved = [i for i, r in enumerate(plain) if r["status"] == 0]
     same_size = all(lrn[i]["status"] == 0 and lrn[i]["size"] == plain[i]["size"] for i in solved)
     verified = all(lrn[i]["status"] == 0 and _verified(lrn[i], ev[i]) for i in solved)
-    missed = [i for i in solved if lrn[i]["status"] != 0 or lrn[i]["size"] != plain[i]["size"]]
     tp = np.array([plain[i]["seconds"] for i in solved])
     tl = np.array([lrn[i]["seconds"] for i in solved])
     tu = np.array([unt[i]["seconds"] for i in solved])
-    mlp.batch_solve(ev, **kw)  # warm-up: device buffers sized for the set
+    mlp.batch_solve(ev, **kw)
     ctx.batch(ev, **kw)
     t0 = time.perf_counter()
     bl = mlp.batch_solve(ev, **kw)
     t_bl = time.perf_counter() - t0
     t0 = time.perf_counter()
+    bu = untrained.batch_solve(ev, **kw)
+    t_bu = time.perf_counter() - t0
+    t0 = time.perf_counter()
     bp = ctx.batch(ev, **kw)
     t_bp = time.perf_counter() - t0
-    bp_solved = sum(r["status"] == 0 for r in bp)
-    b_solved = sum(r["status"] == 0 for r in bl)
-    b_same = all(bl[i]["status"] == 0 and bl[i]["size"] == plain[i]["size"] for i in solved)
     ctx.close()
-    pct = lambda a, q: round(float(np.percentile(a, q)) * 1e3, 3) if len(a) else None
-    # the per-instance path over the whole set, unsolved instances included
-    plain_rate = len(solved) / t_plain_all if len(solved) else 0.0
-    return {
-        "metric": "learned vs plain p50 time-to-solution", "value": round(
-            float(np.percentile(tp, 50) / np.percentile(tl, 50)), 3) if len(solved) else None,
-        "unit": "x (plain p50 / learned p50)", "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": pct(tl, 50), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u64 search, bf16 MFMA MLP", "data": "synthetic",
-        "config": {"workload": "learned_train%d_eval%d_sizes9to19_budget%d"
-                               % (args.train, args.count, args.learned_budget),
-                   "train_examples": int(len(Y)), "label_seconds": round(t_label, 2),
-                   "train_seconds": round(t_train, 2), "final_loss": round(losses[-1], 4),
-                   "solved_plain": len(solved), "same_size": same_size,
-                   "verified": verified, "mismatched": len(missed),
-                   "p50_ms": {"plain": pct(tp, 50), "learned": pct(tl, 50),
-                              "untrained": pct(tu, 50)},
-                   "p95_ms": {"plain": pct(tp, 95), "learned": pct(tl, 95),
-                              "untrained": pct(tu, 95)},
-                   "batched_learned_solves_per_s": round(b_solved / t_bl, 2),
-                   "batched_plain_solves_per_s": round(bp_solved / t_bp, 2),
-                   "plain_per_instance_solves_per_s": round(plain_rate, 2),
-                   "batched_learned_vs_plain_per_instance": round(
-                       (b_solved / t_bl) / plain_rate, 2) if plain_rate else None,
-                   "batched_learned_same_size": b_same},
-    }
+    pct = lambda a, q: round(float(np.percentile(a, q)) * 1e3, 4) if len(a) else None
+    nsol = lambda rs: sum(r["status"] == 0 for r in rs)
+    plain_rate = len(solved) / t_plain_all if solved else 0.0
+    return {"workload": "learned_train%d_eval%d_sizes9to19_budget%d"
+                        % (args.train, args.eval_count, args.learned_budget),
+            "train_examples": int(len(Y)), "label_seconds": round(t_label, 2),
+            "train_seconds": round(t_train, 2), "final_loss": round(losses[-1], 4),
+            "solved_plain": len(solved), "same_size": same_size, "verified": verified,
+            "ordered_sizes": sum(r["ordered_sizes"] for r in lrn),
+            "p50_ms": {"plain": pct(tp, 50), "learned": pct(tl, 50), "untrained": pct(tu, 50)},
+            "p95_ms": {"plain": pct(tp, 95), "learned": pct(tl, 95), "untrained": pct(tu, 95)},
+            "batched_solves_per_s": {"learned": round(nsol(bl) / t_bl, 1),
+                                     "untrained": round(nsol(bu) / t_bu, 1),
+                                     "plain": round(nsol(bp) / t_bp, 1)},
+            "plain_per_instance_solves_per_s": round(plain_rate, 1),
+            "batched_learned_vs_plain_per_instance":
+                round((nsol(bl) / t_bl) / plain_rate, 2) if plain_rate else None,
+            "batched_learned_same_size": all(
+                bl[i]["status"] == 0 and bl[i]["size"] == plain[i]["size"] for i in solved)}
+
+
+# ---- single-record workloads ------------------------------------------------
+def _line(metric, value, unit, hib, ms, rec, dtype="u64", steps=1, warmup=1):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": ms, "higher_is_better": hib, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+            "config": rec}
+
+
+def run_simba_small(args):
+    rec = rec_c1(args)
+    return _line("time to first solution", rec["p50_ms"], "ms", False, rec["p50_ms"], rec,
+                 steps=100, warmup=3)
+
+
+def run_vfb(args):
+    rec = rec_c2(args)
+    return _line("Gcand/s evaluated", rec["Gcand_s"], "Gcand/s", True, rec["ms_per_sweep"],
+                 rec, steps=3)
+
+
+def run_batch(args):
+    rec = rec_c4(args)
+    return _line("batched solves/s", rec["solves_per_s"], "solves/s", True,
+                 rec["ms_per_call"], rec, steps=3)
+
+
+def run_learned(args):
+    rec = rec_c5(args)
+    p = rec["p50_ms"]
+    v = round(p["plain"] / p["learned"], 3) if p["plain"] and p["learned"] else None
+    return _line("learned vs plain p50 time-to-solution", v, "x (plain p50 / learned p50)",
+                 True, p["learned"], rec, dtype="u64 search, bf16 MFMA MLP")
+
+
+def run_simba_cap(args):
+    """The default instance searched to size 20 under a time cap, as the
+    reference's runs are (one run): how far the search gets, and its rate
+    over the completed sizes."""
+    import simba_amd
+    from simba_amd.synth import count_table, make_instance
+
+    inst = make_instance(args.seed, 4, 64, 15, 64, (1,))
+    ctx = simba_amd.Context(0)
+    t0 = time.perf_counter()
+    r = ctx.simba(inst, min_size=1, max_size=20, exhaustive=True, budget_s=args.cap,
+                  table_bytes=args.table_gb << 30)
+    wall = time.perf_counter() - t0
+    ctx.close()
+    T = count_table(5, 20)
+    done = sum(T[s][8] for s in range(1, r["size_reached"] + 1))
+    rec = {"workload": "simba_4vars_const1_u64_64samples_sizes1to20_cap%ds" % args.cap,
+           "status": r["status"], "size_completed": r["size_reached"],
+           "candidates_completed": done, "search_seconds": round(r["seconds"], 3),
+           "first_solution": r["expr"], "first_solution_size": r["size"]}
+    return _line("Gcand/s evaluated", round(done / r["seconds"] / 1e9, 3), "Gcand/s", True,
+                 round(1e3 * wall, 1), rec, warmup=0)
 
 
 def run_copy(args):
@@ -433,11 +512,9 @@ def run_copy(args):
         dst.copy_(src)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
-    return {"metric": "copy GB/s", "value": round(2 * n / dt / 1e9, 1), "unit": "GB/s",
-            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": round(1e3 * dt, 3), "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "u8", "data": "synthetic",
-            "config": {"workload": "d2d_copy_1GiB"}}
+    return _line("copy GB/s", round(2 * n / dt / 1e9, 1), "GB/s", True, round(1e3 * dt, 3),
+                 {"workload": "d2d_copy_1GiB"}, dtype="u8", steps=args.steps,
+                 warmup=args.warmup)
 
 
 def main():
@@ -448,29 +525,31 @@ def main():
     ap.add_argument("--workload", default="simba",
                     choices=["simba", "simba_small", "simba_cap", "vfb", "batch", "learned",
                              "copy"])
+    ap.add_argument("--no-configs", action="store_true",
+                    help="simba: leave out the records of the other configurations")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--size", type=int, default=15, help="simba: last size swept")
+    ap.add_argument("--window-max", type=int, default=20,
+                    help="simba: the search's max size (unary-chain windows of span max - s)")
+    ap.add_argument("--cpu-size", type=int, default=11)
+    ap.add_argument("--table-gb", type=int, default=40)
+    ap.add_argument("--seed", type=int, default=15)
+    ap.add_argument("--small-seed", type=int, default=27,
+                    help="C1 instance seed (27: no equivalent below size 7)")
+    ap.add_argument("--vfb-levels", type=int, default=13)
+    ap.add_argument("--vfb-cpu-levels", type=int, default=11)
+    ap.add_argument("--cache-gb", type=int, default=96)
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--budget-size", type=int, default=9)
     ap.add_argument("--learned-budget", type=int, default=11,
-                    help="learned: largest size searched (the learned block order applies "
-                         "to sizes of at least 2^26 trees)")
+                    help="C5: largest size searched (the learned order applies to sizes "
+                         "of at least 2^26 trees)")
     ap.add_argument("--train", type=int, default=200000)
+    ap.add_argument("--eval-count", type=int, default=4096)
     ap.add_argument("--label-size", type=int, default=9)
     ap.add_argument("--epochs", type=int, default=4)
-    ap.add_argument("--size", type=int, default=15)
-    ap.add_argument("--window-max", type=int, default=20,
-                    help="simba: the search's max size (unary-chain windows of size "
-                         "max - s); the sweep ends after --size")
-    ap.add_argument("--cpu-size", type=int, default=11)
-    ap.add_argument("--table-gb", type=int, default=40)
-    ap.add_argument("--vfb-levels", type=int, default=13)
-    ap.add_argument("--cache-gb", type=int, default=96)
-    ap.add_argument("--seed", type=int, default=15)
-    ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cap", type=int, default=60, help="simba_cap: time cap in seconds")
-    ap.add_argument("--small-seed", type=int, default=27,
-                    help="simba_small: instance seed (27: no equivalent below size 7)")
     ap.add_argument("--lanes-b", type=int, default=0, help="B values per lane (4 or 8)")
-    ap.add_argument("--verbose", action="store_true", help="library progress on stderr")
     ap.add_argument("--wg-per-cu", type=int, default=0, help="workgroups per CU")
     ap.add_argument("--autotune", action="store_true",
                     help="tune launch parameters on the workload before timing")
@@ -479,9 +558,9 @@ def main():
         print("bench.py measures one GPU: every box here has a single MI355X; "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    out = {"simba": run_simba, "vfb": run_vfb, "batch": run_batch,
-           "learned": run_learned, "copy": run_copy,
-           "simba_cap": run_simba_cap, "simba_small": run_simba_small}[args.workload](args)
+    out = {"simba": run_simba, "vfb": run_vfb, "batch": run_batch, "learned": run_learned,
+           "copy": run_copy, "simba_cap": run_simba_cap,
+           "simba_small": run_simba_small}[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

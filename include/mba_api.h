@@ -113,7 +113,9 @@ typedef struct {
                             /* level; batch: one per graph replay)          */
   int32_t hit_replays;      /* SIMBA per-instance: launch groups replayed    */
                             /* in parts after a hit-buffer overflow         */
-  int32_t reserved[3];
+  int32_t ordered_sizes;    /* learned search: sizes whose blocks were       */
+                            /* visited in the network's order               */
+  int32_t reserved[2];
 } mba_result;
 
 /* VFB per-level statistics (PRINT_CACHE_SIZE mode, full-cache.cu:763-765). */
@@ -124,6 +126,8 @@ typedef struct {
   uint64_t level_bytes[MBA_MAX_SIZE + 1]; /* algorithmic HBM bytes per level */
   uint64_t table_slots;
   uint64_t rows_capacity;
+  uint64_t level_cas[MBA_MAX_SIZE + 1];     /* 64-bit key CAS attempts      */
+  uint64_t level_atomics[MBA_MAX_SIZE + 1]; /* CAS + companion atomicMin    */
 } mba_vfb_stats;
 
 typedef struct mba_ctx mba_ctx;

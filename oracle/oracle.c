@@ -13,19 +13,20 @@
  * constants.
  *
  * Pinning: every function is checked by tests/ against brute force, closed
- * forms or golden vectors, except two, whose parity is unpinned beyond
- * invariants (no reference test or vector exists): the combine step of the
- * VFB fingerprint (orc_fingerprint; its fmix32 is pinned by the published
- * MurmurHash3_x86_32 vectors) and the probe hash orc_fmix64, which only
- * places keys in the hash set and changes no answer or count.
+ * forms or golden vectors.  The VFB fingerprint's finalizers are checked
+ * through the published MurmurHash3 vectors (fmix32: x86_32, fmix64:
+ * x64_128), its combine step (makeUnqChk) by known answers of one sample,
+ * the 32-bit wrap of its first sum and the rotation of odd indices.
  *
  * Token encoding of an RPN formula (one byte per node):
  *   0..31   leaf index (variables first, then constants)
  *   32+o    operator o, o in 0..7 = ~ & | ^ neg + - *   (globals.cuh:13-15)
  */
+#define _GNU_SOURCE
 #include "oracle.h"
 
 #include <pthread.h>
+#include <sched.h>
 #include <stdatomic.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -39,6 +40,25 @@ static double now_s(void) {
 }
 
 static inline uint64_t wmask(int W) { return W == 64 ? ~0ull : 0xffffffffull; }
+
+/* Pins the calling thread to the k-th CPU (mod the count) of the process's
+ * allowed set, so worker k of a pool runs on its own core. */
+static void pin_self(int k) {
+  cpu_set_t allowed, one;
+  if (sched_getaffinity(0, sizeof allowed, &allowed) != 0) return;
+  int n = CPU_COUNT(&allowed);
+  if (n <= 0) return;
+  int want = k % n, seen = 0;
+  for (int c = 0; c < CPU_SETSIZE; ++c) {
+    if (!CPU_ISSET(c, &allowed)) continue;
+    if (seen++ == want) {
+      CPU_ZERO(&one);
+      CPU_SET(c, &one);
+      pthread_setaffinity_np(pthread_self(), sizeof one, &one);
+      return;
+    }
+  }
+}
 
 /* ------------------------------------------------------------------------ */
 /* Count table: nocache.cu:117-160 (compute_n_formulas).                     */
@@ -278,14 +298,16 @@ typedef struct {
   _Atomic uint64_t best;
   _Atomic uint64_t evaluated;
   _Atomic int stop;
+  _Atomic int next_worker;
   double deadline;
-  int exhaustive;
+  int exhaustive, pin;
 } simba_job;
 
 #define ORC_CHUNK 16384ull
 
 static void *simba_worker(void *arg) {
   simba_job *J = (simba_job *)arg;
+  if (J->pin) pin_self(atomic_fetch_add(&J->next_worker, 1));
   uint8_t rpn[ORC_MAX_SIZE];
   uint64_t done = 0;
   for (;;) {
@@ -343,10 +365,12 @@ int orc_simba_search(int n_vars, int n_consts, const uint64_t *consts, int W,
     atomic_init(&J.best, UINT64_MAX);
     atomic_init(&J.evaluated, 0);
     atomic_init(&J.stop, 0);
+    atomic_init(&J.next_worker, 0);
     J.deadline = budget_s > 0 ? t0 + budget_s : 0;
     J.exhaustive = exhaustive;
     pthread_t th[256];
     int nt = nthreads > 256 ? 256 : nthreads;
+    J.pin = nt > 1;
     for (int k = 0; k < nt; ++k) pthread_create(&th[k], NULL, simba_worker, &J);
     for (int k = 0; k < nt; ++k) pthread_join(th[k], NULL);
     evaluated += atomic_load(&J.evaluated);
@@ -760,5 +784,384 @@ int orc_vfb_search(int n_vars, int n_consts, const uint64_t *consts, int W,
   free(S.keys);
   free(tgt);
   free(cs);
+  return status;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multithreaded VFB (the CPU baseline's form of orc_vfb_search, intended    */
+/* dedup): the same enumeration order, dedup and solution, computed level by */
+/* level with every level swept fully.  Per level, three passes over the     */
+/* candidate index space, split into chunks pulled from an atomic counter:   */
+/*   1. signature -> fingerprint -> a shared open-addressing set (CAS on the */
+/*      key; the slot's companion takes the atomic minimum of                */
+/*      level << 48 | index, so keys of earlier levels win and the first     */
+/*      occurrence in enumeration order is the representative);             */
+/*   2. keep = companion == own tag; per-chunk counts (prefix on one thread);*/
+/*   3. kept rows written in enumeration order.                              */
+/* The solution is the first candidate equal to the target when its         */
+/* fingerprint's companion holds that candidate (full-cache.cu:293-321).     */
+/* ------------------------------------------------------------------------ */
+#define MT_CHUNK 65536ull
+
+typedef struct {
+  uint64_t begin, count; /* candidate index range of the block */
+  uint64_t l0, nl, r0, nr;
+  int op, unary;
+} mt_block;
+
+typedef struct {
+  int n, W, nblocks, nthreads;
+  const uint64_t *vals; /* rows [row][n] */
+  const uint64_t *tgt;
+  const mt_block *blk;
+  uint64_t total, tag, mask;
+  uint64_t *keys, *comp; /* the set: key and companion per slot */
+  uint32_t *slot_of;     /* per candidate, ~0 = invalid key */
+  uint8_t *keep;
+  uint64_t *chunk_cnt;   /* per chunk: kept count, then base */
+  uint64_t *out_vals;    /* rows of this level */
+  int32_t *out_l, *out_r;
+  uint8_t *out_op;
+  _Atomic uint64_t next;
+  _Atomic uint64_t found;
+  _Atomic int next_worker;
+  int pass;
+} mt_level;
+
+static void mt_decode(const mt_level *M, uint64_t idx, int *op, uint64_t *a, uint64_t *b,
+                      int *unary) {
+  int lo = 0, hi = M->nblocks - 1;
+  while (lo < hi) { /* last block whose begin <= idx */
+    int mid = (lo + hi + 1) / 2;
+    if (M->blk[mid].begin <= idx) lo = mid; else hi = mid - 1;
+  }
+  const mt_block *B = &M->blk[lo];
+  uint64_t local = idx - B->begin;
+  *op = B->op;
+  *unary = B->unary;
+  if (B->unary) {
+    *a = B->l0 + local;
+    *b = *a;
+    return;
+  }
+  int sw = 0;
+  if (B->op == ORC_SUB) {
+    sw = (int)(local & 1);
+    local >>= 1;
+  }
+  uint64_t l = B->l0 + local / B->nr, r = B->r0 + local % B->nr;
+  *a = sw ? r : l;
+  *b = sw ? l : r;
+}
+
+static void mt_sig(const mt_level *M, uint64_t idx, uint64_t *cs, int32_t *l, int32_t *r,
+                   int *op_out) {
+  int op, unary;
+  uint64_t a, b;
+  mt_decode(M, idx, &op, &a, &b, &unary);
+  const uint64_t *A = M->vals + a * M->n, *Bv = M->vals + b * M->n;
+  for (int i = 0; i < M->n; ++i) cs[i] = orc_apply(op, A[i], unary ? 0 : Bv[i], M->W);
+  *l = (int32_t)a;
+  *r = unary ? -1 : (int32_t)b;
+  *op_out = op;
+}
+
+static uint64_t mt_insert(mt_level *M, uint64_t fp) {
+  uint64_t s = orc_fmix64(fp) & M->mask;
+  for (;;) {
+    uint64_t k = __atomic_load_n(&M->keys[s], __ATOMIC_RELAXED);
+    if (k == fp) return s;
+    if (k == ~0ull) {
+      uint64_t expect = ~0ull;
+      if (__atomic_compare_exchange_n(&M->keys[s], &expect, fp, 0, __ATOMIC_RELAXED,
+                                      __ATOMIC_RELAXED) || expect == fp)
+        return s;
+    }
+    s = (s + 1) & M->mask;
+  }
+}
+
+static uint64_t mt_find(const mt_level *M, uint64_t fp) {
+  uint64_t s = orc_fmix64(fp) & M->mask;
+  for (;;) {
+    uint64_t k = M->keys[s];
+    if (k == fp) return s;
+    if (k == ~0ull) return ~0ull;
+    s = (s + 1) & M->mask;
+  }
+}
+
+static void *mt_worker(void *arg) {
+  mt_level *M = (mt_level *)arg;
+  pin_self(atomic_fetch_add(&M->next_worker, 1));
+  uint64_t cs[ORC_MAX_SAMPLES];
+  const uint64_t nchunks = (M->total + MT_CHUNK - 1) / MT_CHUNK;
+  for (;;) {
+    uint64_t c = atomic_fetch_add(&M->next, 1);
+    if (c >= nchunks) break;
+    uint64_t i0 = c * MT_CHUNK, i1 = i0 + MT_CHUNK < M->total ? i0 + MT_CHUNK : M->total;
+    if (M->pass == 1) {
+      for (uint64_t idx = i0; idx < i1; ++idx) {
+        int32_t l, r;
+        int op;
+        mt_sig(M, idx, cs, &l, &r, &op);
+        uint64_t fp = orc_fingerprint(cs, M->n, M->W);
+        if (fp >= ~1ull) { /* invalid key: dropped (hash_set.cuh:200-204) */
+          M->slot_of[idx] = ~0u;
+          continue;
+        }
+        uint64_t s = mt_insert(M, fp);
+        M->slot_of[idx] = (uint32_t)s;
+        uint64_t want = M->tag | idx, cur = __atomic_load_n(&M->comp[s], __ATOMIC_RELAXED);
+        while (want < cur && !__atomic_compare_exchange_n(&M->comp[s], &cur, want, 1,
+                                                          __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+        }
+        if (!memcmp(cs, M->tgt, sizeof(uint64_t) * M->n)) {
+          uint64_t f = atomic_load(&M->found);
+          while (idx < f && !atomic_compare_exchange_weak(&M->found, &f, idx)) {
+          }
+        }
+      }
+    } else if (M->pass == 2) {
+      uint64_t cnt = 0;
+      for (uint64_t idx = i0; idx < i1; ++idx) {
+        uint32_t s = M->slot_of[idx];
+        uint8_t k = s != ~0u && M->comp[s] == (M->tag | idx);
+        M->keep[idx] = k;
+        cnt += k;
+      }
+      M->chunk_cnt[c] = cnt;
+    } else {
+      uint64_t row = M->chunk_cnt[c];
+      for (uint64_t idx = i0; idx < i1; ++idx) {
+        if (!M->keep[idx]) continue;
+        int32_t l, r;
+        int op;
+        mt_sig(M, idx, M->out_vals + row * M->n, &l, &r, &op);
+        M->out_l[row] = l;
+        M->out_r[row] = r;
+        M->out_op[row] = (uint8_t)op;
+        ++row;
+      }
+    }
+  }
+  return NULL;
+}
+
+static void mt_run(mt_level *M, int pass) {
+  M->pass = pass;
+  atomic_store(&M->next, 0);
+  atomic_store(&M->next_worker, 0);
+  pthread_t th[256];
+  int nt = M->nthreads > 256 ? 256 : M->nthreads;
+  for (int k = 0; k < nt; ++k) pthread_create(&th[k], NULL, mt_worker, M);
+  for (int k = 0; k < nt; ++k) pthread_join(th[k], NULL);
+}
+
+/* Set of capacity >= want slots (power of two) holding the keys (and their
+ * companions) of the old set. */
+static int mt_regrow(uint64_t **keys, uint64_t **comp, uint64_t *cap, uint64_t want) {
+  uint64_t c = *cap ? *cap : 1024;
+  while (c < want) c <<= 1;
+  if (c == *cap) return 0;
+  uint64_t *nk = (uint64_t *)malloc(sizeof(uint64_t) * c);
+  uint64_t *nc = (uint64_t *)malloc(sizeof(uint64_t) * c);
+  if (!nk || !nc) { free(nk); free(nc); return -1; }
+  memset(nk, 0xff, sizeof(uint64_t) * c);
+  memset(nc, 0xff, sizeof(uint64_t) * c);
+  for (uint64_t i = 0; i < *cap; ++i) {
+    uint64_t k = (*keys)[i];
+    if (k == ~0ull) continue;
+    uint64_t s = orc_fmix64(k) & (c - 1);
+    while (nk[s] != ~0ull) s = (s + 1) & (c - 1);
+    nk[s] = k;
+    nc[s] = (*comp)[i];
+  }
+  free(*keys);
+  free(*comp);
+  *keys = nk;
+  *comp = nc;
+  *cap = c;
+  return 0;
+}
+
+int orc_vfb_search_mt(int n_vars, int n_consts, const uint64_t *consts, int W,
+                      int n_samples, const uint64_t *inputs, const uint64_t *outputs,
+                      int max_len, int exhaustive, int nthreads, orc_result *res,
+                      uint64_t *level_new, uint64_t *level_cand) {
+  memset(res, 0, sizeof *res);
+  int L = n_vars + n_consts, n = n_samples;
+  if (L < 1 || L > ORC_MAX_LEAVES || n < 1 || n > ORC_MAX_SAMPLES || max_len < 1 ||
+      max_len > 40)
+    return -2;
+  if (nthreads < 1) nthreads = 1;
+  double t0 = now_s();
+  const uint64_t mask = wmask(W);
+  uint64_t *tgt = (uint64_t *)malloc(sizeof(uint64_t) * n);
+  for (int i = 0; i < n; ++i) tgt[i] = outputs[i] & mask;
+  if (level_new) memset(level_new, 0, sizeof(uint64_t) * (max_len + 1));
+  if (level_cand) memset(level_cand, 0, sizeof(uint64_t) * (max_len + 1));
+  vcache C = {0};
+  C.n = n;
+  C.W = W;
+  uint64_t cs[ORC_MAX_SAMPLES];
+  int64_t found_row = -1;
+  int found_level = 0;
+  uint64_t found_index = 0, evaluated = 0;
+  for (int v = 0; v < L; ++v) { /* level 1 (full-cache.cu:694-707) */
+    for (int i = 0; i < n; ++i)
+      cs[i] = (v < n_vars ? inputs[i * n_vars + v] : consts[v - n_vars]) & mask;
+    vc_push(&C, cs, -1, -1, 255);
+    evaluated++;
+    if (found_row < 0 && !memcmp(cs, tgt, sizeof(uint64_t) * n)) {
+      found_row = v;
+      found_level = 1;
+      found_index = (uint64_t)v;
+    }
+  }
+  if (level_new) level_new[1] = L;
+  if (level_cand) level_cand[1] = L;
+  uint64_t *keys = NULL, *comp = NULL, cap = 0, nkeys = 0;
+  int status = ORC_NOT_FOUND, err = 0;
+  if (mt_regrow(&keys, &comp, &cap, 1 << 16)) err = 1;
+  for (int v = 0; v < L && !err; ++v) { /* hashSetInit (full-cache.cu:142-157) */
+    uint64_t fp = orc_fingerprint(C.vals + (size_t)v * n, n, W);
+    if (fp >= ~1ull) continue;
+    uint64_t s = orc_fmix64(fp) & (cap - 1);
+    while (keys[s] != ~0ull && keys[s] != fp) s = (s + 1) & (cap - 1);
+    if (keys[s] == ~0ull) nkeys++;
+    keys[s] = fp;
+    comp[s] = 0;
+  }
+  uint64_t lvl_start[42];
+  lvl_start[1] = 0;
+  lvl_start[2] = C.rows;
+  mt_block blk[8 * 20 + 8];
+  for (int lev = 2; lev <= max_len && (found_row < 0 || exhaustive) && !err; ++lev) {
+    int nb = 0;
+    uint64_t total = 0;
+    for (int op = 0; op < 8; ++op) {
+      int unary = op == ORC_NOT || op == ORC_NEG;
+      int nsplit = unary ? 1 : (lev - 1) / 2;
+      for (int sp = 1; sp <= nsplit; ++sp) {
+        mt_block b;
+        memset(&b, 0, sizeof b);
+        b.op = op;
+        b.unary = unary;
+        if (unary) {
+          b.l0 = lvl_start[lev - 1];
+          b.nl = lvl_start[lev] - b.l0;
+          b.nr = 1;
+        } else {
+          b.l0 = lvl_start[sp];
+          b.nl = lvl_start[sp + 1] - b.l0;
+          b.r0 = lvl_start[lev - 1 - sp];
+          b.nr = lvl_start[lev - sp] - b.r0;
+        }
+        b.count = b.nl * b.nr * (op == ORC_SUB ? 2 : 1);
+        if (!b.count) continue;
+        b.begin = total;
+        total += b.count;
+        blk[nb++] = b;
+      }
+    }
+    if (level_cand) level_cand[lev] = total;
+    evaluated += total;
+    if (total == 0) {
+      lvl_start[lev + 1] = C.rows;
+      continue;
+    }
+    /* room for every key this level can add, at most half full */
+    if (mt_regrow(&keys, &comp, &cap, 2 * (nkeys + total))) { err = 1; break; }
+    mt_level M;
+    memset(&M, 0, sizeof M);
+    M.n = n; M.W = W; M.nblocks = nb; M.nthreads = nthreads;
+    M.vals = C.vals; M.tgt = tgt; M.blk = blk; M.total = total;
+    M.tag = (uint64_t)lev << 48;
+    M.keys = keys; M.comp = comp; M.mask = cap - 1;
+    const uint64_t nchunks = (total + MT_CHUNK - 1) / MT_CHUNK;
+    M.slot_of = (uint32_t *)malloc(sizeof(uint32_t) * total);
+    M.keep = (uint8_t *)malloc(total);
+    M.chunk_cnt = (uint64_t *)malloc(sizeof(uint64_t) * nchunks);
+    if (!M.slot_of || !M.keep || !M.chunk_cnt || cap > 0xffffffffull) {
+      free(M.slot_of); free(M.keep); free(M.chunk_cnt);
+      err = 1;
+      break;
+    }
+    atomic_init(&M.found, UINT64_MAX);
+    mt_run(&M, 1);
+    mt_run(&M, 2);
+    uint64_t kept = 0;
+    for (uint64_t c = 0; c < nchunks; ++c) {
+      uint64_t k = M.chunk_cnt[c];
+      M.chunk_cnt[c] = C.rows + kept;
+      kept += k;
+    }
+    /* rows of this level appended in order */
+    uint64_t need = C.rows + kept;
+    if (need > C.cap) {
+      uint64_t nc = C.cap ? C.cap : 1024;
+      while (nc < need) nc *= 2;
+      C.vals = (uint64_t *)realloc(C.vals, sizeof(uint64_t) * n * nc);
+      C.left = (int32_t *)realloc(C.left, sizeof(int32_t) * nc);
+      C.right = (int32_t *)realloc(C.right, sizeof(int32_t) * nc);
+      C.op = (uint8_t *)realloc(C.op, nc);
+      C.cap = nc;
+    }
+    M.vals = C.vals;
+    M.out_vals = C.vals;
+    M.out_l = C.left;
+    M.out_r = C.right;
+    M.out_op = C.op;
+    mt_run(&M, 3);
+    uint64_t f = atomic_load(&M.found);
+    if (found_row < 0 && f != UINT64_MAX) {
+      uint64_t tfp = orc_fingerprint(tgt, n, W);
+      uint64_t s = tfp >= ~1ull ? ~0ull : mt_find(&M, tfp);
+      if (s != ~0ull && comp[s] == (M.tag | f)) {
+        uint64_t c = f / MT_CHUNK, row = M.chunk_cnt[c];
+        for (uint64_t i = c * MT_CHUNK; i < f; ++i) row += M.keep[i];
+        found_row = (int64_t)row;
+        found_level = lev;
+        found_index = f;
+      }
+    }
+    nkeys += kept;
+    C.rows += kept;
+    if (level_new) level_new[lev] = kept;
+    lvl_start[lev + 1] = C.rows;
+    free(M.slot_of);
+    free(M.keep);
+    free(M.chunk_cnt);
+  }
+  if (err) {
+    status = ORC_OVERFLOW;
+    strcpy(res->expr, "Not found !");
+  } else if (found_row >= 0) {
+    status = ORC_FOUND;
+    res->size = found_level;
+    res->rank = found_index;
+    int len = vfb_rpn(&C, L, found_row, res->rpn, 0, ORC_MAX_RPN);
+    if (len < 0) {
+      strcpy(res->expr, "Size of the output is too big !");
+      res->op = -1;
+    } else {
+      res->op = found_level > 1 ? res->rpn[len - 1] - ORC_TOK_OP : -1;
+      orc_infix(res->rpn, len, n_vars, consts, W, 1, res->expr, sizeof res->expr);
+    }
+  } else {
+    strcpy(res->expr, "Not found !");
+  }
+  res->status = status;
+  res->candidates = evaluated;
+  res->rows = C.rows;
+  res->seconds = now_s() - t0;
+  free(C.vals);
+  free(C.left);
+  free(C.right);
+  free(C.op);
+  free(keys);
+  free(comp);
+  free(tgt);
   return status;
 }

@@ -14,6 +14,7 @@ extern "C" {
 #define ORC_MAX_SIZE 32
 #define ORC_MAX_LEAVES 16
 #define ORC_MAX_RPN 128
+#define ORC_MAX_SAMPLES 128
 #define ORC_TOK_OP 32
 
 enum { ORC_NOT = 0, ORC_AND, ORC_OR, ORC_XOR, ORC_NEG, ORC_ADD, ORC_SUB, ORC_MUL };
@@ -58,6 +59,13 @@ int orc_vfb_search(int n_vars, int n_consts, const uint64_t *consts, int W,
                    int n_samples, const uint64_t *inputs, const uint64_t *outputs,
                    int max_len, int dedup_mode, int exhaustive, uint64_t max_rows,
                    orc_result *res, uint64_t *level_new, uint64_t *level_cand);
+/* Multithreaded VFB with the intended dedup (the CPU baseline): the same
+ * solution, per-level counts and candidates as orc_vfb_search, every level
+ * swept fully; nthreads workers pinned to the process's CPUs. */
+int orc_vfb_search_mt(int n_vars, int n_consts, const uint64_t *consts, int W,
+                      int n_samples, const uint64_t *inputs, const uint64_t *outputs,
+                      int max_len, int exhaustive, int nthreads, orc_result *res,
+                      uint64_t *level_new, uint64_t *level_cand);
 
 #ifdef __cplusplus
 }

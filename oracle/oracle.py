@@ -70,6 +70,9 @@ def lib():
                                      ctypes.c_int, u64p, u64p, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_uint64,
                                      ctypes.POINTER(OrcResult), u64p, u64p]
+        L.orc_vfb_search_mt.argtypes = [ctypes.c_int, ctypes.c_int, u64p, ctypes.c_int,
+                                        ctypes.c_int, u64p, u64p, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.POINTER(OrcResult), u64p, u64p]
         _lib = L
     return _lib
 
@@ -176,6 +179,29 @@ def simba_rtid_search(inputs, outputs, n_vars, consts=(), W=64, min_size=1, max_
 def fingerprint(values, W):
     a, p = _u64(values)
     return int(lib().orc_fingerprint(p, len(a), W))
+
+
+def vfb_search_mt(inputs, outputs, n_vars, consts=(), W=32, max_len=8, threads=1,
+                  exhaustive=True):
+    """Multithreaded VFB (intended dedup; every level swept fully): the CPU
+    baseline.  Same solution and per-level counts as vfb_search."""
+    inputs = np.asarray(inputs, dtype=np.uint64).reshape(-1)
+    outputs = np.asarray(outputs, dtype=np.uint64)
+    n = outputs.shape[0]
+    ia, ip = _u64(inputs)
+    oa, op = _u64(outputs)
+    ca, cp = _u64(list(consts) if len(consts) else [0])
+    new = np.zeros(max_len + 1, dtype=np.uint64)
+    cand = np.zeros(max_len + 1, dtype=np.uint64)
+    r = OrcResult()
+    lib().orc_vfb_search_mt(n_vars, len(consts), cp, W, n, ip, op, max_len, int(exhaustive),
+                            threads, ctypes.byref(r),
+                            new.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                            cand.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+    out = _result(r)
+    out["level_new"] = [int(x) for x in new]
+    out["level_cand"] = [int(x) for x in cand]
+    return out
 
 
 def vfb_search(inputs, outputs, n_vars, consts=(), W=32, max_len=8, dedup="intended",

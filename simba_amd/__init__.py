@@ -54,7 +54,8 @@ class Result(ctypes.Structure):
                 ("rpn", ctypes.c_char * (MAX_SIZE + 1)), ("seconds", ctypes.c_double),
                 ("kernel_seconds", ctypes.c_double), ("candidates", ctypes.c_uint64),
                 ("size_reached", ctypes.c_int32), ("kernel_launches", ctypes.c_uint32),
-                ("hit_replays", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("hit_replays", ctypes.c_int32), ("ordered_sizes", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2)]
 
     def as_dict(self):
         return {"status": self.status, "size": self.size, "rank": int(self.rank),
@@ -63,7 +64,7 @@ class Result(ctypes.Structure):
                 "kernel_seconds": self.kernel_seconds,
                 "kernel_launches": self.kernel_launches,
                 "candidates": int(self.candidates), "size_reached": self.size_reached,
-                "hit_replays": self.hit_replays}
+                "hit_replays": self.hit_replays, "ordered_sizes": self.ordered_sizes}
 
 
 def _result_array(n):
@@ -97,7 +98,9 @@ class VfbStats(ctypes.Structure):
                 ("level_cand", ctypes.c_uint64 * (MAX_SIZE + 1)),
                 ("level_seconds", ctypes.c_double * (MAX_SIZE + 1)),
                 ("level_bytes", ctypes.c_uint64 * (MAX_SIZE + 1)),
-                ("table_slots", ctypes.c_uint64), ("rows_capacity", ctypes.c_uint64)]
+                ("table_slots", ctypes.c_uint64), ("rows_capacity", ctypes.c_uint64),
+                ("level_cas", ctypes.c_uint64 * (MAX_SIZE + 1)),
+                ("level_atomics", ctypes.c_uint64 * (MAX_SIZE + 1))]
 
 
 # Every symbol the C ABI exports (include/mba_api.h).
@@ -239,6 +242,8 @@ class Context:
         d["level_seconds"] = list(st.level_seconds)
         d["level_bytes"] = list(st.level_bytes)
         d["table_slots"] = int(st.table_slots)
+        d["level_cas"] = [int(x) for x in st.level_cas]
+        d["level_atomics"] = [int(x) for x in st.level_atomics]
         return d
 
     def vfb_fingerprints(self, values, width):

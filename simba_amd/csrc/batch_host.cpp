@@ -237,6 +237,9 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   MBA_HIP(hipMemsetAsync(d_found, 0xff, nslots * 8, stq));
   MBA_HIP(hipMemsetAsync(d_flags, 0, nslots * 4, stq));
   const int mult = o.wg_per_cu > 0 ? o.wg_per_cu : 8;  // workgroups per CU
+  // learned order from sizes of this many trees (opts.block_log2 lowers it)
+  const uint64_t learned_min =
+      o.block_log2 > 0 ? (1ull << std::min(o.block_log2, 63)) : kLearnedBlockMin;
   const int grid = ctx.cu_count * mult;
 
   // the step graph names these buffers and the launch shape: it is captured
@@ -305,10 +308,11 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
         }
         // small sizes are swept fused in one round (a round per block
         // group costs a graph replay and a sync)
-        if (st.T->t[s][8] < kLearnedBlockMin) {
+        if (st.T->t[s][8] < learned_min) {
           order[i].clear();
           continue;
         }
+        results[i].ordered_sizes++;
         std::vector<BlockRef> blocks = binary_blocks(*st.T, s);
         const float *sc = scores + i * 256;
         std::stable_sort(blocks.begin(), blocks.end(), [&](const BlockRef &x, const BlockRef &y) {
@@ -387,7 +391,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
           J.kind = 0;
           push(J);
         }
-        if (s >= 3 && round == 0 && (!learned || T.t[s][8] < kLearnedBlockMin)) {
+        if (s >= 3 && round == 0 && (!learned || T.t[s][8] < learned_min)) {
           for (int j = 1; j <= (s - 1) / 2; ++j) pair_job(j, kAllInst);
         }
         if (learned) {

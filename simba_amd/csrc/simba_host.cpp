@@ -873,6 +873,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
         });
       }
       if (block_scores) {
+        ++res.ordered_sizes;
         // learned order: the splits (pairs {j, s-1-j}) in the order of their
         // best-scored block, in rounds of one split, one split and the rest;
         // each split is swept once with all operator instances fused (its B

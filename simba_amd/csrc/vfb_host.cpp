@@ -145,6 +145,8 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
   a.probes = d_probes;
   unsigned long long *d_new = (unsigned long long *)(d_misc + 5);
   a.new_keys = d_new;
+  unsigned long long *d_atomics = (unsigned long long *)(d_misc + 6);  // [6], [7]
+  a.atomics = d_atomics;
   MBA_HIP(vfb_launch_seed(W, a, L, st));
 
   std::vector<uint64_t> lstart(kMaxSize + 3, 0);
@@ -258,7 +260,7 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
     const bool last = lev == max_len;
     a.last = last ? 1 : 0;
     MBA_HIP(hipMemsetAsync(d_found, 0xff, 8, st));
-    MBA_HIP(hipMemsetAsync(d_new, 0, 8, st));
+    MBA_HIP(hipMemsetAsync(d_new, 0, 24, st));  // new keys, CAS attempts, atomicMin
     MBA_HIP(hipEventRecord(ctx.ev0, st));
     {
       const uint64_t pgroups = (npairs + kVfbThreads - 1) / kVfbThreads;
@@ -267,8 +269,8 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
                                       (int)pblocks.size(), npairs, gp, st));
     }
     MBA_HIP(vfb_launch_lookup_target(W, a, d_tmin, st));
-    uint64_t cells[6];
-    MBA_HIP(hipMemcpyAsync(cells, d_misc, 48, hipMemcpyDeviceToHost, st));
+    uint64_t cells[8];
+    MBA_HIP(hipMemcpyAsync(cells, d_misc, 64, hipMemcpyDeviceToHost, st));
     MBA_HIP(hipStreamSynchronize(st));
     if ((uint32_t)cells[2]) {
       status = MBA_OVERFLOW;
@@ -276,6 +278,8 @@ int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result 
     }
     cand_total += total;
     S.level_cand[lev] = total;
+    S.level_cas[lev] = cells[6];
+    S.level_atomics[lev] = cells[6] + cells[7];
     const uint64_t found = cells[0], tmin = cells[1];
     if (sol_idx == -1 && found != ~0ull && tmin == (a.tag | found)) {
       sol_idx = (int64_t)found;

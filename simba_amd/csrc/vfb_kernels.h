@@ -51,6 +51,7 @@ struct VfbLevelArgs {
   uint8_t *row_op;        // root op per row
   unsigned long long *probes;  // total probe steps (statistics)
   unsigned long long *new_keys;  // fingerprints this pass inserted (new keys)
+  unsigned long long *atomics;   // [0] key CAS attempts, [1] companion atomicMin
   int last;               // hash-only level: no slot_of, companions of the
                           // target's fingerprint only
   int pad;

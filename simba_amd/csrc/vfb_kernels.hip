@@ -116,7 +116,8 @@ __device__ __forceinline__ uint64_t signature_fp(const word *rows, uint64_t cap,
 // `inserted` is set when this call filled an empty slot (the key is new).
 __device__ __forceinline__ uint64_t table_insert(uint64_t *keys, uint64_t mask,
                                                  uint64_t key, uint64_t &probes,
-                                                 uint64_t limit, bool &inserted) {
+                                                 uint64_t limit, bool &inserted,
+                                                 uint64_t &cas) {
   uint64_t s = (key ^ (key >> 29)) & mask;
   for (uint64_t step = 0;; ++step) {
     if (step >= limit) return ~0ull;
@@ -124,6 +125,7 @@ __device__ __forceinline__ uint64_t table_insert(uint64_t *keys, uint64_t mask,
     uint64_t k = __atomic_load_n(&keys[2 * s], __ATOMIC_RELAXED);
     if (k == key) return s;
     if (k == kVfbEmpty) {
+      ++cas;
       const uint64_t old = atomicCAS((unsigned long long *)&keys[2 * s],
                                      (unsigned long long)kVfbEmpty,
                                      (unsigned long long)key);
@@ -157,24 +159,29 @@ __device__ __forceinline__ uint64_t table_find(const uint64_t *keys, uint64_t ma
 // The last level is hash-only: no rows are stored, so only the target's
 // fingerprint needs its first occurrence (the companion) and the level's new
 // fingerprints are counted as the empty slots this pass fills.
+struct InsertStats {
+  uint64_t probes = 0, fresh = 0, cas = 0, amin = 0;
+};
+
 template <typename word>
 __device__ __forceinline__ void insert_one(const VfbLevelArgs &a, uint64_t idx, uint64_t fp,
-                                           bool eq, uint64_t tfp, uint64_t &probes,
-                                           uint64_t &fresh) {
+                                           bool eq, uint64_t tfp, InsertStats &is) {
   if (fp >= kVfbTomb) {  // invalid key: dropped (hash_set.cuh:200-204)
     if (!a.last) a.slot_of[idx] = ~0u;
     return;
   }
   bool ins = false;
-  const uint64_t s = table_insert(a.keys, a.mask, fp, probes, a.max_probes, ins);
+  const uint64_t s = table_insert(a.keys, a.mask, fp, is.probes, a.max_probes, ins, is.cas);
   if (s == ~0ull) {
     atomicOr(a.fail, 1u);
     if (!a.last) a.slot_of[idx] = ~0u;
     return;
   }
-  fresh += ins ? 1u : 0u;
-  if ((!a.last || fp == tfp) && a.minidx[2 * s] > (a.tag | idx))
+  is.fresh += ins ? 1u : 0u;
+  if ((!a.last || fp == tfp) && a.minidx[2 * s] > (a.tag | idx)) {
+    ++is.amin;
     atomicMin((unsigned long long *)&a.minidx[2 * s], (unsigned long long)(a.tag | idx));
+  }
   if (!a.last) a.slot_of[idx] = (uint32_t)s;
   if (eq) atomicMin(a.found, (unsigned long long)idx);
 }
@@ -196,7 +203,7 @@ k_vfb_insert_pairs(VfbLevelArgs a, const VfbPairBlock *pblocks, int npb, uint64_
   __syncthreads();
   const uint64_t tfp = s_tfp;
   const word *rows = (const word *)a.rows;
-  uint64_t probes = 0, fresh = 0;
+  InsertStats is;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
     int b = 0;
@@ -215,8 +222,8 @@ k_vfb_insert_pairs(VfbLevelArgs a, const VfbPairBlock *pblocks, int npb, uint64_
         sn = fp_step<word>(sn, vn, i);
         sg = fp_step<word>(sg, vg, i);
       }
-      insert_one<word>(a, B.opbegin[NOT] + local, sn, en, tfp, probes, fresh);
-      insert_one<word>(a, B.opbegin[NEG] + local, sg, eg, tfp, probes, fresh);
+      insert_one<word>(a, B.opbegin[NOT] + local, sn, en, tfp, is);
+      insert_one<word>(a, B.opbegin[NEG] + local, sg, eg, tfp, is);
       continue;
     }
     const uint64_t q = local / B.nr;
@@ -235,16 +242,20 @@ k_vfb_insert_pairs(VfbLevelArgs a, const VfbPairBlock *pblocks, int npb, uint64_
         sd[k] = fp_step<word>(sd[k], v[k], i);
       }
     }
-    insert_one<word>(a, B.opbegin[AND] + local, sd[0], eq[0], tfp, probes, fresh);
-    insert_one<word>(a, B.opbegin[OR] + local, sd[1], eq[1], tfp, probes, fresh);
-    insert_one<word>(a, B.opbegin[XOR] + local, sd[2], eq[2], tfp, probes, fresh);
-    insert_one<word>(a, B.opbegin[ADD] + local, sd[3], eq[3], tfp, probes, fresh);
-    insert_one<word>(a, B.opbegin[MUL] + local, sd[4], eq[4], tfp, probes, fresh);
-    insert_one<word>(a, B.opbegin[SUB] + 2 * local, sd[5], eq[5], tfp, probes, fresh);
-    insert_one<word>(a, B.opbegin[SUB] + 2 * local + 1, sd[6], eq[6], tfp, probes, fresh);
+    insert_one<word>(a, B.opbegin[AND] + local, sd[0], eq[0], tfp, is);
+    insert_one<word>(a, B.opbegin[OR] + local, sd[1], eq[1], tfp, is);
+    insert_one<word>(a, B.opbegin[XOR] + local, sd[2], eq[2], tfp, is);
+    insert_one<word>(a, B.opbegin[ADD] + local, sd[3], eq[3], tfp, is);
+    insert_one<word>(a, B.opbegin[MUL] + local, sd[4], eq[4], tfp, is);
+    insert_one<word>(a, B.opbegin[SUB] + 2 * local, sd[5], eq[5], tfp, is);
+    insert_one<word>(a, B.opbegin[SUB] + 2 * local + 1, sd[6], eq[6], tfp, is);
   }
-  if (a.probes) atomicAdd(a.probes, (unsigned long long)probes);
-  if (fresh) atomicAdd(a.new_keys, (unsigned long long)fresh);
+  if (a.probes) atomicAdd(a.probes, (unsigned long long)is.probes);
+  if (is.fresh) atomicAdd(a.new_keys, (unsigned long long)is.fresh);
+  if (a.atomics && (is.cas | is.amin)) {
+    atomicAdd(a.atomics, (unsigned long long)is.cas);
+    atomicAdd(a.atomics + 1, (unsigned long long)is.amin);
+  }
 }
 
 // keep = first occurrence in this level of a fingerprint unseen before.
@@ -350,9 +361,9 @@ __global__ void k_vfb_seed(VfbLevelArgs a, int L) {
   uint64_t seed = 0;
   for (int i = 0; i < a.n; ++i) seed = fp_step<word>(seed, rows[(uint64_t)i * a.cap + v], i);
   if (seed >= kVfbTomb) return;
-  uint64_t probes = 0;
+  uint64_t probes = 0, cas = 0;
   bool ins = false;
-  const uint64_t s = table_insert(a.keys, a.mask, seed, probes, a.max_probes, ins);
+  const uint64_t s = table_insert(a.keys, a.mask, seed, probes, a.max_probes, ins, cas);
   if (s == ~0ull) {
     atomicOr(a.fail, 1u);
     return;

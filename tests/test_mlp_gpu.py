@@ -97,6 +97,40 @@ def test_learned_solve_is_minimal_size_and_verified(ctx):
     m.close()
 
 
+def test_trained_network_drives_learned_order(ctx):
+    """A network trained on plain-solved labels orders the blocks of every
+    size of at least 2^12 trees (block_log2 lowers the threshold from 2^26),
+    per instance and batched: each solution has the plain search's size and
+    holds on every sample, and the order was engaged (ordered_sizes)."""
+    import simba_amd
+    from simba_amd import learn
+    from simba_amd.synth import make_instance
+    note("learned order engaged: labelling and training")
+    X, Y = learn.build_dataset(ctx, 8000, seed=31, max_size=8)
+    m = simba_amd.Mlp(ctx, seed=3)
+    learn.train(m, X, Y, epochs=2)
+    insts = [make_instance(7300 + s, 2 + s % 3, 32, 6 + s % 4, 64) for s in range(16)]
+    kw = dict(max_size=9, block_log2=12)
+    plain = [ctx.simba(x, max_size=9) for x in insts]
+    ordered = 0
+    for seed, (inst, p) in enumerate(zip(insts, plain)):
+        note(f"learned order engaged: instance {seed}")
+        got = m.solve(inst, **kw)
+        assert got["status"] == p["status"]
+        ordered += got["ordered_sizes"]
+        if p["status"] == 0:
+            assert got["size"] == p["size"] and verify_rpn_chars(got["rpn"], inst)
+    assert ordered > 0
+    note("learned order engaged: batched")
+    got = m.batch_solve(insts, **kw)
+    assert sum(g["ordered_sizes"] for g in got) > 0
+    for inst, g, p in zip(insts, got, plain):
+        assert g["status"] == p["status"]
+        if p["status"] == 0:
+            assert g["size"] == p["size"] and verify_rpn_chars(g["rpn"], inst)
+    m.close()
+
+
 def test_trained_network_proposes_solution_blocks_earlier(ctx):
     """What training gains: on held-out instances the trained network puts
     the block that holds the plain search's solution earlier in its order

@@ -403,3 +403,20 @@ def test_vfb_literal_mode_keeps_second_occurrences():
     b = orc.vfb_search(inp, t, 2, (), 32, 5, dedup="literal")
     assert a["expr"] == "(v0)+(v1)"
     assert b["expr"] == "(v1)+(v0)"
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_vfb_multithreaded_equals_single_threaded(seed):
+    # the CPU baseline's level-synchronous VFB gives the checker's answers
+    from simba_amd.synth import make_instance
+    W = 64 if seed % 2 else 32
+    m = (1 << W) - 1
+    consts = [(), (1,), (1, m)][seed % 3]
+    inst = make_instance(700 + seed, 2 + seed % 2, 8 + 8 * (seed % 3), 4 + seed, W, consts)
+    a = orc.vfb_search(inst.inputs, inst.outputs, inst.n_variables, list(consts), W, 7,
+                       exhaustive=True)
+    b = orc.vfb_search_mt(inst.inputs, inst.outputs, inst.n_variables, list(consts), W, 7,
+                          threads=4)
+    assert a["level_new"] == b["level_new"] and a["level_cand"] == b["level_cand"]
+    assert (a["status"], a["size"], a["rank"], a["expr"]) == \
+        (b["status"], b["size"], b["rank"], b["expr"])
