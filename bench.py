@@ -374,9 +374,13 @@ def rec_c4(args):
 
 # ---- C5: learned proposal ---------------------------------------------------
 def rec_c5(args):
-    """Learned block proposal: trained on plain-solved instances (sizes 3..13,
-    2..4 vars, 32 samples), evaluated on held-out instances (sizes 9..19)
-    against the same network untrained and the plain search."""
+    """Learned block proposal: trained on 200k instances (sizes 3..13, 2..4
+    vars, 32 samples) labelled by the enumerator (the plain search's solution
+    within the labelling budget, else the generating tree at its size),
+    evaluated on held-out instances (sizes 9..19) against the same network
+    untrained and the plain search: time to solution per instance, overall
+    and on the instances whose solution size the network ordered, the
+    held-out ranking of the solution block, and batched solves/s."""
     import numpy as np
 
     import simba_amd
@@ -384,58 +388,76 @@ def rec_c5(args):
 
     ctx = simba_amd.Context(0)
     t0 = time.perf_counter()
-    X, Y = learn.build_dataset(ctx, args.train, args.seed, max_size=args.label_size)
+    X, Y, Z = learn.build_dataset(ctx, args.train, args.seed, max_size=args.label_size)
     t_label = time.perf_counter() - t0
+    Xe, Ye, Ze = learn.build_dataset(ctx, 8000, args.seed + 2 * 10**6, max_size=args.label_size)
     mlp = simba_amd.Mlp(ctx, seed=args.seed)
     untrained = simba_amd.Mlp(ctx, seed=args.seed + 1)
     t0 = time.perf_counter()
     losses = learn.train(mlp, X, Y, epochs=args.epochs)
     t_train = time.perf_counter() - t0
+    big = Ze >= 10
+    ranking = {"held_out_rows": int(len(Ye)),
+               "top1": {"trained": round(learn.top1_accuracy(mlp, Xe, Ye), 3),
+                        "untrained": round(learn.top1_accuracy(untrained, Xe, Ye), 3)},
+               "mean_position": {"trained": round(learn.mean_label_position(mlp, Xe, Ye), 2),
+                                 "untrained": round(learn.mean_label_position(untrained, Xe, Ye),
+                                                    2)},
+               "mean_position_sizes_10_13": {
+                   "trained": round(learn.mean_label_position(mlp, Xe[big], Ye[big]), 2),
+                   "untrained": round(learn.mean_label_position(untrained, Xe[big], Ye[big]),
+                                      2)}}
     ev = learn.sample_instances(args.seed + 10**6, args.eval_count, (9, 19), (2, 4), (32, 32))
     kw = dict(max_size=args.learned_budget)
-    mlp.solve(ev[0], **kw)
-    t0 = time.perf_counter()
-    plain = [ctx.simba(x, **kw) for x in ev]
-    t_plain_all = time.perf_counter() - t0
-    lrn = [mlp.solve(x, **kw) for x in ev]
-    unt = [untrained.solve(x, **kw) for x in ev]
+    runs = {}
+    for name, fn in (("plain", lambda x: ctx.simba(x, **kw)), ("learned", lambda x: mlp.solve(x, **kw)),
+                     ("untrained", lambda x: untrained.solve(x, **kw))):
+        fn(ev[0])
+        t0 = time.perf_counter()
+        runs[name] = [fn(x) for x in ev]
+        runs[name + "_seconds"] = time.perf_counter() - t0
+    plain, lrn, unt = runs["plain"], runs["learned"], runs["untrained"]
     solved = [i for i, r in enumerate(plain) if r["status"] == 0]
-    same_size = all(lrn[i]["status"] == 0 and lrn[i]["size"] == plain[i]["size"] for i in solved)
-    verified = all(lrn[i]["status"] == 0 and _verified(lrn[i], ev[i]) for i in solved)
-    tp = np.array([plain[i]["seconds"] for i in solved])
-    tl = np.array([lrn[i]["seconds"] for i in solved])
-    tu = np.array([unt[i]["seconds"] for i in solved])
+    engaged = [i for i in solved if lrn[i]["ordered_sizes"] > 0 and
+               lrn[i]["size"] >= 10]
+    same = {n: all(runs[n][i]["status"] == 0 and runs[n][i]["size"] == plain[i]["size"]
+                   for i in solved) for n in ("learned", "untrained")}
+    verified = all(_verified(lrn[i], ev[i]) and _verified(unt[i], ev[i]) for i in solved)
+    pct = lambda name, idx, q: round(1e3 * float(np.percentile(
+        [runs[name][i]["seconds"] for i in idx], q)), 4) if idx else None
+    t_rec = {q: {n: pct(n, solved, q) for n in ("plain", "learned", "untrained")}
+             for q in (50, 95)}
+    e_rec = {q: {n: pct(n, engaged, q) for n in ("plain", "learned", "untrained")}
+             for q in (50, 95)}
     mlp.batch_solve(ev, **kw)
-    ctx.batch(ev, **kw)
-    t0 = time.perf_counter()
-    bl = mlp.batch_solve(ev, **kw)
-    t_bl = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    bu = untrained.batch_solve(ev, **kw)
-    t_bu = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    bp = ctx.batch(ev, **kw)
-    t_bp = time.perf_counter() - t0
+    batched = {}
+    for name, fn in (("learned", lambda: mlp.batch_solve(ev, **kw)),
+                     ("untrained", lambda: untrained.batch_solve(ev, **kw)),
+                     ("plain", lambda: ctx.batch(ev, **kw))):
+        t0 = time.perf_counter()
+        rb = fn()
+        dt = time.perf_counter() - t0
+        batched[name] = {"solves_per_s": round(sum(r["status"] == 0 for r in rb) / dt, 1),
+                         "solve_rate": round(sum(r["status"] == 0 for r in rb) / len(ev), 4),
+                         "same_size": all(rb[i]["status"] == 0 and rb[i]["size"] ==
+                                          plain[i]["size"] for i in solved)}
     ctx.close()
-    pct = lambda a, q: round(float(np.percentile(a, q)) * 1e3, 4) if len(a) else None
-    nsol = lambda rs: sum(r["status"] == 0 for r in rs)
-    plain_rate = len(solved) / t_plain_all if solved else 0.0
+    plain_rate = len(solved) / runs["plain_seconds"]
     return {"workload": "learned_train%d_eval%d_sizes9to19_budget%d"
                         % (args.train, args.eval_count, args.learned_budget),
-            "train_examples": int(len(Y)), "label_seconds": round(t_label, 2),
-            "train_seconds": round(t_train, 2), "final_loss": round(losses[-1], 4),
-            "solved_plain": len(solved), "same_size": same_size, "verified": verified,
-            "ordered_sizes": sum(r["ordered_sizes"] for r in lrn),
-            "p50_ms": {"plain": pct(tp, 50), "learned": pct(tl, 50), "untrained": pct(tu, 50)},
-            "p95_ms": {"plain": pct(tp, 95), "learned": pct(tl, 95), "untrained": pct(tu, 95)},
-            "batched_solves_per_s": {"learned": round(nsol(bl) / t_bl, 1),
-                                     "untrained": round(nsol(bu) / t_bu, 1),
-                                     "plain": round(nsol(bp) / t_bp, 1)},
+            "train_instances": args.train, "train_rows": int(len(Y)),
+            "rows_sizes_10_13": int((Z >= 10).sum()),
+            "label_seconds": round(t_label, 2), "train_seconds": round(t_train, 2),
+            "final_loss": round(losses[-1], 4), "ranking": ranking,
+            "solved": len(solved), "solve_rate": round(len(solved) / len(ev), 4),
+            "same_size": same, "verified": verified,
+            "p50_ms": t_rec[50], "p95_ms": t_rec[95],
+            "engaged_instances": len(engaged),
+            "engaged_p50_ms": e_rec[50], "engaged_p95_ms": e_rec[95],
+            "batched": batched,
             "plain_per_instance_solves_per_s": round(plain_rate, 1),
             "batched_learned_vs_plain_per_instance":
-                round((nsol(bl) / t_bl) / plain_rate, 2) if plain_rate else None,
-            "batched_learned_same_size": all(
-                bl[i]["status"] == 0 and bl[i]["size"] == plain[i]["size"] for i in solved)}
+                round(batched["learned"]["solves_per_s"] / plain_rate, 2) if plain_rate else None}
 
 
 # ---- single-record workloads ------------------------------------------------
@@ -541,7 +563,7 @@ def main():
     ap.add_argument("--cache-gb", type=int, default=96)
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--budget-size", type=int, default=9)
-    ap.add_argument("--learned-budget", type=int, default=11,
+    ap.add_argument("--learned-budget", type=int, default=13,
                     help="C5: largest size searched (the learned order applies to sizes "
                          "of at least 2^26 trees)")
     ap.add_argument("--train", type=int, default=200000)

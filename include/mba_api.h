@@ -187,9 +187,16 @@ int mba_vfb_solve(mba_ctx *ctx, const mba_instance *inst, const mba_opts *opts,
 int mba_vfb_fingerprints(mba_ctx *ctx, uint32_t width, const uint64_t *values,
                          uint32_t n_words, size_t count, uint64_t *out);
 
-/* Batched SIMBA: `count` instances solved by one persistent kernel with a
- * device work queue.  Each result equals mba_simba_solve's on that instance
- * within the per-instance budget `opts->time_budget_s` (0 = none). */
+/* Batched SIMBA: `count` instances solved together.  Consecutive sizes of
+ * all instances run as one captured launch sequence: per size, one
+ * persistent kernel drains a device work queue of every instance's jobs,
+ * window hits are resolved on the device, and instances settled at a
+ * smaller size skip their remaining jobs; without a budget or cancel flag a
+ * plain batch is one launch sequence and one sync.  Each result equals
+ * mba_simba_solve's on that instance.  `opts->time_budget_s` (0 = none)
+ * bounds each instance's share of the device time (a launch sequence's time
+ * split over the instances by the tiles of their jobs), checked between
+ * launch sequences: an instance past it returns MBA_BUDGET. */
 int mba_batch_solve(mba_ctx *ctx, const mba_instance *insts, size_t count,
                     const mba_opts *opts, mba_result *results);
 

@@ -114,6 +114,7 @@ EXPORTS = ["mba_ctx_create", "mba_ctx_destroy", "mba_strerror", "mba_ctx_last_er
            "mba_mlp_train_step",
            "mba_learned_solve", "mba_learned_batch_solve"]
 MLP_IN, MLP_HIDDEN, MLP_OUT = 256, 512, 128
+FEAT_HASHED = 240  # hashed sample-bit buckets; the last 16 inputs: size one-hot
 
 _lib = None
 
@@ -274,13 +275,25 @@ def _fptr(a):
 
 
 def features(inst):
-    """256 hashed input/output-bit features of an instance (float32)."""
+    """256 features of an instance (float32): 240 hashed input/output-bit
+    buckets and a size one-hot left at zero (set_size_slot)."""
     ci, keep = inst.to_c()
     out = np.zeros(MLP_IN, dtype=np.float32)
     rc = lib().mba_features(ctypes.byref(ci), _fptr(out))
     if rc != 0:
         raise MbaError(lib().mba_strerror(rc).decode())
     return out
+
+
+def set_size_slot(x, size):
+    """Sets the size one-hot of feature rows x (n x 256, in place): the size
+    whose blocks the network orders (sizes above 16 share the last slot)."""
+    x = np.asarray(x)
+    x[..., FEAT_HASHED:] = 0.0
+    sizes = np.broadcast_to(np.asarray(size), x.shape[:-1])
+    slot = FEAT_HASHED + np.clip(sizes, 1, MLP_IN - FEAT_HASHED) - 1
+    np.put_along_axis(x, slot[..., None].astype(np.int64), 1.0, axis=-1)
+    return x
 
 
 def features_batch(ctx, insts):

@@ -63,7 +63,7 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 }  // namespace
 
 int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opts &o,
-                mba_result *results, const float *scores) {
+                mba_result *results, const std::function<int(int, float *)> *score_size) {
   const auto t_start = Clock::now();
   if (count == 0) return 0;
   const int W = (int)insts[0].width;
@@ -121,9 +121,12 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   // pinned staging (kept in the context, grown when needed): step header at
   // 0, hit and slot counts at 256, then the jobs of a step, the instances'
   // table views, leaf values, outputs and sample layouts
-  size_t job_cap = 0;
-  for (size_t i = 0; i < count; ++i) job_cap += 2 + (size_t)std::max(0, (max_size - 1) / 2);
-  const size_t off_jobs = 512;
+  // jobs of every size of a search (a group of sizes runs as one launch
+  // sequence): the leaf scan, a table build per tabulated size, the pairs
+  size_t jobs_per_inst = 1;
+  for (int z = 2; z <= max_size; ++z) jobs_per_inst += 1 + (size_t)std::max(0, (z - 1) / 2);
+  const size_t job_cap = count * jobs_per_inst;
+  const size_t off_jobs = 2048;
   const size_t off_tvs = align256(off_jobs + sizeof(BatchJob) * job_cap);
   const size_t off_leaf = align256(off_tvs + sizeof(TableView) * count);
   const size_t off_out = align256(off_leaf + 8 * nleaf);
@@ -140,8 +143,10 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     ctx.b_pinned_bytes = want;
   }
   char *pin = (char *)ctx.b_pinned;
-  BatchStep *h_step = (BatchStep *)pin;
-  uint32_t *h_hc = (uint32_t *)(pin + 256);
+  // step headers of a group (one per size, plus the scratch header of a
+  // step run in parts) at 0, the step counters read back at 1024
+  BatchStep *h_steps = (BatchStep *)pin;
+  BatchCounters *h_cnt = (BatchCounters *)(pin + 1024);
   BatchJob *h_jobs = (BatchJob *)(pin + off_jobs);
   TableView *tvs = (TableView *)(pin + off_tvs);
   uint64_t *leafvals = (uint64_t *)(pin + off_leaf);
@@ -213,67 +218,48 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   MBA_HIP(launch_batch_leaves(W, ctx.b_inst.as<TableView>(), ctx.leafvals.as<uint64_t>(),
                               ctx.b_results.as<BatchInstData>(), (uint32_t)count, stq));
 
-  // ---- per-step graph: upload jobs, reset counters, drain, read hit count --
-  MBA_HIP(ctx.b_queue.ensure(sizeof(BatchJob) * job_cap + 256));
-  // hit buffer (opts.hit_cap; smaller buffers exercise the split replays)
+  // ---- device state ---------------------------------------------------------
+  // hit buffer (opts.hit_cap; smaller buffers exercise the split replays) and
+  // its screened candidates (at most two per hit)
   const uint32_t hit_cap = o.hit_cap > 0 ? (uint32_t)o.hit_cap : kBatchHitCap;
   MBA_HIP(ctx.hits.ensure(sizeof(HitRec) * hit_cap));
+  MBA_HIP(ctx.surv.ensure(sizeof(SurvRec) * 2 * (size_t)hit_cap));
   MBA_HIP(ctx.hit_count.ensure(256));
-  BatchJob *d_jobs = (BatchJob *)(ctx.b_queue.as<char>() + 256);
-  BatchStep *d_step = ctx.b_queue.as<BatchStep>();
+  BatchCounters *d_cnt = ctx.hit_count.as<BatchCounters>();
   unsigned long long *d_head = (unsigned long long *)(ctx.hit_count.as<char>() + 64);
-  uint32_t *d_hc = ctx.hit_count.as<uint32_t>();
   HitRec *d_hits = ctx.hits.as<HitRec>();
   const TableView *d_tvs = ctx.b_inst.as<TableView>();
-  // device hit resolution: per-(instance, size) minimum rank, slot flags and
-  // the list of slots a resolve launch lowered
+  MBA_HIP(ctx.b_queue.ensure(sizeof(BatchJob) * job_cap + sizeof(BatchStep) * (kMaxSize + 2)));
+  BatchStep *d_steps = ctx.b_queue.as<BatchStep>();
+  BatchJob *d_jobs =
+      (BatchJob *)(ctx.b_queue.as<char>() + sizeof(BatchStep) * (kMaxSize + 2));
+  // per-(instance, size) slots: the verified minimum rank, the slot flags and
+  // the list of lowered slots of the resolution kernels; per instance, the
+  // smallest size settled on the device
   const size_t nslots = count * (kMaxSize + 1);
-  MBA_HIP(ctx.b_misc.ensure(nslots * 16 + 256));
+  MBA_HIP(ctx.b_misc.ensure(nslots * 16 + count * 4 + 256));
   unsigned long long *d_found = (unsigned long long *)(ctx.b_misc.as<char>() + 256);
   uint32_t *d_flags = (uint32_t *)(d_found + nslots);
   uint32_t *d_touched = d_flags + nslots;
-  uint32_t *d_ntouched = ctx.b_misc.as<uint32_t>();
-  uint32_t *h_nt = h_hc + 4;
+  int32_t *d_settled = (int32_t *)(d_touched + nslots);
   MBA_HIP(hipMemsetAsync(d_found, 0xff, nslots * 8, stq));
   MBA_HIP(hipMemsetAsync(d_flags, 0, nslots * 4, stq));
+  MBA_HIP(hipMemsetAsync(d_settled, 0, count * 4, stq));
+  MBA_HIP(hipMemsetAsync(d_cnt, 0, sizeof(BatchCounters), stq));
   const int mult = o.wg_per_cu > 0 ? o.wg_per_cu : 8;  // workgroups per CU
+  const int grid = ctx.cu_count * mult;
   // learned order from sizes of this many trees (opts.block_log2 lowers it)
   const uint64_t learned_min =
       o.block_log2 > 0 ? (1ull << std::min(o.block_log2, 63)) : kLearnedBlockMin;
-  const int grid = ctx.cu_count * mult;
 
-  // the step graph names these buffers and the launch shape: it is captured
-  // again only when one of them changed since the previous call
-  const uint64_t key[11] = {(uint64_t)W, (uint64_t)grid, (uint64_t)hit_cap,
-                            (uint64_t)(uintptr_t)d_step,
-                            (uint64_t)(uintptr_t)d_jobs, (uint64_t)(uintptr_t)d_tvs,
-                            (uint64_t)(uintptr_t)d_head, (uint64_t)(uintptr_t)d_hits,
-                            (uint64_t)(uintptr_t)d_hc, (uint64_t)(uintptr_t)h_step,
-                            (uint64_t)(uintptr_t)h_hc};
-  MBA_HIP(hipStreamSynchronize(stq));
-  if (!ctx.b_gexec || std::memcmp(key, ctx.b_graph_key, sizeof key) != 0) {
-    if (ctx.b_gexec) MBA_HIP(hipGraphExecDestroy(ctx.b_gexec));
-    if (ctx.b_graph) MBA_HIP(hipGraphDestroy(ctx.b_graph));
-    ctx.b_gexec = nullptr;
-    ctx.b_graph = nullptr;
-    MBA_HIP(hipStreamBeginCapture(stq, hipStreamCaptureModeThreadLocal));
-    MBA_HIP(hipMemcpyAsync(d_step, h_step, sizeof(BatchStep), hipMemcpyHostToDevice, stq));
-    MBA_HIP(launch_batch_reset(d_head, d_hc, stq));
-    MBA_HIP(launch_batch_step(W, d_step, d_jobs, d_tvs, d_head, d_hits, d_hc, hit_cap, grid,
-                              stq));
-    MBA_HIP(hipMemcpyAsync(h_hc, d_hc, 4, hipMemcpyDeviceToHost, stq));
-    MBA_HIP(hipStreamEndCapture(stq, &ctx.b_graph));
-    MBA_HIP(hipGraphInstantiate(&ctx.b_gexec, ctx.b_graph, nullptr, nullptr, 0));
-    std::memcpy(ctx.b_graph_key, key, sizeof key);
-  }
-  hipGraphExec_t gexec = ctx.b_gexec;
-
-  std::vector<HitRec> hrec;
-  std::vector<VerifyCand> vc;
-  std::vector<uint8_t> okv;
   double ktime = 0.0;
   uint32_t klaunch = 0;
+  int replays = 0;  // steps run again in parts after a hit buffer overflow
   bool budget_hit = false;
+  std::vector<double> own(count, 0.0);  // each instance's share of device time
+  std::vector<uint64_t> inst_tiles(count, 0);
+  std::vector<int32_t> settled(count, 0);
+  std::vector<uint64_t> found_h(nslots);
   auto finish = [&](size_t i, int z) {
     InstState &st = S[i];
     fill_result(*st.T, z, st.found[z], (int)insts[i].n_vars, insts[i].consts, W, results[i]);
@@ -282,272 +268,417 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     st.active = false;
   };
   std::vector<std::vector<int>> order(count);  // learned: splits in score order
+  std::vector<float> scores(score_size ? count * 256 : 0);
   if (o.verbose)
     std::fprintf(stderr, "batch: %zu instances set up in %.3f ms\n", count,
                  1e3 * seconds_since(t_start));
-  double t_build = 0, t_gpu = 0, t_resolve = 0;
-  uint64_t n_hits = 0, n_kept = 0, n_cands = 0;
-  int replays = 0;  // steps run again in parts after a hit buffer overflow
-  std::vector<uint32_t> hits_of(o.verbose ? count : 0, 0);
-  for (int s = 0; s <= max_size; ++s) {
-    if ((o.cancel && *o.cancel) ||
-        (o.time_budget_s > 0 && seconds_since(t_start) > o.time_budget_s)) {
+
+  // The jobs of size s for instance i (learned rounds: the splits of
+  // `splits`, else every split; table build and leaf scan in the first
+  // round), appended to h_jobs[nj..] with step-relative first tiles.
+  auto size_jobs = [&](size_t i, int s, bool first_round, const std::vector<int> *splits,
+                       size_t &nj, uint64_t &tiles) {
+    InstState &st = S[i];
+    const CountTable &T = *st.T;
+    auto push = [&](BatchJob &J) {
+      J.tile0 = tiles;
+      tiles += J.ntiles;
+      inst_tiles[i] += J.ntiles;
+      h_jobs[nj++] = J;
+    };
+    auto pair_job = [&](int j) {
+      const CheckTargets tg = targets_for(st.t0, s, max_size, W);
+      BatchJob J;
+      std::memset(&J, 0, sizeof J);
+      J.p.sb = s;
+      J.p.j = j;
+      J.p.m = s - 1 - j;
+      J.p.nA = T.t[j][8];
+      J.p.nB = T.t[J.p.m][8];
+      J.p.ta = kPairTA;
+      J.p.nTA = (J.p.nA + kPairTA - 1) / kPairTA;
+      J.p.gen_a = j > st.Kt;
+      J.p.gen_b = J.p.m > st.Kt;
+      J.p.mask = kAllInst;
+      J.p.rb = kPairRB;
+      J.ntiles = J.p.nTA * ((J.p.nB + kPairTB - 1) / kPairTB);
+      J.lo_e = tg.lo_even;
+      J.lo_o = tg.lo_odd;
+      J.span = tg.span;
+      J.inst = (uint32_t)i;
+      J.kind = 1;
+      push(J);
+    };
+    if (first_round && s == 1) {  // leaves and every unary chain over them
+      BatchJob J;
+      std::memset(&J, 0, sizeof J);
+      const CheckTargets tg = targets_for(st.t0, 1, max_size, W);
+      J.ntiles = 1;
+      J.lo_e = tg.lo_even;
+      J.lo_o = tg.lo_odd;
+      J.span = tg.span;
+      J.inst = (uint32_t)i;
+      J.kind = 2;
+      push(J);
+    }
+    if (first_round && s >= 2 && s <= st.Kt) {  // value table V[s]
+      BatchJob J;
+      std::memset(&J, 0, sizeof J);
+      J.p.m = s;
+      J.p.nB = T.t[s][8];
+      J.ntiles = (J.p.nB + kBuildTile - 1) / kBuildTile;
+      J.inst = (uint32_t)i;
+      J.kind = 0;
+      push(J);
+    }
+    if (s >= 3) {
+      if (splits) {
+        for (int j : *splits) pair_job(j);
+      } else {
+        for (int j = 1; j <= (s - 1) / 2; ++j) pair_job(j);
+      }
+    }
+  };
+
+  // Launch sequence of one size step k of a group: reset the counters, drain
+  // the step's jobs, resolve its hits on the device (screen on two samples,
+  // check on every sample, per-slot minimum rank), mark settled instances.
+  auto step_launches = [&](const BatchStep *d_step) -> hipError_t {
+    hipError_t e = launch_batch_reset(d_head, d_cnt, stq);
+    if (e == hipSuccess)
+      e = launch_batch_step(W, d_step, d_jobs, d_tvs, d_head, d_hits, &d_cnt->hits, hit_cap,
+                            d_settled, grid, stq);
+    if (e == hipSuccess)
+      e = launch_hit_resolve(W, d_tvs, ctx.leafvals.as<uint64_t>(), ctx.outputs.as<uint64_t>(),
+                             ctx.b_results.as<BatchInstData>(), d_hits, &d_cnt->hits, hit_cap,
+                             1024, 2048, min_size, max_size, d_found, ctx.surv.as<SurvRec>(),
+                             &d_cnt->surv, d_flags, d_touched, &d_cnt->touched, stq);
+    if (e == hipSuccess)
+      e = launch_batch_settle(d_step, d_found, d_settled, (uint32_t)count, min_size, d_cnt,
+                              hit_cap, stq);
+    return e;
+  };
+
+  // A group of nsz size steps (headers h_steps[0..nsz), jobs already in
+  // h_jobs[0..njobs)) as one captured graph: the jobs are copied in, the
+  // graph runs every step and copies the counters back; one sync.  The graph
+  // reads the steps from device memory, so it is kept across groups and
+  // calls of the same shape.
+  auto run_group = [&](int nsz, size_t njobs) -> int {
+    MBA_HIP(hipMemcpyAsync(d_steps, h_steps, sizeof(BatchStep) * nsz, hipMemcpyHostToDevice,
+                           stq));
+    MBA_HIP(hipMemcpyAsync(d_jobs, h_jobs, sizeof(BatchJob) * njobs, hipMemcpyHostToDevice,
+                           stq));
+    // every value and buffer the captured launches name
+    const uint64_t key[Ctx::kGraphKey] = {(uint64_t)W,
+                                          (uint64_t)grid,
+                                          (uint64_t)hit_cap,
+                                          (uint64_t)nsz | (uint64_t)count << 8,
+                                          (uint64_t)min_size | (uint64_t)max_size << 8,
+                                          (uint64_t)(uintptr_t)d_steps,
+                                          (uint64_t)(uintptr_t)d_tvs,
+                                          (uint64_t)(uintptr_t)d_hits,
+                                          (uint64_t)(uintptr_t)d_cnt,
+                                          (uint64_t)(uintptr_t)d_found,
+                                          (uint64_t)(uintptr_t)ctx.surv.p,
+                                          (uint64_t)(uintptr_t)ctx.leafvals.p,
+                                          (uint64_t)(uintptr_t)ctx.outputs.p,
+                                          (uint64_t)(uintptr_t)ctx.b_results.p,
+                                          (uint64_t)(uintptr_t)h_cnt};
+    if (!ctx.b_gexec || std::memcmp(key, ctx.b_graph_key, sizeof key) != 0) {
+      MBA_HIP(hipStreamSynchronize(stq));
+      if (ctx.b_gexec) MBA_HIP(hipGraphExecDestroy(ctx.b_gexec));
+      if (ctx.b_graph) MBA_HIP(hipGraphDestroy(ctx.b_graph));
+      ctx.b_gexec = nullptr;
+      ctx.b_graph = nullptr;
+      MBA_HIP(hipStreamBeginCapture(stq, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < nsz; ++k) MBA_HIP(step_launches(d_steps + k));
+      MBA_HIP(hipMemcpyAsync(h_cnt, d_cnt, sizeof(BatchCounters), hipMemcpyDeviceToHost, stq));
+      MBA_HIP(hipStreamEndCapture(stq, &ctx.b_graph));
+      MBA_HIP(hipGraphInstantiate(&ctx.b_gexec, ctx.b_graph, nullptr, nullptr, 0));
+      std::memcpy(ctx.b_graph_key, key, sizeof key);
+    }
+    MBA_HIP(hipEventRecord(ctx.ev0, stq));
+    MBA_HIP(hipGraphLaunch(ctx.b_gexec, stq));
+    MBA_HIP(hipEventRecord(ctx.ev1, stq));
+    MBA_HIP(hipEventSynchronize(ctx.ev1));
+    float ms = 0.f;
+    MBA_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
+    ktime += ms * 1e-3;
+    ++klaunch;
+    return 0;
+  };
+
+  // One size step run by direct launches on jobs [j0, j1) of h_jobs (a
+  // step that overflowed the hit buffer inside a group): its hits are
+  // resolved before returning; an overflowing part is run again as halves
+  // of its jobs, a single job as halves of its tiles.  size_done is the
+  // settle bound (the sizes complete once every part has run).
+  std::function<int(size_t, size_t, int)> run_parts = [&](size_t j0, size_t j1,
+                                                          int size_done) -> int {
+    const uint64_t tbase = h_jobs[j0].tile0;
+    const uint64_t tend = h_jobs[j1 - 1].tile0 + h_jobs[j1 - 1].ntiles;
+    for (size_t k = j0; k < j1; ++k) h_jobs[k].tile0 -= tbase;
+    BatchStep &hs = h_steps[kMaxSize + 1];
+    hs.total_tiles = tend - tbase;
+    hs.job_off = j0;
+    hs.njobs = (uint32_t)(j1 - j0);
+    hs.size_done = size_done;
+    BatchStep *d_hs = d_steps + kMaxSize + 1;
+    MBA_HIP(hipMemcpyAsync(d_hs, &hs, sizeof hs, hipMemcpyHostToDevice, stq));
+    MBA_HIP(hipMemcpyAsync(d_jobs + j0, h_jobs + j0, sizeof(BatchJob) * (j1 - j0),
+                           hipMemcpyHostToDevice, stq));
+    MBA_HIP(hipEventRecord(ctx.ev0, stq));
+    MBA_HIP(step_launches(d_hs));
+    MBA_HIP(hipMemcpyAsync(h_cnt, d_cnt, sizeof(BatchCounters), hipMemcpyDeviceToHost, stq));
+    MBA_HIP(hipEventRecord(ctx.ev1, stq));
+    MBA_HIP(hipEventSynchronize(ctx.ev1));
+    for (size_t k = j0; k < j1; ++k) h_jobs[k].tile0 += tbase;
+    float ms = 0.f;
+    MBA_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
+    ktime += ms * 1e-3;
+    ++klaunch;
+    if (h_cnt->hits <= hit_cap) return 0;
+    ++replays;
+    if (j1 - j0 > 1) {
+      const size_t jm = j0 + (j1 - j0) / 2;
+      if (int rc = run_parts(j0, jm, size_done)) return rc;
+      return run_parts(jm, j1, size_done);
+    }
+    BatchJob &J = h_jobs[j0];
+    const BatchJob keep = J;
+    if (keep.ntiles < 2) {
+      ctx.last_error = "batch: one tile's window hits overflow the hit buffer";
+      return MBA_ERR_LIMIT;
+    }
+    const uint64_t half = keep.ntiles / 2;
+    J.ntiles = half;
+    int rc = run_parts(j0, j1, size_done);
+    if (rc == 0) {
+      J = keep;
+      J.p.tile_begin += half;
+      J.ntiles = keep.ntiles - half;
+      rc = run_parts(j0, j1, size_done);
+    }
+    J = keep;
+    return rc;
+  };
+
+  // Reads the verified slots and the settled sizes back: instances settled
+  // at a size finish there (the smallest size with a verified tree, its
+  // minimum rank; learned rounds: any verified tree of that size).
+  auto collect = [&]() -> int {
+    MBA_HIP(hipMemcpyAsync(found_h.data(), d_found, nslots * 8, hipMemcpyDeviceToHost, stq));
+    MBA_HIP(hipMemcpyAsync(settled.data(), d_settled, count * 4, hipMemcpyDeviceToHost, stq));
+    MBA_HIP(hipStreamSynchronize(stq));
+    for (size_t i = 0; i < count; ++i) {
+      InstState &st = S[i];
+      std::memcpy(st.found, found_h.data() + i * (kMaxSize + 1), 8 * (kMaxSize + 1));
+      if (st.active && settled[i] > 0) finish(i, settled[i]);
+    }
+    return 0;
+  };
+
+  // Charges each active instance its share of a group's device time (the
+  // group's time split by the tiles of its jobs) and stops the instances
+  // past the budget.
+  auto charge = [&](double seconds, uint64_t total_tiles) {
+    if (total_tiles == 0) return;
+    for (size_t i = 0; i < count; ++i) {
+      own[i] += seconds * (double)inst_tiles[i] / (double)total_tiles;
+      inst_tiles[i] = 0;
+    }
+  };
+  auto over_budget = [&](size_t i) {
+    return o.time_budget_s > 0 && own[i] > o.time_budget_s;
+  };
+
+  // sizes whose steps would use the learned order for some active instance
+  auto learned_size = [&](int s) {
+    if (!score_size || s < 3) return false;
+    for (size_t i = 0; i < count; ++i)
+      if (S[i].active && S[i].T->t[s][8] >= learned_min) return true;
+    return false;
+  };
+  // with a budget or a cancel flag, a group ends once it holds this many
+  // tiles (the host checks between groups); without, at a learned size
+  constexpr uint64_t kGroupTiles = 1ull << 16;
+  const bool checkpoints = o.time_budget_s > 0 || o.cancel != nullptr;
+
+  int s = 0;
+  while (s <= max_size) {
+    if (o.cancel && *o.cancel) {
       budget_hit = true;
       break;
     }
-    const bool learned = scores && s >= 3;
-    size_t max_blocks = 0;
-    if (learned) {
-      for (size_t i = 0; i < count; ++i) {
-        InstState &st = S[i];
-        if (!st.active) continue;
-        // a unary-rooted solution of this size is verified already
-        if (s >= min_size && st.found[s] != ~0ull) {
-          finish(i, s);
-          continue;
-        }
-        // small sizes are swept fused in one round (a round per block
-        // group costs a graph replay and a sync)
-        if (st.T->t[s][8] < learned_min) {
-          order[i].clear();
-          continue;
-        }
-        results[i].ordered_sizes++;
-        std::vector<BlockRef> blocks = binary_blocks(*st.T, s);
-        const float *sc = scores + i * 256;
-        std::stable_sort(blocks.begin(), blocks.end(), [&](const BlockRef &x, const BlockRef &y) {
-          return sc[x.op * 32 + x.split] > sc[y.op * 32 + y.split];
-        });
-        order[i].clear();
-        for (const BlockRef &b : blocks)
-          if (std::find(order[i].begin(), order[i].end(), b.pair_j) == order[i].end())
-            order[i].push_back(b.pair_j);
-        max_blocks = std::max(max_blocks, order[i].size());
-      }
-    }
-    // learned order: rounds of the split of the best block, the next split,
-    // then the rest, each split swept once with every operator instance fused
-    // (a round costs a replay and a sync; sizes without a solution need
-    // every split anyway)
-    static const size_t kRoundEnd[3] = {1, 2, ~(size_t)0};
-    for (int round = 0;; ++round) {
-      const size_t rb = round == 0 ? 0 : kRoundEnd[round - 1];
-      const size_t re = kRoundEnd[std::min(round, 2)];
-      size_t nj = 0;
-      uint64_t tiles = 0;
-      const auto t_b0 = Clock::now();
-      for (size_t i = 0; i < count; ++i) {
-        InstState &st = S[i];
-        if (!st.active) continue;
-        auto push = [&](const BatchJob &J) {
-          h_jobs[nj] = J;
-          h_jobs[nj].tile0 = tiles;
-          tiles += J.ntiles;
-          ++nj;
-        };
-        const CountTable &T = *st.T;
-        auto pair_job = [&](int j, uint32_t mask) {
-          const CheckTargets tg = targets_for(st.t0, s, max_size, W);
-          BatchJob J;
-          std::memset(&J, 0, sizeof J);
-          J.p.sb = s;
-          J.p.j = j;
-          J.p.m = s - 1 - j;
-          J.p.nA = T.t[j][8];
-          J.p.nB = T.t[J.p.m][8];
-          J.p.ta = kPairTA;
-          J.p.nTA = (J.p.nA + kPairTA - 1) / kPairTA;
-          J.p.gen_a = j > st.Kt;
-          J.p.gen_b = J.p.m > st.Kt;
-          J.p.mask = mask;
-          J.p.rb = kPairRB;
-          J.ntiles = J.p.nTA * ((J.p.nB + kPairTB - 1) / kPairTB);
-          J.lo_e = tg.lo_even;
-          J.lo_o = tg.lo_odd;
-          J.span = tg.span;
-          J.inst = (uint32_t)i;
-          J.kind = 1;
-          push(J);
-        };
-        if (round == 0 && s == 1) {  // leaves and every unary chain over them
-          BatchJob J;
-          std::memset(&J, 0, sizeof J);
-          const CheckTargets tg = targets_for(st.t0, 1, max_size, W);
-          J.ntiles = 1;
-          J.lo_e = tg.lo_even;
-          J.lo_o = tg.lo_odd;
-          J.span = tg.span;
-          J.inst = (uint32_t)i;
-          J.kind = 2;
-          push(J);
-        }
-        if (round == 0 && s >= 2 && s <= st.Kt) {  // value table V[s]
-          BatchJob J;
-          std::memset(&J, 0, sizeof J);
-          J.p.m = s;
-          J.p.nB = T.t[s][8];
-          J.ntiles = (J.p.nB + kBuildTile - 1) / kBuildTile;
-          J.inst = (uint32_t)i;
-          J.kind = 0;
-          push(J);
-        }
-        if (s >= 3 && round == 0 && (!learned || T.t[s][8] < learned_min)) {
-          for (int j = 1; j <= (s - 1) / 2; ++j) pair_job(j, kAllInst);
-        }
-        if (learned) {
-          for (size_t k = rb; k < re && k < order[i].size(); ++k) pair_job(order[i][k], kAllInst);
-        }
-      }
-      t_build += seconds_since(t_b0);
-      // Runs jobs [j0, j1) of this step (one graph replay) and resolves their
-      // hits; a replay whose hits overflow the buffer is run again as two
-      // halves of its jobs (jobs are independent within a step, table builds
-      // idempotent), so only one job that overflows it alone is an error.
-      std::function<int(size_t, size_t)> run_jobs = [&](size_t j0, size_t j1) -> int {
-        const uint64_t tbase = h_jobs[j0].tile0;
-        const uint64_t tend = h_jobs[j1 - 1].tile0 + h_jobs[j1 - 1].ntiles;
-        for (size_t k = j0; k < j1; ++k) h_jobs[k].tile0 -= tbase;
-        const auto t_g0 = Clock::now();
-        h_step->total_tiles = tend - tbase;
-        h_step->njobs = (uint32_t)(j1 - j0);
-        MBA_HIP(hipEventRecord(ctx.ev0, stq));
-        // these jobs only (the graph then uploads the step header, resets the
-        // queue, drains it and reads the hit count back)
-        MBA_HIP(hipMemcpyAsync(d_jobs, h_jobs + j0, sizeof(BatchJob) * (j1 - j0),
-                               hipMemcpyHostToDevice, stq));
-        MBA_HIP(hipGraphLaunch(gexec, stq));
-        MBA_HIP(hipEventRecord(ctx.ev1, stq));
-        MBA_HIP(hipEventSynchronize(ctx.ev1));
-        for (size_t k = j0; k < j1; ++k) h_jobs[k].tile0 += tbase;
-        float ms = 0.f;
-        MBA_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
-        ktime += ms * 1e-3;
-        ++klaunch;
-        t_gpu += seconds_since(t_g0);
-        if (o.verbose)
-          std::fprintf(stderr, "batch: size %d round %d: %zu jobs, %llu tiles, %.3f ms\n", s,
-                       round, j1 - j0, (unsigned long long)(tend - tbase), ms);
-        const auto t_r0 = Clock::now();
-        const uint32_t nh = *h_hc;
-        if (nh > hit_cap) {
-          ++replays;
-          if (j1 - j0 == 1) {
-            // one job: its tiles in two parts (a tile holds at most
-            // 256 * 2048 * 7 hits, well under the buffer)
-            BatchJob &J = h_jobs[j0];
-            const BatchJob keep = J;
-            if (keep.ntiles < 2) {
-              ctx.last_error = "batch: one tile's window hits overflow the hit buffer";
-              return MBA_ERR_LIMIT;
-            }
-            const uint64_t half = keep.ntiles / 2;
-            J.ntiles = half;
-            int rc = run_jobs(j0, j1);
-            if (rc == 0) {
-              J = keep;
-              J.p.tile_begin += half;
-              J.ntiles = keep.ntiles - half;
-              rc = run_jobs(j0, j1);
-            }
-            J = keep;
-            return rc;
-          }
-          const size_t jm = j0 + (j1 - j0) / 2;
-          if (int rc = run_jobs(j0, jm)) return rc;
-          return run_jobs(jm, j1);
-        }
-        n_hits += nh;
-        if (nh > 0 && !hits_of.empty()) {  // verbose: hits per instance
-          hrec.resize(nh);
-          MBA_HIP(hipMemcpy(hrec.data(), d_hits, sizeof(HitRec) * nh, hipMemcpyDeviceToHost));
-          for (const HitRec &h0 : hrec) hits_of[h0.inst >> 8]++;
-        }
-        if (nh > 0) {
-          // resolve every hit on the device (screen on two samples, check on
-          // every sample, per-slot minimum), then verify the lowered slots'
-          // trees
-          MBA_HIP(ctx.surv.ensure(sizeof(SurvRec) * 2 * (size_t)nh));
-          MBA_HIP(hipMemsetAsync(d_ntouched, 0, 8, stq));  // touched and candidate counts
-          MBA_HIP(launch_hit_resolve(W, d_tvs, ctx.leafvals.as<uint64_t>(),
-                                     ctx.outputs.as<uint64_t>(),
-                                     ctx.b_results.as<BatchInstData>(), d_hits, d_hc, nh,
-                                     (nh + 63) / 64, std::min<uint32_t>(2 * nh, 8192),
-                                     min_size, max_size, d_found, ctx.surv.as<SurvRec>(),
-                                     d_ntouched + 1, d_flags, d_touched, d_ntouched, stq));
-          MBA_HIP(hipMemcpyAsync(h_nt, d_ntouched, 4, hipMemcpyDeviceToHost, stq));
-          MBA_HIP(hipStreamSynchronize(stq));
-          const uint32_t nt = *h_nt;
-          n_kept += nt;
-          if (nt > 0) {
-            MBA_HIP(ctx.vranks.ensure(sizeof(VerifyCand) * nt));
-            MBA_HIP(ctx.vok.ensure(nt));
-            MBA_HIP(launch_found_gather(d_touched, d_ntouched, nt, d_found, d_flags,
-                                        ctx.vranks.as<VerifyCand>(), stq));
-            MBA_HIP(launch_verify_batch(W, d_tvs, ctx.leafvals.as<uint64_t>(),
-                                        ctx.outputs.as<uint64_t>(),
-                                        ctx.b_results.as<BatchInstData>(),
-                                        ctx.vranks.as<VerifyCand>(), nt,
-                                        ctx.vok.as<uint8_t>(), stq));
-            vc.resize(nt);
-            okv.resize(nt);
-            MBA_HIP(hipMemcpyAsync(vc.data(), ctx.vranks.p, sizeof(VerifyCand) * nt,
-                                   hipMemcpyDeviceToHost, stq));
-            MBA_HIP(hipMemcpyAsync(okv.data(), ctx.vok.p, nt, hipMemcpyDeviceToHost, stq));
-            MBA_HIP(hipStreamSynchronize(stq));
-            for (uint32_t c = 0; c < nt; ++c) {
-              if (!okv[c]) continue;
-              ++n_cands;
-              auto &f = S[vc[c].inst].found[vc[c].size];
-              f = std::min(f, vc[c].rank);
-            }
-          }
-        }
-        t_resolve += seconds_since(t_r0);
-        return 0;
-      };
-      if (nj > 0) {
-        if (int rc = run_jobs(0, nj)) return rc;
-      }
-      if (!learned) break;
-      // learned order: any verified solution of this (minimal) size settles
-      if (s >= min_size)
-        for (size_t i = 0; i < count; ++i)
-          if (S[i].active && S[i].found[s] != ~0ull) finish(i, s);
-      if (re >= max_blocks) break;
-    }
-    // sizes completed by this step: 1..2 after the leaf scan, s for s >= 3
-    if (s == 0 || s == 2) continue;
-    const int lo = s == 1 ? 1 : s, hi = s == 1 ? std::min(2, max_size) : s;
+    size_t n_active = 0;
     for (size_t i = 0; i < count; ++i) {
-      InstState &st = S[i];
-      if (!st.active) continue;
-      for (int z = lo; z <= hi; ++z) {
-        st.reached = z;
-        if (z >= min_size && st.found[z] != ~0ull) {
-          finish(i, z);
+      if (S[i].active && over_budget(i)) {
+        S[i].active = false;
+        results[i].status = MBA_BUDGET;
+        results[i].seconds = seconds_since(t_start);
+      }
+      n_active += S[i].active;
+    }
+    if (n_active == 0) break;
+    if (!learned_size(s)) {
+      // ---- a group of plain size steps, one launch sequence -------------
+      size_t nj = 0;
+      uint64_t group_tiles = 0;
+      int nsz = 0;
+      int z = s;
+      for (; z <= max_size && (z == s || !learned_size(z)); ++z) {
+        const size_t j0 = nj;
+        uint64_t tiles = 0;
+        for (size_t i = 0; i < count; ++i)
+          if (S[i].active) size_jobs(i, z, true, nullptr, nj, tiles);
+        BatchStep &hs = h_steps[nsz++];
+        hs.total_tiles = tiles;
+        hs.job_off = j0;
+        hs.njobs = (uint32_t)(nj - j0);
+        hs.size_done = z == 1 ? std::min(2, max_size) : z;
+        group_tiles += tiles;
+        if (z == 0 || z == 2) {  // no jobs of their own: sizes 1..2 complete at step 1
+          hs.size_done = 0;
+        }
+        if (checkpoints && group_tiles >= kGroupTiles) {
+          ++z;
           break;
         }
       }
+      const int z_end = z - 1;
+      const auto t_g = Clock::now();
+      if (nj > 0) {
+        if (int rc = run_group(nsz, nj)) return rc;
+        if (h_cnt->overflow > 0) {
+          // a step's hits overflowed the buffer: run the group's steps again
+          // one at a time, in parts (resolving any subset of the hits only
+          // lowers slots to verified ranks), from the settled sizes of
+          // before the group (a size settled on part of its hits may not
+          // hold its minimum rank yet)
+          MBA_HIP(hipMemsetAsync(&d_cnt->overflow, 0, 4, stq));
+          MBA_HIP(hipMemcpyAsync(d_settled, settled.data(), count * 4, hipMemcpyHostToDevice,
+                                 stq));
+          for (int k = 0; k < nsz; ++k) {
+            const BatchStep hs = h_steps[k];
+            if (hs.njobs == 0) continue;
+            if (int rc = run_parts(hs.job_off, hs.job_off + hs.njobs, std::max(0, hs.size_done - 1)))
+              return rc;
+            // settle the complete size
+            h_steps[kMaxSize + 1] = hs;
+            h_steps[kMaxSize + 1].njobs = 0;
+            h_steps[kMaxSize + 1].total_tiles = 0;
+            MBA_HIP(hipMemcpyAsync(d_steps + kMaxSize + 1, &h_steps[kMaxSize + 1],
+                                   sizeof(BatchStep), hipMemcpyHostToDevice, stq));
+            MBA_HIP(launch_batch_settle(d_steps + kMaxSize + 1, d_found, d_settled,
+                                        (uint32_t)count, min_size, d_cnt, hit_cap, stq));
+          }
+        }
+      }
+      if (o.verbose)
+        std::fprintf(stderr, "batch: sizes %d..%d in one launch sequence: %zu jobs, %llu tiles, "
+                     "%.3f ms\n", s, z_end, nj, (unsigned long long)group_tiles,
+                     1e3 * seconds_since(t_g));
+      charge(seconds_since(t_g), group_tiles);
+      if (int rc = collect()) return rc;
+      for (size_t i = 0; i < count; ++i)
+        if (S[i].active) S[i].reached = z_end;
+      s = z_end + 1;
+      continue;
     }
+    // ---- a learned size: rounds of the best-scored split, the next split,
+    // then the rest (each split swept once with every operator instance
+    // fused); instances with fewer trees at this size take every split in
+    // the first round; any verified solution of this (minimal) size settles
+    if (int rc = (*score_size)(s, scores.data())) return rc;
+    size_t max_blocks = 0;
+    for (size_t i = 0; i < count; ++i) {
+      InstState &st = S[i];
+      order[i].clear();
+      if (!st.active || st.T->t[s][8] < learned_min) continue;
+      results[i].ordered_sizes++;
+      std::vector<BlockRef> blocks = binary_blocks(*st.T, s);
+      const float *sc = scores.data() + i * 256;
+      std::stable_sort(blocks.begin(), blocks.end(), [&](const BlockRef &x, const BlockRef &y) {
+        return sc[x.op * 32 + x.split] > sc[y.op * 32 + y.split];
+      });
+      for (const BlockRef &b : blocks)
+        if (std::find(order[i].begin(), order[i].end(), b.pair_j) == order[i].end())
+          order[i].push_back(b.pair_j);
+      max_blocks = std::max(max_blocks, order[i].size());
+    }
+    static const size_t kRoundEnd[3] = {1, 2, ~(size_t)0};
+    for (int round = 0; round < 3; ++round) {
+      const size_t rb = round == 0 ? 0 : kRoundEnd[round - 1];
+      const size_t re = kRoundEnd[round];
+      size_t nj = 0;
+      uint64_t tiles = 0;
+      std::vector<int> sub;
+      for (size_t i = 0; i < count; ++i) {
+        InstState &st = S[i];
+        if (!st.active) continue;
+        if (order[i].empty()) {
+          if (round == 0) size_jobs(i, s, true, nullptr, nj, tiles);
+          continue;
+        }
+        sub.assign(order[i].begin() + std::min(rb, order[i].size()),
+                   order[i].begin() + std::min(re, order[i].size()));
+        size_jobs(i, s, round == 0, &sub, nj, tiles);
+      }
+      if (nj > 0) {
+        BatchStep &hs = h_steps[0];
+        hs.total_tiles = tiles;
+        hs.job_off = 0;
+        hs.njobs = (uint32_t)nj;
+        hs.size_done = s;  // any verified tree of this size settles
+        const auto t_g = Clock::now();
+        if (int rc = run_group(1, nj)) return rc;
+        if (h_cnt->overflow > 0) {
+          MBA_HIP(hipMemsetAsync(&d_cnt->overflow, 0, 4, stq));
+          if (int rc = run_parts(0, nj, s)) return rc;
+        }
+        if (o.verbose)
+          std::fprintf(stderr, "batch: size %d round %d: %zu jobs, %llu tiles, %.3f ms\n", s,
+                       round, nj, (unsigned long long)tiles, 1e3 * seconds_since(t_g));
+        charge(seconds_since(t_g), tiles);
+        if (int rc = collect()) return rc;
+      }
+      if (re >= max_blocks) break;
+    }
+    for (size_t i = 0; i < count; ++i)
+      if (S[i].active) S[i].reached = s;
+    ++s;
+  }
+
+  // every answer checked once more on all samples by the verify kernel
+  std::vector<VerifyCand> vc;
+  std::vector<size_t> vi;
+  for (size_t i = 0; i < count; ++i)
+    if (results[i].status == MBA_FOUND) {
+      vc.push_back(VerifyCand{(uint32_t)i, (uint32_t)results[i].size, results[i].rank});
+      vi.push_back(i);
+    }
+  if (!vc.empty()) {
+    MBA_HIP(ctx.vranks.ensure(sizeof(VerifyCand) * vc.size()));
+    MBA_HIP(ctx.vok.ensure(vc.size()));
+    MBA_HIP(hipMemcpyAsync(ctx.vranks.p, vc.data(), sizeof(VerifyCand) * vc.size(),
+                           hipMemcpyHostToDevice, stq));
+    MBA_HIP(launch_verify_batch(W, d_tvs, ctx.leafvals.as<uint64_t>(), ctx.outputs.as<uint64_t>(),
+                                ctx.b_results.as<BatchInstData>(), ctx.vranks.as<VerifyCand>(),
+                                (uint32_t)vc.size(), ctx.vok.as<uint8_t>(), stq));
+    std::vector<uint8_t> ok(vc.size());
+    MBA_HIP(hipMemcpyAsync(ok.data(), ctx.vok.p, vc.size(), hipMemcpyDeviceToHost, stq));
+    MBA_HIP(hipStreamSynchronize(stq));
+    for (size_t k = 0; k < vc.size(); ++k)
+      if (!ok[k]) {
+        ctx.last_error = "batch: an answer failed verification on every sample";
+        return MBA_ERR_LIMIT;
+      }
   }
   const double total = seconds_since(t_start);
   if (o.verbose)
-    std::fprintf(stderr,
-                 "batch: jobs %.3f ms, device %.3f ms, hits %.3f ms (%llu hits, %llu lowered "
-                 "slots, %llu verified), total %.3f ms\n",
-                 1e3 * t_build, 1e3 * t_gpu, 1e3 * t_resolve, (unsigned long long)n_hits,
-                 (unsigned long long)n_kept, (unsigned long long)n_cands, 1e3 * total);
-  if (o.verbose && count > 0) {
-    std::vector<uint32_t> idx(count);
-    for (size_t i = 0; i < count; ++i) idx[i] = (uint32_t)i;
-    std::partial_sort(idx.begin(), idx.begin() + std::min<size_t>(8, count), idx.end(),
-                      [&](uint32_t a, uint32_t b) { return hits_of[a] > hits_of[b]; });
-    for (size_t k = 0; k < std::min<size_t>(8, count); ++k) {
-      const uint32_t i = idx[k];
-      std::fprintf(stderr, "batch: instance %u: %u hits, L=%d n=%d, answer size %d (%s)\n", i,
-                   hits_of[i], S[i].L, S[i].n, results[i].size, results[i].expr);
-    }
-  }
+    std::fprintf(stderr, "batch: %u launch sequences, device %.3f ms, total %.3f ms\n", klaunch,
+                 1e3 * ktime, 1e3 * total);
   for (size_t i = 0; i < count; ++i) {
     InstState &st = S[i];
     mba_result &r = results[i];

@@ -66,7 +66,8 @@ struct Ctx {
   size_t b_pinned_bytes = 0;
   hipGraph_t b_graph = nullptr;
   hipGraphExec_t b_gexec = nullptr;
-  uint64_t b_graph_key[11] = {};
+  static constexpr int kGraphKey = 15;
+  uint64_t b_graph_key[kGraphKey] = {};
 };
 
 using Clock = std::chrono::steady_clock;
@@ -86,12 +87,12 @@ inline double seconds_since(Clock::time_point t0) {
 // block_scores: NULL for the plain (rank-order, minimum-rank) search, else
 // per (op, left size) scores at [op * 32 + split] that order the binary
 // blocks of each size (learned proposal; any solution of minimal size).
-// score_init: when given, fills block_scores the first time the search
-// orders a size by them (a search settled below that runs no network).
+// score_size: when given, fills block_scores for a size the first time the
+// search orders that size by them (a search settled below runs no network).
 // rtid: the SIMBA-RTid order instead (nocache_rtid.cu:85-95).
 int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &opts,
                 mba_result &res, const float *block_scores = nullptr, bool rtid = false,
-                const std::function<int()> *score_init = nullptr);
+                const std::function<int(int)> *score_size = nullptr);
 int vfb_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &opts,
               mba_result &res, mba_vfb_stats *stats);
 // The SIMBA search with every device buffer carved from the library's
@@ -103,11 +104,12 @@ int simba_solve_noalloc(Ctx &ctx, const mba_instance &inst, const mba_opts &opts
 // Device fingerprints of `count` signatures of n words (see mba_api.h).
 int vfb_fingerprints(Ctx &ctx, int W, const uint64_t *values, int n, size_t count,
                      uint64_t *out);
-// scores: NULL (plain, minimum rank per instance) or count x 256 block
-// scores as for simba_solve (learned order: rounds of the best-scored split,
-// the next split, then the rest).
+// score_size: NULL (plain, minimum rank per instance) or a function that
+// fills count x 256 block scores (as for simba_solve) for a size; learned
+// order: rounds of the best-scored split, the next split, then the rest.
 int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count,
-                const mba_opts &opts, mba_result *results, const float *scores = nullptr);
+                const mba_opts &opts, mba_result *results,
+                const std::function<int(int, float *)> *score_size = nullptr);
 
 }  // namespace mba
 

@@ -197,8 +197,10 @@ int features_dev(Ctx &ctx, const mba_instance *insts, size_t count, float **d_x)
 // 256 features of an instance: for every sample and bit position k, the
 // pattern of (output bit, input bits of up to 5 variables) at k, hashed
 // together with a bit-position group (k = 0..7 individually, then 8-15,
-// 16-31, 32-63) and the variable count into one of 256 buckets; bucket
-// counts are normalised by samples x width.
+// 16-31, 32-63) and the variable count into one of the first 240 buckets;
+// bucket counts are normalised by samples x width.  The last 16 slots are a
+// one-hot of the size being ordered, left zero here (set per size by the
+// search and by the training rows).
 void instance_features(const mba_instance &in, float *out) {
   std::memset(out, 0, sizeof(float) * kMlpIn);
   const int W = (int)in.width, n = (int)in.n_samples, nv = (int)in.n_vars;
@@ -210,7 +212,7 @@ void instance_features(const mba_instance &in, float *out) {
       for (int v = 0; v < nv && v < 5; ++v)
         p |= (uint32_t)((in.inputs[(size_t)s * nv + v] >> k) & 1) << (v + 1);
       const uint32_t g = k < 8 ? (uint32_t)k : k < 16 ? 8u : k < 32 ? 9u : 10u;
-      const uint32_t b = fmix32(p * 64u + g + 1024u * (uint32_t)nv) & (kMlpIn - 1);
+      const uint32_t b = feat_bucket(fmix32(p * 64u + g + 1024u * (uint32_t)nv));
       out[b] += 1.f;
     }
   }
@@ -394,20 +396,27 @@ int mba_learned_solve(mba_ctx *c, mba_mlp *m, const mba_instance *inst, const mb
     opts = &d;
   }
   // the scores order the blocks of sizes with at least kLearnedBlockMin
-  // trees; the network runs when the search first reaches such a size (its
-  // time is inside the search's, so time to solution includes it), and not
-  // at all for a search settled below
+  // trees; the network runs for each such size the search reaches, with that
+  // size in its input (its time is inside the search's, so time to solution
+  // includes it), and not at all for a search settled below
   float scores[8 * 32];
   std::fill(scores, scores + 8 * 32, 0.f);
-  const std::function<int()> score_init = [&]() -> int {
-    float feat[kMlpIn], logits[kMlpOut];
-    if (int rc = mba_features(inst, feat)) return rc;
+  float feat[kMlpIn];
+  bool have_feat = false;
+  const std::function<int(int)> score_size = [&](int size) -> int {
+    float logits[kMlpOut];
+    if (!have_feat) {
+      if (int rc = mba_features(inst, feat)) return rc;
+      have_feat = true;
+    }
+    std::fill(feat + kFeatHashed, feat + kMlpIn, 0.f);
+    feat[feat_size_slot(size)] = 1.f;
     if (int rc = mba_mlp_forward(m, feat, 1, logits)) return rc;
     block_scores_from_logits(logits, scores);
     return hipSetDevice(c->c.device) == hipSuccess ? 0 : MBA_ERR_HIP;
   };
   if (hipSetDevice(c->c.device) != hipSuccess) return MBA_ERR_HIP;
-  return simba_solve(c->c, *inst, *opts, *res, scores, false, &score_init);
+  return simba_solve(c->c, *inst, *opts, *res, scores, false, &score_size);
 }
 
 int mba_learned_batch_solve(mba_ctx *c, mba_mlp *m, const mba_instance *insts, size_t count,
@@ -419,29 +428,43 @@ int mba_learned_batch_solve(mba_ctx *c, mba_mlp *m, const mba_instance *insts, s
     opts = &d;
   }
   if (count == 0) return 0;
-  // features and forward on the network's device, rows in chunks of 4096
+  // features once on the network's device; for each size the search orders,
+  // the size's one-hot is set and the forward pass runs on every instance
+  // (rows in chunks of 4096)
   Ctx &ctx = m->owner->c;
   MBA_HIP(hipSetDevice(ctx.device));
   float *d_x = nullptr;
   if (int rc = features_dev(ctx, insts, count, &d_x)) return rc;
-  std::vector<float> logits(count * kMlpOut), scores(count * 8 * 32);
-  for (size_t b = 0; b < count; b += 4096) {
-    const int nb = (int)std::min<size_t>(4096, count - b);
-    const int rows = (nb + 15) / 16 * 16;
-    if (int rc = ensure_cap(ctx, m, rows)) return rc;
-    const Acts a = acts_of(m->acts.as<float>(), m->cap);
-    MBA_HIP(hipMemsetAsync(a.X, 0, (size_t)rows * kMlpIn * 4, ctx.stream));
-    MBA_HIP(hipMemcpyAsync(a.X, d_x + b * kMlpIn, (size_t)nb * kMlpIn * 4,
-                           hipMemcpyDeviceToDevice, ctx.stream));
-    if (int rc = forward_dev(ctx, m, rows)) return rc;
-    MBA_HIP(hipMemcpyAsync(logits.data() + b * kMlpOut, a.Z, (size_t)nb * kMlpOut * 4,
-                           hipMemcpyDeviceToHost, ctx.stream));
-  }
+  std::vector<float> feats(count * kMlpIn), logits(count * kMlpOut);
+  MBA_HIP(hipMemcpyAsync(feats.data(), d_x, count * kMlpIn * 4, hipMemcpyDeviceToHost,
+                         ctx.stream));
   MBA_HIP(hipStreamSynchronize(ctx.stream));
-  for (size_t i = 0; i < count; ++i)
-    block_scores_from_logits(logits.data() + i * kMlpOut, scores.data() + i * 256);
+  const std::function<int(int, float *)> score_size = [&](int size, float *scores) -> int {
+    MBA_HIP(hipSetDevice(ctx.device));
+    for (size_t i = 0; i < count; ++i) {
+      float *f = feats.data() + i * kMlpIn;
+      std::fill(f + kFeatHashed, f + kMlpIn, 0.f);
+      f[feat_size_slot(size)] = 1.f;
+    }
+    for (size_t b = 0; b < count; b += 4096) {
+      const int nb = (int)std::min<size_t>(4096, count - b);
+      const int rows = (nb + 15) / 16 * 16;
+      if (int rc = ensure_cap(ctx, m, rows)) return rc;
+      const Acts a = acts_of(m->acts.as<float>(), m->cap);
+      MBA_HIP(hipMemsetAsync(a.X, 0, (size_t)rows * kMlpIn * 4, ctx.stream));
+      MBA_HIP(hipMemcpyAsync(a.X, feats.data() + b * kMlpIn, (size_t)nb * kMlpIn * 4,
+                             hipMemcpyHostToDevice, ctx.stream));
+      if (int rc = forward_dev(ctx, m, rows)) return rc;
+      MBA_HIP(hipMemcpyAsync(logits.data() + b * kMlpOut, a.Z, (size_t)nb * kMlpOut * 4,
+                             hipMemcpyDeviceToHost, ctx.stream));
+    }
+    MBA_HIP(hipStreamSynchronize(ctx.stream));
+    for (size_t i = 0; i < count; ++i)
+      block_scores_from_logits(logits.data() + i * kMlpOut, scores + i * 256);
+    return hipSetDevice(c->c.device) == hipSuccess ? 0 : MBA_ERR_HIP;
+  };
   if (hipSetDevice(c->c.device) != hipSuccess) return MBA_ERR_HIP;
-  return batch_solve(c->c, insts, count, *opts, results, scores.data());
+  return batch_solve(c->c, insts, count, *opts, results, &score_size);
 }
 
 int mba_features_batch(mba_ctx *c, const mba_instance *insts, size_t count, float *out) {

@@ -6,6 +6,17 @@
 namespace mba {
 
 constexpr int kMlpIn = 256;
+// features: kFeatHashed hashed sample-bit buckets, then a one-hot of the size
+// whose blocks are being ordered (slot kFeatHashed + min(size, 16) - 1)
+constexpr int kFeatHashed = 240;
+constexpr int kFeatSizes = kMlpIn - kFeatHashed;
+// bucket of a hashed pattern: multiply-shift range reduction to kFeatHashed
+__host__ __device__ inline uint32_t feat_bucket(uint32_t h) {
+  return (uint32_t)(((uint64_t)h * (uint64_t)kFeatHashed) >> 32);
+}
+__host__ __device__ inline int feat_size_slot(int size) {
+  return kFeatHashed + (size < 1 ? 0 : size > kFeatSizes ? kFeatSizes - 1 : size - 1);
+}
 constexpr int kMlpHidden = 512;
 constexpr int kMlpOut = 128;   // class = op * 16 + min(left size, 15)
 

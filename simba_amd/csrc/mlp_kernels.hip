@@ -122,9 +122,9 @@ hipError_t launch_colsum(const float *D, int rows, int cols, float *out, hipStre
 // Features of a batch of instances, one workgroup (one wave) per instance,
 // lane k = bit position k: the pattern (output bit, bits of the first five
 // variables) of every sample at bit k, hashed with the bit group and the
-// variable count into one of 256 buckets (a histogram in LDS), scaled by
-// 8 / (samples * width).  Counts stay exact in float, so the result equals
-// the host's running float sums.
+// variable count into one of kFeatHashed buckets (a histogram in LDS),
+// scaled by 8 / (samples * width); the size slots stay zero.  Counts stay
+// exact in float, so the result equals the host's running float sums.
 __device__ __forceinline__ uint32_t feat_mix(uint32_t x) {
   x ^= x >> 16;
   x *= 0x85ebca6bu;
@@ -151,7 +151,7 @@ k_features(const uint64_t *__restrict__ inputs, const uint64_t *__restrict__ out
       uint32_t p = (uint32_t)((outputs[f.out_off + smp] >> k) & 1u);
       const uint64_t *in = inputs + f.in_off + (size_t)smp * f.n_vars;
       for (uint32_t v = 0; v < nv5; ++v) p |= (uint32_t)((in[v] >> k) & 1u) << (v + 1);
-      atomicAdd(&hist[feat_mix(p * 64u + g + 1024u * f.n_vars) & (kMlpIn - 1)], 1u);
+      atomicAdd(&hist[feat_bucket(feat_mix(p * 64u + g + 1024u * f.n_vars))], 1u);
     }
   }
   __syncthreads();

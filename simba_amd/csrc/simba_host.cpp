@@ -289,7 +289,7 @@ void fill_result(const CountTable &T, int size, uint64_t rank, int n_vars,
 }
 
 int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_result &res,
-                const float *block_scores, bool rtid, const std::function<int()> *score_init) {
+                const float *block_scores, bool rtid, const std::function<int(int)> *score_size) {
   const auto t_start = Clock::now();
   std::memset(&res, 0, sizeof res);
   res.status = MBA_NOT_FOUND;
@@ -382,13 +382,29 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     else
       MBA_HIP(hipMemcpyAsync(ctx.tables.p, v1w.data(), 4 * L, hipMemcpyHostToDevice, st));
   }
-  // the small sizes in one workgroup (one launch), the rest per block
+  // Value tables are built as the sweep reaches the sizes that read them
+  // (size s reads V[m] for m <= s - 2), so a search settled at a small size
+  // builds only small tables: the small sizes in one workgroup (one
+  // launch), the rest per block.
   int m_small = 1;
   for (uint64_t acc = 0; m_small + 1 <= Kt && acc + T.t[m_small + 1][8] <= kSmallTableValues;)
     acc += T.t[++m_small][8];
-  if (m_small >= 2) MBA_HIP(launch_table_small(W, d_tv, 2, m_small, st));
-  for (int m = m_small + 1; m <= Kt; ++m)
-    MBA_HIP(launch_table_build(W, ctx.tables.p, tv.off, T, m, st));
+  int built = 1;  // V[1] holds the leaf values already
+  auto ensure_tables = [&](int upto) -> int {
+    upto = std::min(upto, Kt);
+    if (upto <= built) return 0;
+    if (built < m_small) {
+      MBA_HIP(launch_table_small(W, d_tv, 2, m_small, st));
+      built = m_small;
+    }
+    for (int m = built + 1; m <= upto; ++m)
+      MBA_HIP(launch_table_build(W, ctx.tables.p, tv.off, T, m, st));
+    built = std::max(built, upto);
+    return 0;
+  };
+  if (rtid) {  // RTid reads every tabulated size directly
+    if (int rc = ensure_tables(Kt)) return rc;
+  }
 
   if (rtid) {
     // SIMBA-RTid: every (size, op) block in order, thread g of the block
@@ -553,6 +569,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   // buffer: those resolved from the truncated buffer stand, as they only
   // lower slots to verified ranks), < 0 on an error.
   auto fused_persistent = [&](int s0, int s1) -> int {
+    if (int rc = ensure_tables(s1 - 2)) return rc;
     constexpr size_t kStageOff = 4096;
     constexpr size_t kMaxJobs = (Ctx::kPinnedBytes - kStageOff - 256) / sizeof(BatchJob);
     auto *h_step = (BatchStep *)((char *)ctx.pinned + kStageOff);
@@ -590,8 +607,9 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     }
     if (nj == 0) return 0;
     h_step->total_tiles = tiles;
+    h_step->job_off = 0;
     h_step->njobs = (uint32_t)nj;
-    h_step->pad = 0;
+    h_step->size_done = 0;
     MBA_HIP(ctx.b_queue.ensure(256 + sizeof(BatchJob) * kMaxJobs));
     MBA_HIP(hipMemcpyAsync(ctx.b_queue.p, h_step, 256 + sizeof(BatchJob) * nj,
                            hipMemcpyHostToDevice, st));
@@ -600,7 +618,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     const int g = (int)std::min<uint64_t>((uint64_t)ctx.cu_count * 8, tiles);
     MBA_HIP(launch_batch_step(W, ctx.b_queue.as<BatchStep>(),
                               (const BatchJob *)(ctx.b_queue.as<char>() + 256), d_tv, d_head,
-                              d_hits, d_hc, kHitCap, g, st));
+                              d_hits, d_hc, kHitCap, nullptr, g, st));
     MBA_HIP(hipEventRecord(ctx.ev1, st));
     ++klaunch;
     if (int rc = process_hits(true)) return rc;  // syncs; rc < 0
@@ -788,6 +806,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
       break;
     }
     const int last = fused_last(s);
+    if (int rc = ensure_tables(std::max(last, s) - 2)) return rc;
     if (last >= s) {
       // fused step: every split of sizes s..last, all seven operator
       // instances (the hit buffer is empty, or holds the leaves' hits).
@@ -863,9 +882,8 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
       // (op, split) blocks in rank order (plain), or splits in the order of
       // the learned scores; stop once the answer is settled
       std::vector<BlockRef> blocks = binary_blocks(T, s);
-      if (block_scores && score_init) {
-        if (int rc = (*score_init)()) return rc;
-        score_init = nullptr;
+      if (block_scores && score_size) {
+        if (int rc = (*score_size)(s)) return rc;
       }
       if (block_scores) {
         std::stable_sort(blocks.begin(), blocks.end(), [&](const BlockRef &x, const BlockRef &y) {
@@ -875,22 +893,25 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
       if (block_scores) {
         ++res.ordered_sizes;
         // learned order: the splits (pairs {j, s-1-j}) in the order of their
-        // best-scored block, in rounds of one split, one split and the rest;
-        // each split is swept once with all operator instances fused (its B
-        // side is formed once), and a round that yields a verified solution
-        // ends the size (any solution of the minimal size is accepted)
+        // best-scored block, each swept once with all operator instances
+        // fused (its B side is formed once), in rounds: the best split alone,
+        // then one split per round, small splits sharing a round until it
+        // holds kRoundPairs pairs (a round costs a sync); a round that yields
+        // a verified solution ends the size (any solution of the minimal
+        // size is accepted)
+        constexpr uint64_t kRoundPairs = 1ull << 26;
         std::vector<int> js;
         for (const BlockRef &b : blocks)
           if (std::find(js.begin(), js.end(), b.pair_j) == js.end()) js.push_back(b.pair_j);
-        for (size_t r0 = 0; r0 < js.size() && !stopped;) {
-          const size_t r1 = r0 < 2 ? r0 + 1 : js.size();
-          for (size_t k = r0; k < r1 && !stopped; ++k) {
-            const int rc = run_pair(s, js[k], kAllInst, tg);
-            if (rc < 0) return MBA_ERR_HIP;
-            stopped = rc == 1;
-          }
-          r0 = r1;
+        uint64_t round_pairs = 0;
+        for (size_t k = 0; k < js.size() && !stopped; ++k) {
+          const int rc = run_pair(s, js[k], kAllInst, tg);
+          if (rc < 0) return MBA_ERR_HIP;
+          stopped = rc == 1;
           if (stopped) break;
+          round_pairs += T.t[js[k]][8] * T.t[s - 1 - js[k]][8];
+          if (k > 0 && k + 1 < js.size() && round_pairs < kRoundPairs) continue;
+          round_pairs = 0;
           if (int rc2 = process_hits()) return rc2;
           if (s >= min_size && found[s] != ~0ull) break;
         }

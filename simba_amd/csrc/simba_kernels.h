@@ -103,11 +103,20 @@ struct BatchJob {
   uint32_t kind;
 };
 
-// Rewritten by the host before each (graph) replay of a step.
+// One size step of a batch: its jobs are jobs_base[job_off, job_off + njobs)
+// (global tiles [0, total_tiles)); once its hits are resolved, every size up
+// to size_done is complete for the step's instances.
 struct BatchStep {
   uint64_t total_tiles;
+  uint64_t job_off;
   uint32_t njobs;
-  uint32_t pad;
+  int32_t size_done;
+};
+
+// Device counters of a batch step: window hits, screened candidates,
+// lowered slots, and the largest hit count of a step above the buffer.
+struct BatchCounters {
+  uint32_t hits, surv, touched, overflow;
 };
 
 struct BatchInstData {
@@ -122,16 +131,27 @@ struct VerifyCand {
   uint64_t rank;
 };
 
-hipError_t launch_batch_reset(unsigned long long *d_head, uint32_t *d_hit_count,
+// Resets the step's work-queue head and the hit, survivor and lowered-slot
+// counts (the overflow record is kept).
+hipError_t launch_batch_reset(unsigned long long *d_head, BatchCounters *d_cnt,
                               hipStream_t stream);
+// After a step's hits are resolved: settled[i] = the smallest size z in
+// [min_size, step->size_done] whose slot d_found[i * (kMaxSize + 1) + z] holds
+// a verified rank (instances already settled keep theirs), and the step's
+// hit count is recorded in d_cnt->overflow when it exceeded the buffer.
+hipError_t launch_batch_settle(const BatchStep *d_step, const unsigned long long *d_found,
+                               int32_t *d_settled, uint32_t count, int min_size,
+                               BatchCounters *d_cnt, uint32_t hit_cap, hipStream_t stream);
 // V[1] (filter-sample leaf values) of every instance of a batch, one launch;
 hipError_t launch_batch_leaves(int width, const TableView *d_tvs, const uint64_t *d_leafvals,
                                const BatchInstData *d_idat, uint32_t count,
                                hipStream_t stream);
-hipError_t launch_batch_step(int width, const BatchStep *d_step, const BatchJob *d_jobs,
+// The persistent kernel of a step: jobs d_jobs_base + step->job_off; jobs of
+// instances with d_settled[inst] > 0 are skipped (d_settled may be NULL).
+hipError_t launch_batch_step(int width, const BatchStep *d_step, const BatchJob *d_jobs_base,
                              const TableView *d_tvs, unsigned long long *d_head,
                              HitRec *d_hits, uint32_t *d_hit_count, uint32_t hit_cap,
-                             int grid, hipStream_t stream);
+                             const int32_t *d_settled, int grid, hipStream_t stream);
 // Full verification of (instance, size, rank) candidates on all samples, one
 // wave per candidate (lane per sample, leaf values and RPN stacks in LDS, exit
 // at the first mismatching sample): ok[i] = 1 when candidate i matches all.

@@ -1458,13 +1458,14 @@ __device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J, ui
 
 template <typename word>
 __global__ void __launch_bounds__(kPairThreads)
-k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jobs,
+k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jobs_base,
              const TableView *__restrict__ tvs, unsigned long long *head, HitRec *hits,
-             uint32_t *hit_count, uint32_t hit_cap) {
+             uint32_t *hit_count, uint32_t hit_cap, const int32_t *__restrict__ settled) {
   __shared__ unsigned long long s_t;
   __shared__ AStage S;
   const uint64_t total = step->total_tiles;
   const int njobs = (int)step->njobs;
+  const BatchJob *__restrict__ jobs = jobs_base + step->job_off;
   // tiles per counter grab: up to 32, fewer when the step has few tiles per
   // workgroup, so the tail stays short
   const uint64_t per = total / ((uint64_t)gridDim.x * 8u);
@@ -1496,6 +1497,8 @@ k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jo
         jcur = jb;
         staged = ~0ull;
       }
+      // an instance settled at a smaller size needs no more work
+      if (settled && settled[J.inst] > 0) continue;
       const TableView *tv = tvs + J.inst;
       const uint64_t lt = t - J.tile0 + J.p.tile_begin;
       if (J.kind == 0) {  // table build: V[m] ranks [lt*kBuildTile, +kBuildTile)
@@ -1535,11 +1538,33 @@ __global__ void k_batch_leaves(const TableView *__restrict__ tvs,
   }
 }
 
-__global__ void k_batch_reset(unsigned long long *head, uint32_t *hit_count) {
+__global__ void k_batch_reset(unsigned long long *head, BatchCounters *cnt) {
   if (threadIdx.x == 0) {
     *head = 0;
-    *hit_count = 0;
+    cnt->hits = 0;
+    cnt->surv = 0;
+    cnt->touched = 0;
   }
+}
+
+__global__ void __launch_bounds__(256)
+k_batch_settle(const BatchStep *__restrict__ step, const unsigned long long *__restrict__ found,
+               int32_t *__restrict__ settled, uint32_t count, int min_size,
+               BatchCounters *cnt, uint32_t hit_cap) {
+  const int done = step->size_done;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    if (settled[i] > 0) continue;
+    const unsigned long long *f = found + (size_t)i * (kMaxSize + 1);
+    for (int z = min_size < 1 ? 1 : min_size; z <= done; ++z) {
+      if (f[z] != ~0ull) {
+        settled[i] = z;
+        break;
+      }
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && cnt->hits > hit_cap)
+    cnt->overflow = cnt->hits > cnt->overflow ? cnt->hits : cnt->overflow;
 }
 
 
@@ -1817,22 +1842,33 @@ hipError_t launch_batch_leaves(int width, const TableView *d_tvs, const uint64_t
   return hipGetLastError();
 }
 
-hipError_t launch_batch_reset(unsigned long long *d_head, uint32_t *d_hit_count,
+hipError_t launch_batch_reset(unsigned long long *d_head, BatchCounters *d_cnt,
                               hipStream_t stream) {
-  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, stream, d_head, d_hit_count);
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, stream, d_head, d_cnt);
   return hipGetLastError();
 }
 
-hipError_t launch_batch_step(int width, const BatchStep *d_step, const BatchJob *d_jobs,
+hipError_t launch_batch_settle(const BatchStep *d_step, const unsigned long long *d_found,
+                               int32_t *d_settled, uint32_t count, int min_size,
+                               BatchCounters *d_cnt, uint32_t hit_cap, hipStream_t stream) {
+  const uint32_t blocks = (count + 255) / 256;
+  hipLaunchKernelGGL(k_batch_settle, dim3(blocks ? blocks : 1), dim3(256), 0, stream, d_step,
+                     d_found, d_settled, count, min_size, d_cnt, hit_cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_step(int width, const BatchStep *d_step, const BatchJob *d_jobs_base,
                              const TableView *d_tvs, unsigned long long *d_head,
                              HitRec *d_hits, uint32_t *d_hit_count, uint32_t hit_cap,
-                             int grid, hipStream_t stream) {
+                             const int32_t *d_settled, int grid, hipStream_t stream) {
   if (width == 32)
     hipLaunchKernelGGL(k_batch_step<uint32_t>, dim3(grid), dim3(kPairThreads), 0, stream,
-                       d_step, d_jobs, d_tvs, d_head, d_hits, d_hit_count, hit_cap);
+                       d_step, d_jobs_base, d_tvs, d_head, d_hits, d_hit_count, hit_cap,
+                       d_settled);
   else
     hipLaunchKernelGGL(k_batch_step<uint64_t>, dim3(grid), dim3(kPairThreads), 0, stream,
-                       d_step, d_jobs, d_tvs, d_head, d_hits, d_hit_count, hit_cap);
+                       d_step, d_jobs_base, d_tvs, d_head, d_hits, d_hit_count, hit_cap,
+                       d_settled);
   return hipGetLastError();
 }
 

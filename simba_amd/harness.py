@@ -6,8 +6,12 @@ and appends each process's stdout line `<path>,<formula>,<seconds>` to a CSV
 whose header is `instance,formula,time`; a run that times out or fails adds
 no line.  Here one process and one device context serve the whole folder:
 the per-instance cap is the library's time budget (a capped search returns
-MBA_BUDGET and adds no line), and with --batch the folder is solved by the
-batched persistent kernel under the same per-instance cap.  Lines match the
+MBA_BUDGET and adds no line; a search that ends past the cap adds none
+either, as the killed process would not), and with --batch the folder is
+solved by the batched search, the cap bounding each instance's share of the
+device time (each step's time split over the instances by their work), so a
+folder may take longer than the cap while every instance stays within it.
+Lines match the
 CLIs' (path as globbed, the reference's printing of the formula, seconds with
 std::cout's default 6 significant digits).
 
@@ -77,6 +81,8 @@ def run(paths, tool="simba", max_len=20, timeout_s=60.0, batch=False, native=Fal
         for p, r in zip(kept, results):
             if r["status"] == simba_amd.BUDGET:
                 continue
+            if not batch and timeout_s > 0 and r["seconds"] > timeout_s:
+                continue  # finished after the cap: `timeout` would have killed it
             lines.append(csv_line(p, r["expr"], r["seconds"]))
     finally:
         if own:

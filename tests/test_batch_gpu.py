@@ -95,3 +95,38 @@ def test_batch_hit_buffer_overflow_split_keeps_answers(ctx, max_size, cap):
     assert got[0]["hit_replays"] >= 1 and base[0]["hit_replays"] == 0
     key = lambda r: (r["status"], r["size"], r["rank"], r["expr"])
     assert [key(r) for r in got] == [key(r) for r in base]
+
+
+def test_batch_plain_search_is_one_launch_sequence(ctx):
+    # without a budget or cancel flag every size step of a plain batch runs in
+    # one captured launch sequence (one sync): sizes settle on the device
+    insts = _instances(31, 64, 64, 7)
+    note("batch one launch sequence")
+    got = ctx.batch(insts, max_size=8)
+    assert got[0]["kernel_launches"] == 1
+    for inst, g in list(zip(insts, got))[:24]:
+        p = ctx.simba(inst, max_size=8)
+        assert (g["status"], g["size"], g["rank"]) == (p["status"], p["size"], p["rank"])
+
+
+def test_batch_budget_is_per_instance(ctx):
+    # the budget bounds each instance's share of the device time, not the
+    # batch's: a batch longer than the budget still gives every instance its
+    # unbudgeted answer when each share stays below it
+    import simba_amd
+    from simba_amd.learn import sample_instances
+    insts = sample_instances(77, 2048, (5, 11), (2, 5), (16, 64))
+    note("batch per-instance budget")
+    base = ctx.batch(insts, max_size=9)
+    budget = 0.002
+    got = ctx.batch(insts, max_size=9, budget_s=budget)
+    assert max(r["seconds"] for r in got) > budget  # the batch outlasts the budget
+    key = lambda r: (r["status"], r["size"], r["rank"], r["expr"])
+    assert [key(r) for r in got] == [key(r) for r in base]
+    # a budget every instance exceeds after its first launch sequence (sizes
+    # up to 11 take several): answers found before it are the unbudgeted ones
+    base11 = ctx.batch(insts[:512], max_size=11)
+    tiny = ctx.batch(insts[:512], max_size=11, budget_s=1e-9)
+    assert any(r["status"] == simba_amd.BUDGET for r in tiny)
+    for t, b in zip(tiny, base11):
+        assert t["status"] == simba_amd.BUDGET or key(t) == key(b)

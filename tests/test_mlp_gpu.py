@@ -106,7 +106,7 @@ def test_trained_network_drives_learned_order(ctx):
     from simba_amd import learn
     from simba_amd.synth import make_instance
     note("learned order engaged: labelling and training")
-    X, Y = learn.build_dataset(ctx, 8000, seed=31, max_size=8)
+    X, Y, _ = learn.build_dataset(ctx, 8000, seed=31, max_size=8)
     m = simba_amd.Mlp(ctx, seed=3)
     learn.train(m, X, Y, epochs=2)
     insts = [make_instance(7300 + s, 2 + s % 3, 32, 6 + s % 4, 64) for s in range(16)]
@@ -138,8 +138,8 @@ def test_trained_network_proposes_solution_blocks_earlier(ctx):
     import simba_amd
     from simba_amd import learn
     note("learned gain: labelling")
-    X, Y = learn.build_dataset(ctx, 24000, seed=11, max_size=8)
-    Xe, Ye = learn.build_dataset(ctx, 4000, seed=10**6 + 11, max_size=8)
+    X, Y, _ = learn.build_dataset(ctx, 24000, seed=11, max_size=8)
+    Xe, Ye, _ = learn.build_dataset(ctx, 4000, seed=10**6 + 11, max_size=8)
     trained = simba_amd.Mlp(ctx, seed=1)
     untrained = simba_amd.Mlp(ctx, seed=1)
     note("learned gain: training")
