@@ -334,42 +334,53 @@ def rec_c2(args):
 # ---- C4: batched solving ----------------------------------------------------
 def rec_c4(args):
     """4096 instances (vars 2..5, target size 5..17, samples 16..64) by one
-    batched call vs one plain call per instance, same size budget: solves/s,
-    solve rate and exact match of the answers."""
+    batched call vs one plain call per instance, with the autotuned launch:
+    (a) the same size budget for both (every size up to --budget-size), and
+    (b) the same time budget per instance (--time-budget-ms: the plain
+    call's wall time; the batched instance's share of the device time) with
+    sizes up to 17, the configuration's largest target.  Solves/s, solve rate
+    and exact match on the instances both solved."""
     import simba_amd
+    from simba_amd import autotune
     from simba_amd.learn import sample_instances
 
     insts = sample_instances(args.seed, args.count, (5, 17), (2, 5), (16, 64))
     ctx = simba_amd.Context(0)
-    kw = dict(max_size=args.budget_size)
-    tuned = {"wg_per_cu": args.wg_per_cu} if args.wg_per_cu else {}
-    kw.update(tuned)
-    if args.autotune:
-        from simba_amd import autotune
-        tuned, _ = autotune.tune_batch(ctx, insts[:1024], args.budget_size)
-        kw.update(tuned)
-    ctx.batch(insts, **kw)
-    reps = 3
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        rb = ctx.batch(insts, **kw)
-    t_batch = (time.perf_counter() - t0) / reps
-    t0 = time.perf_counter()
-    rp = [ctx.simba(x, **kw) for x in insts]
-    t_plain = time.perf_counter() - t0
+    if args.wg_per_cu:
+        tuned = {"wg_per_cu": args.wg_per_cu}
+    else:
+        tuned, _ = autotune.tune_batch(ctx, insts[:1024], args.budget_size, store=False)
+
+    def compare(kw, reps):
+        ctx.batch(insts, **kw)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            rb = ctx.batch(insts, **kw)
+        t_batch = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        rp = [ctx.simba(x, **{k: v for k, v in kw.items() if k != "wg_per_cu"}) for x in insts]
+        t_plain = time.perf_counter() - t0
+        sb = sum(r["status"] == 0 for r in rb)
+        sp = sum(r["status"] == 0 for r in rp)
+        both = [i for i in range(len(insts)) if rb[i]["status"] == 0 and rp[i]["status"] == 0]
+        return {"solves_per_s": round(sb / t_batch, 1), "solve_rate": round(sb / len(insts), 4),
+                "ms_per_call": round(1e3 * t_batch, 2),
+                "plain_per_instance_solves_per_s": round(sp / t_plain, 1),
+                "plain_solve_rate": round(sp / len(insts), 4),
+                "plain_seconds": round(t_plain, 3),
+                "speedup_vs_plain": round((sb / t_batch) / (sp / t_plain), 2) if sp else None,
+                "exact_match_solved_by_both": round(
+                    sum(_key(rb[i]) == _key(rp[i]) for i in both) / max(len(both), 1), 4),
+                "solved_by_both": len(both)}
+
+    size_rec = compare(dict(max_size=args.budget_size, **tuned), 3)
+    time_rec = compare(dict(max_size=17, budget_s=args.time_budget_ms / 1e3, **tuned), 1)
     ctx.close()
-    solved = sum(r["status"] == 0 for r in rb)
-    solved_plain = sum(r["status"] == 0 for r in rp)
-    exact = sum(_key(a) == _key(b) for a, b in zip(rb, rp)) / len(insts)
-    return {"workload": "batch_%d_vars2to5_size5to17_samples16to64_budget%d"
-                        % (args.count, args.budget_size),
-            "solves_per_s": round(solved / t_batch, 1),
-            "solve_rate": round(solved / len(insts), 4),
-            "ms_per_call": round(1e3 * t_batch, 2),
-            "plain_per_instance_solves_per_s": round(solved_plain / t_plain, 1),
-            "plain_solve_rate": round(solved_plain / len(insts), 4),
-            "speedup_vs_plain": round(t_plain / t_batch, 2),
-            "exact_match_vs_plain": exact, "launch": tuned or "default"}
+    return {"workload": "batch_%d_vars2to5_size5to17_samples16to64" % args.count,
+            "launch": tuned,
+            "solves_per_s": size_rec["solves_per_s"], "solve_rate": size_rec["solve_rate"],
+            "size_budget_%d" % args.budget_size: size_rec,
+            "time_budget_%gms_sizes_to_17" % args.time_budget_ms: time_rec}
 
 
 # ---- C5: learned proposal ---------------------------------------------------
@@ -388,13 +399,13 @@ def rec_c5(args):
 
     ctx = simba_amd.Context(0)
     t0 = time.perf_counter()
-    X, Y, Z = learn.build_dataset(ctx, args.train, args.seed, max_size=args.label_size)
+    X, Y, Z = learn.build_dataset(ctx, args.train, args.seed, max_size=args.label_size, log=log)
     t_label = time.perf_counter() - t0
     Xe, Ye, Ze = learn.build_dataset(ctx, 8000, args.seed + 2 * 10**6, max_size=args.label_size)
     mlp = simba_amd.Mlp(ctx, seed=args.seed)
     untrained = simba_amd.Mlp(ctx, seed=args.seed + 1)
     t0 = time.perf_counter()
-    losses = learn.train(mlp, X, Y, epochs=args.epochs)
+    losses = learn.train(mlp, X, Y, epochs=args.epochs, log=log)
     t_train = time.perf_counter() - t0
     big = Ze >= 10
     ranking = {"held_out_rows": int(len(Ye)),
@@ -414,7 +425,13 @@ def rec_c5(args):
                      ("untrained", lambda x: untrained.solve(x, **kw))):
         fn(ev[0])
         t0 = time.perf_counter()
-        runs[name] = [fn(x) for x in ev]
+        out = []
+        for k, x in enumerate(ev):
+            out.append(fn(x))
+            if (k + 1) % 1024 == 0:
+                log("c5 %s per instance: %d/%d, %.1f s" % (name, k + 1, len(ev),
+                                                         time.perf_counter() - t0))
+        runs[name] = out
         runs[name + "_seconds"] = time.perf_counter() - t0
     plain, lrn, unt = runs["plain"], runs["learned"], runs["untrained"]
     solved = [i for i, r in enumerate(plain) if r["status"] == 0]
@@ -429,6 +446,7 @@ def rec_c5(args):
              for q in (50, 95)}
     e_rec = {q: {n: pct(n, engaged, q) for n in ("plain", "learned", "untrained")}
              for q in (50, 95)}
+    log("c5 batched warm-up")
     mlp.batch_solve(ev, **kw)
     batched = {}
     for name, fn in (("learned", lambda: mlp.batch_solve(ev, **kw)),
@@ -437,6 +455,7 @@ def rec_c5(args):
         t0 = time.perf_counter()
         rb = fn()
         dt = time.perf_counter() - t0
+        log("c5 batched %s: %.1f s" % (name, dt))
         batched[name] = {"solves_per_s": round(sum(r["status"] == 0 for r in rb) / dt, 1),
                          "solve_rate": round(sum(r["status"] == 0 for r in rb) / len(ev), 4),
                          "same_size": all(rb[i]["status"] == 0 and rb[i]["size"] ==
@@ -483,7 +502,7 @@ def run_vfb(args):
 def run_batch(args):
     rec = rec_c4(args)
     return _line("batched solves/s", rec["solves_per_s"], "solves/s", True,
-                 rec["ms_per_call"], rec, steps=3)
+                 rec["size_budget_%d" % args.budget_size]["ms_per_call"], rec, steps=3)
 
 
 def run_learned(args):
@@ -563,6 +582,8 @@ def main():
     ap.add_argument("--cache-gb", type=int, default=96)
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--budget-size", type=int, default=9)
+    ap.add_argument("--time-budget-ms", type=float, default=2.0,
+                    help="C4: per-instance time budget of the sizes-to-17 comparison")
     ap.add_argument("--learned-budget", type=int, default=13,
                     help="C5: largest size searched (the learned order applies to sizes "
                          "of at least 2^26 trees)")

@@ -68,15 +68,23 @@ def tune_simba(ctx, inst, size, space=SIMBA_SPACE, repeats=1, store=True):
 
 
 def tune_batch(ctx, insts, max_size, space=BATCH_SPACE, store=True):
+    """Times a batched call on `insts` for every configuration (each must
+    give the same answers); returns (best config, {config: seconds})."""
+    import time
     names = list(space)
     timing = {}
+    answer = None
     for combo in itertools.product(*(space[n] for n in names)):
         cfg = dict(zip(names, combo))
-        import time
         ctx.batch(insts[: min(64, len(insts))], max_size=max_size, **cfg)
         t = time.perf_counter()
-        ctx.batch(insts, max_size=max_size, **cfg)
+        res = ctx.batch(insts, max_size=max_size, **cfg)
         timing[json.dumps(cfg, sort_keys=True)] = time.perf_counter() - t
+        key = [(r["status"], r["size"], r["rank"]) for r in res]
+        if answer is None:
+            answer = key
+        elif key != answer:
+            raise RuntimeError(f"configuration {cfg} changed the answers")
     best_cfg = json.loads(min(timing, key=timing.get))
     if store:
         _store(f"batch:w{insts[0].width}:s{max_size}", best_cfg)

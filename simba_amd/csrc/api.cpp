@@ -74,7 +74,7 @@ void mba_ctx_destroy(mba_ctx *c) {
                     &c->c.vok, &c->c.vfb_rows, &c->c.vfb_left, &c->c.vfb_right,
                     &c->c.vfb_keys, &c->c.vfb_idx, &c->c.vfb_flags, &c->c.vfb_misc,
                     &c->c.vfb_counts, &c->c.vfb_target, &c->c.b_inst, &c->c.b_data,
-                    &c->c.b_queue, &c->c.b_results, &c->c.b_misc, &c->c.mlp_w,
+                    &c->c.b_queue, &c->c.b_results, &c->c.b_misc, &c->c.b_tvup, &c->c.mlp_w,
                     &c->c.mlp_misc, &c->c.f_in, &c->c.f_out, &c->c.f_meta, &c->c.f_x};
   for (DevBuf *b : bufs) b->release();
   if (c->c.pinned) (void)hipHostFree(c->c.pinned);

@@ -46,6 +46,7 @@ struct InstState {
 
 // Runs f(lo, hi) over [0, n) in up to kSetupThreads host threads.
 constexpr size_t kSetupThreads = 8;
+constexpr uint64_t kMinAllowance = 256;  // budgeted batches: tiles an instance always gets
 template <typename F>
 void parallel_ranges(size_t n, F f) {
   const size_t nt = n >= 1024 ? kSetupThreads : 1;
@@ -86,7 +87,9 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   const uint64_t table_budget =
       o.table_bytes ? (uint64_t)o.table_bytes
                     : std::min<uint64_t>(kBatchTableBytes, ctx.hbm_bytes / 5 * 2);
-  const uint64_t per_budget = std::max<uint64_t>(table_budget / count, 1 << 16);
+  // tables up to this size (V[max_size - 2] is only ever the B side of
+  // leaves: formed per tile)
+  const int kt_cap = std::max(1, max_size - 3 - std::max(0, o.table_delta));
   // pass 1 (in order): limits, count tables per leaf count, sample offsets
   uint64_t nleaf = 0, nout = 0;
   for (size_t i = 0; i < count; ++i) {
@@ -118,16 +121,16 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     nleaf += (uint64_t)n * L;
     nout += (uint64_t)n;
   }
-  // pinned staging (kept in the context, grown when needed): step header at
-  // 0, hit and slot counts at 256, then the jobs of a step, the instances'
-  // table views, leaf values, outputs and sample layouts
+  // pinned staging (kept in the context, grown when needed): step headers
+  // at 0, step counters at 1024, then the instances' table views, leaf
+  // values, outputs and sample layouts
   // jobs of every size of a search (a group of sizes runs as one launch
   // sequence): the leaf scan, a table build per tabulated size, the pairs
   size_t jobs_per_inst = 1;
   for (int z = 2; z <= max_size; ++z) jobs_per_inst += 1 + (size_t)std::max(0, (z - 1) / 2);
+  // (B sides above Kt add a job per B block: the host arrays grow with them)
   const size_t job_cap = count * jobs_per_inst;
-  const size_t off_jobs = 2048;
-  const size_t off_tvs = align256(off_jobs + sizeof(BatchJob) * job_cap);
+  const size_t off_tvs = 2048;
   const size_t off_leaf = align256(off_tvs + sizeof(TableView) * count);
   const size_t off_out = align256(off_leaf + 8 * nleaf);
   const size_t off_idat = align256(off_out + 8 * nout);
@@ -147,7 +150,11 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   // step run in parts) at 0, the step counters read back at 1024
   BatchStep *h_steps = (BatchStep *)pin;
   BatchCounters *h_cnt = (BatchCounters *)(pin + 1024);
-  BatchJob *h_jobs = (BatchJob *)(pin + off_jobs);
+  // the jobs of a group and the B blocks they name (block 0: a placeholder
+  // for jobs whose B side is a table)
+  std::vector<BatchJob> jobv(job_cap);
+  BatchJob *h_jobs = jobv.data();
+  std::vector<BBlock> blkv(1), blk_tmp;
   TableView *tvs = (TableView *)(pin + off_tvs);
   uint64_t *leafvals = (uint64_t *)(pin + off_leaf);
   uint64_t *outputs = (uint64_t *)(pin + off_out);
@@ -172,20 +179,15 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       }
       st.fs = pick_filter_sample(lv, outv, n, L, W);
       st.t0 = outv[st.fs];
-      // V[max_size - 2] is only ever the B side of leaves: formed per tile
-      st.Kt = std::max(1, std::min(choose_kt(*st.T, std::max(max_size, 1), W, per_budget),
-                                   max_size - 3) -
-                              std::max(0, o.table_delta));
+      // V[1] only; larger tables are placed as the sweep reaches them
+      st.Kt = 1;
       TableView &tv = tvs[i];
       std::memset(&tv, 0, sizeof tv);
       uint64_t e = 0;
-      for (int m = 1; m <= st.Kt; ++m) {
-        tv.off[m] = e;
-        e += st.T->t[m][8];
-      }
-      st.elems = e;
+      for (int m = 2; m <= kt_cap; ++m) e += st.T->t[m][8];
+      st.elems = e;  // every table the search could place
       std::memcpy(tv.T, st.T->t, sizeof tv.T);
-      tv.Kt = st.Kt;
+      tv.Kt = 1;
       tv.L = L;
       idat[i].leaf_off = leaf_off[i];
       idat[i].out_off = out_off[i];
@@ -193,17 +195,18 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       idat[i].fs = (uint32_t)st.fs;
     }
   });
-  // pass 3: the value-table pool, one slice per instance
-  uint64_t pool_elems = 0;
-  for (size_t i = 0; i < count; ++i) pool_elems += S[i].elems;
-  MBA_HIP(ctx.b_data.ensure(std::max<size_t>(pool_elems * wb, 256)));
-  {
-    uint64_t pe = 0;
-    for (size_t i = 0; i < count; ++i) {
-      tvs[i].base = ctx.b_data.as<char>() + pe * wb;
-      pe += S[i].elems;
-    }
+  // pass 3: the value-table pool: every instance's V[1] first, then tables
+  // placed as the sweep reaches their size (place_table)
+  uint64_t pool_used = 0, pool_need = 0;
+  for (size_t i = 0; i < count; ++i) {
+    tvs[i].off[1] = pool_used;
+    pool_used += (uint64_t)S[i].L;
+    pool_need += S[i].elems;
   }
+  const uint64_t pool_elems = pool_used + std::min<uint64_t>(pool_need, table_budget / wb);
+  const bool pool_all = pool_need <= table_budget / wb;  // room for every table
+  MBA_HIP(ctx.b_data.ensure(std::max<size_t>(pool_elems * wb, 256)));
+  for (size_t i = 0; i < count; ++i) tvs[i].base = ctx.b_data.p;
   MBA_HIP(ctx.b_inst.ensure(sizeof(TableView) * count));
   MBA_HIP(ctx.leafvals.ensure(8 * std::max<uint64_t>(nleaf, 1)));
   MBA_HIP(ctx.outputs.ensure(8 * std::max<uint64_t>(nout, 1)));
@@ -229,10 +232,26 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   unsigned long long *d_head = (unsigned long long *)(ctx.hit_count.as<char>() + 64);
   HitRec *d_hits = ctx.hits.as<HitRec>();
   const TableView *d_tvs = ctx.b_inst.as<TableView>();
-  MBA_HIP(ctx.b_queue.ensure(sizeof(BatchJob) * job_cap + sizeof(BatchStep) * (kMaxSize + 2)));
-  BatchStep *d_steps = ctx.b_queue.as<BatchStep>();
-  BatchJob *d_jobs =
-      (BatchJob *)(ctx.b_queue.as<char>() + sizeof(BatchStep) * (kMaxSize + 2));
+  // device copies of a group: step headers, jobs, blocks (grown with a
+  // group that holds more)
+  BatchStep *d_steps = nullptr;
+  BatchJob *d_jobs = nullptr;
+  BBlock *d_blocks = nullptr;
+  size_t dev_jobs = 0, dev_blocks = 0;
+  auto queue_room = [&](size_t nj, size_t nb) -> int {
+    if (d_steps && nj <= dev_jobs && nb <= dev_blocks) return 0;
+    dev_jobs = std::max(dev_jobs, std::max(nj, job_cap));
+    dev_blocks = std::max(dev_blocks, std::max<size_t>(nb, 4096));
+    const size_t off_j = align256(sizeof(BatchStep) * (kMaxSize + 2));
+    const size_t off_b = align256(off_j + sizeof(BatchJob) * dev_jobs);
+    MBA_HIP(hipStreamSynchronize(stq));
+    MBA_HIP(ctx.b_queue.ensure(off_b + sizeof(BBlock) * dev_blocks));
+    d_steps = ctx.b_queue.as<BatchStep>();
+    d_jobs = (BatchJob *)(ctx.b_queue.as<char>() + off_j);
+    d_blocks = (BBlock *)(ctx.b_queue.as<char>() + off_b);
+    return 0;
+  };
+  if (int rc = queue_room(job_cap, 4096)) return rc;
   // per-(instance, size) slots: the verified minimum rank, the slot flags and
   // the list of lowered slots of the resolution kernels; per instance, the
   // smallest size settled on the device
@@ -273,18 +292,57 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     std::fprintf(stderr, "batch: %zu instances set up in %.3f ms\n", count,
                  1e3 * seconds_since(t_start));
 
+  // table placements of the group being built, applied on the device before
+  // it runs (every record of an instance carries its final Kt)
+  std::vector<TvUpdate> tv_updates;
+  size_t n_active_now = count;
+  auto flush_tables = [&]() -> int {
+    if (tv_updates.empty()) return 0;
+    for (TvUpdate &u : tv_updates) u.kt = S[u.inst].Kt;
+    MBA_HIP(ctx.b_tvup.ensure(sizeof(TvUpdate) * tv_updates.size()));
+    MBA_HIP(hipMemcpyAsync(ctx.b_tvup.p, tv_updates.data(), sizeof(TvUpdate) * tv_updates.size(),
+                           hipMemcpyHostToDevice, stq));
+    MBA_HIP(launch_tv_update(ctx.b_inst.as<TableView>(), ctx.b_tvup.as<TvUpdate>(),
+                             (uint32_t)tv_updates.size(), stq));
+    MBA_HIP(hipStreamSynchronize(stq));  // the record vector is reused
+    tv_updates.clear();
+    return 0;
+  };
+
   // The jobs of size s for instance i (learned rounds: the splits of
   // `splits`, else every split; table build and leaf scan in the first
-  // round), appended to h_jobs[nj..] with step-relative first tiles.
+  // round), appended to h_jobs[nj..] with step-relative first tiles.  Only
+  // the pair tiles [win_from, win_from + win_len) of the size (counted over
+  // its splits in order) are taken, the table build and leaf scan with the
+  // window at 0; returns the end of the window when tiles remain past it,
+  // else 0 (the size is complete).
   auto size_jobs = [&](size_t i, int s, bool first_round, const std::vector<int> *splits,
-                       size_t &nj, uint64_t &tiles) {
+                       size_t &nj, uint64_t &tiles, uint64_t win_from = 0,
+                       uint64_t win_len = ~0ull) -> uint64_t {
     InstState &st = S[i];
     const CountTable &T = *st.T;
-    auto push = [&](BatchJob &J) {
+    const uint64_t win_end = win_len > ~0ull - win_from ? ~0ull : win_from + win_len;
+    uint64_t loc = 0;  // pair tiles of the size before the current job
+    auto push = [&](BatchJob &J) -> bool {
+      if (J.kind == 1) {
+        const uint64_t a = loc, b = loc + J.ntiles;
+        loc = b;
+        const uint64_t lo = std::max(a, win_from), hi = std::min(b, win_end);
+        if (lo >= hi) return false;
+        J.p.tile_begin += lo - a;
+        J.ntiles = hi - lo;
+      } else if (win_from > 0) {
+        return false;
+      }
       J.tile0 = tiles;
       tiles += J.ntiles;
       inst_tiles[i] += J.ntiles;
+      if (nj == jobv.size()) {
+        jobv.resize(2 * nj);
+        h_jobs = jobv.data();
+      }
       h_jobs[nj++] = J;
+      return true;
     };
     auto pair_job = [&](int j) {
       const CheckTargets tg = targets_for(st.t0, s, max_size, W);
@@ -301,13 +359,27 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       J.p.gen_b = J.p.m > st.Kt;
       J.p.mask = kAllInst;
       J.p.rb = kPairRB;
-      J.ntiles = J.p.nTA * ((J.p.nB + kPairTB - 1) / kPairTB);
       J.lo_e = tg.lo_even;
       J.lo_o = tg.lo_odd;
       J.span = tg.span;
       J.inst = (uint32_t)i;
       J.kind = 1;
-      push(J);
+      if (!J.p.gen_b) {
+        J.p.nTB = (J.p.nB + kPairTB - 1) / kPairTB;
+        J.ntiles = J.p.nTA * J.p.nTB;
+        push(J);
+        return;
+      }
+      // a B side above Kt: one job per B block formed from the tables
+      blk_tmp.clear();
+      b_blocks(T, st.Kt, tvs[i].off, J.p.m, 0, 1, {}, kPairRB, blk_tmp);
+      for (const BBlock &b : blk_tmp) {
+        BatchJob Jb = J;
+        Jb.p.nTB = block_tiles(b, kPairRB);
+        Jb.ntiles = Jb.p.nTA * Jb.p.nTB;
+        Jb.bb = (uint32_t)blkv.size();
+        if (push(Jb)) blkv.push_back(b);
+      }
     };
     if (first_round && s == 1) {  // leaves and every unary chain over them
       BatchJob J;
@@ -321,7 +393,21 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       J.kind = 2;
       push(J);
     }
-    if (first_round && s >= 2 && s <= st.Kt) {  // value table V[s]
+    if (first_round && win_from == 0 && s >= 2 && s == st.Kt + 1 && s <= kt_cap) {
+      // V[s] is tabulated when it fits this instance's share of the free
+      // pool (the free elements over the instances still active)
+      const uint64_t n = T.t[s][8];
+      const uint64_t free_elems = pool_elems - pool_used;
+      if (n <= free_elems &&
+          (pool_all || n <= free_elems / std::max<size_t>(n_active_now, 1))) {
+        tvs[i].off[s] = pool_used;
+        tvs[i].Kt = s;
+        st.Kt = s;
+        tv_updates.push_back(TvUpdate{(uint32_t)i, (uint32_t)s, pool_used, s, 0});
+        pool_used += n;
+      }
+    }
+    if (first_round && s >= 2 && s <= st.Kt && st.Kt == s) {  // value table V[s]
       BatchJob J;
       std::memset(&J, 0, sizeof J);
       J.p.m = s;
@@ -338,6 +424,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
         for (int j = 1; j <= (s - 1) / 2; ++j) pair_job(j);
       }
     }
+    return loc > win_end ? win_end : 0;
   };
 
   // Launch sequence of one size step k of a group: reset the counters, drain
@@ -346,8 +433,8 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   auto step_launches = [&](const BatchStep *d_step) -> hipError_t {
     hipError_t e = launch_batch_reset(d_head, d_cnt, stq);
     if (e == hipSuccess)
-      e = launch_batch_step(W, d_step, d_jobs, d_tvs, d_head, d_hits, &d_cnt->hits, hit_cap,
-                            d_settled, grid, stq);
+      e = launch_batch_step(W, d_step, d_jobs, d_blocks, d_tvs, d_head, d_hits, &d_cnt->hits,
+                            hit_cap, d_settled, grid, stq);
     if (e == hipSuccess)
       e = launch_hit_resolve(W, d_tvs, ctx.leafvals.as<uint64_t>(), ctx.outputs.as<uint64_t>(),
                              ctx.b_results.as<BatchInstData>(), d_hits, &d_cnt->hits, hit_cap,
@@ -365,10 +452,16 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   // reads the steps from device memory, so it is kept across groups and
   // calls of the same shape.
   auto run_group = [&](int nsz, size_t njobs) -> int {
+    if (int rc = queue_room(njobs, blkv.size())) return rc;
+    for (int k = 0; k < nsz; ++k) h_steps[k].index = (uint32_t)k;
     MBA_HIP(hipMemcpyAsync(d_steps, h_steps, sizeof(BatchStep) * nsz, hipMemcpyHostToDevice,
                            stq));
+    MBA_HIP(hipMemsetAsync(&d_cnt->overflow, 0, 4, stq));
+    MBA_HIP(hipMemsetAsync(&d_cnt->first_overflow, 0xff, 4, stq));
     MBA_HIP(hipMemcpyAsync(d_jobs, h_jobs, sizeof(BatchJob) * njobs, hipMemcpyHostToDevice,
                            stq));
+    MBA_HIP(hipMemcpyAsync(d_blocks, blkv.data(), sizeof(BBlock) * blkv.size(),
+                           hipMemcpyHostToDevice, stq));
     // every value and buffer the captured launches name
     const uint64_t key[Ctx::kGraphKey] = {(uint64_t)W,
                                           (uint64_t)grid,
@@ -376,6 +469,8 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
                                           (uint64_t)nsz | (uint64_t)count << 8,
                                           (uint64_t)min_size | (uint64_t)max_size << 8,
                                           (uint64_t)(uintptr_t)d_steps,
+                                          (uint64_t)(uintptr_t)d_jobs,
+                                          (uint64_t)(uintptr_t)d_blocks,
                                           (uint64_t)(uintptr_t)d_tvs,
                                           (uint64_t)(uintptr_t)d_hits,
                                           (uint64_t)(uintptr_t)d_cnt,
@@ -414,6 +509,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   // resolved before returning; an overflowing part is run again as halves
   // of its jobs, a single job as halves of its tiles.  size_done is the
   // settle bound (the sizes complete once every part has run).
+  uint32_t parts_lo = 0, parts_hi = 0;  // instances of the step run in parts
   std::function<int(size_t, size_t, int)> run_parts = [&](size_t j0, size_t j1,
                                                           int size_done) -> int {
     const uint64_t tbase = h_jobs[j0].tile0;
@@ -424,6 +520,8 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     hs.job_off = j0;
     hs.njobs = (uint32_t)(j1 - j0);
     hs.size_done = size_done;
+    hs.inst_lo = parts_lo;
+    hs.inst_hi = parts_hi;
     BatchStep *d_hs = d_steps + kMaxSize + 1;
     MBA_HIP(hipMemcpyAsync(d_hs, &hs, sizeof hs, hipMemcpyHostToDevice, stq));
     MBA_HIP(hipMemcpyAsync(d_jobs + j0, h_jobs + j0, sizeof(BatchJob) * (j1 - j0),
@@ -492,6 +590,16 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   auto over_budget = [&](size_t i) {
     return o.time_budget_s > 0 && own[i] > o.time_budget_s;
   };
+  // With a budget, an instance takes at most the tiles its remaining budget
+  // covers at the tile rate of the last launch sequence (one sequence may
+  // otherwise hold a whole large size of one instance, many budgets long);
+  // the rest of the size follows in the next sequences while it has budget.
+  double tile_rate = 2e7;  // tiles per second, updated after every sequence
+  auto allowance = [&](size_t i) -> uint64_t {
+    if (o.time_budget_s <= 0) return ~0ull;
+    const double left = std::max(0.0, o.time_budget_s - own[i]);
+    return std::max<uint64_t>(kMinAllowance, (uint64_t)(left * tile_rate));
+  };
 
   // sizes whose steps would use the learned order for some active instance
   auto learned_size = [&](int s) {
@@ -500,12 +608,25 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       if (S[i].active && S[i].T->t[s][8] >= learned_min) return true;
     return false;
   };
-  // with a budget or a cancel flag, a group ends once it holds this many
-  // tiles (the host checks between groups); without, at a learned size
-  constexpr uint64_t kGroupTiles = 1ull << 16;
+  // A group ends once it holds this many tiles: with a budget or a cancel
+  // flag the host checks between groups (smaller groups); without, large
+  // groups keep small searches in one launch sequence while the host still
+  // drops settled instances from the jobs of large sizes.
   const bool checkpoints = o.time_budget_s > 0 || o.cancel != nullptr;
+  const uint64_t kGroupTiles = checkpoints ? (1ull << 18) : (1ull << 20);
 
   int s = 0;
+  size_t cursor = 0;  // the first instance of size s still to sweep
+  // Per instance: the next size it sweeps, and the pair tiles of that size
+  // swept so far (nonzero only for an instance whose budget window ended
+  // inside the size).  Every active instance is at size s or s + 1.
+  std::vector<int32_t> next_sz(count, 0);
+  std::vector<uint64_t> part_t(count, 0);
+  auto pending_at = [&](int z) {
+    for (size_t i = 0; i < count; ++i)
+      if (S[i].active && next_sz[i] == z) return true;
+    return false;
+  };
   while (s <= max_size) {
     if (o.cancel && *o.cancel) {
       budget_hit = true;
@@ -521,48 +642,92 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       n_active += S[i].active;
     }
     if (n_active == 0) break;
-    if (!learned_size(s)) {
+    n_active_now = n_active;
+    if (cursor > 0 || !learned_size(s)) {
       // ---- a group of plain size steps, one launch sequence -------------
+      // Each step is one size over a range of instances: all of them, or as
+      // many as keep the group near kGroupTiles tiles, the next group
+      // resuming within the size.  With a budget, an instance takes the
+      // window of the size's tiles its budget allows; one whose window ends
+      // inside the size ends the group at the end of the size's pass (its
+      // next window waits for its charge) and settles nothing at that size.
       size_t nj = 0;
+      blkv.resize(1);
       uint64_t group_tiles = 0;
       int nsz = 0;
-      int z = s;
-      for (; z <= max_size && (z == s || !learned_size(z)); ++z) {
+      int step_size[kMaxSize + 1];
+      const int s0 = s;
+      while (s <= max_size && nsz < kMaxSize + 1) {
+        if (nsz > 0 && cursor == 0 && learned_size(s)) break;
         const size_t j0 = nj;
         uint64_t tiles = 0;
-        for (size_t i = 0; i < count; ++i)
-          if (S[i].active) size_jobs(i, z, true, nullptr, nj, tiles);
+        size_t i = cursor;
+        for (; i < count; ++i) {
+          if (!S[i].active || next_sz[i] != s) continue;
+          const uint64_t next =
+              size_jobs(i, s, true, nullptr, nj, tiles, part_t[i], allowance(i));
+          part_t[i] = next;  // > 0: instance i resumes size s in a later pass
+          if (next == 0) next_sz[i] = s + 1;
+          if (group_tiles + tiles >= kGroupTiles) {
+            ++i;
+            break;
+          }
+        }
+        step_size[nsz] = s;
         BatchStep &hs = h_steps[nsz++];
         hs.total_tiles = tiles;
         hs.job_off = j0;
         hs.njobs = (uint32_t)(nj - j0);
-        hs.size_done = z == 1 ? std::min(2, max_size) : z;
+        // sizes 1..2 complete with the leaf scan (step 1); steps 0 and 2
+        // settle nothing
+        hs.size_done = s == 1 ? std::min(2, max_size) : (s == 0 || s == 2) ? 0 : s;
+        hs.inst_lo = (uint32_t)cursor;
+        hs.inst_hi = (uint32_t)i;
         group_tiles += tiles;
-        if (z == 0 || z == 2) {  // no jobs of their own: sizes 1..2 complete at step 1
-          hs.size_done = 0;
-        }
-        if (checkpoints && group_tiles >= kGroupTiles) {
-          ++z;
+        bool skip_rest = true;  // no instance after i sweeps size s
+        for (size_t k = i; k < count && skip_rest; ++k)
+          skip_rest = !(S[k].active && next_sz[k] == s);
+        if (i < count && !skip_rest) {
+          cursor = i;  // the group ends inside size s
           break;
         }
+        cursor = 0;
+        if (pending_at(s)) break;  // size s takes another pass
+        ++s;
+        if (group_tiles >= kGroupTiles) break;
       }
-      const int z_end = z - 1;
+      if (int rc = flush_tables()) return rc;
+      if (o.time_budget_s > 0) {
+        // an instance inside a size's tiles settles no size it has not
+        // swept whole: settled = -(that size) bounds the settle scan below it
+        for (size_t k = 0; k < count; ++k)
+          if (settled[k] <= 0) settled[k] = S[k].active && part_t[k] > 0 ? -next_sz[k] : 0;
+        MBA_HIP(hipMemcpyAsync(d_settled, settled.data(), count * 4, hipMemcpyHostToDevice,
+                               stq));
+      }
       const auto t_g = Clock::now();
       if (nj > 0) {
         if (int rc = run_group(nsz, nj)) return rc;
         if (h_cnt->overflow > 0) {
-          // a step's hits overflowed the buffer: run the group's steps again
+          // step k0 was the first whose hits overflowed the buffer: the
+          // settled sizes are rebuilt from before the group through step
+          // k0 - 1 (their hits were all resolved), and steps k0.. run again
           // one at a time, in parts (resolving any subset of the hits only
-          // lowers slots to verified ranks), from the settled sizes of
-          // before the group (a size settled on part of its hits may not
-          // hold its minimum rank yet)
-          MBA_HIP(hipMemsetAsync(&d_cnt->overflow, 0, 4, stq));
+          // lowers slots to verified ranks; a size settled on part of its
+          // hits may not hold its minimum rank yet)
+          const int k0 = (int)std::min<uint32_t>(h_cnt->first_overflow, (uint32_t)nsz);
           MBA_HIP(hipMemcpyAsync(d_settled, settled.data(), count * 4, hipMemcpyHostToDevice,
                                  stq));
-          for (int k = 0; k < nsz; ++k) {
+          for (int k = 0; k < k0; ++k)
+            MBA_HIP(launch_batch_settle(d_steps + k, d_found, d_settled, (uint32_t)count,
+                                        min_size, d_cnt, hit_cap, stq));
+          for (int k = k0; k < nsz; ++k) {
             const BatchStep hs = h_steps[k];
             if (hs.njobs == 0) continue;
-            if (int rc = run_parts(hs.job_off, hs.job_off + hs.njobs, std::max(0, hs.size_done - 1)))
+            parts_lo = hs.inst_lo;
+            parts_hi = hs.inst_hi;
+            if (int rc = run_parts(hs.job_off, hs.job_off + hs.njobs,
+                                   std::max(0, hs.size_done - 1)))
               return rc;
             // settle the complete size
             h_steps[kMaxSize + 1] = hs;
@@ -576,14 +741,18 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
         }
       }
       if (o.verbose)
-        std::fprintf(stderr, "batch: sizes %d..%d in one launch sequence: %zu jobs, %llu tiles, "
-                     "%.3f ms\n", s, z_end, nj, (unsigned long long)group_tiles,
-                     1e3 * seconds_since(t_g));
-      charge(seconds_since(t_g), group_tiles);
+        std::fprintf(stderr, "batch: sizes %d..%d in one launch sequence (%d steps): %zu jobs, "
+                     "%llu tiles, %.3f ms, last step %u hits, replays %d\n", s0,
+                     cursor ? s : s - 1, nsz, nj, (unsigned long long)group_tiles,
+                     1e3 * seconds_since(t_g), h_cnt->hits, replays);
+      const double t_seq = seconds_since(t_g);
+      if (group_tiles >= 1024 && t_seq > 0) tile_rate = (double)group_tiles / t_seq;
+      charge(t_seq, group_tiles);
       if (int rc = collect()) return rc;
-      for (size_t i = 0; i < count; ++i)
-        if (S[i].active) S[i].reached = z_end;
-      s = z_end + 1;
+      for (int k = 0; k < nsz; ++k)  // sizes completed by the group's steps
+        for (size_t i = h_steps[k].inst_lo; i < h_steps[k].inst_hi; ++i)
+          if (S[i].active && h_steps[k].size_done > 0 && next_sz[i] > step_size[k])
+            S[i].reached = std::max(S[i].reached, h_steps[k].size_done);
       continue;
     }
     // ---- a learned size: rounds of the best-scored split, the next split,
@@ -612,6 +781,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       const size_t rb = round == 0 ? 0 : kRoundEnd[round - 1];
       const size_t re = kRoundEnd[round];
       size_t nj = 0;
+      blkv.resize(1);
       uint64_t tiles = 0;
       std::vector<int> sub;
       for (size_t i = 0; i < count; ++i) {
@@ -631,22 +801,31 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
         hs.job_off = 0;
         hs.njobs = (uint32_t)nj;
         hs.size_done = s;  // any verified tree of this size settles
+        hs.inst_lo = 0;
+        hs.inst_hi = (uint32_t)count;
+        if (int rc = flush_tables()) return rc;
         const auto t_g = Clock::now();
         if (int rc = run_group(1, nj)) return rc;
         if (h_cnt->overflow > 0) {
           MBA_HIP(hipMemsetAsync(&d_cnt->overflow, 0, 4, stq));
+          parts_lo = 0;
+          parts_hi = (uint32_t)count;
           if (int rc = run_parts(0, nj, s)) return rc;
         }
         if (o.verbose)
-          std::fprintf(stderr, "batch: size %d round %d: %zu jobs, %llu tiles, %.3f ms\n", s,
-                       round, nj, (unsigned long long)tiles, 1e3 * seconds_since(t_g));
+          std::fprintf(stderr, "batch: size %d round %d: %zu jobs, %llu tiles, %.3f ms, "
+                       "%u hits, replays %d\n", s, round, nj, (unsigned long long)tiles,
+                       1e3 * seconds_since(t_g), h_cnt->hits, replays);
         charge(seconds_since(t_g), tiles);
         if (int rc = collect()) return rc;
       }
       if (re >= max_blocks) break;
     }
     for (size_t i = 0; i < count; ++i)
-      if (S[i].active) S[i].reached = s;
+      if (S[i].active) {
+        S[i].reached = s;
+        next_sz[i] = s + 1;
+      }
     ++s;
   }
 

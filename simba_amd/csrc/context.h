@@ -53,7 +53,7 @@ struct Ctx {
   DevBuf vfb_rows, vfb_left, vfb_right, vfb_keys, vfb_idx, vfb_flags, vfb_misc,
       vfb_counts, vfb_target;
   // batched / learned
-  DevBuf b_inst, b_data, b_queue, b_results, b_misc, mlp_w, mlp_misc;
+  DevBuf b_inst, b_data, b_queue, b_results, b_misc, b_tvup, mlp_w, mlp_misc;
   // instance features on the device (packed inputs/outputs, layout, rows)
   DevBuf f_in, f_out, f_meta, f_x;
   // pinned host staging for the per-step read backs of the search drivers
@@ -66,7 +66,7 @@ struct Ctx {
   size_t b_pinned_bytes = 0;
   hipGraph_t b_graph = nullptr;
   hipGraphExec_t b_gexec = nullptr;
-  static constexpr int kGraphKey = 15;
+  static constexpr int kGraphKey = 17;
   uint64_t b_graph_key[kGraphKey] = {};
 };
 

@@ -54,7 +54,7 @@ static BBlock shape_block(BBlock b, int rb) {
   return b;
 }
 
-static uint64_t block_tiles(const BBlock &b, int rb) {
+uint64_t block_tiles(const BBlock &b, int rb) {
   const uint64_t tile = (uint64_t)kPairThreads * rb;
   if (b.generic) return (b.nA + tile - 1) / tile;
   const uint64_t nY = b.trans ? b.nB : b.nA;
@@ -71,9 +71,9 @@ static uint64_t block_tiles(const BBlock &b, int rb) {
 // generic block.
 constexpr uint64_t kFixedSiblings = 64;
 
-static void b_blocks(const CountTable &T, int Kt, const uint64_t *off, int m, uint64_t start,
-                     uint64_t stride, const std::vector<BStep> &steps, int rb,
-                     std::vector<BBlock> &out) {
+void b_blocks(const CountTable &T, int Kt, const uint64_t *off, int m, uint64_t start,
+              uint64_t stride, const std::vector<BStep> &steps, int rb,
+              std::vector<BBlock> &out) {
   auto emit = [&](BBlock b, const std::vector<BStep> &st) {
     b.stride = stride;
     if ((int)st.size() > kMaxBSteps && !b.generic) {
@@ -563,24 +563,36 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   };
 
   // A fused step as one persistent launch (k_batch_step over the step's
-  // splits as jobs, narrow A tiles for the small ones).  Returns 1 when its
-  // hits were resolved, 0 when the caller must run the splits one launch at
-  // a time (too many jobs for the staging area, or hits that overflowed the
-  // buffer: those resolved from the truncated buffer stand, as they only
-  // lower slots to verified ranks), < 0 on an error.
+  // splits as jobs, narrow A tiles for the small ones; a B side above Kt --
+  // V[max_size - 2], the B side of the leaves -- as one job per B block).
+  // Returns 1 when its hits were resolved; otherwise the caller runs the
+  // splits one launch at a time: 0 when nothing was launched (more jobs and
+  // blocks than the staging area holds; the buffer keeps its hits), 2 when
+  // the hits overflowed the buffer (those resolved from the truncated buffer
+  // stand, as they only lower slots to verified ranks; the buffer is empty);
+  // < 0 on an error.
+  std::vector<BatchJob> f_jobs;
+  std::vector<BBlock> f_blocks, f_tmp;
   auto fused_persistent = [&](int s0, int s1) -> int {
     if (int rc = ensure_tables(s1 - 2)) return rc;
-    constexpr size_t kStageOff = 4096;
-    constexpr size_t kMaxJobs = (Ctx::kPinnedBytes - kStageOff - 256) / sizeof(BatchJob);
-    auto *h_step = (BatchStep *)((char *)ctx.pinned + kStageOff);
-    auto *h_jobs = (BatchJob *)((char *)ctx.pinned + kStageOff + 256);
-    size_t nj = 0;
+    f_jobs.clear();
+    f_blocks.assign(1, BBlock{});  // block 0: named by jobs with a tabulated B side
     uint64_t tiles = 0;
+    auto push = [&](BatchJob J, uint64_t ntb) {
+      J.p.nTB = ntb;
+      J.p.ta = kPairTA;
+      while (J.p.ta > kMinTA && (J.p.nA + J.p.ta - 1) / J.p.ta * ntb < kSmallLaunchTiles)
+        J.p.ta /= 2;
+      J.p.nTA = (J.p.nA + J.p.ta - 1) / J.p.ta;
+      J.ntiles = J.p.nTA * ntb;
+      J.tile0 = tiles;
+      tiles += J.ntiles;
+      f_jobs.push_back(J);
+    };
     for (int z = s0; z <= s1; ++z) {
       const CheckTargets tgz = targets_for(t0, z, max_size, W);
       for (int j = 1; j <= (z - 1) / 2; ++j) {
-        if (nj == kMaxJobs) return 0;
-        BatchJob &J = h_jobs[nj++];
+        BatchJob J;
         std::memset(&J, 0, sizeof J);
         J.p.sb = z;
         J.p.j = j;
@@ -591,41 +603,54 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
         J.p.gen_b = J.p.m > Kt;
         J.p.mask = kAllInst;
         J.p.rb = kPairRB;
-        const uint64_t ntb = (J.p.nB + kPairTB - 1) / kPairTB;
-        J.p.ta = kPairTA;
-        while (J.p.ta > kMinTA && (J.p.nA + J.p.ta - 1) / J.p.ta * ntb < kSmallLaunchTiles)
-          J.p.ta /= 2;
-        J.p.nTA = (J.p.nA + J.p.ta - 1) / J.p.ta;
-        J.ntiles = J.p.nTA * ntb;
         J.lo_e = tgz.lo_even;
         J.lo_o = tgz.lo_odd;
         J.span = tgz.span;
         J.kind = 1;
-        J.tile0 = tiles;
-        tiles += J.ntiles;
+        if (!J.p.gen_b) {
+          push(J, (J.p.nB + kPairTB - 1) / kPairTB);
+          continue;
+        }
+        f_tmp.clear();
+        b_blocks(T, Kt, tv.off, J.p.m, 0, 1, {}, kPairRB, f_tmp);
+        for (const BBlock &b : f_tmp) {
+          J.bb = (uint32_t)f_blocks.size();
+          f_blocks.push_back(b);
+          push(J, block_tiles(b, kPairRB));
+        }
       }
     }
+    const size_t nj = f_jobs.size();
     if (nj == 0) return 0;
+    // staging (pinned, then one copy to the device): step header, jobs,
+    // blocks
+    constexpr size_t kStageOff = 4096;
+    const size_t off_blk = (256 + sizeof(BatchJob) * nj + 255) / 256 * 256;
+    const size_t bytes = off_blk + sizeof(BBlock) * f_blocks.size();
+    if (kStageOff + bytes > Ctx::kPinnedBytes) return 0;
+    char *stage = (char *)ctx.pinned + kStageOff;
+    auto *h_step = (BatchStep *)stage;
+    std::memset(h_step, 0, sizeof *h_step);
     h_step->total_tiles = tiles;
-    h_step->job_off = 0;
     h_step->njobs = (uint32_t)nj;
-    h_step->size_done = 0;
-    MBA_HIP(ctx.b_queue.ensure(256 + sizeof(BatchJob) * kMaxJobs));
-    MBA_HIP(hipMemcpyAsync(ctx.b_queue.p, h_step, 256 + sizeof(BatchJob) * nj,
-                           hipMemcpyHostToDevice, st));
+    std::memcpy(stage + 256, f_jobs.data(), sizeof(BatchJob) * nj);
+    std::memcpy(stage + off_blk, f_blocks.data(), sizeof(BBlock) * f_blocks.size());
+    MBA_HIP(ctx.b_queue.ensure(Ctx::kPinnedBytes - kStageOff));
+    MBA_HIP(hipMemcpyAsync(ctx.b_queue.p, stage, bytes, hipMemcpyHostToDevice, st));
     MBA_HIP(hipMemsetAsync(d_head, 0, 8, st));
     MBA_HIP(hipEventRecord(ctx.ev0, st));
     const int g = (int)std::min<uint64_t>((uint64_t)ctx.cu_count * 8, tiles);
     MBA_HIP(launch_batch_step(W, ctx.b_queue.as<BatchStep>(),
-                              (const BatchJob *)(ctx.b_queue.as<char>() + 256), d_tv, d_head,
-                              d_hits, d_hc, kHitCap, nullptr, g, st));
+                              (const BatchJob *)(ctx.b_queue.as<char>() + 256),
+                              (const BBlock *)(ctx.b_queue.as<char>() + off_blk), d_tv,
+                              d_head, d_hits, d_hc, kHitCap, nullptr, g, st));
     MBA_HIP(hipEventRecord(ctx.ev1, st));
     ++klaunch;
     if (int rc = process_hits(true)) return rc;  // syncs; rc < 0
     float ms = 0.f;
     MBA_HIP(hipEventElapsedTime(&ms, ctx.ev0, ctx.ev1));
     ktime += ms * 1e-3;
-    return *(const uint32_t *)ctx.pinned > kHitCap ? 0 : 1;
+    return *(const uint32_t *)ctx.pinned > kHitCap ? 2 : 1;
   };
 
   // Step 1: leaves (size 1) and, through unary chains, every unary tree over a
@@ -814,8 +839,20 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
       // side by side; a step whose hits overflow the buffer (or with more
       // jobs than the staging holds) runs as one launch per split instead.
       int rc = 0;
-      const int prc = fused_persistent(s, last);
+      int prc = fused_persistent(s, last);
       if (prc < 0) return prc;
+      if (prc == 2) {
+        // the persistent launch overflowed the hit buffer: the leaf scan's
+        // hits (resolved only up to the buffer's capacity) are scanned
+        // again on their own, then the splits run one launch at a time
+        ++replays;
+        if (leaf_pending) {
+          MBA_HIP(launch_leaf_scan(W, d_tv, L, targets_for(t0, 1, max_size, W), d_hits, d_hc,
+                                   kHitCap, st));
+          if (int lrc = process_hits()) return lrc;  // leaf hits above the buffer: an error
+        }
+        prc = 0;
+      }
       if (prc == 0) {
         for (int z = s; z <= last && rc == 0; ++z) {
           const CheckTargets tgz = targets_for(t0, z, max_size, W);

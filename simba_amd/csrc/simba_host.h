@@ -28,6 +28,14 @@ constexpr uint64_t kBlockModeMin = 1ull << 31;
 // The same threshold for the learned order, whose point is to stop early.
 constexpr uint64_t kLearnedBlockMin = 1ull << 26;
 
+// The B-side trees of size m > Kt in rank order as blocks formed from the
+// tables at element offsets off[] (see b_blocks in simba_host.cpp), for
+// tiles of 256 * rb B values; block_tiles: the tiles of one block.
+void b_blocks(const CountTable &T, int Kt, const uint64_t *off, int m, uint64_t start,
+              uint64_t stride, const std::vector<BStep> &steps, int rb,
+              std::vector<BBlock> &out);
+uint64_t block_tiles(const BBlock &b, int rb);
+
 // Windows for the checks of base size sb (chains up to max_size).
 CheckTargets targets_for(uint64_t t0, int sb, int max_size, int W);
 // Left child size of the root of an RPN (0 for unary roots and leaves).

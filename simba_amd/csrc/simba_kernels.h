@@ -98,26 +98,44 @@ constexpr int kBuildTile = 1024;  // table values per build tile
 struct BatchJob {
   uint64_t tile0, ntiles;
   uint64_t lo_e, lo_o, span;  // windows (kinds 1 and 2)
-  PairCore p;                 // B sides above Kt are stepped per tile (no blocks)
+  PairCore p;                 // p.nTB: tiles of the B block when p.gen_b
   uint32_t inst;
   uint32_t kind;
+  uint32_t bb;                // B block of a B side above Kt (index in the step's blocks)
+  uint32_t pad;
 };
 
 // One size step of a batch: its jobs are jobs_base[job_off, job_off + njobs)
 // (global tiles [0, total_tiles)); once its hits are resolved, every size up
-// to size_done is complete for the step's instances.
+// to size_done is complete for the instances [inst_lo, inst_hi).
 struct BatchStep {
   uint64_t total_tiles;
   uint64_t job_off;
   uint32_t njobs;
   int32_t size_done;
+  uint32_t inst_lo, inst_hi;
+  uint32_t index;  // position of the step in its launch sequence
+  uint32_t pad;
 };
 
 // Device counters of a batch step: window hits, screened candidates,
-// lowered slots, and the largest hit count of a step above the buffer.
+// lowered slots; across a launch sequence, the largest hit count of a step
+// above the buffer and the first step that overflowed it (~0: none).
 struct BatchCounters {
-  uint32_t hits, surv, touched, overflow;
+  uint32_t hits, surv, touched, overflow, first_overflow, pad[3];
 };
+
+// A value table V[m] placed at element `off` of a batch's table pool for
+// instance `inst`, whose largest tabulated size is then `kt`.
+struct TvUpdate {
+  uint32_t inst, m;
+  uint64_t off;
+  int32_t kt, pad;
+};
+// Applies n table placements to the device table views (records of one
+// instance carry the same kt).
+hipError_t launch_tv_update(TableView *d_tvs, const TvUpdate *d_up, uint32_t n,
+                            hipStream_t stream);
 
 struct BatchInstData {
   uint64_t leaf_off;   // [sample][leaf] values of this instance in the batch array
@@ -135,10 +153,14 @@ struct VerifyCand {
 // counts (the overflow record is kept).
 hipError_t launch_batch_reset(unsigned long long *d_head, BatchCounters *d_cnt,
                               hipStream_t stream);
-// After a step's hits are resolved: settled[i] = the smallest size z in
-// [min_size, step->size_done] whose slot d_found[i * (kMaxSize + 1) + z] holds
-// a verified rank (instances already settled keep theirs), and the step's
-// hit count is recorded in d_cnt->overflow when it exceeded the buffer.
+// After a step's hits are resolved: for i in [step->inst_lo, step->inst_hi),
+// settled[i] = the smallest size z in [min_size, step->size_done] whose slot
+// d_found[i * (kMaxSize + 1) + z] holds a verified rank (instances already
+// settled, settled[i] > 0, keep theirs; settled[i] = -y < 0 bounds z below
+// y: the instance has swept part of size y only), and the step's hit count
+// is recorded in d_cnt->overflow (first_overflow: the step's index) when it
+// exceeded the buffer.  The grid covers `count` instances (the largest
+// range).
 hipError_t launch_batch_settle(const BatchStep *d_step, const unsigned long long *d_found,
                                int32_t *d_settled, uint32_t count, int min_size,
                                BatchCounters *d_cnt, uint32_t hit_cap, hipStream_t stream);
@@ -146,11 +168,14 @@ hipError_t launch_batch_settle(const BatchStep *d_step, const unsigned long long
 hipError_t launch_batch_leaves(int width, const TableView *d_tvs, const uint64_t *d_leafvals,
                                const BatchInstData *d_idat, uint32_t count,
                                hipStream_t stream);
-// The persistent kernel of a step: jobs d_jobs_base + step->job_off; jobs of
-// instances with d_settled[inst] > 0 are skipped (d_settled may be NULL).
+// The persistent kernel of a step: jobs d_jobs_base + step->job_off, their
+// B blocks d_blocks[job.bb] (at least one element: table jobs name block
+// 0); jobs of instances with d_settled[inst] > 0 are skipped (d_settled may
+// be NULL).
 hipError_t launch_batch_step(int width, const BatchStep *d_step, const BatchJob *d_jobs_base,
-                             const TableView *d_tvs, unsigned long long *d_head,
-                             HitRec *d_hits, uint32_t *d_hit_count, uint32_t hit_cap,
+                             const BBlock *d_blocks, const TableView *d_tvs,
+                             unsigned long long *d_head, HitRec *d_hits,
+                             uint32_t *d_hit_count, uint32_t hit_cap,
                              const int32_t *d_settled, int grid, hipStream_t stream);
 // Full verification of (instance, size, rank) candidates on all samples, one
 // wave per candidate (lane per sample, leaf values and RPN stacks in LDS, exit

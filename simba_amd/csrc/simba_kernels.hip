@@ -343,99 +343,6 @@ __device__ __forceinline__ uint64_t uniform(uint64_t x) {
          __builtin_amdgcn_readfirstlane((uint32_t)x);
 }
 
-// Values of V[m] for m = Kt + 1 (children tabulated) at the B indices of a
-// tile: the tile's first index `first` is decoded once (op block, split,
-// left/right rank -- the same for every lane), and each lane steps from it:
-// inside one (op, split) block whose right side has at least a tile's worth
-// of trees, the lane's right rank is first's plus its offset with at most
-// one carry into the left rank.  Lanes past that block decode on their own.
-template <typename word, int RB>
-__device__ __forceinline__ void gen_b_tile(const TableView *__restrict__ tv, int m,
-                                           uint64_t first, uint64_t lane_b0, word *bv,
-                                           const bool *ok) {
-  const word *V = global_ptr<const word>(tv->base);
-  const int cs = m - 1;
-  const uint64_t last = first + (uint64_t)(kPairThreads * RB);  // end of the tile
-  bool todo[RB];
-#pragma unroll
-  for (int k = 0; k < RB; ++k) {
-    todo[k] = ok[k];
-    bv[k] = word(0);
-  }
-  // the (op, split) block of the tile's first tree and its offset there
-  uint64_t x = first;
-  int op = 0;
-  while (op < 8 && x >= tv->T[m][op]) x -= tv->T[m][op++];
-  int j = 1;
-  if (op < 8 && !is_unary(op)) {
-    for (;; ++j) {
-      const uint64_t blk = tv->T[j][8] * tv->T[cs - j][8];
-      if (x < blk) break;
-      x -= blk;
-    }
-  }
-  // every block the tile touches, in rank order (workgroup-uniform)
-  uint64_t base = first;
-  while (op < 8 && base < last) {
-    const bool un = is_unary(op);
-    uint64_t rem, rs = 1, a0 = 0, r0 = 0;
-    const word *VA, *VB;
-    if (un) {
-      rem = tv->T[m][op] - x;
-      VA = V + tv->off[m - 1];
-      VB = V;
-    } else {
-      rem = tv->T[j][8] * tv->T[cs - j][8] - x;
-      rs = tv->T[cs - j][8];
-      a0 = x / rs;
-      r0 = x - a0 * rs;
-      VA = V + tv->off[j];
-      VB = V + tv->off[cs - j];
-    }
-    // a tile spans at most two rows of the block when its right side has at
-    // least a tile of trees (one carry); otherwise (ra, rb) by a 32-bit
-    // division (r0 + off < rs + tile < 2^32)
-    const bool one_carry = un || rs >= (uint64_t)(kPairThreads * RB);
-    const uint32_t rs32 = (uint32_t)rs;
-#pragma unroll
-    for (int k = 0; k < RB; ++k) {
-      if (!todo[k]) continue;
-      const uint64_t off = lane_b0 + (uint64_t)k * kPairThreads - base;
-      if (off >= rem) continue;  // a later block
-      todo[k] = false;
-      if (un) {
-        bv[k] = apply_op<word>(op, VA[x + off], word(0));
-      } else {
-        uint64_t rb, ra;
-        if (one_carry) {
-          rb = r0 + off;
-          ra = a0;
-          if (rb >= rs) {
-            rb -= rs;
-            ra += 1;
-          }
-        } else {
-          const uint32_t t = (uint32_t)(r0 + off);
-          const uint32_t q = t / rs32;
-          rb = t - q * rs32;
-          ra = a0 + q;
-        }
-        bv[k] = apply_op<word>(op, VA[ra], VB[rb]);
-      }
-    }
-    // the next block: the next split of this op, else the next op
-    base += rem;
-    x = 0;
-    if (!un && j < (op == SUB ? m - 2 : (m - 1) / 2)) {
-      ++j;
-      continue;
-    }
-    j = 1;
-    ++op;
-    while (op < 8 && tv->T[m][op] == 0) ++op;
-  }
-}
-
 // Per-A values of a tile, staged in LDS by the workgroup (one A value per
 // lane) and read back as wave-uniform broadcasts inside the loop:
 //   exact mode:   q0 = {a, a ^ t, t - a, a - t},
@@ -532,7 +439,7 @@ __device__ __forceinline__ bool stage_a(AStage &S, const TableView *__restrict__
 }
 
 // ---- B side sources --------------------------------------------------------
-enum : int { kBTable = 0, kBDesc = 1, kBGeneric = 2, kBStep = 3 };
+enum : int { kBTable = 0, kBDesc = 1, kBGeneric = 2 };
 
 template <typename word>
 __device__ __forceinline__ uint32_t lo32_at(const word *p, uint64_t i) {
@@ -664,9 +571,9 @@ __device__ __forceinline__ void desc_lo(const BBlock &bb, const word *V, uint64_
 
 // Full B value and rank (within size m) of the k-th value of this lane
 // (used by the exact re-walk only).
-template <typename word, int BSRC, int RB>
+template <typename word, int BSRC, int RB, typename PL>
 __device__ __forceinline__ word b_full(const TableView *__restrict__ tv,
-                                       const word *__restrict__ B, const PairLaunch &p,
+                                       const word *__restrict__ B, const PL &p,
                                        uint64_t bt, int k, uint64_t &rank) {
   if (BSRC == kBDesc) {
     uint64_t ra, rb;
@@ -685,9 +592,9 @@ __device__ __forceinline__ word b_full(const TableView *__restrict__ tv,
 }
 
 // Low halves of this lane's B values of tile bt (and their validity).
-template <typename word, int BSRC, int RB>
+template <typename word, int BSRC, int RB, typename PL>
 __device__ __forceinline__ void b_lo(const TableView *__restrict__ tv,
-                                     const word *__restrict__ B, const PairLaunch &p,
+                                     const word *__restrict__ B, const PL &p,
                                      uint64_t bt, uint32_t *bl, bool *ok) {
   if (BSRC == kBDesc) {
     desc_lo<word, RB>(p.bb, global_ptr<const word>(tv->base), bt, bl, ok);
@@ -695,21 +602,6 @@ __device__ __forceinline__ void b_lo(const TableView *__restrict__ tv,
   }
   const uint64_t b0 = bt * (kPairThreads * RB) + threadIdx.x;
   const uint64_t n = BSRC == kBGeneric ? p.bb.nA : p.nB;
-  if (BSRC == kBStep) {
-    word bv[RB];
-#pragma unroll
-    for (int k = 0; k < RB; ++k) ok[k] = b0 + (uint64_t)k * kPairThreads < n;
-    if (p.m == tv->Kt + 1) {
-      gen_b_tile<word, RB>(tv, p.m, bt * (kPairThreads * RB), b0, bv, ok);
-    } else {
-#pragma unroll
-      for (int k = 0; k < RB; ++k)
-        bv[k] = ok[k] ? value_of<word>(tv, p.m, b0 + (uint64_t)k * kPairThreads) : word(0);
-    }
-#pragma unroll
-    for (int k = 0; k < RB; ++k) bl[k] = lo32(bv[k]);
-    return;
-  }
 #pragma unroll
   for (int k = 0; k < RB; ++k) {
     const uint64_t b = b0 + (uint64_t)k * kPairThreads;
@@ -933,9 +825,9 @@ __device__ __forceinline__ void pair_rewalk(const AStage &S, uint32_t na, uint64
 // column values are loaded once (low halves) and combined with each of its
 // rows (up to bb.rows_pt block rows, one uniform load each), so a column
 // value read from memory serves every row of the tile.
-template <typename word, bool SAME, int RB>
+template <typename word, bool SAME, int RB, typename PL>
 __device__ __forceinline__ void desc_band_tile(const TableView *__restrict__ tv,
-                                               const AStage &S, const PairLaunch &p,
+                                               const AStage &S, const PL &p,
                                                uint32_t na, uint64_t a0, uint64_t bt,
                                                bool bitops, word lo_e, word lo_o, word span,
                                                HitRec *__restrict__ hits,
@@ -1020,10 +912,10 @@ __device__ __forceinline__ void desc_band_tile(const TableView *__restrict__ tv,
 // The A values of the tile must be staged in S (stage_a).  p.mask selects the
 // operator instances (bit i: instance i, see HitRec); `tag` is or-ed into
 // HitRec::inst above bit 8 (the batch's instance index).
-template <typename word, bool SAME, int BSRC, int RB>
+template <typename word, bool SAME, int BSRC, int RB, typename PL>
 __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
                                           const AStage &S, const word *__restrict__ A,
-                                          const word *__restrict__ B, const PairLaunch &p,
+                                          const word *__restrict__ B, const PL &p,
                                           uint64_t at, uint64_t bt, bool bitops, word lo_e,
                                           word lo_o, word span, HitRec *__restrict__ hits,
                                           uint32_t *__restrict__ hit_count, uint32_t hit_cap,
@@ -1032,20 +924,20 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
   const uint64_t a1 = a0 + p.ta < p.nA ? a0 + p.ta : p.nA;
   const uint32_t na = (uint32_t)(a1 - a0);
   if (BSRC == kBDesc && (1u << p.bb.logTB) == (uint32_t)(kPairThreads * RB)) {
-    desc_band_tile<word, SAME, RB>(tv, S, p, na, a0, bt, bitops, lo_e, lo_o, span, hits,
-                                   hit_count, hit_cap, tag);
+    desc_band_tile<word, SAME, RB, PL>(tv, S, p, na, a0, bt, bitops, lo_e, lo_o, span, hits,
+                                       hit_count, hit_cap, tag);
     return;
   }
   uint32_t bl[RB];
   bool ok[RB];
-  b_lo<word, BSRC, RB>(tv, B, p, bt, bl, ok);
+  b_lo<word, BSRC, RB, PL>(tv, B, p, bt, bl, ok);
   const bool hit = pair_prefilter<SAME, RB>(S, na, bl, ok, p.mask, bitops, lo32(lo_e),
                                             lo32(lo_o), (uint32_t)span);
   if (__any(hit) && hit) {
     const uint32_t pair = ((uint32_t)p.sb << 16) | (uint32_t)p.j;
     pair_rewalk<word, SAME, RB>(S, na, a0, ok, p.mask, lo_e, lo_o, span, pair, tag, hits,
                                 hit_count, hit_cap, [&](int k, uint64_t &rank) -> word {
-                                  return b_full<word, BSRC, RB>(tv, B, p, bt, k, rank);
+                                  return b_full<word, BSRC, RB, PL>(tv, B, p, bt, k, rank);
                                 });
   }
 }
@@ -1070,8 +962,8 @@ k_pair_check(const TableView *__restrict__ tv, const word *__restrict__ A,
       bitops = stage_a<word>(S, tv, A, p.gen_a, p.j, a0, a1, mode, le, lo, (uint32_t)span);
       staged = at;
     }
-    pair_tile<word, SAME, BSRC, RB>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o, span, hits,
-                                    hit_count, hit_cap, 0u);
+    pair_tile<word, SAME, BSRC, RB, PairLaunch>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o,
+                                                span, hits, hit_count, hit_cap, 0u);
   }
 }
 
@@ -1420,13 +1312,19 @@ k_rtid_scan(const TableView *__restrict__ tv, const uint64_t *__restrict__ leafv
 // staged A tile while they do not change.  Every workgroup leaves once the
 // counter passes the step's total, so the grid drains.
 // ---------------------------------------------------------------------------
+// A batch job's pair core with its B block read in place (the job's block
+// in the step's block array; every lane of the workgroup reads the same one).
+struct BatchPair : PairCore {
+  const BBlock &bb;
+  __device__ BatchPair(const PairCore &c, const BBlock &b) : PairCore(c), bb(b) {}
+};
+
 template <typename word>
-__device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J, uint64_t lt,
-                           uint64_t &staged, bool &bitops, HitRec *hits, uint32_t *hc,
-                           uint32_t cap) {
+__device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J,
+                           const BBlock *__restrict__ blocks, uint64_t lt, uint64_t &staged,
+                           bool &bitops, HitRec *hits, uint32_t *hc, uint32_t cap) {
   const word *base = global_ptr<const word>(tv->base);
-  PairLaunch p;
-  static_cast<PairCore &>(p) = J.p;  // p.bb is not read by the table/stepped sources
+  const BatchPair p(J.p, blocks[J.bb]);  // the table source reads no block
   const word *A = base + (p.gen_a ? 0 : tv->off[p.j]);
   const word *B = base + (p.gen_b ? 0 : tv->off[p.m]);
   const word lo_e = (word)J.lo_e, lo_o = (word)J.lo_o, span = (word)J.span;
@@ -1440,27 +1338,29 @@ __device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J, ui
                            lo32(lo_o), (uint32_t)span);
     staged = at;
   }
+  // the B side: its table, a block formed from tables, or a generic rank range
+  const int bs = !p.gen_b ? kBTable : p.bb.generic ? kBGeneric : kBDesc;
+#define BATCH_TILE(SAME_, BS)                                                              \
+  pair_tile<word, SAME_, BS, kPairRB, BatchPair>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o, \
+                                                 span, hits, hc, cap, tag)
   if (p.j == p.m) {
-    if (p.gen_b)
-      pair_tile<word, true, kBStep, kPairRB>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o, span,
-                                             hits, hc, cap, tag);
-    else
-      pair_tile<word, true, kBTable, kPairRB>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o, span,
-                                              hits, hc, cap, tag);
-  } else if (p.gen_b) {
-    pair_tile<word, false, kBStep, kPairRB>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o, span,
-                                            hits, hc, cap, tag);
+    if (bs == kBTable) BATCH_TILE(true, kBTable);
+    else if (bs == kBDesc) BATCH_TILE(true, kBDesc);
+    else BATCH_TILE(true, kBGeneric);
   } else {
-    pair_tile<word, false, kBTable, kPairRB>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o, span,
-                                             hits, hc, cap, tag);
+    if (bs == kBTable) BATCH_TILE(false, kBTable);
+    else if (bs == kBDesc) BATCH_TILE(false, kBDesc);
+    else BATCH_TILE(false, kBGeneric);
   }
+#undef BATCH_TILE
 }
 
 template <typename word>
 __global__ void __launch_bounds__(kPairThreads)
 k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jobs_base,
-             const TableView *__restrict__ tvs, unsigned long long *head, HitRec *hits,
-             uint32_t *hit_count, uint32_t hit_cap, const int32_t *__restrict__ settled) {
+             const BBlock *__restrict__ blocks, const TableView *__restrict__ tvs,
+             unsigned long long *head, HitRec *hits, uint32_t *hit_count, uint32_t hit_cap,
+             const int32_t *__restrict__ settled) {
   __shared__ unsigned long long s_t;
   __shared__ AStage S;
   const uint64_t total = step->total_tiles;
@@ -1508,7 +1408,7 @@ k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jo
         for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x)
           out[r] = build_value<word>(tv, J.p.m, r);
       } else if (J.kind == 1) {
-        batch_pair<word>(S, tv, J, lt, staged, bitops, hits, hit_count, hit_cap);
+        batch_pair<word>(S, tv, J, blocks, lt, staged, bitops, hits, hit_count, hit_cap);
       } else {  // leaves against the windows of base size 1
         const word *V = global_ptr<const word>(tv->base) + tv->off[1];
         if ((int)threadIdx.x < tv->L) {
@@ -1534,8 +1434,26 @@ __global__ void k_batch_leaves(const TableView *__restrict__ tvs,
   const uint32_t l = threadIdx.x;
   if (l < (uint32_t)tv.L) {
     const BatchInstData &d = idat[i];
-    ((word *)tv.base)[l] = (word)leafvals[d.leaf_off + (uint64_t)d.fs * tv.L + l];
+    ((word *)tv.base)[tv.off[1] + l] = (word)leafvals[d.leaf_off + (uint64_t)d.fs * tv.L + l];
   }
+}
+
+__global__ void __launch_bounds__(256)
+k_tv_update(TableView *__restrict__ tvs, const TvUpdate *__restrict__ up, uint32_t n) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const TvUpdate u = up[k];
+    tvs[u.inst].off[u.m] = u.off;
+    tvs[u.inst].Kt = u.kt;
+  }
+}
+
+hipError_t launch_tv_update(TableView *d_tvs, const TvUpdate *d_up, uint32_t n,
+                            hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  const uint32_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_tv_update, dim3(blocks < 1024 ? blocks : 1024), dim3(256), 0, stream,
+                     d_tvs, d_up, n);
+  return hipGetLastError();
 }
 
 __global__ void k_batch_reset(unsigned long long *head, BatchCounters *cnt) {
@@ -1553,18 +1471,25 @@ k_batch_settle(const BatchStep *__restrict__ step, const unsigned long long *__r
                BatchCounters *cnt, uint32_t hit_cap) {
   const int done = step->size_done;
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
-    if (settled[i] > 0) continue;
+  const uint32_t hi = step->inst_hi < count ? step->inst_hi : count;
+  for (uint32_t i = step->inst_lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi;
+       i += stride) {
+    const int st = settled[i];
+    if (st > 0) continue;
+    // st = -z: the instance has swept the sizes below z only
+    const int lim = st < 0 && -st - 1 < done ? -st - 1 : done;
     const unsigned long long *f = found + (size_t)i * (kMaxSize + 1);
-    for (int z = min_size < 1 ? 1 : min_size; z <= done; ++z) {
+    for (int z = min_size < 1 ? 1 : min_size; z <= lim; ++z) {
       if (f[z] != ~0ull) {
         settled[i] = z;
         break;
       }
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0 && cnt->hits > hit_cap)
+  if (blockIdx.x == 0 && threadIdx.x == 0 && cnt->hits > hit_cap) {
     cnt->overflow = cnt->hits > cnt->overflow ? cnt->hits : cnt->overflow;
+    if (step->index < cnt->first_overflow) cnt->first_overflow = step->index;
+  }
 }
 
 
@@ -1858,17 +1783,18 @@ hipError_t launch_batch_settle(const BatchStep *d_step, const unsigned long long
 }
 
 hipError_t launch_batch_step(int width, const BatchStep *d_step, const BatchJob *d_jobs_base,
-                             const TableView *d_tvs, unsigned long long *d_head,
-                             HitRec *d_hits, uint32_t *d_hit_count, uint32_t hit_cap,
+                             const BBlock *d_blocks, const TableView *d_tvs,
+                             unsigned long long *d_head, HitRec *d_hits,
+                             uint32_t *d_hit_count, uint32_t hit_cap,
                              const int32_t *d_settled, int grid, hipStream_t stream) {
   if (width == 32)
     hipLaunchKernelGGL(k_batch_step<uint32_t>, dim3(grid), dim3(kPairThreads), 0, stream,
-                       d_step, d_jobs_base, d_tvs, d_head, d_hits, d_hit_count, hit_cap,
-                       d_settled);
+                       d_step, d_jobs_base, d_blocks, d_tvs, d_head, d_hits, d_hit_count,
+                       hit_cap, d_settled);
   else
     hipLaunchKernelGGL(k_batch_step<uint64_t>, dim3(grid), dim3(kPairThreads), 0, stream,
-                       d_step, d_jobs_base, d_tvs, d_head, d_hits, d_hit_count, hit_cap,
-                       d_settled);
+                       d_step, d_jobs_base, d_blocks, d_tvs, d_head, d_hits, d_hit_count,
+                       hit_cap, d_settled);
   return hipGetLastError();
 }
 

@@ -143,6 +143,23 @@ def test_hit_buffer_overflow_replay_keeps_answers(ctx, idx, max_size, cap):
         assert verify_rpn_chars(got["rpn"], inst)
 
 
+def test_leaf_hits_above_the_buffer_are_an_error(ctx):
+    # v1 = t and v0 = t - 1 on every sample: both leaves hit the size-1
+    # windows.  A one-record buffer overflows in the fused step; the leaf
+    # scan runs again on its own and its two hits overflow it: an error, not
+    # an answer missing the leaf solution.  The default buffer finds v1.
+    import simba_amd
+    rng = np.random.default_rng(5)
+    v0 = rng.integers(0, 2**63, 16, dtype=np.uint64)
+    v1 = v0 + np.uint64(1)
+    inst = simba_amd.MbaInstance(np.stack([v0, v1], 1), v1, 2, width=64)
+    note("simba leaf hits above the buffer")
+    base = ctx.simba(inst, max_size=7)
+    assert (base["status"], base["size"], base["expr"]) == (0, 1, "v1")
+    with pytest.raises(simba_amd.MbaError):
+        ctx.simba(inst, max_size=7, hit_cap=1)
+
+
 def test_autotune_picks_a_config_with_identical_answers(ctx, tmp_path, monkeypatch):
     from simba_amd import autotune
     from simba_amd.synth import make_instance
