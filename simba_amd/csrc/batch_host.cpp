@@ -33,6 +33,8 @@ constexpr uint32_t kBatchHitCap = 1u << 24;  // 512 MiB of hit records
 // V[8] in tables at a size budget of 11, so the last size's B sides are
 // formed from tabulated children
 constexpr uint64_t kBatchTableBytes = 96ull << 30;
+// the table pool is packed again (see table_steps) from this size on
+constexpr int kRepackMin = 8;
 
 struct InstState {
   int L = 0, n = 0, fs = 0, Kt = 1;
@@ -122,7 +124,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     nout += (uint64_t)n;
   }
   // pinned staging (kept in the context, grown when needed): step headers
-  // at 0, step counters at 1024, then the instances' table views, leaf
+  // at 0, step counters at 1536, then the instances' table views, leaf
   // values, outputs and sample layouts
   // jobs of every size of a search (a group of sizes runs as one launch
   // sequence): the leaf scan, a table build per tabulated size, the pairs
@@ -147,9 +149,10 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   }
   char *pin = (char *)ctx.b_pinned;
   // step headers of a group (one per size, plus the scratch header of a
-  // step run in parts) at 0, the step counters read back at 1024
+  // step run in parts) at 0, the step counters read back at 1536
   BatchStep *h_steps = (BatchStep *)pin;
-  BatchCounters *h_cnt = (BatchCounters *)(pin + 1024);
+  static_assert(sizeof(BatchStep) * (kMaxSize + 2) <= 1536, "step headers");
+  BatchCounters *h_cnt = (BatchCounters *)(pin + 1536);
   // the jobs of a group and the B blocks they name (block 0: a placeholder
   // for jobs whose B side is a table)
   std::vector<BatchJob> jobv(job_cap);
@@ -274,6 +277,8 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   double ktime = 0.0;
   uint32_t klaunch = 0;
   int replays = 0;  // steps run again in parts after a hit buffer overflow
+  double host_build = 0.0, host_collect = 0.0;  // host time (verbose report)
+  size_t blocks_sent = 0;
   bool budget_hit = false;
   std::vector<double> own(count, 0.0);  // each instance's share of device time
   std::vector<uint64_t> inst_tiles(count, 0);
@@ -296,7 +301,8 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   // it runs (every record of an instance carries its final Kt)
   std::vector<TvUpdate> tv_updates;
   size_t n_active_now = count;
-  auto flush_tables = [&]() -> int {
+  bool leaves_pending = false;  // V[1] of every view to rewrite (pool packed again)
+  auto flush_views = [&]() -> int {
     if (tv_updates.empty()) return 0;
     for (TvUpdate &u : tv_updates) u.kt = S[u.inst].Kt;
     MBA_HIP(ctx.b_tvup.ensure(sizeof(TvUpdate) * tv_updates.size()));
@@ -307,6 +313,16 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     MBA_HIP(hipStreamSynchronize(stq));  // the record vector is reused
     tv_updates.clear();
     return 0;
+  };
+  auto flush_tables = [&]() -> int {
+    if (leaves_pending) {  // after the views: V[1] rewritten where the pool was packed again
+      leaves_pending = false;
+      if (int rc = flush_views()) return rc;
+      MBA_HIP(launch_batch_leaves(W, ctx.b_inst.as<TableView>(), ctx.leafvals.as<uint64_t>(),
+                                  ctx.b_results.as<BatchInstData>(), (uint32_t)count, stq));
+      return 0;
+    }
+    return flush_views();
   };
 
   // The jobs of size s for instance i (learned rounds: the splits of
@@ -601,6 +617,118 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     return std::max<uint64_t>(kMinAllowance, (uint64_t)(left * tile_rate));
   };
 
+  // Tables for the B sides of size s (V[m], m <= s - 2).  An active
+  // instance whose tables end below min(kt_cap, s - 2) gets the missing
+  // sizes when the free pool holds its share of them.  Once the active
+  // instances have halved since the pool was last packed, it is packed anew
+  // from its start: every active instance's tables are placed again (V[1]
+  // rewritten, V[2..] rebuilt), each up to the largest size its share of
+  // the whole pool holds, and the finished instances' tables are dropped.
+  // The builds run as steps of their own, one per table size and smaller
+  // sizes first (a table is formed from the smaller ones), at the start of
+  // a launch sequence: the views change before the sequence runs, so no
+  // earlier step of it may read them.  Returns 1 when tables are due and
+  // the group already holds steps (they open the next group).
+  size_t packed_active = count;
+  int tables_for = -1;  // the size whose tables were last provided
+  std::vector<std::pair<int, uint32_t>> builds;  // (table size, instance)
+  int step_size[kMaxSize + 2];
+  auto table_steps = [&](int s, size_t &nj, int &nsz, uint64_t &group_tiles) -> int {
+    if (tables_for == s) return 0;
+    const int want = std::min(kt_cap, s - 2);
+    if (want < 2) {
+      tables_for = s;
+      return 0;
+    }
+    const size_t act = std::max<size_t>(n_active_now, 1);
+    const bool repack = s >= kRepackMin && n_active_now * 2 <= packed_active;
+    const uint64_t free_share =
+        pool_all ? pool_elems - pool_used : (pool_elems - pool_used) / act;
+    bool due = false;  // some table to place
+    for (size_t i = 0; i < count && !due; ++i)
+      due = S[i].active && S[i].Kt < want && (repack || S[i].T->t[S[i].Kt + 1][8] <= free_share);
+    if (!due) {
+      tables_for = s;
+      return 0;
+    }
+    if (nsz > 0) return 1;
+    tables_for = s;
+    builds.clear();
+    auto place = [&](size_t i, int m) {
+      tvs[i].off[m] = pool_used;
+      tv_updates.push_back(TvUpdate{(uint32_t)i, (uint32_t)m, pool_used, 0, 0});
+      pool_used += S[i].T->t[m][8];
+      if (m >= 2) builds.emplace_back(m, (uint32_t)i);
+    };
+    if (repack) {
+      packed_active = n_active_now;
+      pool_used = 0;
+      const uint64_t share = pool_elems / act;
+      for (size_t i = 0; i < count; ++i) {
+        InstState &st = S[i];
+        if (!st.active) {  // tables dropped: the view keeps Kt 0
+          if (st.Kt > 0) tv_updates.push_back(TvUpdate{(uint32_t)i, 1u, 0, 0, 0});
+          st.Kt = 0;
+          continue;
+        }
+        const int top = std::min(kt_cap, std::max(want, st.Kt));
+        uint64_t used = (uint64_t)st.L;
+        int k = 1;
+        while (k < top && used + st.T->t[k + 1][8] <= share) used += st.T->t[++k][8];
+        for (int m = 1; m <= k; ++m) place(i, m);
+        st.Kt = k;
+        tvs[i].Kt = k;
+      }
+      leaves_pending = true;
+    } else {
+      const uint64_t share = pool_all ? ~0ull : free_share;
+      for (size_t i = 0; i < count; ++i) {
+        InstState &st = S[i];
+        if (!st.active || st.Kt >= want) continue;
+        uint64_t got = 0;
+        while (st.Kt < want) {
+          const uint64_t n = st.T->t[st.Kt + 1][8];
+          if (n > pool_elems - pool_used || got + n > share) break;
+          got += n;
+          place(i, ++st.Kt);
+          tvs[i].Kt = st.Kt;
+        }
+      }
+    }
+    std::sort(builds.begin(), builds.end());
+    for (size_t b = 0; b < builds.size();) {
+      const int m = builds[b].first;
+      const size_t j0 = nj;
+      uint64_t tiles = 0;
+      for (; b < builds.size() && builds[b].first == m; ++b) {
+        const uint32_t i = builds[b].second;
+        BatchJob J;
+        std::memset(&J, 0, sizeof J);
+        J.p.m = m;
+        J.p.nB = S[i].T->t[m][8];
+        J.ntiles = (J.p.nB + kBuildTile - 1) / kBuildTile;
+        J.inst = i;
+        J.kind = 0;
+        J.tile0 = tiles;
+        tiles += J.ntiles;
+        inst_tiles[i] += J.ntiles;
+        if (nj == jobv.size()) {
+          jobv.resize(2 * nj);
+          h_jobs = jobv.data();
+        }
+        h_jobs[nj++] = J;
+      }
+      step_size[nsz] = 0;
+      BatchStep &hs = h_steps[nsz++];
+      std::memset(&hs, 0, sizeof hs);
+      hs.total_tiles = tiles;
+      hs.job_off = j0;
+      hs.njobs = (uint32_t)(nj - j0);
+      group_tiles += tiles;
+    }
+    return 0;
+  };
+
   // sizes whose steps would use the learned order for some active instance
   auto learned_size = [&](int s) {
     if (!score_size || s < 3) return false;
@@ -651,14 +779,16 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       // window of the size's tiles its budget allows; one whose window ends
       // inside the size ends the group at the end of the size's pass (its
       // next window waits for its charge) and settles nothing at that size.
+      const auto t_b = Clock::now();
       size_t nj = 0;
       blkv.resize(1);
       uint64_t group_tiles = 0;
       int nsz = 0;
-      int step_size[kMaxSize + 1];
       const int s0 = s;
       while (s <= max_size && nsz < kMaxSize + 1) {
         if (nsz > 0 && cursor == 0 && learned_size(s)) break;
+        if (table_steps(s, nj, nsz, group_tiles)) break;  // no room: the next group
+        if (nsz == kMaxSize + 1) break;
         const size_t j0 = nj;
         uint64_t tiles = 0;
         size_t i = cursor;
@@ -706,6 +836,8 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
                                stq));
       }
       const auto t_g = Clock::now();
+      host_build += seconds_since(t_b);
+      blocks_sent += blkv.size();
       if (nj > 0) {
         if (int rc = run_group(nsz, nj)) return rc;
         if (h_cnt->overflow > 0) {
@@ -748,7 +880,9 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       const double t_seq = seconds_since(t_g);
       if (group_tiles >= 1024 && t_seq > 0) tile_rate = (double)group_tiles / t_seq;
       charge(t_seq, group_tiles);
+      const auto t_c = Clock::now();
       if (int rc = collect()) return rc;
+      host_collect += seconds_since(t_c);
       for (int k = 0; k < nsz; ++k)  // sizes completed by the group's steps
         for (size_t i = h_steps[k].inst_lo; i < h_steps[k].inst_hi; ++i)
           if (S[i].active && h_steps[k].size_done > 0 && next_sz[i] > step_size[k])
@@ -759,6 +893,19 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     // then the rest (each split swept once with every operator instance
     // fused); instances with fewer trees at this size take every split in
     // the first round; any verified solution of this (minimal) size settles
+    {  // the size's tables first (a launch sequence of build steps, if any)
+      size_t nj = 0;
+      int nsz = 0;
+      uint64_t tiles = 0;
+      blkv.resize(1);
+      table_steps(s, nj, nsz, tiles);  // an empty group has room for them
+      if (int rc = flush_tables()) return rc;
+      if (nj > 0) {
+        const auto t_g = Clock::now();
+        if (int rc = run_group(nsz, nj)) return rc;
+        charge(seconds_since(t_g), tiles);
+      }
+    }
     if (int rc = (*score_size)(s, scores.data())) return rc;
     size_t max_blocks = 0;
     for (size_t i = 0; i < count; ++i) {
@@ -856,8 +1003,9 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   }
   const double total = seconds_since(t_start);
   if (o.verbose)
-    std::fprintf(stderr, "batch: %u launch sequences, device %.3f ms, total %.3f ms\n", klaunch,
-                 1e3 * ktime, 1e3 * total);
+    std::fprintf(stderr, "batch: %u launch sequences, device %.3f ms, total %.3f ms (host: "
+                 "job building %.3f ms, %zu blocks; read backs %.3f ms)\n", klaunch, 1e3 * ktime,
+                 1e3 * total, 1e3 * host_build, blocks_sent, 1e3 * host_collect);
   for (size_t i = 0; i < count; ++i) {
     InstState &st = S[i];
     mba_result &r = results[i];

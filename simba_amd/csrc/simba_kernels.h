@@ -164,7 +164,8 @@ hipError_t launch_batch_reset(unsigned long long *d_head, BatchCounters *d_cnt,
 hipError_t launch_batch_settle(const BatchStep *d_step, const unsigned long long *d_found,
                                int32_t *d_settled, uint32_t count, int min_size,
                                BatchCounters *d_cnt, uint32_t hit_cap, hipStream_t stream);
-// V[1] (filter-sample leaf values) of every instance of a batch, one launch;
+// V[1] (filter-sample leaf values) of every instance of a batch whose view
+// has Kt > 0, one launch;
 hipError_t launch_batch_leaves(int width, const TableView *d_tvs, const uint64_t *d_leafvals,
                                const BatchInstData *d_idat, uint32_t count,
                                hipStream_t stream);

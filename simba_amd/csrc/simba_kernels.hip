@@ -1432,7 +1432,7 @@ __global__ void k_batch_leaves(const TableView *__restrict__ tvs,
   if (i >= count) return;
   const TableView &tv = tvs[i];
   const uint32_t l = threadIdx.x;
-  if (l < (uint32_t)tv.L) {
+  if (tv.Kt > 0 && l < (uint32_t)tv.L) {  // Kt 0: an instance without tables
     const BatchInstData &d = idat[i];
     ((word *)tv.base)[tv.off[1] + l] = (word)leafvals[d.leaf_off + (uint64_t)d.fs * tv.L + l];
   }

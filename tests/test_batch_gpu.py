@@ -82,6 +82,21 @@ def test_batch_b_sides_above_the_tables(ctx):
             (p["status"], p["size"], p["rank"], p["expr"])
 
 
+@pytest.mark.parametrize("table_mb", [0, 48])
+def test_batch_tables_packed_again_keep_answers(ctx, table_mb):
+    # sizes to 11 on instances that settle at many sizes: as the active
+    # instances halve the table pool is packed again and the tables of the
+    # remaining ones grow (a small pool: most B sides stay above the tables
+    # for a while); every answer stays the per-instance search's
+    from simba_amd.learn import sample_instances
+    insts = sample_instances(10**6 + 15, 160, (9, 19), (2, 4), (32, 32))
+    note(f"batch tables packed again table_mb={table_mb}")
+    got = ctx.batch(insts, max_size=11, table_bytes=table_mb << 20)
+    key = lambda r: (r["status"], r["size"], r["rank"], r["expr"])
+    for inst, g in zip(insts, got):
+        assert key(g) == key(ctx.simba(inst, max_size=11))
+
+
 # (max size, hit buffer): a step whose window hits overflow the small buffer
 # is run again with its jobs (and then one job's tiles) in parts; the answers
 # stay those of the default buffer
