@@ -154,10 +154,33 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   static_assert(sizeof(BatchStep) * (kMaxSize + 2) <= 1536, "step headers");
   BatchCounters *h_cnt = (BatchCounters *)(pin + 1536);
   // the jobs of a group and the B blocks they name (block 0: a placeholder
-  // for jobs whose B side is a table)
+  // for jobs whose B side is a table).  A B side above the tables is a list
+  // of blocks that depends on the leaf count, the tabulated depth and the B
+  // size only (table references are resolved through each instance's view),
+  // so each list is built once per call and named by every job of that shape.
   std::vector<BatchJob> jobv(job_cap);
   BatchJob *h_jobs = jobv.data();
   std::vector<BBlock> blkv(1), blk_tmp;
+  struct BList {
+    uint32_t first, n;  // blocks [first, first + n) of blkv
+    uint64_t tiles;     // B tiles of the list
+  };
+  std::map<uint64_t, BList> blists;
+  auto block_list = [&](const InstState &st, int m) -> BList {
+    const uint64_t key = (uint64_t)st.L | (uint64_t)st.Kt << 8 | (uint64_t)m << 16;
+    auto it = blists.find(key);
+    if (it != blists.end()) return it->second;
+    blk_tmp.clear();
+    b_blocks(*st.T, st.Kt, m, 0, 1, {}, kPairRB, blk_tmp);
+    BList bl{(uint32_t)blkv.size(), (uint32_t)blk_tmp.size(), 0};
+    for (BBlock &b : blk_tmp) {
+      b.tile_pre = bl.tiles;
+      bl.tiles += block_tiles(b, kPairRB);
+      blkv.push_back(b);
+    }
+    blists.emplace(key, bl);
+    return bl;
+  };
   TableView *tvs = (TableView *)(pin + off_tvs);
   uint64_t *leafvals = (uint64_t *)(pin + off_leaf);
   uint64_t *outputs = (uint64_t *)(pin + off_out);
@@ -386,16 +409,13 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
         push(J);
         return;
       }
-      // a B side above Kt: one job per B block formed from the tables
-      blk_tmp.clear();
-      b_blocks(T, st.Kt, tvs[i].off, J.p.m, 0, 1, {}, kPairRB, blk_tmp);
-      for (const BBlock &b : blk_tmp) {
-        BatchJob Jb = J;
-        Jb.p.nTB = block_tiles(b, kPairRB);
-        Jb.ntiles = Jb.p.nTA * Jb.p.nTB;
-        Jb.bb = (uint32_t)blkv.size();
-        if (push(Jb)) blkv.push_back(b);
-      }
+      // a B side above Kt: the blocks formed from the tables, one job
+      const BList bl = block_list(st, J.p.m);
+      J.bb = bl.first;
+      J.nbb = bl.n;
+      J.p.nTB = bl.tiles;
+      J.ntiles = J.p.nTA * bl.tiles;
+      push(J);
     };
     if (first_round && s == 1) {  // leaves and every unary chain over them
       BatchJob J;
@@ -781,7 +801,6 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       // next window waits for its charge) and settles nothing at that size.
       const auto t_b = Clock::now();
       size_t nj = 0;
-      blkv.resize(1);
       uint64_t group_tiles = 0;
       int nsz = 0;
       const int s0 = s;
@@ -837,7 +856,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       }
       const auto t_g = Clock::now();
       host_build += seconds_since(t_b);
-      blocks_sent += blkv.size();
+      blocks_sent = blkv.size();
       if (nj > 0) {
         if (int rc = run_group(nsz, nj)) return rc;
         if (h_cnt->overflow > 0) {
@@ -897,7 +916,6 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       size_t nj = 0;
       int nsz = 0;
       uint64_t tiles = 0;
-      blkv.resize(1);
       table_steps(s, nj, nsz, tiles);  // an empty group has room for them
       if (int rc = flush_tables()) return rc;
       if (nj > 0) {
@@ -928,7 +946,6 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       const size_t rb = round == 0 ? 0 : kRoundEnd[round - 1];
       const size_t re = kRoundEnd[round];
       size_t nj = 0;
-      blkv.resize(1);
       uint64_t tiles = 0;
       std::vector<int> sub;
       for (size_t i = 0; i < count; ++i) {

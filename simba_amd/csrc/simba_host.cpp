@@ -71,9 +71,8 @@ uint64_t block_tiles(const BBlock &b, int rb) {
 // generic block.
 constexpr uint64_t kFixedSiblings = 64;
 
-void b_blocks(const CountTable &T, int Kt, const uint64_t *off, int m, uint64_t start,
-              uint64_t stride, const std::vector<BStep> &steps, int rb,
-              std::vector<BBlock> &out) {
+void b_blocks(const CountTable &T, int Kt, int m, uint64_t start, uint64_t stride,
+              const std::vector<BStep> &steps, int rb, std::vector<BBlock> &out) {
   auto emit = [&](BBlock b, const std::vector<BStep> &st) {
     b.stride = stride;
     if ((int)st.size() > kMaxBSteps && !b.generic) {
@@ -119,13 +118,13 @@ void b_blocks(const CountTable &T, int Kt, const uint64_t *off, int m, uint64_t 
       if (m - 1 <= Kt) {
         BBlock b{};
         b.start = start + stride * r;
-        b.offA = off[m - 1];
+        b.offA = tab_ref(m - 1, 0);
         b.nA = T.t[m - 1][8];
         b.nB = 1;
         b.op = -1;
         emit(b, st);
       } else {
-        b_blocks(T, Kt, off, m - 1, start + stride * r, stride, st, rb, out);
+        b_blocks(T, Kt, m - 1, start + stride * r, stride, st, rb, out);
       }
     } else {
       const int cs = m - 1;
@@ -138,8 +137,8 @@ void b_blocks(const CountTable &T, int Kt, const uint64_t *off, int m, uint64_t 
         if (l <= Kt && rr <= Kt) {
           BBlock b{};
           b.start = base;
-          b.offA = off[l];
-          b.offB = off[rr];
+          b.offA = tab_ref(l, 0);
+          b.offB = tab_ref(rr, 0);
           b.nA = nl;
           b.nB = nr;
           b.op = op;
@@ -147,13 +146,13 @@ void b_blocks(const CountTable &T, int Kt, const uint64_t *off, int m, uint64_t 
         } else if (l > Kt && rr <= Kt && nr <= kFixedSiblings) {
           // right child fixed to tree b: rank base + stride*(ra*nr + b)
           for (uint64_t bi = 0; bi < nr; ++bi)
-            b_blocks(T, Kt, off, l, base + stride * bi, stride * nr,
-                     with(BStep{2, op, off[rr] + bi}), rb, out);
+            b_blocks(T, Kt, l, base + stride * bi, stride * nr,
+                     with(BStep{2, op, tab_ref(rr, bi)}), rb, out);
         } else if (rr > Kt && l <= Kt && nl <= kFixedSiblings) {
           // left child fixed to tree a: rank base + stride*(a*nr + rb)
           for (uint64_t ai = 0; ai < nl; ++ai)
-            b_blocks(T, Kt, off, rr, base + stride * ai * nr, stride,
-                     with(BStep{1, op, off[l] + ai}), rb, out);
+            b_blocks(T, Kt, rr, base + stride * ai * nr, stride,
+                     with(BStep{1, op, tab_ref(l, ai)}), rb, out);
         } else {
           BBlock b{};
           b.start = base;
@@ -612,7 +611,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
           continue;
         }
         f_tmp.clear();
-        b_blocks(T, Kt, tv.off, J.p.m, 0, 1, {}, kPairRB, f_tmp);
+        b_blocks(T, Kt, J.p.m, 0, 1, {}, kPairRB, f_tmp);
         for (const BBlock &b : f_tmp) {
           J.bb = (uint32_t)f_blocks.size();
           f_blocks.push_back(b);
@@ -783,7 +782,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
       auto it = bcache.find(m * 16 + p.rb);
       if (it == bcache.end()) {
         std::vector<BBlock> v;
-        b_blocks(T, Kt, tv.off, m, 0, 1, {}, p.rb, v);
+        b_blocks(T, Kt, m, 0, 1, {}, p.rb, v);
         it = bcache.emplace(m * 16 + p.rb, std::move(v)).first;
       }
       blocks = &it->second;

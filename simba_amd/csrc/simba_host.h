@@ -29,11 +29,11 @@ constexpr uint64_t kBlockModeMin = 1ull << 31;
 constexpr uint64_t kLearnedBlockMin = 1ull << 26;
 
 // The B-side trees of size m > Kt in rank order as blocks formed from the
-// tables at element offsets off[] (see b_blocks in simba_host.cpp), for
-// tiles of 256 * rb B values; block_tiles: the tiles of one block.
-void b_blocks(const CountTable &T, int Kt, const uint64_t *off, int m, uint64_t start,
-              uint64_t stride, const std::vector<BStep> &steps, int rb,
-              std::vector<BBlock> &out);
+// tables V[1..Kt] (table references resolved through the instance's
+// TableView; see b_blocks in simba_host.cpp), for tiles of 256 * rb B
+// values; block_tiles: the tiles of one block.
+void b_blocks(const CountTable &T, int Kt, int m, uint64_t start, uint64_t stride,
+              const std::vector<BStep> &steps, int rb, std::vector<BBlock> &out);
 uint64_t block_tiles(const BBlock &b, int rb);
 
 // Windows for the checks of base size sb (chains up to max_size).

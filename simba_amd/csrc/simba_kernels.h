@@ -45,22 +45,37 @@ struct CheckTargets {
 // start + stride * (ra * nB + rb).  A tile covers 2^logTB columns (over ra
 // when trans, else over rb) by (256*RB) >> logTB rows; ntc tiles span a row
 // band (ntc_div: the division by ntc).  When the columns span a whole tile
-// (2^logTB = 256*RB) a tile covers rows_pt consecutive rows of its columns.  generic: ranks start + stride * i,
-// i < nA, each evaluated by a table-assisted unrank (children above Kt).
+// (2^logTB = 256*RB) a tile covers rows_pt consecutive rows of its columns.
+// generic: ranks start + stride * i, i < nA, each evaluated by a
+// table-assisted unrank (children above Kt).
+// Table references (offA, offB, the fixed siblings' c) name a table and an
+// element of it, (m << kTabShift) | index, resolved through the instance's
+// TableView (tab_off), so one block list serves every instance of the same
+// leaf count and tabulated depth.  tile_pre: the B tiles of the blocks
+// before this one in its list (a job's B side runs through a whole list).
 constexpr int kMaxBSteps = 8;
+constexpr int kTabShift = 56;
+constexpr uint64_t kTabIndex = (1ull << kTabShift) - 1;
+__host__ __device__ inline uint64_t tab_ref(int m, uint64_t index) {
+  return ((uint64_t)m << kTabShift) | index;
+}
+__device__ inline uint64_t tab_off(const TableView *tv, uint64_t ref) {
+  return tv->off[ref >> kTabShift] + (ref & kTabIndex);
+}
 struct BStep {
   int32_t kind;  // 0 affine, 1 op(c, x), 2 op(x, c)
   int32_t op;    // kinds 1-2: the binary operator; kind 0: neg flag
-  uint64_t c;    // kind 0: k; kinds 1-2: table element offset of the fixed tree
+  uint64_t c;    // kind 0: k; kinds 1-2: table reference of the fixed tree
 };
 struct BBlock {
   uint64_t start, stride;
-  uint64_t offA, offB;
+  uint64_t offA, offB;     // table references (tab_ref)
   uint64_t nA, nB;
   uint64_t ntc;
   int32_t op, generic, trans, logTB, nsteps, rows_pt;
   DivMagic ntc_div;
   uint32_t pad2;
+  uint64_t tile_pre;       // B tiles of the earlier blocks of its list
   BStep steps[kMaxBSteps];
 };
 
@@ -101,8 +116,9 @@ struct BatchJob {
   PairCore p;                 // p.nTB: tiles of the B block when p.gen_b
   uint32_t inst;
   uint32_t kind;
-  uint32_t bb;                // B block of a B side above Kt (index in the step's blocks)
-  uint32_t pad;
+  uint32_t bb;                // B side above Kt: its first block (index in the blocks)
+  uint32_t nbb;               // blocks of the B side (0 or 1: block bb alone); the
+                              // job's B tiles run through blocks [bb, bb + nbb)
 };
 
 // One size step of a batch: its jobs are jobs_base[job_off, job_off + njobs)

@@ -475,14 +475,15 @@ __device__ __forceinline__ bool desc_coords(const BBlock &bb, uint64_t bt, int k
 
 // The path steps of a block (innermost first) on one value.
 template <typename word>
-__device__ __forceinline__ word apply_steps(const BBlock &bb, const word *V, word v) {
+__device__ __forceinline__ word apply_steps(const BBlock &bb, const TableView *__restrict__ tv,
+                                            const word *V, word v) {
   for (int i = 0; i < bb.nsteps; ++i) {
     const BStep &st = bb.steps[i];
     if (st.kind == 0) {
       if (st.op) v = word(0) - v;
       v = word(v + (word)st.c);
     } else {
-      const word c = V[st.c];
+      const word c = V[tab_off(tv, st.c)];
       v = st.kind == 1 ? apply_op<word>(st.op, c, v) : apply_op<word>(st.op, v, c);
     }
   }
@@ -490,11 +491,11 @@ __device__ __forceinline__ word apply_steps(const BBlock &bb, const word *V, wor
 }
 
 template <typename word>
-__device__ __forceinline__ word desc_value(const BBlock &bb, const word *V, uint64_t ra,
-                                           uint64_t rb) {
-  word v = V[bb.offA + ra];
-  if (bb.op >= 0) v = apply_op<word>(bb.op, v, V[bb.offB + rb]);
-  return apply_steps<word>(bb, V, v);
+__device__ __forceinline__ word desc_value(const BBlock &bb, const TableView *__restrict__ tv,
+                                           const word *V, uint64_t ra, uint64_t rb) {
+  word v = V[tab_off(tv, bb.offA) + ra];
+  if (bb.op >= 0) v = apply_op<word>(bb.op, v, V[tab_off(tv, bb.offB) + rb]);
+  return apply_steps<word>(bb, tv, V, v);
 }
 
 __device__ __forceinline__ uint32_t lo_op(int op, uint32_t a, uint32_t b) {
@@ -512,7 +513,8 @@ __device__ __forceinline__ uint32_t lo_op(int op, uint32_t a, uint32_t b) {
 // -v = (v ^ ~0) - ~0, so s*v + k = (v ^ sg) + (k - sg) with sg = 0 or ~0; a
 // fixed sibling's low half is one uniform load.
 template <typename word, int RB>
-__device__ __forceinline__ void steps_lo(const BBlock &bb, const word *V, uint32_t *bl) {
+__device__ __forceinline__ void steps_lo(const BBlock &bb, const TableView *__restrict__ tv,
+                                         const word *V, uint32_t *bl) {
   for (int i = 0; i < bb.nsteps; ++i) {
     const BStep &st = bb.steps[i];
     if (st.kind == 0) {
@@ -521,7 +523,7 @@ __device__ __forceinline__ void steps_lo(const BBlock &bb, const word *V, uint32
 #pragma unroll
       for (int k = 0; k < RB; ++k) bl[k] = (bl[k] ^ sg) + kk;
     } else {
-      const uint32_t c = lo32_at<word>(V, st.c);
+      const uint32_t c = lo32_at<word>(V, tab_off(tv, st.c));
       if (st.kind == 1) {
 #pragma unroll
         for (int k = 0; k < RB; ++k) bl[k] = lo_op(st.op, c, bl[k]);
@@ -537,35 +539,38 @@ __device__ __forceinline__ void steps_lo(const BBlock &bb, const word *V, uint32
 // 2-D mapping (blocks whose columns span less than a tile; row-band tiles go
 // through desc_band_tile): inner value, then the path steps.
 template <typename word, int RB, int OP>
-__device__ __forceinline__ void desc_lo_op(const BBlock &bb, const word *V, uint64_t bt,
-                                           uint32_t *bl, bool *ok) {
+__device__ __forceinline__ void desc_lo_op(const BBlock &bb, const TableView *__restrict__ tv,
+                                           const word *V, uint64_t bt, uint32_t *bl,
+                                           bool *ok) {
   auto comb = [](uint32_t a, uint32_t b) -> uint32_t {
     return OP == AND ? (a & b) : OP == OR ? (a | b) : OP == XOR ? (a ^ b)
          : OP == ADD ? (a + b) : OP == SUB ? (a - b) : OP == MUL ? (a * b) : a;
   };
+  const word *VA = V + tab_off(tv, bb.offA);
+  const word *VB = OP >= 0 ? V + tab_off(tv, bb.offB) : V;
 #pragma unroll
   for (int k = 0; k < RB; ++k) {
     uint64_t ra, rb;
     ok[k] = desc_coords<RB>(bb, bt, k, ra, rb);
     if (!ok[k]) ra = 0, rb = 0;
-    uint32_t v = lo32_at<word>(V, bb.offA + ra);
-    if (OP >= 0) v = comb(v, lo32_at<word>(V, bb.offB + rb));
+    uint32_t v = lo32_at<word>(VA, ra);
+    if (OP >= 0) v = comb(v, lo32_at<word>(VB, rb));
     bl[k] = v;
   }
-  if (bb.nsteps) steps_lo<word, RB>(bb, V, bl);
+  if (bb.nsteps) steps_lo<word, RB>(bb, tv, V, bl);
 }
 
 template <typename word, int RB>
-__device__ __forceinline__ void desc_lo(const BBlock &bb, const word *V, uint64_t bt,
-                                        uint32_t *bl, bool *ok) {
+__device__ __forceinline__ void desc_lo(const BBlock &bb, const TableView *__restrict__ tv,
+                                        const word *V, uint64_t bt, uint32_t *bl, bool *ok) {
   switch (bb.op) {
-    case AND: desc_lo_op<word, RB, AND>(bb, V, bt, bl, ok); break;
-    case OR: desc_lo_op<word, RB, OR>(bb, V, bt, bl, ok); break;
-    case XOR: desc_lo_op<word, RB, XOR>(bb, V, bt, bl, ok); break;
-    case ADD: desc_lo_op<word, RB, ADD>(bb, V, bt, bl, ok); break;
-    case SUB: desc_lo_op<word, RB, SUB>(bb, V, bt, bl, ok); break;
-    case MUL: desc_lo_op<word, RB, MUL>(bb, V, bt, bl, ok); break;
-    default: desc_lo_op<word, RB, -1>(bb, V, bt, bl, ok); break;
+    case AND: desc_lo_op<word, RB, AND>(bb, tv, V, bt, bl, ok); break;
+    case OR: desc_lo_op<word, RB, OR>(bb, tv, V, bt, bl, ok); break;
+    case XOR: desc_lo_op<word, RB, XOR>(bb, tv, V, bt, bl, ok); break;
+    case ADD: desc_lo_op<word, RB, ADD>(bb, tv, V, bt, bl, ok); break;
+    case SUB: desc_lo_op<word, RB, SUB>(bb, tv, V, bt, bl, ok); break;
+    case MUL: desc_lo_op<word, RB, MUL>(bb, tv, V, bt, bl, ok); break;
+    default: desc_lo_op<word, RB, -1>(bb, tv, V, bt, bl, ok); break;
   }
 }
 
@@ -579,7 +584,7 @@ __device__ __forceinline__ word b_full(const TableView *__restrict__ tv,
     uint64_t ra, rb;
     desc_coords<RB>(p.bb, bt, k, ra, rb);
     rank = p.bb.start + p.bb.stride * (ra * p.bb.nB + rb);
-    return desc_value<word>(p.bb, global_ptr<const word>(tv->base), ra, rb);
+    return desc_value<word>(p.bb, tv, global_ptr<const word>(tv->base), ra, rb);
   }
   const uint64_t local = bt * (kPairThreads * RB) + threadIdx.x + (uint64_t)k * kPairThreads;
   if (BSRC == kBGeneric) {
@@ -597,7 +602,7 @@ __device__ __forceinline__ void b_lo(const TableView *__restrict__ tv,
                                      const word *__restrict__ B, const PL &p,
                                      uint64_t bt, uint32_t *bl, bool *ok) {
   if (BSRC == kBDesc) {
-    desc_lo<word, RB>(p.bb, global_ptr<const word>(tv->base), bt, bl, ok);
+    desc_lo<word, RB>(p.bb, tv, global_ptr<const word>(tv->base), bt, bl, ok);
     return;
   }
   const uint64_t b0 = bt * (kPairThreads * RB) + threadIdx.x;
@@ -842,7 +847,8 @@ __device__ __forceinline__ void desc_band_tile(const TableView *__restrict__ tv,
   const uint64_t nX = bb.trans ? bb.nA : bb.nB, nY = bb.trans ? bb.nB : bb.nA;
   const uint64_t left = nX - x0;
   const uint32_t rem = left > tile ? tile : (uint32_t)left;
-  const word *col = V + (bb.trans ? bb.offA : bb.offB) + x0;
+  const word *col = V + tab_off(tv, bb.trans ? bb.offA : bb.offB) + x0;
+  const word *rowp = V + (bb.op < 0 ? 0 : tab_off(tv, bb.trans ? bb.offB : bb.offA));
   uint32_t cl[RB], bl[RB];
   bool ok[RB];
 #pragma unroll
@@ -857,7 +863,7 @@ __device__ __forceinline__ void desc_band_tile(const TableView *__restrict__ tv,
   const uint32_t le = lo32(lo_e), lo = lo32(lo_o), sp = (uint32_t)span;
   for (uint32_t r = 0; r < nr; ++r) {  // workgroup-uniform
     const uint64_t y = y0 + r;
-    const uint32_t rowv = bb.op < 0 ? 0u : lo32_at<word>(V, (bb.trans ? bb.offB : bb.offA) + y);
+    const uint32_t rowv = bb.op < 0 ? 0u : lo32_at<word>(rowp, y);
     switch (bb.op) {  // workgroup-uniform; only '-' depends on the order
       case AND:
 #pragma unroll
@@ -893,7 +899,7 @@ __device__ __forceinline__ void desc_band_tile(const TableView *__restrict__ tv,
         for (int k = 0; k < RB; ++k) bl[k] = cl[k];
         break;
     }
-    if (bb.nsteps) steps_lo<word, RB>(bb, V, bl);
+    if (bb.nsteps) steps_lo<word, RB>(bb, tv, V, bl);
     const bool hit = pair_prefilter<SAME, RB>(S, na, bl, ok, p.mask, bitops, le, lo, sp);
     if (__any(hit) && hit) {
       pair_rewalk<word, SAME, RB>(
@@ -902,7 +908,7 @@ __device__ __forceinline__ void desc_band_tile(const TableView *__restrict__ tv,
             const uint64_t x = x0 + threadIdx.x + (uint64_t)k * kPairThreads;
             const uint64_t ra = bb.trans ? x : y, rb = bb.trans ? y : x;
             rank = bb.start + bb.stride * (ra * bb.nB + rb);
-            return desc_value<word>(bb, V, ra, rb);
+            return desc_value<word>(bb, tv, V, ra, rb);
           });
     }
   }
@@ -1324,12 +1330,26 @@ __device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J,
                            const BBlock *__restrict__ blocks, uint64_t lt, uint64_t &staged,
                            bool &bitops, HitRec *hits, uint32_t *hc, uint32_t cap) {
   const word *base = global_ptr<const word>(tv->base);
-  const BatchPair p(J.p, blocks[J.bb]);  // the table source reads no block
+  const uint64_t at = J.p.nTA == 1 ? 0 : lt % J.p.nTA;
+  uint64_t bt = J.p.nTA == 1 ? lt : lt / J.p.nTA;
+  // a B side above the tables runs through the blocks [bb, bb + nbb): the
+  // last one whose first tile (tile_pre) is at most bt holds tile bt
+  const BBlock *bp = blocks + J.bb;  // the table source reads no block
+  if (J.p.gen_b && J.nbb > 1) {
+    int lo = 0, hi = (int)J.nbb - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (bp[mid].tile_pre <= bt) lo = mid;
+      else hi = mid - 1;
+    }
+    bp += lo;
+    bt -= bp->tile_pre;
+  }
+  const BatchPair p(J.p, *bp);
   const word *A = base + (p.gen_a ? 0 : tv->off[p.j]);
   const word *B = base + (p.gen_b ? 0 : tv->off[p.m]);
   const word lo_e = (word)J.lo_e, lo_o = (word)J.lo_o, span = (word)J.span;
   const uint32_t tag = J.inst << 8;
-  const uint64_t at = p.nTA == 1 ? 0 : lt % p.nTA, bt = p.nTA == 1 ? lt : lt / p.nTA;
   if (at != staged) {  // workgroup-uniform; the caller resets `staged` per job
     const uint64_t a0 = at * p.ta;
     const uint64_t a1 = a0 + p.ta < p.nA ? a0 + p.ta : p.nA;
