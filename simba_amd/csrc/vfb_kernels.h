@@ -9,6 +9,7 @@ constexpr uint64_t kVfbTomb = ~0ull - 1;     // tombstone key (full-cache.cu:739
 constexpr int kVfbMaxBlocks = 96;
 constexpr int kVfbChunk = 4096;              // candidates per workgroup step
 constexpr int kVfbThreads = 256;
+constexpr int kVfbBand = 256;               // right rows per band of a split (insert pass)
 
 // One (op, split[, swap]) block of a level's candidate space.  Candidate
 // indices of the level are the concatenation of the blocks in enumeration

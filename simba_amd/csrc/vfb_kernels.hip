@@ -20,6 +20,8 @@
 //           candidates recompute their signature and append it in order.
 // Rows are stored SoA, rows[sample * cap + row], so the lanes of a wave that
 // walk consecutive right rows read consecutive words of every sample.
+#include <type_traits>
+
 #include "vfb_kernels.h"
 
 namespace mba {
@@ -94,35 +96,39 @@ __device__ __forceinline__ Cand decode(const VfbBlock *blk, int nblocks, uint64_
   return c;
 }
 
-template <typename word>
-__device__ __forceinline__ uint64_t signature_fp(const word *rows, uint64_t cap, int n,
-                                                 const Cand &c, const word *tgt,
-                                                 bool &eq) {
-  uint64_t seed = 0;
-  eq = true;
-  for (int i = 0; i < n; ++i) {
-    const word x = rows[(uint64_t)i * cap + c.l];
-    const word y = rows[(uint64_t)i * cap + c.r];
-    const word v = apply_op<word>(c.op, x, y);
-    eq &= (v == tgt[i]);
-    seed = fp_step<word>(seed, v, i);
+// fp_step at a sample index of known parity (odd: the rotation).
+template <typename word, int ODD>
+__device__ __forceinline__ uint64_t fp_step_p(uint64_t seed, word x) {
+  uint64_t v;
+  if (sizeof(word) == 4) {
+    const uint32_t h = fmix32((uint32_t)x) + 0x9e3779b9u;
+    v = (uint64_t)h + (seed << 6) + (seed >> 2);
+  } else {
+    v = fmix64((uint64_t)x) + 0x9e3779b9ull + (seed << 6) + (seed >> 2);
   }
-  return seed;
+  if (ODD) v = (v >> 32) | (v << 32);
+  return seed ^ v;
 }
 
-// Open addressing with linear probing; returns the slot of `key`, or ~0 when
-// the probe sequence exceeds `limit` (table too full: the host reports a
-// capacity overflow), so every lane terminates.
-// `inserted` is set when this call filled an empty slot (the key is new).
+__device__ __forceinline__ uint64_t probe_start(uint64_t key, uint64_t mask) {
+  return (key ^ (key >> 29)) & mask;
+}
+
+// Open addressing with linear probing, continued from slot s (its `step`-th
+// probe); returns the slot of `key`, or ~0 when the probe sequence reaches
+// `limit` (table too full: the host reports a capacity overflow), so every
+// lane terminates.  Keys only ever change from EMPTY to a key, so a plain
+// (cached, possibly stale) load is enough: a stale EMPTY is caught by the
+// compare-and-swap, which returns the key that is there.  `inserted` is set
+// when this call filled an empty slot (the key is new).
 __device__ __forceinline__ uint64_t table_insert(uint64_t *keys, uint64_t mask,
-                                                 uint64_t key, uint64_t &probes,
-                                                 uint64_t limit, bool &inserted,
-                                                 uint64_t &cas) {
-  uint64_t s = (key ^ (key >> 29)) & mask;
-  for (uint64_t step = 0;; ++step) {
+                                                 uint64_t key, uint64_t s, uint64_t step,
+                                                 uint64_t &probes, uint64_t limit,
+                                                 bool &inserted, uint64_t &cas) {
+  for (;; ++step) {
     if (step >= limit) return ~0ull;
     ++probes;
-    uint64_t k = __atomic_load_n(&keys[2 * s], __ATOMIC_RELAXED);
+    const uint64_t k = keys[2 * s];
     if (k == key) return s;
     if (k == kVfbEmpty) {
       ++cas;
@@ -141,7 +147,7 @@ __device__ __forceinline__ uint64_t table_insert(uint64_t *keys, uint64_t mask,
 
 __device__ __forceinline__ uint64_t table_find(const uint64_t *keys, uint64_t mask,
                                                uint64_t key) {
-  uint64_t s = (key ^ (key >> 29)) & mask;
+  uint64_t s = probe_start(key, mask);
   for (uint64_t step = 0; step <= mask; ++step) {
     const uint64_t k = keys[2 * s];
     if (k == key) return s;
@@ -151,41 +157,92 @@ __device__ __forceinline__ uint64_t table_find(const uint64_t *keys, uint64_t ma
   return ~0ull;
 }
 
+struct InsertStats {
+  uint64_t probes = 0, fresh = 0, cas = 0, amin = 0;
+};
+
+// Inserts the N fingerprints of one lane's candidates (one row pair).  The
+// first probes of all of them are issued together, then the compare-and-
+// swaps into the slots that read EMPTY, so a lane waits on memory about
+// twice per pair rather than twice per candidate; the rest of a probe
+// sequence (an occupied first slot, mostly in the same cache line) follows
+// per candidate.  Then the companions (first occurrence: the smallest level
+// tag | index) and the candidates' slots.  An invalid fingerprint (EMPTY or
+// TOMBSTONE) is dropped (hash_set.cuh:200-204).
+template <int N>
+__device__ __forceinline__ void insert_group(const VfbLevelArgs &a, const uint64_t (&idx)[N],
+                                             const uint64_t (&fp)[N], uint64_t tfp,
+                                             InsertStats &is) {
+  uint64_t s[N], k[N];
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    s[c] = probe_start(fp[c], a.mask);
+    k[c] = fp[c] < kVfbTomb ? a.keys[2 * s[c]] : 0;
+  }
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    if (fp[c] < kVfbTomb && k[c] == kVfbEmpty) {
+      ++is.cas;
+      const uint64_t old = atomicCAS((unsigned long long *)&a.keys[2 * s[c]],
+                                     (unsigned long long)kVfbEmpty, (unsigned long long)fp[c]);
+      // k = the key now in the slot; EMPTY marks this lane's insert
+      k[c] = old == kVfbEmpty ? kVfbEmpty - 1 : old;
+      if (old == kVfbEmpty) ++is.fresh;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    if (fp[c] >= kVfbTomb) {
+      if (!a.last) a.slot_of[idx[c]] = ~0u;
+      continue;
+    }
+    ++is.probes;
+    uint64_t sl = s[c];
+    if (k[c] != kVfbEmpty - 1 && k[c] != fp[c]) {  // occupied by another key
+      bool ins = false;
+      sl = table_insert(a.keys, a.mask, fp[c], (s[c] + 1) & a.mask, 1, is.probes, a.max_probes,
+                        ins, is.cas);
+      is.fresh += ins ? 1u : 0u;
+      if (sl == ~0ull) {
+        atomicOr(a.fail, 1u);
+        if (!a.last) a.slot_of[idx[c]] = ~0u;
+        continue;
+      }
+    }
+    if ((!a.last || fp[c] == tfp) && a.minidx[2 * sl] > (a.tag | idx[c])) {
+      ++is.amin;
+      atomicMin((unsigned long long *)&a.minidx[2 * sl], (unsigned long long)(a.tag | idx[c]));
+    }
+    if (!a.last) a.slot_of[idx[c]] = (uint32_t)sl;
+  }
+}
+
+// Whether op(l, r) equals the target on every sample (a candidate whose
+// fingerprint is the target's).
+template <typename word>
+__device__ __noinline__ bool equals_target(const word *rows, uint64_t cap, int n, int op,
+                                           uint32_t l, uint32_t r, const word *tgt) {
+  for (int i = 0; i < n; ++i)
+    if (apply_op<word>(op, rows[(uint64_t)i * cap + l], rows[(uint64_t)i * cap + r]) != tgt[i])
+      return false;
+  return true;
+}
+
 // Pair-major form of the insert pass: one lane takes one (left, right) row
 // pair of a split (or one row for the unary ops) and produces every
 // candidate built on it -- & | ^ + * and both orders of '-' (or ~ and neg) --
 // so each pair of rows is read once for up to seven candidates.  Candidate
 // indices are those of the enumeration order (opbegin[op] + local index).
+// Within a split the pairs are taken in bands of kVfbBand right rows, every
+// left row of a band before the next band, so a band's right rows are read
+// by neighbouring lanes and workgroups close together in time (cache hits)
+// rather than once per left row across the whole right level; the
+// candidate indices stay those of the left-major order.  Consecutive
+// chunks of pairs go to workgroups of one XCD (workgroup w runs on XCD
+// w mod 8), whose L2 then holds the band.
 // The last level is hash-only: no rows are stored, so only the target's
 // fingerprint needs its first occurrence (the companion) and the level's new
 // fingerprints are counted as the empty slots this pass fills.
-struct InsertStats {
-  uint64_t probes = 0, fresh = 0, cas = 0, amin = 0;
-};
-
-template <typename word>
-__device__ __forceinline__ void insert_one(const VfbLevelArgs &a, uint64_t idx, uint64_t fp,
-                                           bool eq, uint64_t tfp, InsertStats &is) {
-  if (fp >= kVfbTomb) {  // invalid key: dropped (hash_set.cuh:200-204)
-    if (!a.last) a.slot_of[idx] = ~0u;
-    return;
-  }
-  bool ins = false;
-  const uint64_t s = table_insert(a.keys, a.mask, fp, is.probes, a.max_probes, ins, is.cas);
-  if (s == ~0ull) {
-    atomicOr(a.fail, 1u);
-    if (!a.last) a.slot_of[idx] = ~0u;
-    return;
-  }
-  is.fresh += ins ? 1u : 0u;
-  if ((!a.last || fp == tfp) && a.minidx[2 * s] > (a.tag | idx)) {
-    ++is.amin;
-    atomicMin((unsigned long long *)&a.minidx[2 * s], (unsigned long long)(a.tag | idx));
-  }
-  if (!a.last) a.slot_of[idx] = (uint32_t)s;
-  if (eq) atomicMin(a.found, (unsigned long long)idx);
-}
-
 template <typename word>
 __global__ void __launch_bounds__(kVfbThreads)
 k_vfb_insert_pairs(VfbLevelArgs a, const VfbPairBlock *pblocks, int npb, uint64_t npairs) {
@@ -203,9 +260,12 @@ k_vfb_insert_pairs(VfbLevelArgs a, const VfbPairBlock *pblocks, int npb, uint64_
   __syncthreads();
   const uint64_t tfp = s_tfp;
   const word *rows = (const word *)a.rows;
+  const int n = a.n;
   InsertStats is;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
+  const uint32_t G = gridDim.x;
+  const uint32_t wg = G % 8 == 0 ? (blockIdx.x % 8) * (G / 8) + blockIdx.x / 8 : blockIdx.x;
+  const uint64_t stride = (uint64_t)G * blockDim.x;
+  for (uint64_t p = (uint64_t)wg * blockDim.x + threadIdx.x; p < npairs; p += stride) {
     int b = 0;
     while (b + 1 < npb && p >= s_pb[b + 1].pbegin) ++b;
     const VfbPairBlock &B = s_pb[b];
@@ -213,42 +273,66 @@ k_vfb_insert_pairs(VfbLevelArgs a, const VfbPairBlock *pblocks, int npb, uint64_
     if (B.unary) {
       const uint32_t l = B.l0 + (uint32_t)local;
       uint64_t sn = 0, sg = 0;
-      bool en = true, eg = true;
-      for (int i = 0; i < a.n; ++i) {
-        const word x = rows[(uint64_t)i * a.cap + l];
-        const word vn = ~x, vg = word(0) - x;
-        en &= vn == s_tgt[i];
-        eg &= vg == s_tgt[i];
-        sn = fp_step<word>(sn, vn, i);
-        sg = fp_step<word>(sg, vg, i);
+      int i = 0;
+      for (; i + 1 < n; i += 2) {
+        const word x0 = rows[(uint64_t)i * a.cap + l], x1 = rows[(uint64_t)(i + 1) * a.cap + l];
+        sn = fp_step_p<word, 0>(sn, word(~x0));
+        sg = fp_step_p<word, 0>(sg, word(word(0) - x0));
+        sn = fp_step_p<word, 1>(sn, word(~x1));
+        sg = fp_step_p<word, 1>(sg, word(word(0) - x1));
       }
-      insert_one<word>(a, B.opbegin[NOT] + local, sn, en, tfp, is);
-      insert_one<word>(a, B.opbegin[NEG] + local, sg, eg, tfp, is);
+      if (i < n) {
+        const word x0 = rows[(uint64_t)i * a.cap + l];
+        sn = fp_step_p<word, 0>(sn, word(~x0));
+        sg = fp_step_p<word, 0>(sg, word(word(0) - x0));
+      }
+      const uint64_t idx[2] = {B.opbegin[NOT] + local, B.opbegin[NEG] + local};
+      const uint64_t fp[2] = {sn, sg};
+      insert_group<2>(a, idx, fp, tfp, is);
+      if (sn == tfp && equals_target<word>(rows, a.cap, n, NOT, l, l, s_tgt))
+        atomicMin(a.found, (unsigned long long)idx[0]);
+      if (sg == tfp && equals_target<word>(rows, a.cap, n, NEG, l, l, s_tgt))
+        atomicMin(a.found, (unsigned long long)idx[1]);
       continue;
     }
-    const uint64_t q = local / B.nr;
-    const uint32_t l = B.l0 + (uint32_t)q, r = B.r0 + (uint32_t)(local - q * B.nr);
+    // band of kVfbBand right rows (the last band may be narrower)
+    const uint64_t bw = (uint64_t)B.nl * kVfbBand;
+    const uint64_t band = local / bw;
+    const uint64_t in_band = local - band * bw;
+    const uint64_t r0 = band * kVfbBand;
+    const uint64_t w = B.nr - r0 < (uint64_t)kVfbBand ? B.nr - r0 : (uint64_t)kVfbBand;
+    const uint64_t q = in_band / w;
+    const uint32_t l = B.l0 + (uint32_t)q, r = B.r0 + (uint32_t)(r0 + (in_band - q * w));
+    const uint64_t lr = q * B.nr + r0 + (in_band - q * w);  // left-major pair index
     uint64_t sd[7] = {0, 0, 0, 0, 0, 0, 0};
-    bool eq[7] = {true, true, true, true, true, true, true};
-    for (int i = 0; i < a.n; ++i) {
+    auto sample = [&](auto odd, int i) {
+      constexpr int O = decltype(odd)::value;
       const word x = rows[(uint64_t)i * a.cap + l];
       const word y = rows[(uint64_t)i * a.cap + r];
-      const word v[7] = {word(x & y), word(x | y), word(x ^ y), word(x + y),
-                         word(x * y), word(x - y), word(y - x)};
-      const word t = s_tgt[i];
-#pragma unroll
-      for (int k = 0; k < 7; ++k) {
-        eq[k] &= v[k] == t;
-        sd[k] = fp_step<word>(sd[k], v[k], i);
-      }
+      sd[0] = fp_step_p<word, O>(sd[0], word(x & y));
+      sd[1] = fp_step_p<word, O>(sd[1], word(x | y));
+      sd[2] = fp_step_p<word, O>(sd[2], word(x ^ y));
+      sd[3] = fp_step_p<word, O>(sd[3], word(x + y));
+      sd[4] = fp_step_p<word, O>(sd[4], word(x * y));
+      sd[5] = fp_step_p<word, O>(sd[5], word(x - y));
+      sd[6] = fp_step_p<word, O>(sd[6], word(y - x));
+    };
+    int i = 0;
+    for (; i + 1 < n; i += 2) {
+      sample(std::integral_constant<int, 0>(), i);
+      sample(std::integral_constant<int, 1>(), i + 1);
     }
-    insert_one<word>(a, B.opbegin[AND] + local, sd[0], eq[0], tfp, is);
-    insert_one<word>(a, B.opbegin[OR] + local, sd[1], eq[1], tfp, is);
-    insert_one<word>(a, B.opbegin[XOR] + local, sd[2], eq[2], tfp, is);
-    insert_one<word>(a, B.opbegin[ADD] + local, sd[3], eq[3], tfp, is);
-    insert_one<word>(a, B.opbegin[MUL] + local, sd[4], eq[4], tfp, is);
-    insert_one<word>(a, B.opbegin[SUB] + 2 * local, sd[5], eq[5], tfp, is);
-    insert_one<word>(a, B.opbegin[SUB] + 2 * local + 1, sd[6], eq[6], tfp, is);
+    if (i < n) sample(std::integral_constant<int, 0>(), i);
+    const uint64_t idx[7] = {B.opbegin[AND] + lr, B.opbegin[OR] + lr, B.opbegin[XOR] + lr,
+                             B.opbegin[ADD] + lr, B.opbegin[MUL] + lr, B.opbegin[SUB] + 2 * lr,
+                             B.opbegin[SUB] + 2 * lr + 1};
+    insert_group<7>(a, idx, sd, tfp, is);
+    static constexpr int kOps[7] = {AND, OR, XOR, ADD, MUL, SUB, SUB};
+#pragma unroll
+    for (int c = 0; c < 7; ++c)
+      if (sd[c] == tfp &&
+          equals_target<word>(rows, a.cap, n, kOps[c], c == 6 ? r : l, c == 6 ? l : r, s_tgt))
+        atomicMin(a.found, (unsigned long long)idx[c]);
   }
   if (a.probes) atomicAdd(a.probes, (unsigned long long)is.probes);
   if (is.fresh) atomicAdd(a.new_keys, (unsigned long long)is.fresh);
@@ -363,7 +447,8 @@ __global__ void k_vfb_seed(VfbLevelArgs a, int L) {
   if (seed >= kVfbTomb) return;
   uint64_t probes = 0, cas = 0;
   bool ins = false;
-  const uint64_t s = table_insert(a.keys, a.mask, seed, probes, a.max_probes, ins, cas);
+  const uint64_t s = table_insert(a.keys, a.mask, seed, probe_start(seed, a.mask), 0, probes,
+                                  a.max_probes, ins, cas);
   if (s == ~0ull) {
     atomicOr(a.fail, 1u);
     return;
