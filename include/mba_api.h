@@ -80,7 +80,7 @@ typedef struct {
   int32_t block_log2;       /* sweep one (op, split) block per launch from   */
                             /* 2^block_log2 trees per size (0: defaults)    */
   int32_t hit_cap;          /* window-hit buffer, in hits (0: per-instance   */
-                            /* 2^22, batched 2^24; both above the hits of    */
+                            /* 2^22, batched 2^26; both above the hits of    */
                             /* one tile); a launch group (batched: a step)   */
                             /* that overflows it is replayed in parts, the   */
                             /* batched step by halves of its jobs, then of   */

@@ -27,7 +27,7 @@ namespace mba {
 
 namespace {
 
-constexpr uint32_t kBatchHitCap = 1u << 24;  // 512 MiB of hit records
+constexpr uint32_t kBatchHitCap = 1u << 26;  // 2 GiB of hit records
 // default value-table budget of a batch, split evenly over its instances
 // (at most 2/5 of the device memory): 4096 instances of up to 4 leaves keep
 // V[8] in tables at a size budget of 11, so the last size's B sides are
@@ -278,6 +278,21 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     return 0;
   };
   if (int rc = queue_room(job_cap, 4096)) return rc;
+  // Orders a step's jobs [j0, j1) as its kernels take them (see BatchStep):
+  // tabulated B sides, table builds and leaf scans first, then the B sides
+  // formed from blocks, each part in its former order; renumbers their
+  // tiles from 0 and returns the tiles of the first part.
+  auto order_step = [&](size_t j0, size_t j1) -> uint64_t {
+    auto tabulated = [](const BatchJob &J) { return !(J.kind == 1 && J.p.gen_b); };
+    std::stable_partition(h_jobs + j0, h_jobs + j1, tabulated);
+    uint64_t t = 0, first = ~0ull;
+    for (size_t k = j0; k < j1; ++k) {
+      if (first == ~0ull && !tabulated(h_jobs[k])) first = t;
+      h_jobs[k].tile0 = t;
+      t += h_jobs[k].ntiles;
+    }
+    return first == ~0ull ? t : first;
+  };
   // per-(instance, size) slots: the verified minimum rank, the slot flags and
   // the list of lowered slots of the resolution kernels; per instance, the
   // smallest size settled on the device
@@ -314,7 +329,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     st.reached = z;
     st.active = false;
   };
-  std::vector<std::vector<int>> order(count);  // learned: splits in score order
+  std::vector<std::vector<int>> order(count);  // learned: the splits in proposed order
   std::vector<float> scores(score_size ? count * 256 : 0);
   if (o.verbose)
     std::fprintf(stderr, "batch: %zu instances set up in %.3f ms\n", count,
@@ -349,13 +364,15 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   };
 
   // The jobs of size s for instance i (learned rounds: the splits of
-  // `splits`, else every split; table build and leaf scan in the first
+  // `splits` -- (split, instance mask) pairs -- else every split with every
+  // operator instance; table build and leaf scan in the first
   // round), appended to h_jobs[nj..] with step-relative first tiles.  Only
   // the pair tiles [win_from, win_from + win_len) of the size (counted over
   // its splits in order) are taken, the table build and leaf scan with the
   // window at 0; returns the end of the window when tiles remain past it,
   // else 0 (the size is complete).
-  auto size_jobs = [&](size_t i, int s, bool first_round, const std::vector<int> *splits,
+  auto size_jobs = [&](size_t i, int s, bool first_round,
+                       const std::vector<std::pair<int, uint32_t>> *splits,
                        size_t &nj, uint64_t &tiles, uint64_t win_from = 0,
                        uint64_t win_len = ~0ull) -> uint64_t {
     InstState &st = S[i];
@@ -383,7 +400,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       h_jobs[nj++] = J;
       return true;
     };
-    auto pair_job = [&](int j) {
+    auto pair_job = [&](int j, uint32_t mask) {
       const CheckTargets tg = targets_for(st.t0, s, max_size, W);
       BatchJob J;
       std::memset(&J, 0, sizeof J);
@@ -396,7 +413,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       J.p.nTA = (J.p.nA + kPairTA - 1) / kPairTA;
       J.p.gen_a = j > st.Kt;
       J.p.gen_b = J.p.m > st.Kt;
-      J.p.mask = kAllInst;
+      J.p.mask = mask;
       J.p.rb = kPairRB;
       J.lo_e = tg.lo_even;
       J.lo_o = tg.lo_odd;
@@ -455,9 +472,9 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     }
     if (s >= 3) {
       if (splits) {
-        for (int j : *splits) pair_job(j);
+        for (const auto &jm : *splits) pair_job(jm.first, jm.second);
       } else {
-        for (int j = 1; j <= (s - 1) / 2; ++j) pair_job(j);
+        for (int j = 1; j <= (s - 1) / 2; ++j) pair_job(j, kAllInst);
       }
     }
     return loc > win_end ? win_end : 0;
@@ -553,7 +570,13 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     for (size_t k = j0; k < j1; ++k) h_jobs[k].tile0 -= tbase;
     BatchStep &hs = h_steps[kMaxSize + 1];
     hs.total_tiles = tend - tbase;
-    hs.job_off = j0;
+    hs.tiles_table = hs.total_tiles;
+    for (size_t k = j0; k < j1; ++k)
+      if (h_jobs[k].kind == 1 && h_jobs[k].p.gen_b) {
+        hs.tiles_table = h_jobs[k].tile0;
+        break;
+      }
+    hs.job_off = (uint32_t)j0;
     hs.njobs = (uint32_t)(j1 - j0);
     hs.size_done = size_done;
     hs.inst_lo = parts_lo;
@@ -649,6 +672,15 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   // a launch sequence: the views change before the sequence runs, so no
   // earlier step of it may read them.  Returns 1 when tables are due and
   // the group already holds steps (they open the next group).
+  // Cohorts: from the size where the live instances' tables no longer fit
+  // the pool whole, only a cohort of instances whose whole tables fit is
+  // swept (the others wait at their next size, holding no tables), so every
+  // B side of a large size comes from full tables rather than from deep
+  // blocks over small shares; the next cohort starts once the current one
+  // has finished.  Instances are independent, so the order changes no answer.
+  std::vector<uint8_t> waiting(count, 0), swept(count, 0);  // swept: every size done
+  auto live = [&](size_t i) { return S[i].active && !waiting[i] && !swept[i]; };
+  bool force_repack = false;  // a new cohort: its tables placed whole
   size_t packed_active = count;
   int tables_for = -1;  // the size whose tables were last provided
   std::vector<std::pair<int, uint32_t>> builds;  // (table size, instance)
@@ -661,12 +693,12 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       return 0;
     }
     const size_t act = std::max<size_t>(n_active_now, 1);
-    const bool repack = s >= kRepackMin && n_active_now * 2 <= packed_active;
+    const bool repack = force_repack || (s >= kRepackMin && n_active_now * 2 <= packed_active);
     const uint64_t free_share =
         pool_all ? pool_elems - pool_used : (pool_elems - pool_used) / act;
     bool due = false;  // some table to place
     for (size_t i = 0; i < count && !due; ++i)
-      due = S[i].active && S[i].Kt < want && (repack || S[i].T->t[S[i].Kt + 1][8] <= free_share);
+      due = live(i) && S[i].Kt < want && (repack || S[i].T->t[S[i].Kt + 1][8] <= free_share);
     if (!due) {
       tables_for = s;
       return 0;
@@ -683,10 +715,12 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     if (repack) {
       packed_active = n_active_now;
       pool_used = 0;
-      const uint64_t share = pool_elems / act;
+      // a cohort's tables fit whole; else an equal share each
+      const uint64_t share = force_repack ? ~0ull : pool_elems / act;
+      force_repack = false;
       for (size_t i = 0; i < count; ++i) {
         InstState &st = S[i];
-        if (!st.active) {  // tables dropped: the view keeps Kt 0
+        if (!live(i)) {  // tables dropped: the view keeps Kt 0
           if (st.Kt > 0) tv_updates.push_back(TvUpdate{(uint32_t)i, 1u, 0, 0, 0});
           st.Kt = 0;
           continue;
@@ -704,7 +738,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       const uint64_t share = pool_all ? ~0ull : free_share;
       for (size_t i = 0; i < count; ++i) {
         InstState &st = S[i];
-        if (!st.active || st.Kt >= want) continue;
+        if (!live(i) || st.Kt >= want) continue;
         uint64_t got = 0;
         while (st.Kt < want) {
           const uint64_t n = st.T->t[st.Kt + 1][8];
@@ -742,7 +776,8 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       BatchStep &hs = h_steps[nsz++];
       std::memset(&hs, 0, sizeof hs);
       hs.total_tiles = tiles;
-      hs.job_off = j0;
+      hs.tiles_table = tiles;
+      hs.job_off = (uint32_t)j0;
       hs.njobs = (uint32_t)(nj - j0);
       group_tiles += tiles;
     }
@@ -753,7 +788,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   auto learned_size = [&](int s) {
     if (!score_size || s < 3) return false;
     for (size_t i = 0; i < count; ++i)
-      if (S[i].active && S[i].T->t[s][8] >= learned_min) return true;
+      if (live(i) && S[i].T->t[s][8] >= learned_min) return true;
     return false;
   };
   // A group ends once it holds this many tiles: with a budget or a cancel
@@ -770,16 +805,34 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   // inside the size).  Every active instance is at size s or s + 1.
   std::vector<int32_t> next_sz(count, 0);
   std::vector<uint64_t> part_t(count, 0);
+  // elements of V[1..want] of instance i, and whether the live instances'
+  // whole tables for size s exceed the pool
+  auto table_need = [&](size_t i, int want) {
+    uint64_t e = (uint64_t)S[i].L;
+    for (int m = 2; m <= want; ++m) e += S[i].T->t[m][8];
+    return e;
+  };
+  auto cohort_over = [&](int z) {
+    const int want = std::min(kt_cap, z - 2);
+    if (pool_all || want < 2) return false;
+    uint64_t need = 0;
+    for (size_t i = 0; i < count; ++i)
+      if (live(i)) need += table_need(i, want);
+    return need > pool_elems;
+  };
   auto pending_at = [&](int z) {
     for (size_t i = 0; i < count; ++i)
-      if (S[i].active && next_sz[i] == z) return true;
+      if (live(i) && next_sz[i] == z) return true;
     return false;
   };
-  while (s <= max_size) {
+  for (;;) {
     if (o.cancel && *o.cancel) {
       budget_hit = true;
       break;
     }
+    if (s > max_size)  // the live instances have swept every size
+      for (size_t i = 0; i < count; ++i)
+        if (live(i)) swept[i] = 1;
     size_t n_active = 0;
     for (size_t i = 0; i < count; ++i) {
       if (S[i].active && over_budget(i)) {
@@ -787,9 +840,39 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
         results[i].status = MBA_BUDGET;
         results[i].seconds = seconds_since(t_start);
       }
-      n_active += S[i].active;
+      n_active += live(i);
     }
-    if (n_active == 0) break;
+    if (n_active == 0) {
+      // the next cohort: the waiting instances at the smallest next size
+      int z = kMaxSize + 2;
+      for (size_t i = 0; i < count; ++i)
+        if (S[i].active && waiting[i]) z = std::min(z, (int)next_sz[i]);
+      if (z > max_size) break;
+      for (size_t i = 0; i < count; ++i)
+        if (S[i].active && waiting[i] && next_sz[i] == z) waiting[i] = 0, ++n_active;
+      s = z;
+      cursor = 0;
+      tables_for = -1;
+      force_repack = true;
+    }
+    if (cursor == 0 && cohort_over(s)) {
+      // keep the instances (in order) whose whole tables fit the pool
+      const int want = std::min(kt_cap, s - 2);
+      uint64_t used = 0;
+      n_active = 0;
+      for (size_t i = 0; i < count; ++i) {
+        if (!live(i)) continue;
+        const uint64_t need = table_need(i, want);
+        if (n_active > 0 && used + need > pool_elems) {
+          waiting[i] = 1;
+          continue;
+        }
+        used += need;
+        ++n_active;
+      }
+      tables_for = -1;
+      force_repack = true;
+    }
     n_active_now = n_active;
     if (cursor > 0 || !learned_size(s)) {
       // ---- a group of plain size steps, one launch sequence -------------
@@ -805,14 +888,14 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       int nsz = 0;
       const int s0 = s;
       while (s <= max_size && nsz < kMaxSize + 1) {
-        if (nsz > 0 && cursor == 0 && learned_size(s)) break;
+        if (nsz > 0 && cursor == 0 && (learned_size(s) || cohort_over(s))) break;
         if (table_steps(s, nj, nsz, group_tiles)) break;  // no room: the next group
         if (nsz == kMaxSize + 1) break;
         const size_t j0 = nj;
         uint64_t tiles = 0;
         size_t i = cursor;
         for (; i < count; ++i) {
-          if (!S[i].active || next_sz[i] != s) continue;
+          if (!live(i) || next_sz[i] != s) continue;
           const uint64_t next =
               size_jobs(i, s, true, nullptr, nj, tiles, part_t[i], allowance(i));
           part_t[i] = next;  // > 0: instance i resumes size s in a later pass
@@ -825,7 +908,8 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
         step_size[nsz] = s;
         BatchStep &hs = h_steps[nsz++];
         hs.total_tiles = tiles;
-        hs.job_off = j0;
+        hs.tiles_table = order_step(j0, nj);
+        hs.job_off = (uint32_t)j0;
         hs.njobs = (uint32_t)(nj - j0);
         // sizes 1..2 complete with the leaf scan (step 1); steps 0 and 2
         // settle nothing
@@ -835,7 +919,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
         group_tiles += tiles;
         bool skip_rest = true;  // no instance after i sweeps size s
         for (size_t k = i; k < count && skip_rest; ++k)
-          skip_rest = !(S[k].active && next_sz[k] == s);
+          skip_rest = !(live(k) && next_sz[k] == s);
         if (i < count && !skip_rest) {
           cursor = i;  // the group ends inside size s
           break;
@@ -925,43 +1009,39 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       }
     }
     if (int rc = (*score_size)(s, scores.data())) return rc;
-    size_t max_blocks = 0;
+    size_t max_splits = 0;
     for (size_t i = 0; i < count; ++i) {
       InstState &st = S[i];
       order[i].clear();
-      if (!st.active || st.T->t[s][8] < learned_min) continue;
+      if (!live(i) || st.T->t[s][8] < learned_min) continue;
       results[i].ordered_sizes++;
-      std::vector<BlockRef> blocks = binary_blocks(*st.T, s);
-      const float *sc = scores.data() + i * 256;
-      std::stable_sort(blocks.begin(), blocks.end(), [&](const BlockRef &x, const BlockRef &y) {
-        return sc[x.op * 32 + x.split] > sc[y.op * 32 + y.split];
-      });
-      for (const BlockRef &b : blocks)
-        if (std::find(order[i].begin(), order[i].end(), b.pair_j) == order[i].end())
-          order[i].push_back(b.pair_j);
-      max_blocks = std::max(max_blocks, order[i].size());
+      order[i] = learned_split_order(*st.T, s, scores.data() + i * 256);
+      max_splits = std::max(max_splits, order[i].size());
     }
+    // rounds of splits (each with every operator instance fused): the first
+    // proposed split, the second, the rest
     static const size_t kRoundEnd[3] = {1, 2, ~(size_t)0};
+    std::vector<std::pair<int, uint32_t>> sub;
     for (int round = 0; round < 3; ++round) {
       const size_t rb = round == 0 ? 0 : kRoundEnd[round - 1];
       const size_t re = kRoundEnd[round];
       size_t nj = 0;
       uint64_t tiles = 0;
-      std::vector<int> sub;
       for (size_t i = 0; i < count; ++i) {
-        InstState &st = S[i];
-        if (!st.active) continue;
+        if (!live(i)) continue;
         if (order[i].empty()) {
           if (round == 0) size_jobs(i, s, true, nullptr, nj, tiles);
           continue;
         }
-        sub.assign(order[i].begin() + std::min(rb, order[i].size()),
-                   order[i].begin() + std::min(re, order[i].size()));
+        sub.clear();
+        for (size_t k = std::min(rb, order[i].size()); k < std::min(re, order[i].size()); ++k)
+          sub.emplace_back(order[i][k], kAllInst);
         size_jobs(i, s, round == 0, &sub, nj, tiles);
       }
       if (nj > 0) {
         BatchStep &hs = h_steps[0];
         hs.total_tiles = tiles;
+        hs.tiles_table = order_step(0, nj);
         hs.job_off = 0;
         hs.njobs = (uint32_t)nj;
         hs.size_done = s;  // any verified tree of this size settles
@@ -983,10 +1063,10 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
         charge(seconds_since(t_g), tiles);
         if (int rc = collect()) return rc;
       }
-      if (re >= max_blocks) break;
+      if (re >= max_splits) break;
     }
     for (size_t i = 0; i < count; ++i)
-      if (S[i].active) {
+      if (live(i)) {
         S[i].reached = s;
         next_sz[i] = s + 1;
       }

@@ -7,6 +7,7 @@
 // (size, op) launch with a match, nocache.cu:102-105, which is one of the
 // trees of that size and op block).
 #include <algorithm>
+#include <cmath>
 #include <functional>
 #include <cstdio>
 #include <cstring>
@@ -39,7 +40,6 @@ constexpr uint64_t kFuseTrees = 1ull << 22;
 // (256*rb) >> logTB lanes take successive rows.  When the columns span a
 // whole tile, a tile takes up to kBandRows rows of its columns (the column
 // values are read once for all of them).
-constexpr uint64_t kBandRows = 8;
 
 static BBlock shape_block(BBlock b, int rb) {
   const uint64_t tile = (uint64_t)kPairThreads * rb;
@@ -50,7 +50,7 @@ static BBlock shape_block(BBlock b, int rb) {
   b.logTB = lg;
   b.ntc = (nX + (1ull << lg) - 1) >> lg;
   b.ntc_div = div_magic(b.ntc <= 0xffffffffull ? (uint32_t)b.ntc : 1u);
-  b.rows_pt = (1ull << lg) == tile ? (int32_t)std::min(nY, kBandRows) : 1;
+  b.rows_pt = (1ull << lg) == tile ? (int32_t)std::min<uint64_t>(nY, kBandRows) : 1;
   return b;
 }
 
@@ -165,6 +165,20 @@ void b_blocks(const CountTable &T, int Kt, int m, uint64_t start, uint64_t strid
     }
     r += cnt;
   }
+}
+
+std::vector<int> learned_split_order(const CountTable &T, int s, const float *scores) {
+  const std::vector<BlockRef> blocks = binary_blocks(T, s);
+  float top = -3.4e38f;
+  for (const BlockRef &b : blocks) top = std::max(top, scores[b.op * 32 + b.split]);
+  const int nj = (s - 1) / 2;
+  std::vector<double> mass(nj + 1, 0.0);
+  for (const BlockRef &b : blocks) mass[b.pair_j] += std::exp((double)(scores[b.op * 32 + b.split] - top));
+  std::vector<int> js;
+  for (int j = 1; j <= nj; ++j) js.push_back(j);
+  auto key = [&](int j) { return mass[j] / (double)(T.t[j][8] * T.t[s - 1 - j][8]); };
+  std::stable_sort(js.begin(), js.end(), [&](int x, int y) { return key(x) > key(y); });
+  return js;
 }
 
 int pick_filter_sample(const uint64_t *lv, const uint64_t *outv, int n, int L, int W,
@@ -631,12 +645,24 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     auto *h_step = (BatchStep *)stage;
     std::memset(h_step, 0, sizeof *h_step);
     h_step->total_tiles = tiles;
+    // tabulated B sides first, then those formed from blocks (BatchStep)
+    {
+      auto tabulated = [](const BatchJob &J) { return !J.p.gen_b; };
+      std::stable_partition(f_jobs.begin(), f_jobs.end(), tabulated);
+      uint64_t t = 0;
+      h_step->tiles_table = tiles;
+      for (BatchJob &J : f_jobs) {
+        if (h_step->tiles_table == tiles && !tabulated(J)) h_step->tiles_table = t;
+        J.tile0 = t;
+        t += J.ntiles;
+      }
+    }
     h_step->njobs = (uint32_t)nj;
     std::memcpy(stage + 256, f_jobs.data(), sizeof(BatchJob) * nj);
     std::memcpy(stage + off_blk, f_blocks.data(), sizeof(BBlock) * f_blocks.size());
     MBA_HIP(ctx.b_queue.ensure(Ctx::kPinnedBytes - kStageOff));
     MBA_HIP(hipMemcpyAsync(ctx.b_queue.p, stage, bytes, hipMemcpyHostToDevice, st));
-    MBA_HIP(hipMemsetAsync(d_head, 0, 8, st));
+    MBA_HIP(hipMemsetAsync(d_head, 0, 16, st));  // the two parts' tile counters
     MBA_HIP(hipEventRecord(ctx.ev0, st));
     const int g = (int)std::min<uint64_t>((uint64_t)ctx.cu_count * 8, tiles);
     MBA_HIP(launch_batch_step(W, ctx.b_queue.as<BatchStep>(),
@@ -922,23 +948,17 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
         if (int rc = (*score_size)(s)) return rc;
       }
       if (block_scores) {
-        std::stable_sort(blocks.begin(), blocks.end(), [&](const BlockRef &x, const BlockRef &y) {
-          return block_scores[x.op * 32 + x.split] > block_scores[y.op * 32 + y.split];
-        });
-      }
-      if (block_scores) {
         ++res.ordered_sizes;
-        // learned order: the splits (pairs {j, s-1-j}) in the order of their
-        // best-scored block, each swept once with all operator instances
-        // fused (its B side is formed once), in rounds: the best split alone,
-        // then one split per round, small splits sharing a round until it
-        // holds kRoundPairs pairs (a round costs a sync); a round that yields
-        // a verified solution ends the size (any solution of the minimal
-        // size is accepted)
+        // learned order: the splits (pairs {j, s-1-j}) in the order of the
+        // scores' probability of their blocks per pair checked, each swept
+        // once with all operator instances fused (its B side is formed
+        // once), in rounds: the first split alone, then one split per
+        // round, small splits sharing a round until it holds kRoundPairs
+        // pairs (a round costs a sync); a round that yields a verified
+        // solution ends the size (any solution of the minimal size is
+        // accepted)
         constexpr uint64_t kRoundPairs = 1ull << 26;
-        std::vector<int> js;
-        for (const BlockRef &b : blocks)
-          if (std::find(js.begin(), js.end(), b.pair_j) == js.end()) js.push_back(b.pair_j);
+        const std::vector<int> js = learned_split_order(T, s, block_scores);
         uint64_t round_pairs = 0;
         for (size_t k = 0; k < js.size() && !stopped; ++k) {
           const int rc = run_pair(s, js[k], kAllInst, tg);

@@ -23,6 +23,10 @@ struct BlockRef {
 };
 // Binary blocks of size s in rank order (nocache.cu:52-79).
 std::vector<BlockRef> binary_blocks(const CountTable &T, int s);
+// The splits {j, s-1-j} of size s in the order learned block scores
+// (scores[op * 32 + left size]) propose: by the probability their blocks
+// get (softmax over the size's blocks) per pair checked, highest first.
+std::vector<int> learned_split_order(const CountTable &T, int s, const float *scores);
 // Below this many trees a size is swept with fused pairs and one sync.
 constexpr uint64_t kBlockModeMin = 1ull << 31;
 // The same threshold for the learned order, whose point is to stop early.

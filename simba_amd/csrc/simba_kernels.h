@@ -54,6 +54,7 @@ struct CheckTargets {
 // leaf count and tabulated depth.  tile_pre: the B tiles of the blocks
 // before this one in its list (a job's B side runs through a whole list).
 constexpr int kMaxBSteps = 8;
+constexpr uint32_t kBandRows = 8;  // rows of a row-band tile (at most)
 constexpr int kTabShift = 56;
 constexpr uint64_t kTabIndex = (1ull << kTabShift) - 1;
 __host__ __device__ inline uint64_t tab_ref(int m, uint64_t index) {
@@ -124,14 +125,19 @@ struct BatchJob {
 // One size step of a batch: its jobs are jobs_base[job_off, job_off + njobs)
 // (global tiles [0, total_tiles)); once its hits are resolved, every size up
 // to size_done is complete for the instances [inst_lo, inst_hi).
+// The step's jobs with a tabulated B side (and table builds, leaf scans) come
+// first, tiles [0, tiles_table); the jobs whose B side is formed from blocks
+// follow, tiles [tiles_table, total_tiles): each part runs in a kernel of
+// its own (the first needs far fewer registers, so more waves hide its
+// memory latency).
 struct BatchStep {
   uint64_t total_tiles;
-  uint64_t job_off;
+  uint64_t tiles_table;
+  uint32_t job_off;
   uint32_t njobs;
   int32_t size_done;
   uint32_t inst_lo, inst_hi;
   uint32_t index;  // position of the step in its launch sequence
-  uint32_t pad;
 };
 
 // Device counters of a batch step: window hits, screened candidates,

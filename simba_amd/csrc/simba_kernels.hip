@@ -217,6 +217,35 @@ __device__ __forceinline__ word build_value(const TableView *__restrict__ tv, in
   return apply_op<word>(op, V[tv->off[j] + a], V[tv->off[cs - j] + (x - a * rs)]);
 }
 
+// The children of V[m][r] as table element offsets (ib unused for a unary
+// root) and the root op: build_value without the loads, so a lane can issue
+// the loads of several values together.
+__device__ __forceinline__ int build_refs(const TableView *__restrict__ tv, int m, uint64_t r,
+                                          uint64_t &ia, uint64_t &ib) {
+  uint64_t x = r;
+  int op = 0;
+  while (x >= tv->T[m][op]) x -= tv->T[m][op++];
+  if (is_unary(op)) {
+    ia = tv->off[m - 1] + x;
+    ib = ia;
+    return op;
+  }
+  const int cs = m - 1;
+  int j = 1;
+  for (;; ++j) {
+    const uint64_t blk = tv->T[j][8] * tv->T[cs - j][8];
+    if (x < blk) break;
+    x -= blk;
+  }
+  const uint64_t rs = tv->T[cs - j][8];
+  uint64_t a;
+  if ((x >> 32) == 0 && rs <= 0xffffffffull) a = (uint32_t)x / (uint32_t)rs;
+  else a = x / rs;
+  ia = tv->off[j] + a;
+  ib = tv->off[cs - j] + (x - a * rs);
+  return op;
+}
+
 template <typename word>
 __global__ void __launch_bounds__(256)
 k_table_build(const TableView *__restrict__ tv, word *__restrict__ out, int m,
@@ -861,9 +890,17 @@ __device__ __forceinline__ void desc_band_tile(const TableView *__restrict__ tv,
   const uint32_t nr = (uint32_t)(nY - y0 < (uint64_t)bb.rows_pt ? nY - y0 : bb.rows_pt);
   const uint32_t pair = ((uint32_t)p.sb << 16) | (uint32_t)p.j;
   const uint32_t le = lo32(lo_e), lo = lo32(lo_o), sp = (uint32_t)span;
+  // the tile's row values, loaded together (up to kBandRows)
+  uint32_t rows_lo[kBandRows];
+#pragma unroll
+  for (uint32_t r = 0; r < kBandRows; ++r)
+    rows_lo[r] = uniform(bb.op < 0 || r >= nr ? 0u : lo32_at<word>(rowp, y0 + r));
+#pragma unroll 1
   for (uint32_t r = 0; r < nr; ++r) {  // workgroup-uniform
     const uint64_t y = y0 + r;
-    const uint32_t rowv = bb.op < 0 ? 0u : lo32_at<word>(rowp, y);
+    uint32_t rowv = rows_lo[0];
+#pragma unroll
+    for (uint32_t q = 1; q < kBandRows; ++q) rowv = r == q ? rows_lo[q] : rowv;
     switch (bb.op) {  // workgroup-uniform; only '-' depends on the order
       case AND:
 #pragma unroll
@@ -1325,7 +1362,7 @@ struct BatchPair : PairCore {
   __device__ BatchPair(const PairCore &c, const BBlock &b) : PairCore(c), bb(b) {}
 };
 
-template <typename word>
+template <typename word, bool BLOCKS>
 __device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J,
                            const BBlock *__restrict__ blocks, uint64_t lt, uint64_t &staged,
                            bool &bitops, HitRec *hits, uint32_t *hc, uint32_t cap) {
@@ -1358,24 +1395,25 @@ __device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J,
                            lo32(lo_o), (uint32_t)span);
     staged = at;
   }
-  // the B side: its table, a block formed from tables, or a generic rank range
-  const int bs = !p.gen_b ? kBTable : p.bb.generic ? kBGeneric : kBDesc;
+  // the B side: its table (BLOCKS false), a block formed from tables or a
+  // generic rank range (BLOCKS true)
 #define BATCH_TILE(SAME_, BS)                                                              \
   pair_tile<word, SAME_, BS, kPairRB, BatchPair>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o, \
                                                  span, hits, hc, cap, tag)
-  if (p.j == p.m) {
-    if (bs == kBTable) BATCH_TILE(true, kBTable);
-    else if (bs == kBDesc) BATCH_TILE(true, kBDesc);
-    else BATCH_TILE(true, kBGeneric);
+  if (!BLOCKS) {
+    if (p.j == p.m) BATCH_TILE(true, kBTable);
+    else BATCH_TILE(false, kBTable);
+  } else if (p.j == p.m) {
+    if (p.bb.generic) BATCH_TILE(true, kBGeneric);
+    else BATCH_TILE(true, kBDesc);
   } else {
-    if (bs == kBTable) BATCH_TILE(false, kBTable);
-    else if (bs == kBDesc) BATCH_TILE(false, kBDesc);
-    else BATCH_TILE(false, kBGeneric);
+    if (p.bb.generic) BATCH_TILE(false, kBGeneric);
+    else BATCH_TILE(false, kBDesc);
   }
 #undef BATCH_TILE
 }
 
-template <typename word>
+template <typename word, bool BLOCKS>
 __global__ void __launch_bounds__(kPairThreads)
 k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jobs_base,
              const BBlock *__restrict__ blocks, const TableView *__restrict__ tvs,
@@ -1383,19 +1421,22 @@ k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jo
              const int32_t *__restrict__ settled) {
   __shared__ unsigned long long s_t;
   __shared__ AStage S;
-  const uint64_t total = step->total_tiles;
+  // this kernel's part of the step's tiles (see BatchStep), its own counter
+  const uint64_t first = BLOCKS ? step->tiles_table : 0;
+  const uint64_t total = BLOCKS ? step->total_tiles : step->tiles_table;
+  if (BLOCKS) ++head;
   const int njobs = (int)step->njobs;
   const BatchJob *__restrict__ jobs = jobs_base + step->job_off;
   // tiles per counter grab: up to 32, fewer when the step has few tiles per
   // workgroup, so the tail stays short
-  const uint64_t per = total / ((uint64_t)gridDim.x * 8u);
+  const uint64_t per = (total - first) / ((uint64_t)gridDim.x * 8u);
   const uint64_t K = per < 1 ? 1 : (per > 32 ? 32 : per);
   int jb = 0, jcur = -1;  // workgroup-uniform job cursor
   BatchJob J;
   uint64_t staged = ~0ull;  // A tile of job jcur staged in S
   bool bitops = false;
   for (;;) {
-    if (threadIdx.x == 0) s_t = atomicAdd(head, (unsigned long long)K);
+    if (threadIdx.x == 0) s_t = first + atomicAdd(head, (unsigned long long)K);
     __syncthreads();
     const uint64_t t0 = s_t;
     __syncthreads();
@@ -1421,15 +1462,31 @@ k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jo
       if (settled && settled[J.inst] > 0) continue;
       const TableView *tv = tvs + J.inst;
       const uint64_t lt = t - J.tile0 + J.p.tile_begin;
-      if (J.kind == 0) {  // table build: V[m] ranks [lt*kBuildTile, +kBuildTile)
+      if (!BLOCKS && J.kind == 0) {  // table build: V[m] ranks [lt*kBuildTile, +kBuildTile)
+        // each lane's ranks: child references first, then every load, then
+        // the stores (the table being written is not read here)
+        constexpr int kPer = kBuildTile / kPairThreads;
+        const word *V = global_ptr<const word>(tv->base);
         word *out = global_ptr<word>(tv->base) + tv->off[J.p.m];
-        const uint64_t r0 = lt * kBuildTile;
-        const uint64_t r1 = r0 + kBuildTile < J.p.nB ? r0 + kBuildTile : J.p.nB;
-        for (uint64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x)
-          out[r] = build_value<word>(tv, J.p.m, r);
+        const uint64_t r0 = lt * kBuildTile + threadIdx.x;
+        uint64_t ia[kPer], ib[kPer];
+        int op[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+          const uint64_t r = r0 + (uint64_t)k * kPairThreads;
+          op[k] = r < J.p.nB ? build_refs(tv, J.p.m, r, ia[k], ib[k]) : -1;
+          if (op[k] < 0) ia[k] = ib[k] = tv->off[1];
+        }
+        word x[kPer], y[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) x[k] = V[ia[k]], y[k] = V[ib[k]];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+          if (op[k] >= 0) out[r0 + (uint64_t)k * kPairThreads] = apply_op<word>(op[k], x[k], y[k]);
       } else if (J.kind == 1) {
-        batch_pair<word>(S, tv, J, blocks, lt, staged, bitops, hits, hit_count, hit_cap);
-      } else {  // leaves against the windows of base size 1
+        batch_pair<word, BLOCKS>(S, tv, J, blocks, lt, staged, bitops, hits, hit_count,
+                                 hit_cap);
+      } else if (!BLOCKS) {  // leaves against the windows of base size 1
         const word *V = global_ptr<const word>(tv->base) + tv->off[1];
         if ((int)threadIdx.x < tv->L) {
           const word c = V[threadIdx.x];
@@ -1478,7 +1535,8 @@ hipError_t launch_tv_update(TableView *d_tvs, const TvUpdate *d_up, uint32_t n,
 
 __global__ void k_batch_reset(unsigned long long *head, BatchCounters *cnt) {
   if (threadIdx.x == 0) {
-    *head = 0;
+    head[0] = 0;
+    head[1] = 0;
     cnt->hits = 0;
     cnt->surv = 0;
     cnt->touched = 0;
@@ -1807,14 +1865,23 @@ hipError_t launch_batch_step(int width, const BatchStep *d_step, const BatchJob 
                              unsigned long long *d_head, HitRec *d_hits,
                              uint32_t *d_hit_count, uint32_t hit_cap,
                              const int32_t *d_settled, int grid, hipStream_t stream) {
-  if (width == 32)
-    hipLaunchKernelGGL(k_batch_step<uint32_t>, dim3(grid), dim3(kPairThreads), 0, stream,
-                       d_step, d_jobs_base, d_blocks, d_tvs, d_head, d_hits, d_hit_count,
-                       hit_cap, d_settled);
-  else
-    hipLaunchKernelGGL(k_batch_step<uint64_t>, dim3(grid), dim3(kPairThreads), 0, stream,
-                       d_step, d_jobs_base, d_blocks, d_tvs, d_head, d_hits, d_hit_count,
-                       hit_cap, d_settled);
+  // the tabulated part, then the part formed from blocks (an empty part's
+  // kernel returns at once)
+  if (width == 32) {
+    hipLaunchKernelGGL((k_batch_step<uint32_t, false>), dim3(grid), dim3(kPairThreads), 0,
+                       stream, d_step, d_jobs_base, d_blocks, d_tvs, d_head, d_hits,
+                       d_hit_count, hit_cap, d_settled);
+    hipLaunchKernelGGL((k_batch_step<uint32_t, true>), dim3(grid), dim3(kPairThreads), 0,
+                       stream, d_step, d_jobs_base, d_blocks, d_tvs, d_head, d_hits,
+                       d_hit_count, hit_cap, d_settled);
+  } else {
+    hipLaunchKernelGGL((k_batch_step<uint64_t, false>), dim3(grid), dim3(kPairThreads), 0,
+                       stream, d_step, d_jobs_base, d_blocks, d_tvs, d_head, d_hits,
+                       d_hit_count, hit_cap, d_settled);
+    hipLaunchKernelGGL((k_batch_step<uint64_t, true>), dim3(grid), dim3(kPairThreads), 0,
+                       stream, d_step, d_jobs_base, d_blocks, d_tvs, d_head, d_hits,
+                       d_hit_count, hit_cap, d_settled);
+  }
   return hipGetLastError();
 }
 
