@@ -1362,25 +1362,39 @@ struct BatchPair : PairCore {
   __device__ BatchPair(const PairCore &c, const BBlock &b) : PairCore(c), bb(b) {}
 };
 
+// The block of a job's B side that the last tile fell in: its B tiles
+// [lo, hi) and index (workgroup-uniform), so consecutive tiles of a block
+// skip the search.
+struct BlockCursor {
+  uint64_t lo = 0, hi = 0;
+  int k = 0;
+};
+
 template <typename word, bool BLOCKS>
 __device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J,
                            const BBlock *__restrict__ blocks, uint64_t lt, uint64_t &staged,
-                           bool &bitops, HitRec *hits, uint32_t *hc, uint32_t cap) {
+                           bool &bitops, BlockCursor &bc, HitRec *hits, uint32_t *hc,
+                           uint32_t cap) {
   const word *base = global_ptr<const word>(tv->base);
   const uint64_t at = J.p.nTA == 1 ? 0 : lt % J.p.nTA;
   uint64_t bt = J.p.nTA == 1 ? lt : lt / J.p.nTA;
   // a B side above the tables runs through the blocks [bb, bb + nbb): the
   // last one whose first tile (tile_pre) is at most bt holds tile bt
   const BBlock *bp = blocks + J.bb;  // the table source reads no block
-  if (J.p.gen_b && J.nbb > 1) {
-    int lo = 0, hi = (int)J.nbb - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (bp[mid].tile_pre <= bt) lo = mid;
-      else hi = mid - 1;
+  if (BLOCKS && J.nbb > 1) {
+    if (bt < bc.lo || bt >= bc.hi) {
+      int lo = 0, hi = (int)J.nbb - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bp[mid].tile_pre <= bt) lo = mid;
+        else hi = mid - 1;
+      }
+      bc.k = lo;
+      bc.lo = bp[lo].tile_pre;
+      bc.hi = lo + 1 < (int)J.nbb ? bp[lo + 1].tile_pre : ~0ull;
     }
-    bp += lo;
-    bt -= bp->tile_pre;
+    bp += bc.k;
+    bt -= bc.lo;
   }
   const BatchPair p(J.p, *bp);
   const word *A = base + (p.gen_a ? 0 : tv->off[p.j]);
@@ -1413,8 +1427,10 @@ __device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J,
 #undef BATCH_TILE
 }
 
+// At least 4 waves per SIMD: 128 registers hold every path of the kernel
+// without spills, and the extra waves hide the small tiles' memory latency.
 template <typename word, bool BLOCKS>
-__global__ void __launch_bounds__(kPairThreads)
+__global__ void __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(4)))
 k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jobs_base,
              const BBlock *__restrict__ blocks, const TableView *__restrict__ tvs,
              unsigned long long *head, HitRec *hits, uint32_t *hit_count, uint32_t hit_cap,
@@ -1432,6 +1448,8 @@ k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jo
   const uint64_t per = (total - first) / ((uint64_t)gridDim.x * 8u);
   const uint64_t K = per < 1 ? 1 : (per > 32 ? 32 : per);
   int jb = 0, jcur = -1;  // workgroup-uniform job cursor
+  bool done = false;      // the job's instance is settled
+  BlockCursor bc;         // the job's B block of the last tile
   BatchJob J;
   uint64_t staged = ~0ull;  // A tile of job jcur staged in S
   bool bitops = false;
@@ -1457,9 +1475,12 @@ k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jo
         J = jobs[jb];
         jcur = jb;
         staged = ~0ull;
+        bc.lo = bc.hi = 0;
+        // an instance settled at a smaller size needs no more work (the
+        // settled sizes change only between steps)
+        done = settled && settled[J.inst] > 0;
       }
-      // an instance settled at a smaller size needs no more work
-      if (settled && settled[J.inst] > 0) continue;
+      if (done) continue;
       const TableView *tv = tvs + J.inst;
       const uint64_t lt = t - J.tile0 + J.p.tile_begin;
       if (!BLOCKS && J.kind == 0) {  // table build: V[m] ranks [lt*kBuildTile, +kBuildTile)
@@ -1484,7 +1505,7 @@ k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jo
         for (int k = 0; k < kPer; ++k)
           if (op[k] >= 0) out[r0 + (uint64_t)k * kPairThreads] = apply_op<word>(op[k], x[k], y[k]);
       } else if (J.kind == 1) {
-        batch_pair<word, BLOCKS>(S, tv, J, blocks, lt, staged, bitops, hits, hit_count,
+        batch_pair<word, BLOCKS>(S, tv, J, blocks, lt, staged, bitops, bc, hits, hit_count,
                                  hit_cap);
       } else if (!BLOCKS) {  // leaves against the windows of base size 1
         const word *V = global_ptr<const word>(tv->base) + tv->off[1];
