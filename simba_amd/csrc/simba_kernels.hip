@@ -1484,26 +1484,74 @@ k_batch_step(const BatchStep *__restrict__ step, const BatchJob *__restrict__ jo
       const TableView *tv = tvs + J.inst;
       const uint64_t lt = t - J.tile0 + J.p.tile_begin;
       if (!BLOCKS && J.kind == 0) {  // table build: V[m] ranks [lt*kBuildTile, +kBuildTile)
-        // each lane's ranks: child references first, then every load, then
-        // the stores (the table being written is not read here)
+        // The tile's first rank is located once (workgroup-uniform): its
+        // (op, left size) block of V[m], the block's children and the
+        // rank's row a0 and column b0 in it.  A lane's ranks inside that
+        // block are rows and columns of it found with a compare (a block
+        // at least kBuildTile wide) or a 32-bit division; ranks past the
+        // block's end (a tile spanning blocks) decompose on their own.
         constexpr int kPer = kBuildTile / kPairThreads;
+        const int m = J.p.m;
         const word *V = global_ptr<const word>(tv->base);
-        word *out = global_ptr<word>(tv->base) + tv->off[J.p.m];
-        const uint64_t r0 = lt * kBuildTile + threadIdx.x;
+        word *out = global_ptr<word>(tv->base) + tv->off[m];
+        const uint64_t rt = lt * kBuildTile;
+        uint64_t x = rt, blk_end = 0, nB = 1, offA, offB;
+        int op = 0;
+        while (x >= tv->T[m][op]) x -= tv->T[m][op++];
+        if (is_unary(op)) {
+          offA = tv->off[m - 1];
+          offB = offA;
+          blk_end = rt - x + tv->T[m][op];
+        } else {
+          int j = 1;
+          for (;; ++j) {
+            const uint64_t blk = tv->T[j][8] * tv->T[m - 1 - j][8];
+            if (x < blk) break;
+            x -= blk;
+          }
+          nB = tv->T[m - 1 - j][8];
+          offA = tv->off[j];
+          offB = tv->off[m - 1 - j];
+          blk_end = rt - x + tv->T[j][8] * nB;
+        }
+        const uint64_t a0 = uniform(x / nB), b0 = uniform(x - a0 * nB);
         uint64_t ia[kPer], ib[kPer];
-        int op[kPer];
+        int opk[kPer];
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
-          const uint64_t r = r0 + (uint64_t)k * kPairThreads;
-          op[k] = r < J.p.nB ? build_refs(tv, J.p.m, r, ia[k], ib[k]) : -1;
-          if (op[k] < 0) ia[k] = ib[k] = tv->off[1];
+          const uint32_t d = threadIdx.x + (uint32_t)k * kPairThreads;
+          const uint64_t r = rt + d;
+          if (r >= J.p.nB) {
+            opk[k] = -1;
+            ia[k] = ib[k] = tv->off[1];
+          } else if (r < blk_end) {
+            const uint64_t t = b0 + d;
+            uint64_t a, b;
+            if (nB >= kBuildTile) {
+              const bool wrap = t >= nB;
+              a = a0 + (wrap ? 1 : 0);
+              b = wrap ? t - nB : t;
+            } else {
+              const uint32_t q = (uint32_t)t / (uint32_t)nB;
+              a = a0 + q;
+              b = t - (uint64_t)q * nB;
+            }
+            opk[k] = op;
+            ia[k] = offA + (is_unary(op) ? x + d : a);
+            ib[k] = is_unary(op) ? ia[k] : offB + b;
+          } else {
+            opk[k] = build_refs(tv, m, r, ia[k], ib[k]);
+          }
         }
-        word x[kPer], y[kPer];
+        // every load, then the stores (the table being written is not read)
+        word xv[kPer], yv[kPer];
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) x[k] = V[ia[k]], y[k] = V[ib[k]];
+        for (int k = 0; k < kPer; ++k) xv[k] = V[ia[k]], yv[k] = V[ib[k]];
 #pragma unroll
         for (int k = 0; k < kPer; ++k)
-          if (op[k] >= 0) out[r0 + (uint64_t)k * kPairThreads] = apply_op<word>(op[k], x[k], y[k]);
+          if (opk[k] >= 0)
+            out[rt + threadIdx.x + (uint64_t)k * kPairThreads] =
+                apply_op<word>(opk[k], xv[k], yv[k]);
       } else if (J.kind == 1) {
         batch_pair<word, BLOCKS>(S, tv, J, blocks, lt, staged, bitops, bc, hits, hit_count,
                                  hit_cap);
