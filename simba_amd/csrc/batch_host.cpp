@@ -422,7 +422,11 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       J.kind = 1;
       if (!J.p.gen_b) {
         J.p.nTB = (J.p.nB + kPairTB - 1) / kPairTB;
-        J.ntiles = J.p.nTA * J.p.nTB;
+        // few A values per tile: up to 8 B tiles per job tile, so a tile
+        // holds about 64 A values' worth of pairs
+        const uint64_t na = std::min<uint64_t>(J.p.nA, kPairTA);
+        J.bsub = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(8, 64 / na));
+        J.ntiles = J.p.nTA * ((J.p.nTB + J.bsub - 1) / J.bsub);
         push(J);
         return;
       }

@@ -120,6 +120,9 @@ struct BatchJob {
   uint32_t bb;                // B side above Kt: its first block (index in the blocks)
   uint32_t nbb;               // blocks of the B side (0 or 1: block bb alone); the
                               // job's B tiles run through blocks [bb, bb + nbb)
+  uint32_t bsub;              // tabulated B side: B tiles per job tile (0 or 1: one),
+                              // so a job of few A values has tiles of more pairs
+  uint32_t pad;
 };
 
 // One size step of a batch: its jobs are jobs_base[job_off, job_off + njobs)

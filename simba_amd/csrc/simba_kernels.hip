@@ -1415,8 +1415,14 @@ __device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J,
   pair_tile<word, SAME_, BS, kPairRB, BatchPair>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o, \
                                                  span, hits, hc, cap, tag)
   if (!BLOCKS) {
-    if (p.j == p.m) BATCH_TILE(true, kBTable);
-    else BATCH_TILE(false, kBTable);
+    // J.bsub consecutive B tiles per job tile
+    const uint32_t sub = J.bsub > 1 ? J.bsub : 1;
+    const uint64_t b0 = bt * sub;
+    for (uint32_t q = 0; q < sub && b0 + q < p.nTB; ++q) {
+      bt = b0 + q;
+      if (p.j == p.m) BATCH_TILE(true, kBTable);
+      else BATCH_TILE(false, kBTable);
+    }
   } else if (p.j == p.m) {
     if (p.bb.generic) BATCH_TILE(true, kBGeneric);
     else BATCH_TILE(true, kBDesc);
