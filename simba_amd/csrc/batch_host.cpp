@@ -352,7 +352,12 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     tv_updates.clear();
     return 0;
   };
+  bool pool_overrun = false;  // a table placed past the pool (never: reported as an error)
   auto flush_tables = [&]() -> int {
+    if (pool_overrun) {
+      ctx.last_error = "batch: a value table placed past the table pool";
+      return MBA_ERR_LIMIT;
+    }
     if (leaves_pending) {  // after the views: V[1] rewritten where the pool was packed again
       leaves_pending = false;
       if (int rc = flush_views()) return rc;
@@ -711,6 +716,10 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     tables_for = s;
     builds.clear();
     auto place = [&](size_t i, int m) {
+      if (pool_used + S[i].T->t[m][8] > pool_elems) {  // never: the callers leave room
+        pool_overrun = true;
+        return;
+      }
       tvs[i].off[m] = pool_used;
       tv_updates.push_back(TvUpdate{(uint32_t)i, (uint32_t)m, pool_used, 0, 0});
       pool_used += S[i].T->t[m][8];
@@ -719,9 +728,15 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     if (repack) {
       packed_active = n_active_now;
       pool_used = 0;
-      // a cohort's tables fit whole; else an equal share each
+      // a cohort's tables whole (its first instance may alone need more
+      // than the pool: it gets what fits); else an equal share each.  An
+      // instance never takes the room the later live instances' V[1] need,
+      // so every placement stays inside the pool.
       const uint64_t share = force_repack ? ~0ull : pool_elems / act;
       force_repack = false;
+      uint64_t leaves_after = 0;  // V[1] elements of the live instances not placed yet
+      for (size_t i = 0; i < count; ++i)
+        if (live(i)) leaves_after += (uint64_t)S[i].L;
       for (size_t i = 0; i < count; ++i) {
         InstState &st = S[i];
         if (!live(i)) {  // tables dropped: the view keeps Kt 0
@@ -729,10 +744,13 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
           st.Kt = 0;
           continue;
         }
+        leaves_after -= (uint64_t)st.L;
+        const uint64_t room = pool_elems - pool_used - leaves_after;  // >= st.L
+        const uint64_t lim = std::min(share, room);
         const int top = std::min(kt_cap, std::max(want, st.Kt));
         uint64_t used = (uint64_t)st.L;
         int k = 1;
-        while (k < top && used + st.T->t[k + 1][8] <= share) used += st.T->t[++k][8];
+        while (k < top && used + st.T->t[k + 1][8] <= lim) used += st.T->t[++k][8];
         for (int m = 1; m <= k; ++m) place(i, m);
         st.Kt = k;
         tvs[i].Kt = k;
@@ -811,6 +829,23 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
   std::vector<uint64_t> part_t(count, 0);
   // elements of V[1..want] of instance i, and whether the live instances'
   // whole tables for size s exceed the pool
+  // An active instance inside a size's tiles (a budget window) or waiting
+  // for its cohort settles no size from its next one on: settled = -(next
+  // size) bounds the device's settle scan below it (a window hit of a
+  // smaller tree may already hold a larger size's slot).  Written to the
+  // device before a launch sequence whenever some bound is or was set.
+  bool bounds_set = false;
+  auto bound_settle = [&]() -> int {
+    bool any = o.time_budget_s > 0;
+    for (size_t k = 0; k < count && !any; ++k) any = S[k].active && waiting[k];
+    if (!any && !bounds_set) return 0;
+    bounds_set = any;
+    for (size_t k = 0; k < count; ++k)
+      if (settled[k] <= 0)
+        settled[k] = S[k].active && (waiting[k] || part_t[k] > 0) ? -next_sz[k] : 0;
+    MBA_HIP(hipMemcpyAsync(d_settled, settled.data(), count * 4, hipMemcpyHostToDevice, stq));
+    return 0;
+  };
   auto table_need = [&](size_t i, int want) {
     uint64_t e = (uint64_t)S[i].L;
     for (int m = 2; m <= want; ++m) e += S[i].T->t[m][8];
@@ -934,14 +969,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
         if (group_tiles >= kGroupTiles) break;
       }
       if (int rc = flush_tables()) return rc;
-      if (o.time_budget_s > 0) {
-        // an instance inside a size's tiles settles no size it has not
-        // swept whole: settled = -(that size) bounds the settle scan below it
-        for (size_t k = 0; k < count; ++k)
-          if (settled[k] <= 0) settled[k] = S[k].active && part_t[k] > 0 ? -next_sz[k] : 0;
-        MBA_HIP(hipMemcpyAsync(d_settled, settled.data(), count * 4, hipMemcpyHostToDevice,
-                               stq));
-      }
+      if (int rc = bound_settle()) return rc;
       const auto t_g = Clock::now();
       host_build += seconds_since(t_b);
       blocks_sent = blkv.size();
@@ -1008,6 +1036,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       if (int rc = flush_tables()) return rc;
       if (nj > 0) {
         const auto t_g = Clock::now();
+        if (int rc = bound_settle()) return rc;
         if (int rc = run_group(nsz, nj)) return rc;
         charge(seconds_since(t_g), tiles);
       }
@@ -1053,6 +1082,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
         hs.inst_hi = (uint32_t)count;
         if (int rc = flush_tables()) return rc;
         const auto t_g = Clock::now();
+        if (int rc = bound_settle()) return rc;
         if (int rc = run_group(1, nj)) return rc;
         if (h_cnt->overflow > 0) {
           MBA_HIP(hipMemsetAsync(&d_cnt->overflow, 0, 4, stq));

@@ -47,6 +47,23 @@ def test_batch_matches_plain_and_oracle(ctx, W):
             assert (g["size"], g["rank"], g["expr"]) == (w["size"], w["rank"], w["expr"])
 
 
+@pytest.mark.parametrize("table_kib", [256, 1024])
+def test_batch_cohorts_settle_only_swept_sizes(ctx, table_kib):
+    # a table pool far below the instances' tables: from the size where they
+    # no longer fit, the instances are swept in cohorts, the first of a
+    # cohort with whatever of its tables fits; a waiting instance may already
+    # hold a window hit of a larger size (a unary chain over a smaller tree)
+    # and must not settle there before its own sizes are swept
+    from simba_amd.learn import sample_instances
+    insts = sample_instances(77, 256, (5, 13), (2, 5), (16, 64))
+    note(f"batch cohorts, max size 10, {table_kib} KiB of tables")
+    got = ctx.batch(insts, max_size=10, table_bytes=table_kib << 10)
+    for inst, g in zip(insts, got):
+        p = ctx.simba(inst, max_size=10)
+        assert (g["status"], g["size"], g["rank"], g["expr"]) == \
+            (p["status"], p["size"], p["rank"], p["expr"])
+
+
 def test_batch_small_table_budget_and_compat(ctx):
     # tiny table budget: most B sides are formed per block from the few
     # tables (or evaluated per value)
