@@ -589,7 +589,8 @@ def main():
                          "of at least 2^26 trees)")
     ap.add_argument("--train", type=int, default=200000)
     ap.add_argument("--eval-count", type=int, default=4096)
-    ap.add_argument("--label-size", type=int, default=9)
+    ap.add_argument("--label-size", type=int, default=11,
+                    help="C5: labels from the search up to this size (else the generating tree)")
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--cap", type=int, default=60, help="simba_cap: time cap in seconds")
     ap.add_argument("--lanes-b", type=int, default=0, help="B values per lane (4 or 8)")
