@@ -188,11 +188,13 @@ int mba_vfb_fingerprints(mba_ctx *ctx, uint32_t width, const uint64_t *values,
                          uint32_t n_words, size_t count, uint64_t *out);
 
 /* Batched SIMBA: `count` instances solved together.  Consecutive sizes of
- * all instances run as one captured launch sequence: per size, one
- * persistent kernel drains a device work queue of every instance's jobs,
- * window hits are resolved on the device, and instances settled at a
- * smaller size skip their remaining jobs; without a budget or cancel flag a
- * plain batch is one launch sequence and one sync.  Each result equals
+ * all instances run as captured launch sequences: per size step, two
+ * persistent kernels (tabulated B sides, then B sides formed from blocks)
+ * drain a device work queue of every instance's jobs, window hits are
+ * resolved on the device, and instances settled at a smaller size skip
+ * their remaining jobs.  From the size at which the instances' value tables
+ * no longer fit the table pool (opts->table_bytes, 0: 96 GiB at most 2/5 of
+ * the device), instances are swept in cohorts whose tables do.  Each result equals
  * mba_simba_solve's on that instance.  `opts->time_budget_s` (0 = none)
  * bounds each instance's share of the device time (a launch sequence's time
  * split over the instances by the tiles of their jobs), checked between
@@ -233,11 +235,13 @@ int mba_mlp_grads(mba_mlp *m, const float *x, const int32_t *labels, int batch, 
 /* One Adam step (beta 0.9 / 0.999, eps 1e-8) on a batch; returns the loss. */
 int mba_mlp_train_step(mba_mlp *m, const float *x, const int32_t *labels, int batch, float lr,
                        float *loss);
-/* SIMBA search with the binary blocks of each size visited in descending
- * learned score; returns a verified solution of the minimal size. */
+/* SIMBA search with the splits of each size of at least 2^26 trees
+ * (opts->block_log2 lowers it) visited in descending probability the
+ * network gives their blocks per pair checked; returns a verified solution
+ * of the minimal size. */
 int mba_learned_solve(mba_ctx *ctx, mba_mlp *m, const mba_instance *inst,
                       const mba_opts *opts, mba_result *res);
-/* Batched form of mba_learned_solve (one persistent kernel per round). */
+/* Batched form of mba_learned_solve (rounds of splits over all instances). */
 int mba_learned_batch_solve(mba_ctx *ctx, mba_mlp *m, const mba_instance *insts,
                             size_t count, const mba_opts *opts, mba_result *results);
 
