@@ -357,10 +357,13 @@ def rec_c4(args):
         for _ in range(reps):
             rb = ctx.batch(insts, **kw)
         t_batch = (time.perf_counter() - t0) / reps
+        sb = sum(r["status"] == 0 for r in rb)
+        if args.batch_only:  # the batched calls alone (for profiling them)
+            return {"solves_per_s": round(sb / t_batch, 1),
+                    "solve_rate": round(sb / len(insts), 4), "ms_per_call": round(1e3 * t_batch, 2)}
         t0 = time.perf_counter()
         rp = [ctx.simba(x, **{k: v for k, v in kw.items() if k != "wg_per_cu"}) for x in insts]
         t_plain = time.perf_counter() - t0
-        sb = sum(r["status"] == 0 for r in rb)
         sp = sum(r["status"] == 0 for r in rp)
         both = [i for i in range(len(insts)) if rb[i]["status"] == 0 and rp[i]["status"] == 0]
         return {"solves_per_s": round(sb / t_batch, 1), "solve_rate": round(sb / len(insts), 4),
@@ -374,13 +377,15 @@ def rec_c4(args):
                 "solved_by_both": len(both)}
 
     size_rec = compare(dict(max_size=args.budget_size, **tuned), 3)
-    time_rec = compare(dict(max_size=17, budget_s=args.time_budget_ms / 1e3, **tuned), 1)
+    out = {"workload": "batch_%d_vars2to5_size5to17_samples16to64" % args.count,
+           "launch": tuned,
+           "solves_per_s": size_rec["solves_per_s"], "solve_rate": size_rec["solve_rate"],
+           "size_budget_%d" % args.budget_size: size_rec}
+    if not args.batch_only:
+        out["time_budget_%gms_sizes_to_17" % args.time_budget_ms] = compare(
+            dict(max_size=17, budget_s=args.time_budget_ms / 1e3, **tuned), 1)
     ctx.close()
-    return {"workload": "batch_%d_vars2to5_size5to17_samples16to64" % args.count,
-            "launch": tuned,
-            "solves_per_s": size_rec["solves_per_s"], "solve_rate": size_rec["solve_rate"],
-            "size_budget_%d" % args.budget_size: size_rec,
-            "time_budget_%gms_sizes_to_17" % args.time_budget_ms: time_rec}
+    return out
 
 
 # ---- C5: learned proposal ---------------------------------------------------
@@ -582,6 +587,8 @@ def main():
     ap.add_argument("--cache-gb", type=int, default=96)
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--budget-size", type=int, default=9)
+    ap.add_argument("--batch-only", action="store_true",
+                    help="batch: time the batched calls alone (no per-instance comparison)")
     ap.add_argument("--time-budget-ms", type=float, default=2.0,
                     help="C4: per-instance time budget of the sizes-to-17 comparison")
     ap.add_argument("--learned-budget", type=int, default=13,
