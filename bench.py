@@ -520,8 +520,9 @@ def run_learned(args):
 
 def run_simba_cap(args):
     """The default instance searched to size 20 under a time cap, as the
-    reference's runs are (one run): how far the search gets, and its rate
-    over the completed sizes."""
+    reference's runs are (one run): how far the search gets (the completed
+    sizes and the splits of the next size checked whole), and its rate over
+    the candidates those decided."""
     import simba_amd
     from simba_amd.synth import count_table, make_instance
 
@@ -531,15 +532,32 @@ def run_simba_cap(args):
     r = ctx.simba(inst, min_size=1, max_size=20, exhaustive=True, budget_s=args.cap,
                   table_bytes=args.table_gb << 30)
     wall = time.perf_counter() - t0
-    ctx.close()
     T = count_table(5, 20)
     done = sum(T[s][8] for s in range(1, r["size_reached"] + 1))
+    # the size the cap stopped: its splits {j, s-1-j} checked whole (every
+    # binary operator instance; the unary trees of that size are the windows
+    # of the smaller sizes, already checked)
+    s1, part = r["size_reached"] + 1, 0
+    for j in range(1, r["split_reached"] + 1):
+        m = s1 - 1 - j
+        part += (5 + (1 if j == m else 2)) * T[j][8] * T[m][8]
+    part += 2 * T[s1 - 1][8] if r["split_reached"] else 0
+    # the completed sizes alone, same windows, no cap: their own time
+    rs = ctx.simba(inst, min_size=1, max_size=20, exhaustive=True, stop_size=r["size_reached"],
+                   table_bytes=args.table_gb << 30)
     rec = {"workload": "simba_4vars_const1_u64_64samples_sizes1to20_cap%ds" % args.cap,
            "status": r["status"], "size_completed": r["size_reached"],
-           "candidates_completed": done, "search_seconds": round(r["seconds"], 3),
+           "candidates_completed": done,
+           "completed_sizes_alone": {"seconds": round(rs["seconds"], 3),
+                                     "Gcand_s": round(done / rs["seconds"] / 1e9, 3)},
+           "furthest": {"size": s1, "splits_checked": r["split_reached"],
+                        "of_splits": (s1 - 1) // 2, "candidates": part},
+           "search_seconds": round(r["seconds"], 3),
+           "rate_completed_sizes_Gcand_s": round(done / r["seconds"] / 1e9, 3),
            "first_solution": r["expr"], "first_solution_size": r["size"]}
-    return _line("Gcand/s evaluated", round(done / r["seconds"] / 1e9, 3), "Gcand/s", True,
-                 round(1e3 * wall, 1), rec, warmup=0)
+    ctx.close()
+    return _line("Gcand/s evaluated", round((done + part) / r["seconds"] / 1e9, 3), "Gcand/s",
+                 True, round(1e3 * wall, 1), rec, warmup=0)
 
 
 def run_copy(args):

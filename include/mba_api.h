@@ -115,7 +115,11 @@ typedef struct {
                             /* in parts after a hit-buffer overflow         */
   int32_t ordered_sizes;    /* learned search: sizes whose blocks were       */
                             /* visited in the network's order               */
-  int32_t reserved[2];
+  int32_t split_reached;    /* SIMBA exhaustive sweep stopped by a budget    */
+                            /* inside size size_reached + 1: its splits      */
+                            /* {j, s-1-j}, j = 1..split_reached, were fully  */
+                            /* checked (0: none)                             */
+  int32_t reserved[1];
 } mba_result;
 
 /* VFB per-level statistics (PRINT_CACHE_SIZE mode, full-cache.cu:763-765). */

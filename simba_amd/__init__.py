@@ -55,7 +55,7 @@ class Result(ctypes.Structure):
                 ("kernel_seconds", ctypes.c_double), ("candidates", ctypes.c_uint64),
                 ("size_reached", ctypes.c_int32), ("kernel_launches", ctypes.c_uint32),
                 ("hit_replays", ctypes.c_int32), ("ordered_sizes", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 2)]
+                ("split_reached", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1)]
 
     def as_dict(self):
         return {"status": self.status, "size": self.size, "rank": int(self.rank),
@@ -64,7 +64,8 @@ class Result(ctypes.Structure):
                 "kernel_seconds": self.kernel_seconds,
                 "kernel_launches": self.kernel_launches,
                 "candidates": int(self.candidates), "size_reached": self.size_reached,
-                "hit_replays": self.hit_replays, "ordered_sizes": self.ordered_sizes}
+                "hit_replays": self.hit_replays, "ordered_sizes": self.ordered_sizes,
+                "split_reached": self.split_reached}
 
 
 def _result_array(n):

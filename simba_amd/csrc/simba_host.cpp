@@ -930,11 +930,14 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
       // launch group (unless large) whose hits are resolved in its closing
       // sync; once the answer is settled (an exhaustive sweep going on to
       // max_size) the windows' hits no longer matter
+      int splits_done = 0;
       for (int j = 1; j <= (s - 1) / 2 && !stopped; ++j) {
         const int rc = run_pair(s, j, kAllInst, tg, true);
         if (rc < 0) return MBA_ERR_HIP;
         stopped = rc == 1;
+        if (rc == 0) splits_done = j;
       }
+      if (stopped) res.split_reached = splits_done;
       if (!stopped) {
         const int rc = close_group(answer < 0);
         if (rc < 0) return MBA_ERR_HIP;
