@@ -425,6 +425,13 @@ def rec_c5(args):
                                       2)}}
     ev = learn.sample_instances(args.seed + 10**6, args.eval_count, (9, 19), (2, 4), (32, 32))
     kw = dict(max_size=args.learned_budget)
+    # the batched calls' launch, autotuned on a part of the set at a smaller budget
+    if args.wg_per_cu:
+        tuned = {"wg_per_cu": args.wg_per_cu}
+    else:
+        from simba_amd import autotune
+        tuned, _ = autotune.tune_batch(ctx, ev[:1024], min(11, args.learned_budget), store=False)
+    bkw = dict(kw, **tuned)
     runs = {}
     for name, fn in (("plain", lambda x: ctx.simba(x, **kw)), ("learned", lambda x: mlp.solve(x, **kw)),
                      ("untrained", lambda x: untrained.solve(x, **kw))):
@@ -452,11 +459,11 @@ def rec_c5(args):
     e_rec = {q: {n: pct(n, engaged, q) for n in ("plain", "learned", "untrained")}
              for q in (50, 95)}
     log("c5 batched warm-up")
-    mlp.batch_solve(ev, **kw)
-    batched = {}
-    for name, fn in (("learned", lambda: mlp.batch_solve(ev, **kw)),
-                     ("untrained", lambda: untrained.batch_solve(ev, **kw)),
-                     ("plain", lambda: ctx.batch(ev, **kw))):
+    mlp.batch_solve(ev, **bkw)
+    batched = {"launch": tuned}
+    for name, fn in (("learned", lambda: mlp.batch_solve(ev, **bkw)),
+                     ("untrained", lambda: untrained.batch_solve(ev, **bkw)),
+                     ("plain", lambda: ctx.batch(ev, **bkw))):
         t0 = time.perf_counter()
         rb = fn()
         dt = time.perf_counter() - t0

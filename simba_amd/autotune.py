@@ -15,7 +15,7 @@ import json
 import os
 
 SIMBA_SPACE = {"lanes_b": (4, 8), "wg_per_cu": (8, 16, 32, 48), "table_delta": (0, 1)}
-BATCH_SPACE = {"wg_per_cu": (2, 4, 8, 16)}
+BATCH_SPACE = {"wg_per_cu": (4, 8, 16, 24, 32)}
 
 
 def cache_path():

@@ -28,6 +28,10 @@ namespace mba {
 namespace {
 
 constexpr uint32_t kBatchHitCap = 1u << 26;  // 2 GiB of hit records
+// Workgroups (one wave each) of the all-sample candidate check: the kernel
+// is latency-bound (a serial unrank per candidate), so more waves than fit
+// at once keep every SIMD busy while the survivors last
+constexpr uint32_t kConfirmBlocks = 8192;
 // default value-table budget of a batch, split evenly over its instances
 // (at most 2/5 of the device memory): 4096 instances of up to 4 leaves keep
 // V[8] in tables at a size budget of 11, so the last size's B sides are
@@ -500,7 +504,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
     if (e == hipSuccess)
       e = launch_hit_resolve(W, d_tvs, ctx.leafvals.as<uint64_t>(), ctx.outputs.as<uint64_t>(),
                              ctx.b_results.as<BatchInstData>(), d_hits, &d_cnt->hits, hit_cap,
-                             1024, 2048, min_size, max_size, d_found, ctx.surv.as<SurvRec>(),
+                             1024, kConfirmBlocks, min_size, max_size, d_found, ctx.surv.as<SurvRec>(),
                              &d_cnt->surv, d_flags, d_touched, &d_cnt->touched, stq);
     if (e == hipSuccess)
       e = launch_batch_settle(d_step, d_found, d_settled, (uint32_t)count, min_size, d_cnt,

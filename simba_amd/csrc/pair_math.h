@@ -29,7 +29,8 @@
 //    windows (fold(b) is formed once per B value, fold(t -+ a) - h once per
 //    A value).
 // The & and | instances are checked separately, only for the A values whose
-// bits allow them to reach a window.
+// bits allow them to reach a window, and only against the windows they can
+// reach.
 #pragma once
 #include <stdint.h>
 
@@ -105,24 +106,46 @@ MBA_HD inline uint32_t win_pair(const WinPre &w, uint32_t al, uint32_t nal, uint
   return min3_u32(acc, fb - f1, fb - f2);
 }
 
-// Whether a & b or a | b can land in the window [w, w + span] for some b
-// (low halves): every value of the window shares its bits above the span's
-// bit length p with w or with w + span, and a & b lies inside a (a | b
-// contains a), so one of those two high parts must lie inside a (contain a).
-MBA_HD inline bool window_bitops(uint32_t al, uint32_t w, uint32_t span) {
+// Whether a & b (AND) or a | b (OR) can land in the window [w, w + span]
+// for some b (low halves): every value of the window shares its bits above
+// the span's bit length p with w or with w + span, and a & b lies inside a
+// (a | b contains a), so one of those two high parts must lie inside a
+// (contain a).
+template <bool AND>
+MBA_HD inline bool window_bitop(uint32_t al, uint32_t w, uint32_t span) {
   int p = 0;
   while (p < 32 && (span >> p) != 0u) ++p;
   if (p >= 32) return true;
   const uint32_t ah = al >> p, h0 = w >> p, h1 = (w + span) >> p;
-  return (h0 & ~ah) == 0u || (h1 & ~ah) == 0u || (ah & ~h0) == 0u || (ah & ~h1) == 0u;
+  return AND ? ((h0 & ~ah) == 0u || (h1 & ~ah) == 0u) : ((ah & ~h0) == 0u || (ah & ~h1) == 0u);
 }
 
-// & and | distances of the pair (a, b) folded into acc.
+// The (instance, window) combinations an A value can reach: bit 0 a & b in
+// the even window, bit 1 a & b in the odd one, bit 2 a | b in the even one,
+// bit 3 a | b in the odd one.
+enum : uint32_t { kAndEven = 1u, kAndOdd = 2u, kOrEven = 4u, kOrOdd = 8u };
+MBA_HD inline uint32_t bitop_flags(uint32_t al, uint32_t le, uint32_t lo, uint32_t span) {
+  return (window_bitop<true>(al, le, span) ? kAndEven : 0u) |
+         (window_bitop<true>(al, lo, span) ? kAndOdd : 0u) |
+         (window_bitop<false>(al, le, span) ? kOrEven : 0u) |
+         (window_bitop<false>(al, lo, span) ? kOrOdd : 0u);
+}
+
+// Distances of x = a & b (or a | b) to the windows its flags allow (two
+// flag bits, even then odd), folded into acc.
+MBA_HD inline uint32_t win_bitop_dist(const WinPre &w, uint32_t x, uint32_t fl2,
+                                      uint32_t acc) {
+  const uint32_t de = (fl2 & 1u) ? x - w.le : 0xffffffffu;
+  const uint32_t dd = (fl2 & 2u) ? x - w.lo : 0xffffffffu;
+  return min3_u32(acc, de, dd);
+}
+
+// & and | distances of the pair (a, b) folded into acc, for the
+// combinations in `flags` (bitop_flags).
 MBA_HD inline uint32_t win_pair_bitops(const WinPre &w, uint32_t al, uint32_t b,
-                                       uint32_t acc) {
-  const uint32_t x_and = al & b, x_or = al | b;
-  acc = min3_u32(acc, x_and - w.le, x_and - w.lo);
-  return min3_u32(acc, x_or - w.le, x_or - w.lo);
+                                       uint32_t flags, uint32_t acc) {
+  acc = win_bitop_dist(w, al & b, flags & 3u, acc);
+  return win_bitop_dist(w, al | b, flags >> 2, acc);
 }
 
 // Quotient of n < 2^32 by an invariant divisor d >= 1 (Granlund and

@@ -378,10 +378,10 @@ __device__ __forceinline__ uint64_t uniform(uint64_t x) {
 //                 q1 = {t + a, bit-op gap, product residue, shift | kind << 8}
 //   window mode, every instance (pair_math.h):
 //                 q0 = {a, ~a, fold(t - a) - h, fold(t + a) - h},
-//                 q1 = {-, -, -, & | possible}
+//                 q1 = {-, -, -, bitop_flags: the & | windows a can reach}
 //   window mode, some instances:
 //                 q0 = {a, lo_e - a, lo_o - a, a - lo_e},
-//                 q1 = {a - lo_o, a + lo_e, a + lo_o, & | possible}
+//                 q1 = {a - lo_o, a + lo_e, a + lo_o, -}
 // (low halves; t = lo_e in exact mode).
 struct AStage {
   uint4 q0[kPairTA];
@@ -455,13 +455,11 @@ __device__ __forceinline__ bool stage_a(AStage &S, const TableView *__restrict__
       const WinPre w = win_pre(le, lo, span);
       uint32_t f1, f2;
       win_stage(w, al, f1, f2);
-      const bool wb = window_bitops(al, le, span) || window_bitops(al, lo, span);
       S.q0[i] = make_uint4(al, ~al, f1, f2);
-      S.q1[i] = make_uint4(0u, 0u, 0u, wb ? 1u : 0u);
+      S.q1[i] = make_uint4(0u, 0u, 0u, bitop_flags(al, le, lo, span));
     } else {
-      const bool wb = window_bitops(al, le, span) || window_bitops(al, lo, span);
       S.q0[i] = make_uint4(al, le - al, lo - al, al - le);
-      S.q1[i] = make_uint4(al - lo, al + le, al + lo, wb ? 1u : 0u);
+      S.q1[i] = make_uint4(al - lo, al + le, al + lo, 0u);
     }
   }
   return __syncthreads_or(bit) != 0;
@@ -719,19 +717,43 @@ __device__ __forceinline__ bool exact_tile(const AStage &S, uint32_t na, const u
 // values whose staged flag says they can reach a window (a wave-uniform
 // branch per A value).  The result is compared with w.psp.  (SAME pairs need
 // no separate B - A check: its windows are those of the additive tests.)
+// The & (AND) or | distances of one A value and a lane's RB B values to the
+// windows in f2 (bit 0 even, bit 1 odd; workgroup-uniform): one subtraction
+// per B value when a single window is reachable.
+template <bool AND, int RB>
+__device__ __forceinline__ uint32_t bitop_tile(const WinPre &w, uint32_t al,
+                                               const uint32_t *bl, uint32_t f2, uint32_t acc) {
+  if (f2 == 3u) {
+#pragma unroll
+    for (int k = 0; k < RB; ++k) {
+      const uint32_t x = AND ? (al & bl[k]) : (al | bl[k]);
+      acc = min3u(acc, x - w.le, x - w.lo);
+    }
+  } else {
+    const uint32_t ws = f2 == 1u ? w.le : w.lo;
+#pragma unroll
+    for (int k = 0; k < RB; k += 2) {
+      const uint32_t x0 = AND ? (al & bl[k]) : (al | bl[k]);
+      const uint32_t x1 = AND ? (al & bl[k + 1]) : (al | bl[k + 1]);
+      acc = min3u(acc, x0 - ws, x1 - ws);
+    }
+  }
+  return acc;
+}
+
 template <int RB>
 __device__ __forceinline__ uint32_t window_tile(const AStage &S, uint32_t na,
                                                 const uint32_t *bl, const WinPre &w) {
+  static_assert(RB % 2 == 0, "B values per lane come in pairs");
   uint32_t fb[RB];
 #pragma unroll
   for (int k = 0; k < RB; ++k) fb[k] = fold32(bl[k]);
   uint32_t acc = 0xffffffffu;
   for (uint32_t i = 0; i < na; ++i) {
     const uint4 q0 = S.q0[i];
-    if (uniform(S.q1[i].w)) {
-#pragma unroll
-      for (int k = 0; k < RB; ++k) acc = win_pair_bitops(w, q0.x, bl[k], acc);
-    }
+    const uint32_t fl = uniform(S.q1[i].w);  // bitop_flags of this A value
+    if (fl & (kAndEven | kAndOdd)) acc = bitop_tile<true, RB>(w, q0.x, bl, fl & 3u, acc);
+    if (fl & (kOrEven | kOrOdd)) acc = bitop_tile<false, RB>(w, q0.x, bl, fl >> 2, acc);
 #pragma unroll
     for (int k = 0; k < RB; ++k) acc = win_pair(w, q0.x, q0.y, q0.z, q0.w, bl[k], fb[k], acc);
   }
@@ -1238,7 +1260,7 @@ k_hit_screen(const TableView *__restrict__ tvs, const uint64_t *__restrict__ lea
 
 // Check of the listed candidates on every sample, one wave per candidate
 // (grid-stride over *n_surv): the instance's table view is staged in LDS,
-// lane 0 unranks the hit's two subtrees, each lane evaluates its samples
+// lanes 0 and 1 unrank the hit's two subtrees, each lane evaluates its samples
 // (RPN stacks in LDS).  A candidate that holds lowers found[slot]; a slot
 // lowered in this launch is listed once in `touched` (its flag is cleared by
 // k_found_gather).  Candidates that can no longer lower their slot are
@@ -1272,10 +1294,8 @@ k_hit_confirm(const TableView *__restrict__ tvs, const uint64_t *__restrict__ le
     const uint64_t *lv = leafvals + d.leaf_off, *out = outputs + d.out_off;
     const word x0 = (word)g.v, t0 = (word)out[d.fs];
     const word k = odd ? word(t0 + x0) : word(t0 - x0);
-    if (lane == 0 && inst < 7) {
-      unrank_rpn(&s_tv, j, g.a, s_ra);
-      unrank_rpn(&s_tv, m, g.b, s_rb);
-    }
+    // the two subtrees are unranked side by side (lanes 0 and 1)
+    if (lane < 2 && inst < 7) unrank_rpn(&s_tv, lane ? m : j, lane ? g.b : g.a, lane ? s_rb : s_ra);
     __syncthreads();
     bool bad = false;
     for (uint32_t s = (uint32_t)lane; s < d.n_samples; s += 64) {

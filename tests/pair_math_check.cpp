@@ -55,8 +55,8 @@ bool prefilter(const Windows &ws, uint32_t a, uint32_t b) {
   uint32_t f1, f2;
   mba::win_stage(w, a, f1, f2);
   uint32_t acc = 0xffffffffu;
-  if (mba::window_bitops(a, ws.le, ws.span) || mba::window_bitops(a, ws.lo, ws.span))
-    acc = mba::win_pair_bitops(w, a, b, acc);
+  const uint32_t fl = mba::bitop_flags(a, ws.le, ws.lo, ws.span);
+  if (fl) acc = mba::win_pair_bitops(w, a, b, fl, acc);
   acc = mba::win_pair(w, a, ~a, f1, f2, b, mba::fold32(b), acc);
   return acc <= w.psp;
 }
