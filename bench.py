@@ -33,6 +33,7 @@ has its own workload for a full line.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -317,6 +318,7 @@ def rec_c2(args):
     for k in ("level_cas", "level_atomics"):
         if k in r:
             rec[k.replace("level_", "") + "_per_s"] = round(sum(r[k]) / ks, 1)
+    rec["insert_roofline"] = vfb_insert_ceiling(r, args.vfb_levels)
     if not args.no_cpu:
         threads, model = cpu_info()
         lv = args.vfb_cpu_levels
@@ -329,6 +331,41 @@ def rec_c2(args):
                                          "(%.1f s), kept rows equal to the GPU's"
                                          % (lv, w["seconds"])}
     return rec
+
+
+def vfb_insert_ceiling(r, last):
+    """The hash-only last level's insert against the random-access ceiling
+    of its table code: the same table size, as many keys as the level has
+    candidates, as many distinct keys as make the level's count of new keys,
+    after as many other keys as the earlier levels added (the ceiling's own
+    insert and probe-only passes, no fingerprint work)."""
+    import simba_amd
+    count, new = r["level_cand"][last], r["level_new"][last]
+    prefill = sum(r["level_new"][1:last])
+    lo, hi = float(new), 8.0 * count  # distinct values whose draws give `new` keys
+    for _ in range(60):
+        mid = (lo + hi) / 2
+        if mid * -math.expm1(-count / mid) < new:
+            lo = mid
+        else:
+            hi = mid
+    slots = r["table_slots"]
+    c = simba_amd.Context(0)
+    try:
+        ce = c.vfb_probe_ceiling(slots.bit_length() - 1, count, int(lo), prefill)
+    finally:
+        c.close()
+    t_level = r["level_seconds"][last]
+    return {"bound": "random access: a probe of a random 16-byte slot per candidate, "
+                     "a 64-bit compare-and-swap per new key",
+            "level": last, "candidates": count, "new_keys": new,
+            "level_ms": round(1e3 * t_level, 2),
+            "ceiling_insert_ms": round(1e3 * ce["insert_seconds"], 2),
+            "ceiling_probe_ms": round(1e3 * ce["probe_seconds"], 2),
+            "ceiling_fresh": ce["fresh"],
+            "achieved_Gprobes_s": round(count / t_level / 1e9, 2),
+            "peak_Gprobes_s": round(count / ce["insert_seconds"] / 1e9, 2),
+            "frac": round(ce["insert_seconds"] / t_level, 3)}
 
 
 # ---- C4: batched solving ----------------------------------------------------

@@ -191,6 +191,20 @@ int mba_vfb_solve(mba_ctx *ctx, const mba_instance *inst, const mba_opts *opts,
 int mba_vfb_fingerprints(mba_ctx *ctx, uint32_t width, const uint64_t *values,
                          uint32_t n_words, size_t count, uint64_t *out);
 
+/* Random-access ceiling of the VFB insert pass on this device.  No
+ * counterpart in the reference: a measurement of what bounds the insert
+ * kernel (one probe of a random 16-byte slot per candidate, a 64-bit
+ * compare-and-swap per new key).  A table of 2^slots_log2 slots in the
+ * insert pass's layout holds `prefill` keys; then `count` keys with
+ * `distinct` different values (SplitMix64, random order) are probed once
+ * each (*probe_s: seconds) and, on the same starting table, inserted by the
+ * insert pass's own table code with no fingerprint work (*insert_s;
+ * *fresh: keys the insert added).  Allocates and frees its own buffers
+ * (16 * 2^slots_log2 + 8 * (count + prefill) bytes). */
+int mba_vfb_probe_ceiling(mba_ctx *ctx, uint32_t slots_log2, uint64_t count,
+                          uint64_t distinct, uint64_t prefill, double *probe_s,
+                          double *insert_s, uint64_t *fresh);
+
 /* Batched SIMBA: `count` instances solved together.  Consecutive sizes of
  * all instances run as captured launch sequences: per size step, two
  * persistent kernels (tabulated B sides, then B sides formed from blocks)

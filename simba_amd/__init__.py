@@ -108,7 +108,7 @@ class VfbStats(ctypes.Structure):
 EXPORTS = ["mba_ctx_create", "mba_ctx_destroy", "mba_strerror", "mba_ctx_last_error",
            "mba_default_opts", "mba_simba_solve", "mba_simba_solve_noalloc",
            "mba_simba_rtid_solve", "mba_vfb_solve",
-           "mba_batch_solve", "mba_vfb_fingerprints",
+           "mba_batch_solve", "mba_vfb_fingerprints", "mba_vfb_probe_ceiling",
            "mba_instance_read_json", "mba_instance_free", "mba_rpn_to_infix",
            "mba_features", "mba_features_batch", "mba_mlp_create", "mba_mlp_destroy",
            "mba_mlp_get_weights", "mba_mlp_set_weights", "mba_mlp_forward", "mba_mlp_grads",
@@ -146,6 +146,10 @@ def lib():
         L.mba_vfb_fingerprints.argtypes = [ctypes.c_void_p, ctypes.c_uint32,
                                            P(ctypes.c_uint64), ctypes.c_uint32,
                                            ctypes.c_size_t, P(ctypes.c_uint64)]
+        L.mba_vfb_probe_ceiling.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64,
+                                            ctypes.c_uint64, ctypes.c_uint64,
+                                            P(ctypes.c_double), P(ctypes.c_double),
+                                            P(ctypes.c_uint64)]
         L.mba_instance_read_json.argtypes = [ctypes.c_char_p, P(Instance)]
         L.mba_instance_free.argtypes = [P(Instance)]
         L.mba_rpn_to_infix.argtypes = [ctypes.c_char_p, ctypes.c_uint32, P(ctypes.c_uint64),
@@ -259,6 +263,16 @@ class Context:
         self._check(lib().mba_vfb_fingerprints(self._h, width, v.ctypes.data_as(P), v.shape[1],
                                                v.shape[0], out.ctypes.data_as(P)))
         return out
+
+    def vfb_probe_ceiling(self, slots_log2, count, distinct, prefill=0):
+        """Random-access ceiling of the VFB insert pass (mba_vfb_probe_ceiling):
+        seconds of one probe per key and of the insert pass's table code on
+        `count` keys with `distinct` values, after `prefill` other keys."""
+        ps, ins, fresh = ctypes.c_double(), ctypes.c_double(), ctypes.c_uint64()
+        self._check(lib().mba_vfb_probe_ceiling(self._h, slots_log2, count, distinct, prefill,
+                                                ctypes.byref(ps), ctypes.byref(ins),
+                                                ctypes.byref(fresh)))
+        return {"probe_seconds": ps.value, "insert_seconds": ins.value, "fresh": fresh.value}
 
     def batch(self, insts, **kw):
         """Solves the instances in one batched call; returns a BatchResults

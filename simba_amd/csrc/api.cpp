@@ -145,6 +145,15 @@ int mba_vfb_fingerprints(mba_ctx *c, uint32_t width, const uint64_t *values,
   return vfb_fingerprints(c->c, (int)width, values, (int)n_words, count, out);
 }
 
+int mba_vfb_probe_ceiling(mba_ctx *c, uint32_t slots_log2, uint64_t count, uint64_t distinct,
+                          uint64_t prefill, double *probe_s, double *insert_s,
+                          uint64_t *fresh) {
+  if (!c) return MBA_ERR_ARG;
+  if (hipSetDevice(c->c.device) != hipSuccess) return MBA_ERR_HIP;
+  return vfb_probe_ceiling(c->c, (int)slots_log2, count, distinct, prefill, probe_s, insert_s,
+                           fresh);
+}
+
 int mba_batch_solve(mba_ctx *c, const mba_instance *insts, size_t count,
                     const mba_opts *opts, mba_result *results) {
   if (!c || (!insts && count) || (!results && count)) return MBA_ERR_ARG;

@@ -104,6 +104,9 @@ int simba_solve_noalloc(Ctx &ctx, const mba_instance &inst, const mba_opts &opts
 // Device fingerprints of `count` signatures of n words (see mba_api.h).
 int vfb_fingerprints(Ctx &ctx, int W, const uint64_t *values, int n, size_t count,
                      uint64_t *out);
+// Random-access ceiling of the VFB insert pass (see mba_api.h).
+int vfb_probe_ceiling(Ctx &ctx, int slots_log2, uint64_t count, uint64_t distinct,
+                      uint64_t prefill, double *probe_s, double *insert_s, uint64_t *fresh);
 // score_size: NULL (plain, minimum rank per instance) or a function that
 // fills count x 256 block scores (as for simba_solve) for a size; learned
 // order: rounds of the best-scored split, the next split, then the rest.

@@ -120,15 +120,22 @@ MBA_HD inline bool window_bitop(uint32_t al, uint32_t w, uint32_t span) {
   return AND ? ((h0 & ~ah) == 0u || (h1 & ~ah) == 0u) : ((ah & ~h0) == 0u || (ah & ~h1) == 0u);
 }
 
-// The (instance, window) combinations an A value can reach: bit 0 a & b in
-// the even window, bit 1 a & b in the odd one, bit 2 a | b in the even one,
-// bit 3 a | b in the odd one.
+// The (instance, window) combinations an A value must be checked for: bit 0
+// a & b in the even window, bit 1 a & b in the odd one, bit 2 a | b in the
+// even one, bit 3 a | b in the odd one.  Besides the bit test above, a small
+// A value needs neither: for 0 <= a <= h = ceil(span/2), b <= a | b <= b + a,
+// so a | b in a window W puts b in [W - a, W + a] (W's ends moved out by a),
+// which the additive tests already cover (b in W - a from a + b, b in W + a
+// from b - a; the two meet as 2a <= span + 1); likewise a & b for
+// -h <= a <= -1, as b - (-a - 1) <= a & b <= b.
 enum : uint32_t { kAndEven = 1u, kAndOdd = 2u, kOrEven = 4u, kOrOdd = 8u };
 MBA_HD inline uint32_t bitop_flags(uint32_t al, uint32_t le, uint32_t lo, uint32_t span) {
-  return (window_bitop<true>(al, le, span) ? kAndEven : 0u) |
-         (window_bitop<true>(al, lo, span) ? kAndOdd : 0u) |
-         (window_bitop<false>(al, le, span) ? kOrEven : 0u) |
-         (window_bitop<false>(al, lo, span) ? kOrOdd : 0u);
+  const uint32_t h = (span + 1u) >> 1;
+  const bool and_covered = 0u - al <= h && al != 0u, or_covered = al <= h;
+  return (!and_covered && window_bitop<true>(al, le, span) ? kAndEven : 0u) |
+         (!and_covered && window_bitop<true>(al, lo, span) ? kAndOdd : 0u) |
+         (!or_covered && window_bitop<false>(al, le, span) ? kOrEven : 0u) |
+         (!or_covered && window_bitop<false>(al, lo, span) ? kOrOdd : 0u);
 }
 
 // Distances of x = a & b (or a | b) to the windows its flags allow (two

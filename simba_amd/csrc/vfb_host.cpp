@@ -441,4 +441,75 @@ int vfb_fingerprints(Ctx &ctx, int W, const uint64_t *values, int n, size_t coun
   return 0;
 }
 
+int vfb_probe_ceiling(Ctx &ctx, int slots_log2, uint64_t count, uint64_t distinct,
+                      uint64_t prefill, double *probe_s, double *insert_s, uint64_t *fresh) {
+  if (slots_log2 < 10 || slots_log2 > 31 || count == 0 || distinct == 0 ||
+      count + prefill > (1ull << 40)) {
+    ctx.last_error = "probe ceiling: 10 <= slots_log2 <= 31, count >= 1, distinct >= 1";
+    return MBA_ERR_ARG;
+  }
+  const uint64_t slots = 1ull << slots_log2;
+  const size_t need = (size_t)(16 * slots + 8 * (count + prefill) + 64);
+  if (ctx.hbm_bytes && need > ctx.hbm_bytes / 10 * 8) {
+    ctx.last_error = "probe ceiling: table and keys exceed 80% of device memory";
+    return MBA_ERR_LIMIT;
+  }
+  hipStream_t st = ctx.stream;
+  void *mem = nullptr;
+  MBA_HIP(hipMalloc(&mem, need));
+  uint64_t *table = (uint64_t *)mem;
+  uint64_t *keys = table + 2 * slots, *pre = keys + count;
+  unsigned long long *cnt = (unsigned long long *)(pre + prefill);  // new keys, fail, occupied
+  VfbLevelArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.keys = table;
+  a.minidx = table + 1;
+  a.mask = slots - 1;
+  a.max_probes = slots;
+  a.last = 1;
+  a.new_keys = cnt;
+  a.fail = (unsigned int *)(cnt + 1);
+  const int grid = ctx.cu_count * 8;
+  float ms_probe = 0.f, ms_insert = 0.f;
+  unsigned long long host[3] = {0, 0, 0};
+  hipError_t e = vfb_launch_ceiling(0, a, keys, count, distinct, 1, nullptr, grid, st);
+  if (e == hipSuccess)  // prefill keys: distinct values of their own (another salt)
+    e = vfb_launch_ceiling(0, a, pre, prefill, ~0ull >> 2, 1ull << 62, nullptr, grid, st);
+  // the measured pass on a table holding the prefill keys only
+  auto reset = [&]() -> hipError_t {
+    hipError_t r = vfb_fill_u64(table, 2 * slots, kVfbEmpty, st);
+    if (r == hipSuccess) r = hipMemsetAsync(cnt, 0, 24, st);
+    if (r == hipSuccess) r = vfb_launch_ceiling(2, a, pre, prefill, 0, 0, nullptr, grid, st);
+    if (r == hipSuccess) r = hipMemsetAsync(cnt, 0, 24, st);
+    return r;
+  };
+  if (e == hipSuccess) e = reset();
+  if (e == hipSuccess) e = hipEventRecord(ctx.ev0, st);
+  if (e == hipSuccess) e = vfb_launch_ceiling(1, a, keys, count, 0, 0, cnt + 2, grid, st);
+  if (e == hipSuccess) e = hipEventRecord(ctx.ev1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(ctx.ev1);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms_probe, ctx.ev0, ctx.ev1);
+  if (e == hipSuccess) e = reset();
+  if (e == hipSuccess) e = hipEventRecord(ctx.ev0, st);
+  if (e == hipSuccess) e = vfb_launch_ceiling(2, a, keys, count, 0, 0, nullptr, grid, st);
+  if (e == hipSuccess) e = hipEventRecord(ctx.ev1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(ctx.ev1);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms_insert, ctx.ev0, ctx.ev1);
+  if (e == hipSuccess) e = hipMemcpy(host, cnt, 24, hipMemcpyDeviceToHost);
+  const hipError_t ef = hipFree(mem);
+  if (e == hipSuccess) e = ef;
+  if (e != hipSuccess) {
+    ctx.last_error = std::string("probe ceiling: ") + hipGetErrorString(e);
+    return MBA_ERR_HIP;
+  }
+  if ((uint32_t)host[1]) {
+    ctx.last_error = "probe ceiling: table too full";
+    return MBA_OVERFLOW;
+  }
+  if (probe_s) *probe_s = ms_probe * 1e-3;
+  if (insert_s) *insert_s = ms_insert * 1e-3;
+  if (fresh) *fresh = host[0];
+  return 0;
+}
+
 }  // namespace mba

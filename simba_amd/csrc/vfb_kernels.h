@@ -87,6 +87,14 @@ hipError_t vfb_launch_rpn(const int32_t *left, const int32_t *right, const uint8
                           int L, int32_t root_op, int32_t root_l, int32_t root_r,
                           uint8_t *out, int32_t *out_len, int cap, hipStream_t s);
 hipError_t vfb_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s);
+// Random-access ceiling of the insert pass: kind 0 writes `count` keys with
+// `distinct` values (salted) to `keys`; kind 1 runs one probe load per key
+// against a.keys (occupied slots counted in *counter); kind 2 inserts the
+// keys with the insert pass's table code (hash-only, fresh keys counted in
+// *a.new_keys).
+hipError_t vfb_launch_ceiling(int kind, const VfbLevelArgs &a, uint64_t *keys, uint64_t count,
+                              uint64_t distinct, uint64_t salt, unsigned long long *counter,
+                              int grid, hipStream_t s);
 // out[r] = fingerprint of the n words vals[r * n .. r * n + n) (the insert
 // pass's makeUnqChk), r < count.
 hipError_t vfb_launch_fingerprints(int width, const uint64_t *d_vals, int n, uint64_t count,

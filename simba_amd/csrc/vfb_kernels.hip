@@ -528,6 +528,85 @@ k_vfb_fingerprints(const uint64_t *__restrict__ vals, int n, uint64_t count,
   }
 }
 
+// --- random-access ceiling of the insert pass -------------------------------
+// Keys of the measurement: keys[i] = SplitMix64(SplitMix64(i + salt) mod
+// distinct), i < count, kept below TOMBSTONE (Steele, Lea and Flood, "Fast
+// splittable pseudorandom number generators", OOPSLA 2014).
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+__global__ void k_vfb_ceiling_keys(uint64_t *keys, uint64_t count, uint64_t distinct,
+                                   uint64_t salt) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    const uint64_t k = splitmix64(splitmix64(i + salt) % distinct);
+    keys[i] = k >= kVfbTomb ? k - 2 : k;
+  }
+}
+
+// One probe load per key (the insert pass's first probe, no write); the
+// keys that read an occupied slot are counted so every load is used.
+__global__ void __launch_bounds__(kVfbThreads)
+k_vfb_ceiling_probe(const uint64_t *__restrict__ keys, uint64_t count,
+                    const uint64_t *__restrict__ slots, uint64_t mask,
+                    unsigned long long *occupied) {
+  constexpr int N = 7;
+  uint64_t occ = 0;
+  const uint64_t groups = (count + N - 1) / N;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups; g += stride) {
+    uint64_t k[N];
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+      const uint64_t i = g * N + c;
+      k[c] = i < count ? slots[2 * probe_start(keys[i], mask)] : kVfbEmpty;
+    }
+#pragma unroll
+    for (int c = 0; c < N; ++c) occ += k[c] != kVfbEmpty;
+  }
+  if (occ) atomicAdd(occupied, (unsigned long long)occ);
+}
+
+// The insert pass's table code (insert_group, hash-only form) on ready-made
+// keys, seven per lane as in the pass; no fingerprint work.
+__global__ void __launch_bounds__(kVfbThreads)
+k_vfb_ceiling_insert(VfbLevelArgs a, const uint64_t *__restrict__ keys, uint64_t count) {
+  constexpr int N = 7;
+  InsertStats is;
+  const uint64_t groups = (count + N - 1) / N;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups; g += stride) {
+    uint64_t idx[N], fp[N];
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+      idx[c] = g * N + c;
+      fp[c] = idx[c] < count ? keys[idx[c]] : kVfbTomb;  // TOMBSTONE: dropped
+    }
+    insert_group<N>(a, idx, fp, kVfbEmpty, is);
+  }
+  if (is.fresh) atomicAdd(a.new_keys, (unsigned long long)is.fresh);
+}
+
+hipError_t vfb_launch_ceiling(int kind, const VfbLevelArgs &a, uint64_t *keys, uint64_t count,
+                              uint64_t distinct, uint64_t salt, unsigned long long *counter,
+                              int grid, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  if (kind == 0)
+    hipLaunchKernelGGL(k_vfb_ceiling_keys, dim3(grid), dim3(256), 0, s, keys, count, distinct,
+                       salt);
+  else if (kind == 1)
+    hipLaunchKernelGGL(k_vfb_ceiling_probe, dim3(grid), dim3(kVfbThreads), 0, s, keys, count,
+                       a.keys, a.mask, counter);
+  else
+    hipLaunchKernelGGL(k_vfb_ceiling_insert, dim3(grid), dim3(kVfbThreads), 0, s, a, keys,
+                       count);
+  return hipGetLastError();
+}
+
 __global__ void k_fill_u64(uint64_t *p, uint64_t n, uint64_t v) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)

@@ -141,6 +141,8 @@ int main() {
     const Windows ws = windows(t, U);
     uint32_t a = (trial / 3) % 7 == 0 ? edges[trial % 10] : r32();
     if (trial % 11 == 0) a = r32() & ~(uint32_t)rng() & 0xffff0000u;  // sparse bits
+    if (trial % 13 == 0) a = (uint32_t)(trial / 13 % 81) - 40u;  // small |a|: 0, +-1, ...,
+                                                                  // the covered bit-op flags
     for (int inst = 0; inst < 7; ++inst) {
       for (int par = 0; par < 2; ++par) {
         const uint32_t start = par ? ws.lo : ws.le;
@@ -163,6 +165,23 @@ int main() {
             ++misses;
           }
         }
+      }
+    }
+  }
+  // small A values against random and aimed B values, every window span
+  for (int trial = 0; trial < 200000; ++trial) {
+    const uint32_t t = r32(), U = 1u + (uint32_t)(trial % 31);
+    const Windows ws = windows(t, U);
+    const uint32_t a = (uint32_t)(trial % 81) - 40u;
+    const uint32_t w0 = (trial & 1) ? ws.lo : ws.le;
+    const uint32_t b = w0 + (uint32_t)(trial / 2 % 50) - 20u;  // around a window
+    for (int inst = 0; inst < 7; ++inst) {
+      if (!in_windows(ws, inst_value(inst, a, b))) continue;
+      ++checked;
+      if (!prefilter(ws, a, b)) {
+        if (misses < 10)
+          std::printf("MISS small a inst %d t %08x U %u a %08x b %08x\n", inst, t, U, a, b);
+        ++misses;
       }
     }
   }

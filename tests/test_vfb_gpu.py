@@ -86,3 +86,35 @@ def test_vfb_largest_hash_table_keeps_counts():
     assert got["table_slots"] == 1 << 31
     assert got["level_new"][1:6] == want["level_new"][1:6]
     assert (got["size"], got["rank"]) == (want["size"], want["rank"])
+
+
+def _splitmix64(x):
+    """SplitMix64 finalizer on a uint64 array (wrapping arithmetic)."""
+    import numpy as np
+    with np.errstate(over="ignore"):
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return x ^ (x >> np.uint64(31))
+
+
+def _ceiling_keys(count, distinct, salt):
+    import numpy as np
+    k = _splitmix64(_splitmix64(np.arange(count, dtype=np.uint64) + np.uint64(salt))
+                    % np.uint64(distinct))
+    return np.where(k >= np.uint64(2**64 - 2), k - np.uint64(2), k)
+
+
+@pytest.mark.parametrize("slots_log2,count,distinct,prefill", [(20, 700001, 300007, 5000),
+                                                               (16, 20000, 25000, 30000)])
+def test_probe_ceiling_inserts_each_new_key_once(ctx, slots_log2, count, distinct, prefill):
+    """The insert pass's table code on the measurement's keys adds exactly the
+    keys that are neither duplicates nor prefilled (counted on the CPU from
+    the same SplitMix64 draws)."""
+    note(f"vfb probe ceiling slots 2^{slots_log2} count={count} distinct={distinct} "
+         f"prefill={prefill}")
+    r = ctx.vfb_probe_ceiling(slots_log2, count, distinct, prefill)
+    keys = set(_ceiling_keys(count, distinct, 1).tolist())
+    pre = set(_ceiling_keys(prefill, 2**62 - 1, 2**62).tolist())
+    assert r["fresh"] == len(keys - pre)
+    assert r["probe_seconds"] > 0 and r["insert_seconds"] > 0
