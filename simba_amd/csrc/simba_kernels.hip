@@ -804,7 +804,7 @@ __device__ __forceinline__ uint32_t inst_tile(const AStage &S, uint32_t na, int 
 // Low-half prefilter of a lane's RB B values against the staged A values of
 // a tile: whether some pair of this lane may hit (exact, folded-window or
 // per-instance window test, by the launch's layout).
-template <bool SAME, int RB>
+template <bool SAME, int RB, int FIX = -1>
 __device__ __forceinline__ bool pair_prefilter(const AStage &S, uint32_t na, const uint32_t *bl,
                                                const bool *ok, uint32_t mask, bool bitops,
                                                uint32_t le, uint32_t lo, uint32_t sp) {
@@ -812,7 +812,7 @@ __device__ __forceinline__ bool pair_prefilter(const AStage &S, uint32_t na, con
 #pragma unroll
   for (int k = 0; k < RB; ++k) any_ok |= ok[k];
   const uint32_t full = SAME ? 0x3fu : 0x7fu;
-  const int mode = stage_mode(mask, SAME, sp);
+  const int mode = FIX >= 0 ? FIX : stage_mode(mask, SAME, sp);
   bool hit;
   if (mode == kStageExact) {
     hit = exact_tile<SAME, RB>(S, na, bl, le, bitops);
@@ -881,7 +881,7 @@ __device__ __forceinline__ void pair_rewalk(const AStage &S, uint32_t na, uint64
 // column values are loaded once (low halves) and combined with each of its
 // rows (up to bb.rows_pt block rows, one uniform load each), so a column
 // value read from memory serves every row of the tile.
-template <typename word, bool SAME, int RB, typename PL>
+template <typename word, bool SAME, int RB, typename PL, int FIX = -1>
 __device__ __forceinline__ void desc_band_tile(const TableView *__restrict__ tv,
                                                const AStage &S, const PL &p,
                                                uint32_t na, uint64_t a0, uint64_t bt,
@@ -959,7 +959,7 @@ __device__ __forceinline__ void desc_band_tile(const TableView *__restrict__ tv,
         break;
     }
     if (bb.nsteps) steps_lo<word, RB>(bb, tv, V, bl);
-    const bool hit = pair_prefilter<SAME, RB>(S, na, bl, ok, p.mask, bitops, le, lo, sp);
+    const bool hit = pair_prefilter<SAME, RB, FIX>(S, na, bl, ok, p.mask, bitops, le, lo, sp);
     if (__any(hit) && hit) {
       pair_rewalk<word, SAME, RB>(
           S, na, a0, ok, p.mask, lo_e, lo_o, span, pair, tag, hits, hit_count, hit_cap,
@@ -977,7 +977,7 @@ __device__ __forceinline__ void desc_band_tile(const TableView *__restrict__ tv,
 // The A values of the tile must be staged in S (stage_a).  p.mask selects the
 // operator instances (bit i: instance i, see HitRec); `tag` is or-ed into
 // HitRec::inst above bit 8 (the batch's instance index).
-template <typename word, bool SAME, int BSRC, int RB, typename PL>
+template <typename word, bool SAME, int BSRC, int RB, typename PL, int FIX = -1>
 __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
                                           const AStage &S, const word *__restrict__ A,
                                           const word *__restrict__ B, const PL &p,
@@ -989,15 +989,15 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
   const uint64_t a1 = a0 + p.ta < p.nA ? a0 + p.ta : p.nA;
   const uint32_t na = (uint32_t)(a1 - a0);
   if (BSRC == kBDesc && (1u << p.bb.logTB) == (uint32_t)(kPairThreads * RB)) {
-    desc_band_tile<word, SAME, RB, PL>(tv, S, p, na, a0, bt, bitops, lo_e, lo_o, span, hits,
-                                       hit_count, hit_cap, tag);
+    desc_band_tile<word, SAME, RB, PL, FIX>(tv, S, p, na, a0, bt, bitops, lo_e, lo_o, span,
+                                            hits, hit_count, hit_cap, tag);
     return;
   }
   uint32_t bl[RB];
   bool ok[RB];
   b_lo<word, BSRC, RB, PL>(tv, B, p, bt, bl, ok);
-  const bool hit = pair_prefilter<SAME, RB>(S, na, bl, ok, p.mask, bitops, lo32(lo_e),
-                                            lo32(lo_o), (uint32_t)span);
+  const bool hit = pair_prefilter<SAME, RB, FIX>(S, na, bl, ok, p.mask, bitops, lo32(lo_e),
+                                                 lo32(lo_o), (uint32_t)span);
   if (__any(hit) && hit) {
     const uint32_t pair = ((uint32_t)p.sb << 16) | (uint32_t)p.j;
     pair_rewalk<word, SAME, RB>(S, na, a0, ok, p.mask, lo_e, lo_o, span, pair, tag, hits,
@@ -1007,7 +1007,10 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
   }
 }
 
-template <typename word, bool SAME, int BSRC, int RB>
+// FIX: the stage mode when the launcher knows it (kStageWindow: every
+// operator instance against windows, the bulk of a search), so the kernel
+// holds that path only; -1: chosen per launch on the device.
+template <typename word, bool SAME, int BSRC, int RB, int FIX>
 __global__ void __launch_bounds__(kPairThreads)
 k_pair_check(const TableView *__restrict__ tv, const word *__restrict__ A,
              const word *__restrict__ B, PairLaunch p, uint64_t lo_e64, uint64_t lo_o64,
@@ -1015,7 +1018,7 @@ k_pair_check(const TableView *__restrict__ tv, const word *__restrict__ A,
              uint32_t hit_cap) {
   __shared__ AStage S;
   const word lo_e = (word)lo_e64, lo_o = (word)lo_o64, span = (word)span64;
-  const int mode = stage_mode(p.mask, SAME, (uint32_t)span);
+  const int mode = FIX >= 0 ? FIX : stage_mode(p.mask, SAME, (uint32_t)span);
   const uint32_t le = lo32(lo_e), lo = lo32(lo_o);
   uint64_t staged = ~0ull;
   bool bitops = false;
@@ -1027,8 +1030,8 @@ k_pair_check(const TableView *__restrict__ tv, const word *__restrict__ A,
       bitops = stage_a<word>(S, tv, A, p.gen_a, p.j, a0, a1, mode, le, lo, (uint32_t)span);
       staged = at;
     }
-    pair_tile<word, SAME, BSRC, RB, PairLaunch>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o,
-                                                span, hits, hit_count, hit_cap, 0u);
+    pair_tile<word, SAME, BSRC, RB, PairLaunch, FIX>(tv, S, A, B, p, at, bt, bitops, lo_e,
+                                                     lo_o, span, hits, hit_count, hit_cap, 0u);
   }
 }
 
@@ -1820,14 +1823,22 @@ static void pair_dispatch(const TableView *d_tv, const word *A, const word *B,
                           const PairLaunch &p, const CheckTargets &tg, HitRec *h,
                           uint32_t *hc, uint32_t cap, int grid, hipStream_t s) {
   const bool same = p.j == p.m;
-#define PAIR_LAUNCH(S, BS)                                                           \
-  do {                                                                               \
-    if (p.rb == 8)                                                                   \
-      hipLaunchKernelGGL((k_pair_check<word, S, BS, 8>), dim3(grid), dim3(kPairThreads), \
-                         0, s, d_tv, A, B, p, tg.lo_even, tg.lo_odd, tg.span, h, hc, cap); \
-    else                                                                             \
-      hipLaunchKernelGGL((k_pair_check<word, S, BS, 4>), dim3(grid), dim3(kPairThreads), \
-                         0, s, d_tv, A, B, p, tg.lo_even, tg.lo_odd, tg.span, h, hc, cap); \
+  // the folded window layout (stage_mode): every instance, a span above 0
+  const uint32_t full = same ? 0x3fu : 0x7fu;
+  const bool win = (p.mask & full) == full && tg.span != 0;
+#define PAIR_LAUNCH_F(S, BS, F)                                                          \
+  do {                                                                                   \
+    if (p.rb == 8)                                                                       \
+      hipLaunchKernelGGL((k_pair_check<word, S, BS, 8, F>), dim3(grid), dim3(kPairThreads), \
+                         0, s, d_tv, A, B, p, tg.lo_even, tg.lo_odd, tg.span, h, hc, cap);  \
+    else                                                                                 \
+      hipLaunchKernelGGL((k_pair_check<word, S, BS, 4, F>), dim3(grid), dim3(kPairThreads), \
+                         0, s, d_tv, A, B, p, tg.lo_even, tg.lo_odd, tg.span, h, hc, cap);  \
+  } while (0)
+#define PAIR_LAUNCH(S, BS)                   \
+  do {                                       \
+    if (win) PAIR_LAUNCH_F(S, BS, kStageWindow); \
+    else PAIR_LAUNCH_F(S, BS, -1);           \
   } while (0)
   // the B side: its table, a block formed from tables, or a generic rank range
   const int bs = !p.gen_b ? kBTable : p.bb.generic ? kBGeneric : kBDesc;
@@ -1841,6 +1852,7 @@ static void pair_dispatch(const TableView *d_tv, const word *A, const word *B,
     else PAIR_LAUNCH(false, kBGeneric);
   }
 #undef PAIR_LAUNCH
+#undef PAIR_LAUNCH_F
 }
 
 hipError_t launch_pair_check(int width, const TableView *d_tv, const void *d_base,
