@@ -27,7 +27,9 @@ the host share of one GPU) on the same instance, every size 1..11; the first
 solution must be the GPU's.
 
 The same line carries compact records of the other configurations
-(`configs`): C1 time to first solution, C2 VFB levels 1..13, C4 batched
+(`configs`): C1 time to first solution, C2 VFB levels 1..13 (with the
+insert pass against its measured random-access ceiling), C3 as the
+reference runs it (one search to size 20 under a 60 s cap), C4 batched
 solves/s, C5 learned vs untrained proposal.  `--no-configs` skips them; each
 has its own workload for a full line.
 """
@@ -241,8 +243,8 @@ def run_simba(args):
                       % (args.cpu_size, cb["candidates"], cb["seconds"], S)}
     if not args.no_configs:
         out["configs"] = {}
-        for name, fn in (("c1_latency", rec_c1), ("c2_vfb", rec_c2), ("c4_batch", rec_c4),
-                         ("c5_learned", rec_c5)):
+        for name, fn in (("c1_latency", rec_c1), ("c2_vfb", rec_c2), ("c3_capped", rec_c3_cap),
+                         ("c4_batch", rec_c4), ("c5_learned", rec_c5)):
             log("config record: " + name)
             out["configs"][name] = _safe(fn, args)
     return out
@@ -563,6 +565,12 @@ def run_learned(args):
 
 
 def run_simba_cap(args):
+    rec = rec_c3_cap(args)
+    return _line("Gcand/s evaluated", rec["Gcand_s"], "Gcand/s", True, rec["wall_ms"], rec,
+                 warmup=0)
+
+
+def rec_c3_cap(args):
     """The default instance searched to size 20 under a time cap, as the
     reference's runs are (one run): how far the search gets (the completed
     sizes and the splits of the next size checked whole), and its rate over
@@ -598,10 +606,11 @@ def run_simba_cap(args):
                         "of_splits": (s1 - 1) // 2, "candidates": part},
            "search_seconds": round(r["seconds"], 3),
            "rate_completed_sizes_Gcand_s": round(done / r["seconds"] / 1e9, 3),
-           "first_solution": r["expr"], "first_solution_size": r["size"]}
+           "first_solution": r["expr"], "first_solution_size": r["size"],
+           "Gcand_s": round((done + part) / r["seconds"] / 1e9, 3),
+           "wall_ms": round(1e3 * wall, 1)}
     ctx.close()
-    return _line("Gcand/s evaluated", round((done + part) / r["seconds"] / 1e9, 3), "Gcand/s",
-                 True, round(1e3 * wall, 1), rec, warmup=0)
+    return rec
 
 
 def run_copy(args):
