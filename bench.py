@@ -593,7 +593,15 @@ def rec_c3_cap(args):
     for j in range(1, r["split_reached"] + 1):
         m = s1 - 1 - j
         part += (5 + (1 if j == m else 2)) * T[j][8] * T[m][8]
-    part += 2 * T[s1 - 1][8] if r["split_reached"] else 0
+    # and of the split after those, the value pairs of the B blocks whose
+    # launches had completed (each pair decides the split's 6 or 7 trees)
+    j1 = r["split_reached"] + 1
+    m1 = s1 - 1 - j1
+    part += (5 + (1 if j1 == m1 else 2)) * r["split_pairs"]
+    # the unary-rooted trees of that size, decided by the windows of the
+    # completed sizes (counted once the size is entered)
+    unary = 2 * T[s1 - 1][8] if (r["split_reached"] or r["split_pairs"]) else 0
+    part += unary
     # the completed sizes alone, same windows, no cap: their own time
     rs = ctx.simba(inst, min_size=1, max_size=20, exhaustive=True, stop_size=r["size_reached"],
                    table_bytes=args.table_gb << 30)
@@ -603,7 +611,10 @@ def rec_c3_cap(args):
            "completed_sizes_alone": {"seconds": round(rs["seconds"], 3),
                                      "Gcand_s": round(done / rs["seconds"] / 1e9, 3)},
            "furthest": {"size": s1, "splits_checked": r["split_reached"],
-                        "of_splits": (s1 - 1) // 2, "candidates": part},
+                        "of_splits": (s1 - 1) // 2,
+                        "next_split_pairs_checked": r["split_pairs"],
+                        "next_split_pairs": T[j1][8] * T[m1][8],
+                        "unary_trees": unary, "candidates": part},
            "search_seconds": round(r["seconds"], 3),
            "rate_completed_sizes_Gcand_s": round(done / r["seconds"] / 1e9, 3),
            "first_solution": r["expr"], "first_solution_size": r["size"],

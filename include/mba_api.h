@@ -119,7 +119,10 @@ typedef struct {
                             /* inside size size_reached + 1: its splits      */
                             /* {j, s-1-j}, j = 1..split_reached, were fully  */
                             /* checked (0: none)                             */
-  int32_t reserved[1];
+  int32_t reserved;
+  uint64_t split_pairs;     /* and of split split_reached + 1, the (A, B)    */
+                            /* value pairs of the B blocks whose launches    */
+                            /* had completed when the budget stopped it      */
 } mba_result;
 
 /* VFB per-level statistics (PRINT_CACHE_SIZE mode, full-cache.cu:763-765). */

@@ -55,7 +55,8 @@ class Result(ctypes.Structure):
                 ("kernel_seconds", ctypes.c_double), ("candidates", ctypes.c_uint64),
                 ("size_reached", ctypes.c_int32), ("kernel_launches", ctypes.c_uint32),
                 ("hit_replays", ctypes.c_int32), ("ordered_sizes", ctypes.c_int32),
-                ("split_reached", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1)]
+                ("split_reached", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("split_pairs", ctypes.c_uint64)]
 
     def as_dict(self):
         return {"status": self.status, "size": self.size, "rank": int(self.rank),
@@ -65,7 +66,7 @@ class Result(ctypes.Structure):
                 "kernel_launches": self.kernel_launches,
                 "candidates": int(self.candidates), "size_reached": self.size_reached,
                 "hit_replays": self.hit_replays, "ordered_sizes": self.ordered_sizes,
-                "split_reached": self.split_reached}
+                "split_reached": self.split_reached, "split_pairs": int(self.split_pairs)}
 
 
 def _result_array(n):

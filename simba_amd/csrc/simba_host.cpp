@@ -700,6 +700,9 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
   // hits overflow the buffer.
   bool group_open = false;
   uint64_t group_tiles = 0;
+  // pairs of the split being swept: of B blocks whose launches were all
+  // issued (pending) and of those whose launches have completed (done)
+  uint64_t split_pairs_pending = 0, split_pairs_done = 0;
   std::vector<std::pair<PairLaunch, CheckTargets>> group;
   // Replays a launch's tile range in `pieces` parts, resolving each part's
   // hits before the next; a part that still overflows the hit buffer is
@@ -765,6 +768,8 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     ktime += ms * 1e-3;
     group_open = false;
     group_tiles = 0;
+    split_pairs_done += split_pairs_pending;  // every launch issued so far is done
+    split_pairs_pending = 0;
     if (resolve || answer < 0) {
       if (cnt > kHitCap) {
         if (group.empty()) {
@@ -824,6 +829,8 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
       while (p.ta > kMinTA && (p.nA + p.ta - 1) / p.ta * p.nTB < kSmallLaunchTiles) p.ta /= 2;
       p.nTA = (p.nA + p.ta - 1) / p.ta;
       const uint64_t tiles = p.nTA * p.nTB;
+      const uint64_t block_pairs =
+          p.nA * (!p.gen_b ? p.nB : bb.generic ? bb.nA : bb.nA * bb.nB);
       for (uint64_t t0 = 0; t0 < tiles; t0 += tiles_per_launch) {
         p.tile_begin = t0;
         p.tile_end = std::min(tiles, t0 + tiles_per_launch);
@@ -842,6 +849,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
         ++klaunch;
         group.emplace_back(p, tg);
         group_tiles += ntl;
+        if (p.tile_end == tiles) split_pairs_pending += block_pairs;  // block fully issued
         if (group_tiles >= tiles_per_launch) {
           if (int rc = close_group(false)) return rc;
         }
@@ -932,12 +940,16 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
       // max_size) the windows' hits no longer matter
       int splits_done = 0;
       for (int j = 1; j <= (s - 1) / 2 && !stopped; ++j) {
+        split_pairs_pending = split_pairs_done = 0;
         const int rc = run_pair(s, j, kAllInst, tg, true);
         if (rc < 0) return MBA_ERR_HIP;
         stopped = rc == 1;
         if (rc == 0) splits_done = j;
       }
-      if (stopped) res.split_reached = splits_done;
+      if (stopped) {
+        res.split_reached = splits_done;
+        res.split_pairs = split_pairs_done;
+      }
       if (!stopped) {
         const int rc = close_group(answer < 0);
         if (rc < 0) return MBA_ERR_HIP;

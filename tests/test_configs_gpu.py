@@ -161,3 +161,23 @@ def test_device_fingerprint_matches_oracle(ctx, W):
         got = ctx.vfb_fingerprints(np.array(rows, dtype=np.uint64), W)
         want = [orc.fingerprint(r, W) for r in rows]
         assert [int(x) for x in got] == want
+
+
+def test_capped_sweep_reports_checked_splits_and_pairs(ctx):
+    """A search to size 20 stopped by a time budget (the C3 run, shortened)
+    keeps the answer of the sizes it completed and reports how far it got:
+    the whole splits of the next size and, of the split after them, the
+    value pairs of the B blocks whose launches completed -- within that
+    split's count."""
+    from simba_amd.synth import count_table, make_instance
+    inst = make_instance(15, 4, 64, 15, 64, (1,))
+    note("capped C3 sweep, budget 1.5 s")
+    r = ctx.simba(inst, min_size=1, max_size=20, exhaustive=True, budget_s=1.5,
+                  table_bytes=40 << 30)
+    T = count_table(5, 20)
+    assert r["status"] == 0 and r["size"] == 11
+    assert 11 <= r["size_reached"] < 20
+    s1 = r["size_reached"] + 1
+    j1 = r["split_reached"] + 1
+    assert 0 <= r["split_reached"] <= (s1 - 1) // 2
+    assert 0 <= r["split_pairs"] <= T[j1][8] * T[s1 - 1 - j1][8]
