@@ -179,4 +179,22 @@ MBA_HD inline uint32_t div_u32(uint32_t n, const DivMagic &g) {
   return (t1 + ((n - t1) >> g.sh1)) >> g.sh2;
 }
 
+// floor(n / d), d >= 1, for 64-bit operands without the software 64-bit
+// division: 32-bit operands divide directly; otherwise the double-precision
+// quotient (within a few units of the true one: each operand is rounded to
+// 53 bits) is corrected by whole steps of d.
+MBA_HD inline uint64_t udiv64(uint64_t n, uint64_t d) {
+  if ((n >> 32) == 0 && (d >> 32) == 0) return (uint32_t)n / (uint32_t)d;
+  double qd = (double)n / (double)d;
+  uint64_t q = qd >= 18446744073709551615.0 ? ~0ull : (uint64_t)qd;
+  // lower q while q * d exceeds n (or overflows 64 bits)
+  for (;;) {
+    const unsigned __int128 p = (unsigned __int128)q * d;
+    if (p <= n) break;
+    --q;
+  }
+  while (n - q * d >= d) ++q;  // q * d <= n here, so no wrap
+  return q;
+}
+
 }  // namespace mba

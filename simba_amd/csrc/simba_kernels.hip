@@ -83,7 +83,7 @@ __device__ word value_of(const TableView *__restrict__ tv, int m, uint64_t r) {
       n -= blk;
     }
     const uint64_t rs = tv->T[cs - j][8];
-    const uint64_t a = n / rs;
+    const uint64_t a = udiv64(n, rs);
     st[sp++] = {a, j, it.pos};
     st[sp++] = {n - a * rs, cs - j, it.pos + j};
   }
@@ -139,7 +139,7 @@ __device__ __forceinline__ void unrank_rpn(const TableView *__restrict__ tv, int
       n -= blk;
     }
     const uint64_t rs = tv->T[cs - j][8];
-    const uint64_t a = n / rs;
+    const uint64_t a = udiv64(n, rs);
     st[sp++] = {a, j, it.pos};
     st[sp++] = {n - a * rs, cs - j, it.pos + j};
   }
@@ -1281,7 +1281,12 @@ k_hit_confirm(const TableView *__restrict__ tvs, const uint64_t *__restrict__ le
   __shared__ word s_stack[kVerifyDepth][64];
   const int lane = (int)threadIdx.x;
   const uint32_t ns = *n_surv;
-  for (uint32_t e = blockIdx.x; e < ns; e += gridDim.x) {  // workgroup-uniform
+  // a contiguous run of candidates per workgroup (consecutive candidates
+  // mostly share an instance, whose table view then stays staged)
+  const uint32_t per = (ns + gridDim.x - 1) / gridDim.x;
+  const uint32_t e0 = blockIdx.x * per, e1 = min(ns, e0 + per);
+  uint32_t staged = ~0u;
+  for (uint32_t e = e0; e < e1; ++e) {  // workgroup-uniform
     const SurvRec c = surv[e];
     const uint32_t slot = c.slot & 0x7fffffffu;
     const bool odd = (c.slot >> 31) != 0u;
@@ -1290,7 +1295,10 @@ k_hit_confirm(const TableView *__restrict__ tvs, const uint64_t *__restrict__ le
     const HitRec g = hits[c.hit];
     const uint32_t I = g.inst >> 8, inst = g.inst & 0xffu;
     const BatchInstData d = idat[I];
-    stage_view(&s_tv, tvs + I);
+    if (I != staged) {
+      stage_view(&s_tv, tvs + I);
+      staged = I;
+    }
     __syncthreads();
     const int L = s_tv.L;
     const int sb = (int)(g.pair >> 16), j = (int)(g.pair & 0xffffu), m = sb - 1 - j;

@@ -2,7 +2,8 @@
 // (simba_amd/csrc/pair_math.h).
 //
 // Division by an invariant divisor: div_u32(n, div_magic(d)) == n / d on
-// every small (n, d), around powers of two and at random.
+// every small (n, d), around powers of two and at random; udiv64(n, d) ==
+// n / d around 2^32, 2^53, 2^63 and 2^64 and at random.
 //
 // Window prefilter: for every operator instance and parity,
 // a pair whose value lies inside an exact unary-chain window must pass the
@@ -128,10 +129,42 @@ int check_division(std::mt19937_64 &rng) {
   return bad ? 5 : 0;
 }
 
+int check_udiv64(std::mt19937_64 &rng) {
+  long bad = 0, n_checked = 0;
+  auto check = [&](uint64_t n, uint64_t d) {
+    if (d == 0) return;
+    ++n_checked;
+    if (mba::udiv64(n, d) != n / d) {
+      if (bad < 10)
+        std::printf("UDIV64 n %llu d %llu got %llu\n", (unsigned long long)n,
+                    (unsigned long long)d, (unsigned long long)mba::udiv64(n, d));
+      ++bad;
+    }
+  };
+  const uint64_t big[] = {0ull, 1ull, 2ull, 3ull, 0xffffffffull, 0x100000000ull, 0x100000001ull,
+                          (1ull << 53) - 1, 1ull << 53, (1ull << 53) + 1, 1ull << 63,
+                          (1ull << 63) + 1, ~0ull - 1, ~0ull};
+  for (uint64_t n : big)
+    for (uint64_t d : big) {
+      for (int dn = -2; dn <= 2; ++dn) {
+        check(n + dn, d);
+        check(n, d + dn);
+      }
+    }
+  for (int i = 0; i < 3000000; ++i) {
+    const uint64_t n = rng() >> (rng() % 64), d = rng() >> (rng() % 64);
+    check(n, d);
+    if (d) check(n / d * d, d), check(n / d * d + d - 1 >= n / d * d ? n / d * d + d - 1 : n, d);
+  }
+  std::printf("64-bit divisions checked %ld, wrong %ld\n", n_checked, bad);
+  return bad ? 6 : 0;
+}
+
 int main() {
   std::mt19937_64 rng(20261204);
   auto r32 = [&]() { return (uint32_t)rng(); };
   if (int rc = check_division(rng)) return rc;
+  if (int rc = check_udiv64(rng)) return rc;
   const uint32_t edges[] = {0u, 1u, 2u, 0x7fffffffu, 0x80000000u, 0x80000001u, 0xfffffffeu,
                             0xffffffffu, 0x55555555u, 0xaaaaaaaau};
   long checked = 0, misses = 0;
