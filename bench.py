@@ -100,9 +100,15 @@ def naive_pair_kernel_ops(T, C, L, max_size, W):
 
 def pair_count(T, s_lo, s_hi):
     """(A, B) value pairs the pair kernel checks for sizes s_lo..s_hi: one per
-    pair of trees of an unordered split {j, s-1-j}."""
-    return sum(T[j][8] * T[s - 1 - j][8] for s in range(max(s_lo, 3), s_hi + 1)
-               for j in range(1, (s - 1) // 2 + 1))
+    pair of trees of an unordered split {j, s-1-j}, j < s-1-j; an equal-size
+    split {j, j} (both sides tabulated) one per unordered pair of its trees,
+    T[j] (T[j] + 1) / 2, every operator instance giving both orders."""
+    tot = 0
+    for s in range(max(s_lo, 3), s_hi + 1):
+        for j in range(1, (s - 1) // 2 + 1):
+            n, m = T[j][8], T[s - 1 - j][8]
+            tot += n * (n + 1) // 2 if j == s - 1 - j else n * m
+    return tot
 
 
 def profile_fields(name):

@@ -807,6 +807,9 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
     p.gen_a = j > Kt;
     p.gen_b = m > Kt;
     p.mask = mask;
+    // an equal-size split with both sides tabulated: only the tiles that
+    // reach the diagonal, every operator instance (min-rank trees a <= b)
+    p.tri = j == m && !p.gen_a && !p.gen_b && (mask & kAllInst) == kAllInst;
     std::vector<BBlock> one(1);
     const std::vector<BBlock> *blocks = &one;
     if (p.gen_b) {

@@ -87,6 +87,9 @@ struct PairCore {
   uint64_t nTA;            // number of A tiles
   uint64_t nTB;            // number of B tiles (of the block when gen_b)
   bool gen_a, gen_b;       // side is above Kt: values computed per lane
+  bool tri;                // j == m, both sides tabulated, checked as a j < m
+                           // split (all seven instances) over the tiles that
+                           // reach A index <= B index only
   uint32_t mask;           // operator instances to check (bit i = inst i)
   int rb;                  // B values per lane: 4 or 8 (tile = 256 * rb B values)
   uint32_t ta;             // A values per tile (<= kPairTA; nTA = ceil(nA / ta))

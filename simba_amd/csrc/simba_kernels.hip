@@ -1024,6 +1024,11 @@ k_pair_check(const TableView *__restrict__ tv, const word *__restrict__ A,
   bool bitops = false;
   for (uint64_t t = p.tile_begin + blockIdx.x; t < p.tile_end; t += gridDim.x) {
     const uint64_t at = p.nTA == 1 ? 0 : t % p.nTA, bt = p.nTA == 1 ? t : t / p.nTA;
+    // an equal-size split as a triangle: a tile whose every A index exceeds
+    // every B index holds only pairs (a, b), a > b, whose mirror (b, a) lies
+    // in a kept tile, where all seven instances give both orders' trees
+    if (p.tri && at * p.ta > bt * (uint64_t)(kPairThreads * RB) + (kPairThreads * RB - 1))
+      continue;
     if (at != staged) {  // workgroup-uniform: every wave takes the same tiles
       const uint64_t a0 = at * p.ta;
       const uint64_t a1 = a0 + p.ta < p.nA ? a0 + p.ta : p.nA;
@@ -1832,7 +1837,7 @@ static void pair_dispatch(const TableView *d_tv, const word *A, const word *B,
                           uint32_t *hc, uint32_t cap, int grid, hipStream_t s) {
   const bool same = p.j == p.m;
   // the folded window layout (stage_mode): every instance, a span above 0
-  const uint32_t full = same ? 0x3fu : 0x7fu;
+  const uint32_t full = same && !p.tri ? 0x3fu : 0x7fu;
   const bool win = (p.mask & full) == full && tg.span != 0;
 #define PAIR_LAUNCH_F(S, BS, F)                                                          \
   do {                                                                                   \
@@ -1850,7 +1855,9 @@ static void pair_dispatch(const TableView *d_tv, const word *A, const word *B,
   } while (0)
   // the B side: its table, a block formed from tables, or a generic rank range
   const int bs = !p.gen_b ? kBTable : p.bb.generic ? kBGeneric : kBDesc;
-  if (same) {
+  if (same && p.tri) {
+    PAIR_LAUNCH(false, kBTable);
+  } else if (same) {
     if (bs == kBTable) PAIR_LAUNCH(true, kBTable);
     else if (bs == kBDesc) PAIR_LAUNCH(true, kBDesc);
     else PAIR_LAUNCH(true, kBGeneric);
