@@ -423,6 +423,7 @@ int batch_solve(Ctx &ctx, const mba_instance *insts, size_t count, const mba_opt
       J.p.gen_a = j > st.Kt;
       J.p.gen_b = J.p.m > st.Kt;
       J.p.mask = mask;
+      J.p.tri = j == J.p.m && !J.p.gen_a && !J.p.gen_b && (mask & kAllInst) == kAllInst;
       J.p.rb = kPairRB;
       J.lo_e = tg.lo_even;
       J.lo_o = tg.lo_odd;

@@ -615,6 +615,7 @@ int simba_solve(Ctx &ctx, const mba_instance &inst, const mba_opts &o, mba_resul
         J.p.gen_a = j > Kt;
         J.p.gen_b = J.p.m > Kt;
         J.p.mask = kAllInst;
+        J.p.tri = j == J.p.m && !J.p.gen_a && !J.p.gen_b;
         J.p.rb = kPairRB;
         J.lo_e = tgz.lo_even;
         J.lo_o = tgz.lo_odd;

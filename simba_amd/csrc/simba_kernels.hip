@@ -1451,12 +1451,14 @@ __device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J,
   pair_tile<word, SAME_, BS, kPairRB, BatchPair>(tv, S, A, B, p, at, bt, bitops, lo_e, lo_o, \
                                                  span, hits, hc, cap, tag)
   if (!BLOCKS) {
-    // J.bsub consecutive B tiles per job tile
+    // J.bsub consecutive B tiles per job tile; an equal-size split as a
+    // triangle (PairCore::tri) skips the tiles wholly below the diagonal
     const uint32_t sub = J.bsub > 1 ? J.bsub : 1;
-    const uint64_t b0 = bt * sub;
+    const uint64_t b0 = bt * sub, tbw = (uint64_t)(kPairThreads * kPairRB);
     for (uint32_t q = 0; q < sub && b0 + q < p.nTB; ++q) {
       bt = b0 + q;
-      if (p.j == p.m) BATCH_TILE(true, kBTable);
+      if (p.tri && at * p.ta > bt * tbw + tbw - 1) continue;
+      if (p.j == p.m && !p.tri) BATCH_TILE(true, kBTable);
       else BATCH_TILE(false, kBTable);
     }
   } else if (p.j == p.m) {
