@@ -912,17 +912,14 @@ __device__ __forceinline__ void desc_band_tile(const TableView *__restrict__ tv,
   const uint32_t nr = (uint32_t)(nY - y0 < (uint64_t)bb.rows_pt ? nY - y0 : bb.rows_pt);
   const uint32_t pair = ((uint32_t)p.sb << 16) | (uint32_t)p.j;
   const uint32_t le = lo32(lo_e), lo = lo32(lo_o), sp = (uint32_t)span;
-  // the tile's row values, loaded together (up to kBandRows)
-  uint32_t rows_lo[kBandRows];
-#pragma unroll
-  for (uint32_t r = 0; r < kBandRows; ++r)
-    rows_lo[r] = uniform(bb.op < 0 || r >= nr ? 0u : lo32_at<word>(rowp, y0 + r));
+  // the tile's row values in one register: lane q < nr loads row q, each
+  // row is read back with one lane read (up to kBandRows rows)
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t rows_v = bb.op < 0 || lane >= nr ? 0u : lo32_at<word>(rowp, y0 + lane);
 #pragma unroll 1
   for (uint32_t r = 0; r < nr; ++r) {  // workgroup-uniform
     const uint64_t y = y0 + r;
-    uint32_t rowv = rows_lo[0];
-#pragma unroll
-    for (uint32_t q = 1; q < kBandRows; ++q) rowv = r == q ? rows_lo[q] : rowv;
+    const uint32_t rowv = __builtin_amdgcn_readlane(rows_v, r);
     switch (bb.op) {  // workgroup-uniform; only '-' depends on the order
       case AND:
 #pragma unroll
