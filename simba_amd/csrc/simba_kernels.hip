@@ -466,7 +466,11 @@ __device__ __forceinline__ bool stage_a(AStage &S, const TableView *__restrict__
 }
 
 // ---- B side sources --------------------------------------------------------
-enum : int { kBTable = 0, kBDesc = 1, kBGeneric = 2 };
+// kBDesc: blocks formed from tables, row-band or general tiles chosen per
+// block on the device; kBBand / kBGrid: only row-band / only general block
+// tiles (the per-instance launcher knows which, and each instance holds one
+// path).
+enum : int { kBTable = 0, kBDesc = 1, kBGeneric = 2, kBBand = 3, kBGrid = 4 };
 
 template <typename word>
 __device__ __forceinline__ uint32_t lo32_at(const word *p, uint64_t i) {
@@ -607,7 +611,7 @@ template <typename word, int BSRC, int RB, typename PL>
 __device__ __forceinline__ word b_full(const TableView *__restrict__ tv,
                                        const word *__restrict__ B, const PL &p,
                                        uint64_t bt, int k, uint64_t &rank) {
-  if (BSRC == kBDesc) {
+  if (BSRC == kBDesc || BSRC == kBBand || BSRC == kBGrid) {
     uint64_t ra, rb;
     desc_coords<RB>(p.bb, bt, k, ra, rb);
     rank = p.bb.start + p.bb.stride * (ra * p.bb.nB + rb);
@@ -628,7 +632,7 @@ template <typename word, int BSRC, int RB, typename PL>
 __device__ __forceinline__ void b_lo(const TableView *__restrict__ tv,
                                      const word *__restrict__ B, const PL &p,
                                      uint64_t bt, uint32_t *bl, bool *ok) {
-  if (BSRC == kBDesc) {
+  if (BSRC == kBDesc || BSRC == kBGrid) {
     desc_lo<word, RB>(p.bb, tv, global_ptr<const word>(tv->base), bt, bl, ok);
     return;
   }
@@ -985,7 +989,8 @@ __device__ __forceinline__ void pair_tile(const TableView *__restrict__ tv,
   const uint64_t a0 = at * p.ta;
   const uint64_t a1 = a0 + p.ta < p.nA ? a0 + p.ta : p.nA;
   const uint32_t na = (uint32_t)(a1 - a0);
-  if (BSRC == kBDesc && (1u << p.bb.logTB) == (uint32_t)(kPairThreads * RB)) {
+  if (BSRC == kBBand ||
+      (BSRC == kBDesc && (1u << p.bb.logTB) == (uint32_t)(kPairThreads * RB))) {
     desc_band_tile<word, SAME, RB, PL, FIX>(tv, S, p, na, a0, bt, bitops, lo_e, lo_o, span,
                                             hits, hit_count, hit_cap, tag);
     return;
@@ -1853,16 +1858,19 @@ static void pair_dispatch(const TableView *d_tv, const word *A, const word *B,
     else PAIR_LAUNCH_F(S, BS, -1);           \
   } while (0)
   // the B side: its table, a block formed from tables, or a generic rank range
-  const int bs = !p.gen_b ? kBTable : p.bb.generic ? kBGeneric : kBDesc;
+  const int bs = !p.gen_b ? kBTable : p.bb.generic ? kBGeneric
+                 : (1u << p.bb.logTB) == (uint32_t)(kPairThreads * p.rb) ? kBBand : kBGrid;
   if (same && p.tri) {
     PAIR_LAUNCH(false, kBTable);
   } else if (same) {
     if (bs == kBTable) PAIR_LAUNCH(true, kBTable);
-    else if (bs == kBDesc) PAIR_LAUNCH(true, kBDesc);
+    else if (bs == kBBand) PAIR_LAUNCH(true, kBBand);
+    else if (bs == kBGrid) PAIR_LAUNCH(true, kBGrid);
     else PAIR_LAUNCH(true, kBGeneric);
   } else {
     if (bs == kBTable) PAIR_LAUNCH(false, kBTable);
-    else if (bs == kBDesc) PAIR_LAUNCH(false, kBDesc);
+    else if (bs == kBBand) PAIR_LAUNCH(false, kBBand);
+    else if (bs == kBGrid) PAIR_LAUNCH(false, kBGrid);
     else PAIR_LAUNCH(false, kBGeneric);
   }
 #undef PAIR_LAUNCH
