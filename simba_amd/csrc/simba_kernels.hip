@@ -1409,7 +1409,7 @@ struct BlockCursor {
 };
 
 template <typename word, bool BLOCKS>
-__device__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J,
+__device__ __forceinline__ void batch_pair(AStage &S, const TableView *tv, const BatchJob &J,
                            const BBlock *__restrict__ blocks, uint64_t lt, uint64_t &staged,
                            bool &bitops, BlockCursor &bc, HitRec *hits, uint32_t *hc,
                            uint32_t cap) {
