@@ -387,6 +387,8 @@ struct AStage {
   uint4 q0[kPairTA];
   uint4 q1[kPairTA];
   uint64_t full[kPairTA];  // the full-width A values (exact re-walk of a tile)
+  uint32_t fmask[kPairTA / 32];  // window mode: bit i % 32 of word i / 32 set
+                                 // when A value i has bitop flags
 };
 
 // a * b = t (mod 2^32) as a compare: with a = 2^k a' (a' odd), it holds iff
@@ -440,6 +442,7 @@ __device__ __forceinline__ bool stage_a(AStage &S, const TableView *__restrict__
   __syncthreads();  // the previous tile's readers are done
   const uint32_t i = threadIdx.x;
   bool bit = false;
+  uint32_t fl = 0u;  // window mode: this A value's bitop flags
   if (a0 + i < a1) {
     const word av = gen_a ? value_of<word>(tv, j, a0 + i) : A[a0 + i];
     S.full[i] = (uint64_t)av;
@@ -455,11 +458,19 @@ __device__ __forceinline__ bool stage_a(AStage &S, const TableView *__restrict__
       const WinPre w = win_pre(le, lo, span);
       uint32_t f1, f2;
       win_stage(w, al, f1, f2);
+      fl = bitop_flags(al, le, lo, span);
       S.q0[i] = make_uint4(al, ~al, f1, f2);
-      S.q1[i] = make_uint4(0u, 0u, 0u, bitop_flags(al, le, lo, span));
+      S.q1[i] = make_uint4(0u, 0u, 0u, fl);
     } else {
       S.q0[i] = make_uint4(al, le - al, lo - al, al - le);
       S.q1[i] = make_uint4(al - lo, al + le, al + lo, 0u);
+    }
+  }
+  {  // the flagged A values as bit masks, 32 per word (a wave covers two)
+    const uint64_t m = __ballot(fl != 0u);
+    if ((i & 63u) == 0u) {
+      S.fmask[i >> 5] = (uint32_t)m;
+      S.fmask[(i >> 5) + 1] = (uint32_t)(m >> 32);
     }
   }
   return __syncthreads_or(bit) != 0;
@@ -753,11 +764,15 @@ __device__ __forceinline__ uint32_t window_tile(const AStage &S, uint32_t na,
 #pragma unroll
   for (int k = 0; k < RB; ++k) fb[k] = fold32(bl[k]);
   uint32_t acc = 0xffffffffu;
+  uint32_t fm = 0u;  // flagged A values of the current 32
   for (uint32_t i = 0; i < na; ++i) {
+    if ((i & 31u) == 0u) fm = uniform(S.fmask[i >> 5]);
     const uint4 q0 = S.q0[i];
-    const uint32_t fl = uniform(S.q1[i].w);  // bitop_flags of this A value
-    if (fl & (kAndEven | kAndOdd)) acc = bitop_tile<true, RB>(w, q0.x, bl, fl & 3u, acc);
-    if (fl & (kOrEven | kOrOdd)) acc = bitop_tile<false, RB>(w, q0.x, bl, fl >> 2, acc);
+    if ((fm >> (i & 31u)) & 1u) {
+      const uint32_t fl = uniform(S.q1[i].w);  // bitop_flags of this A value
+      if (fl & (kAndEven | kAndOdd)) acc = bitop_tile<true, RB>(w, q0.x, bl, fl & 3u, acc);
+      if (fl & (kOrEven | kOrOdd)) acc = bitop_tile<false, RB>(w, q0.x, bl, fl >> 2, acc);
+    }
 #pragma unroll
     for (int k = 0; k < RB; ++k) acc = win_pair(w, q0.x, q0.y, q0.z, q0.w, bl[k], fb[k], acc);
   }
